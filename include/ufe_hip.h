@@ -1,0 +1,258 @@
+/*
+ * ufe_hip.h -- C ABI of the MI355X engine for UFEMISM's velocity solve and
+ * vertical heat columns (libufe_hip.so).
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 = ok, > 0 = completed with a warning
+ *     (UFE_WARN_*), < 0 = error (UFE_ERR_*); ufe_last_error() gives the text.
+ *   - Arrays are plain host pointers owned by the caller.  Indices are 0-based
+ *     (the Fortran shim converts).  Two-dimensional (vertex or triangle, layer)
+ *     fields are stored with the vertex/triangle index fastest, which is
+ *     Fortran's (nV, nz) column-major layout.
+ *   - Calls are synchronous.  One context per host thread; a context is bound
+ *     to the device it was created on.
+ *   - Units follow the reference: m, yr, Pa, K, J.
+ */
+#ifndef UFE_HIP_H
+#define UFE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define UFE_OK                     0
+#define UFE_WARN_NOT_CONVERGED     1  /* Picard hit visc_it_nit (reference warns and exits the loop) */
+#define UFE_ERR_ARGUMENT          -1
+#define UFE_ERR_DEVICE            -2  /* HIP runtime error */
+#define UFE_ERR_DIVERGED          -3  /* Picard diverged below relax 0.05 or above eps0^2 1e-5 (reference crash) */
+#define UFE_ERR_UNSUPPORTED       -4
+#define UFE_ERR_UNSTABLE          -5  /* heat columns unstable for >= 1% of vertices (reference crash) */
+#define UFE_ERR_MESH              -6  /* operator construction could not find a solvable neighbourhood */
+
+/* ---- choices ----------------------------------------------------------- */
+#define UFE_SLIDING_NO_SLIDING     0
+#define UFE_SLIDING_WEERTMAN       1
+#define UFE_SLIDING_BUDD           2  /* needs till_yield_stress in place of beta_sq */
+#define UFE_SLIDING_COULOMB        3  /* needs till_yield_stress in place of beta_sq */
+
+#define UFE_BC_INFINITE            0
+#define UFE_BC_ZERO                1
+
+#define UFE_PREC_FP64              0  /* plain form: FP64 Krylov, reference tolerances */
+#define UFE_PREC_MIXED             1  /* FP32 Krylov inner solves, FP64 refinement residual */
+
+#define UFE_KRYLOV_BICGSTAB        0
+#define UFE_KRYLOV_GMRES           1
+
+#define UFE_PC_NONE                0
+#define UFE_PC_JACOBI              1  /* point Jacobi */
+#define UFE_PC_BLOCK_JACOBI        2  /* 2x2 (u,v) blocks per triangle */
+#define UFE_PC_AMG                 3  /* aggregation multigrid V-cycle (coarse correction + block-Jacobi smoothing) */
+
+#define UFE_STOP_L2UV              0  /* reference stop test: L2_uv < visc_it_norm_dUV_tol */
+#define UFE_STOP_RESIDUAL          1  /* FP64 nonlinear residual ||b(u)-A(u)u||/||b(u)|| <= nonlinear_rtol */
+
+#define UFE_STRESS_SSA             0
+#define UFE_STRESS_DIVA            1
+
+/* ---- mesh and operators ------------------------------------------------
+ * Operator families share one sparsity pattern (ptr, ind) each:
+ *   ab : map/ddx/ddy a->b  (nT rows, nV cols)  mesh%M_{map,ddx,ddy}_a_b
+ *   ba : map/ddx/ddy b->a  (nV rows, nT cols)  mesh%M_{map,ddx,ddy}_b_a
+ *   bb : ddx/ddy/d2dx2/d2dxdy/d2dy2 b->b, 2nd order (nT rows, nT cols)
+ *        mesh%M2_*_b_b
+ * (mesh_types.f90:180-192; built by mesh_disc_calc_matrix_operators_2D.f90)
+ */
+typedef struct ufe_mesh_desc {
+  int nV, nT, nz;
+  const double *V;        /* [nV*2]  x,y interleaved                 mesh%V      */
+  const int    *Tri;      /* [nT*3]  CCW vertices                     mesh%Tri    */
+  const int    *TriC;     /* [nT*3]  neighbour across vertex n or -1  mesh%TriC   */
+  const int    *TriBI;    /* [nT]    border index 0..8                mesh%TriBI  */
+  const double *TriGC;    /* [nT*2]  geometric centres                mesh%TriGC  */
+  const double *zeta;     /* [nz]    scaled vertical coordinate       mesh%zeta   */
+  int ab_nnz; const int *ab_ptr, *ab_ind; const double *ab_map, *ab_ddx, *ab_ddy;
+  int ba_nnz; const int *ba_ptr, *ba_ind; const double *ba_map, *ba_ddx, *ba_ddy;
+  int bb_nnz; const int *bb_ptr, *bb_ind;
+  const double *bb_ddx, *bb_ddy, *bb_d2dx2, *bb_d2dxdy, *bb_d2dy2;
+} ufe_mesh_desc;
+
+/* Config fields of the reference's C% struct that the velocity solve reads
+ * (model_configuration.f90:274-349,577-601), plus the engine's method
+ * choices.  ufe_params_default() fills the reference defaults. */
+typedef struct ufe_params {
+  double glens_n;                    /* Glens_flow_law_exponent          3      */
+  double epsilon_sq_0;               /* Glens_flow_law_epsilon_sq_0      1e-8   */
+  double visc_eff_min;               /* visc_eff_min                     1e4    */
+  double vel_max;                    /* vel_max                          5000   */
+  int    visc_it_nit;                /* visc_it_nit                      50     */
+  double visc_it_relax;              /* visc_it_relax                    0.2    */
+  double visc_it_norm_dUV_tol;       /* visc_it_norm_dUV_tol             5e-5   */
+  double rtol;                       /* stress_balance_PETSc_rtol        1e-7   */
+  double abstol;                     /* stress_balance_PETSc_abstol      1e-5   */
+  int    include_crossterms;         /* do_include_SSADIVA_crossterms    1      */
+  int    sliding_law;                /* choice_sliding_law               WEERTMAN */
+  double slid_weertman_m;            /* slid_Weertman_m                  3      */
+  double slid_budd_q_plastic;        /* slid_Budd_q_plastic              0.3    */
+  double slid_budd_u_threshold;      /* slid_Budd_u_threshold            100    */
+  double slid_delta_v;               /* slid_delta_v                     1e-3   */
+  double slid_beta_max;              /* slid_beta_max                    1e20   */
+  int    do_gl_subgrid_friction;     /* do_GL_subgrid_friction           1      */
+  double subgrid_friction_exponent;  /* subgrid_friction_exponent_on_B_grid 2   */
+  int    bc_u[4];                    /* BC_u_{north,east,south,west}     INFINITE */
+  int    bc_v[4];                    /* BC_v_{north,east,south,west}     INFINITE */
+  /* engine */
+  int    precision;                  /* UFE_PREC_*                       FP64   */
+  int    krylov;                     /* UFE_KRYLOV_*                     BICGSTAB */
+  int    precond;                    /* UFE_PC_*                         BLOCK_JACOBI */
+  int    gmres_restart;              /*                                  30     */
+  int    krylov_maxit;               /* per linear solve                 10000  */
+  int    warm_start;                 /* 0: zero initial guess (PETSc default), 1: previous iterate */
+  double inner_rtol;                 /* MIXED: inner FP32 solve tolerance relative to the FP64 residual */
+  int    stop_rule;                  /* UFE_STOP_*                       L2UV   */
+  double nonlinear_rtol;             /* for UFE_STOP_RESIDUAL            1e-6   */
+  int    check_every;                /* host polls Krylov convergence every k iterations */
+} ufe_params;
+
+/* Per-solve diagnostics. */
+typedef struct ufe_solve_info {
+  int    n_visc_its;                 /* Picard iterations                                  */
+  int    n_Axb_its;                  /* Krylov iterations summed over Picard iterations     */
+  int    n_refinements;              /* FP64 refinement residuals evaluated (MIXED)          */
+  double L2_uv;                      /* last reference convergence norm                     */
+  double rel_residual;               /* FP64 ||b(u)-A(u)u||/||b(u)|| at the returned u       */
+  double visc_it_relax_applied;
+  double epsilon_sq_0_applied;
+  double seconds;                    /* wall time of the Picard loop                        */
+} ufe_solve_info;
+
+typedef struct ufe_ctx ufe_ctx;
+
+void        ufe_params_default(ufe_params *p);
+const char *ufe_last_error(const ufe_ctx *ctx);   /* ctx may be NULL: last global error */
+int         ufe_version(void);
+
+/* Context: device copies of the mesh, operators and solver work arrays.
+ * Replaces the per-call PETSc matrix/KSP set-up of
+ * petsc_basic.f90:32-141 and the mesh arrays read by SSA_main.f90:87-242. */
+int  ufe_ctx_create(ufe_ctx **ctx, int device, const ufe_mesh_desc *mesh);
+int  ufe_ctx_destroy(ufe_ctx *ctx);
+int  ufe_set_params(ufe_ctx *ctx, const ufe_params *p);
+
+/* ---- drop-in entry points ---------------------------------------------- */
+
+/* solve_SSA (SSA_main.f90:87-242).
+ *   Hi, Hs, beta_sq, mask_grounded_ice : [nV]   (mask may be NULL = all grounded)
+ *   A_flow                             : [nV*nz] ice%A_flow after calc_ice_rheology_Glen
+ *   fraction_gr_b                      : [nT]
+ *   BC_prescr_mask_b/_u_b/_v_b         : [nT] or all three NULL
+ *   u_b, v_b                           : [nT] in: initial guess, out: solution
+ *   info                               : may be NULL */
+int ufe_solve_ssa(ufe_ctx *ctx, const double *Hi, const double *Hs, const double *A_flow,
+                  const double *beta_sq, const double *fraction_gr_b, const int *mask_grounded_ice,
+                  const int *BC_prescr_mask_b, const double *BC_prescr_u_b, const double *BC_prescr_v_b,
+                  double *u_b, double *v_b, ufe_solve_info *info);
+
+/* solve_DIVA (DIVA_main.f90:88-262).  DIVA state that the reference keeps in
+ * its solver type across calls is inout: eta_3D_b [nT*nz], tau_bx_b, tau_by_b,
+ * u_base_b, v_base_b [nT].  u_3D_b, v_3D_b [nT*nz] are outputs
+ * (calc_3D_velocities, DIVA_main.f90:636-676). */
+int ufe_solve_diva(ufe_ctx *ctx, const double *Hi, const double *Hs, const double *A_flow,
+                   const double *beta_sq, const double *fraction_gr_b, const int *mask_grounded_ice,
+                   const int *BC_prescr_mask_b, const double *BC_prescr_u_b, const double *BC_prescr_v_b,
+                   double *u_vav_b, double *v_vav_b, double *u_base_b, double *v_base_b,
+                   double *u_3D_b, double *v_3D_b, double *eta_3D_b,
+                   double *tau_bx_b, double *tau_by_b, ufe_solve_info *info);
+
+/* ---- device-resident form (inputs uploaded once, solve in HBM) -------- */
+int ufe_upload_inputs(ufe_ctx *ctx, const double *Hi, const double *Hs, const double *A_flow,
+                      const double *beta_sq, const double *fraction_gr_b, const int *mask_grounded_ice,
+                      const int *BC_prescr_mask_b, const double *BC_prescr_u_b, const double *BC_prescr_v_b);
+int ufe_upload_velocity(ufe_ctx *ctx, const double *u_b, const double *v_b);
+int ufe_run(ufe_ctx *ctx, int stress_balance, ufe_solve_info *info);
+int ufe_download_velocity(ufe_ctx *ctx, double *u_b, double *v_b);
+/* DIVA state up/download (eta_3D_b, tau_b, u_base_b) and 3-D outputs */
+int ufe_upload_diva_state(ufe_ctx *ctx, const double *eta_3D_b, const double *tau_bx_b,
+                          const double *tau_by_b, const double *u_base_b, const double *v_base_b);
+int ufe_download_diva(ufe_ctx *ctx, double *u_base_b, double *v_base_b, double *u_3D_b,
+                      double *v_3D_b, double *eta_3D_b, double *tau_bx_b, double *tau_by_b);
+/* Nonlinear SSA residual r = b(u) - A(u) u at a given velocity, with the
+ * uploaded inputs (r, b: [2*nT] rows 2*ti+uv, either may be NULL).  This is
+ * the linearised system of solve_SSA_DIVA_linearised
+ * (solve_linearised_SSA_DIVA.f90:23-178) assembled at u, applied to u. */
+int ufe_ssa_residual(ufe_ctx *ctx, const double *u_b, const double *v_b, double *r, double *b,
+                     double *rel_residual);
+/* Time of the last ufe_run spent per stage, from HIP events, in ms:
+ * [0] vertex/viscosity, [1] triangle/a->b, [2] assembly, [3] SpMV, [4] Krylov vector ops */
+int ufe_last_timings(const ufe_ctx *ctx, double *ms5);
+
+/* ---- linear-algebra boundary ------------------------------------------ */
+
+/* y = A x for a CSR matrix (0-based).  Replaces MatMult as exercised by
+ * multiply_PETSc_matrix_with_vector_1D (ut_petsc.f90:57-160). */
+int ufe_csr_matvec(int m, int n, int nnz, const int *ptr, const int *ind, const double *val,
+                   const double *x, double *y);
+
+/* Solve A x = b for a general square CSR matrix with the engine's Krylov
+ * solver (x in: initial guess if warm_start, out: solution).  Replaces
+ * solve_matrix_equation_CSR_PETSc (petsc_basic.f90:32-64). */
+int ufe_csr_solve(int n, int nnz, const int *ptr, const int *ind, const double *val,
+                  const double *b, double *x, double rtol, double abstol, int maxit,
+                  int precond, int *n_its, double *rel_residual);
+
+/* ---- grounded fractions ------------------------------------------------
+ * calc_grounded_fractions_bilin_interp_TAF_b
+ * (subgrid_grounded_fractions_bilin_TAF.f90:182-370), FP64 on the device.
+ * TAF = Hi - max(0, (SL - Hb) * rho_sw/rho_i) is formed per vertex
+ * (ice_geometry_basics.f90:43-56). */
+int ufe_fraction_gr_b(ufe_ctx *ctx, const double *Hi, const double *Hb, const double *SL,
+                      double *fraction_gr_b);
+
+/* ---- vertical heat columns --------------------------------------------
+ * The per-vertex part of solve_3D_heat_equation
+ * (thermodynamics_3D_heat_equation.f90:114-293) with solve_1D_heat_equation
+ * (:309-479), Pounder heat capacity, Ritz conductivity and the pressure
+ * melting point recomputed per layer (thermodynamics_utilities.f90:130-229).
+ * Per-layer inputs are [nV*nz] vertex-fastest; per-vertex inputs [nV].
+ *   mask: 0 = column below Hi_min_thermo (set to T_surf), 1 = grounded
+ *   (basal flux BC), 2 = floating (T_base_float BC)
+ * Columns still unstable after 10 halvings are reported in is_unstable and
+ * counted in *n_unstable; the caller applies its fallback profile. */
+typedef struct ufe_heat_inputs {
+  int nV, nz;
+  const double *zeta;                                   /* [nz] */
+  const double *Ti;                                     /* [nV*nz] temperature at t            */
+  const double *u_3D, *v_3D, *w_3D;                     /* [nV*nz] velocities                  */
+  const double *u_dTdxp_upwind, *v_dTdyp_upwind;        /* [nV*nz] explicit advection terms    */
+  const double *dzeta_dx, *dzeta_dy, *dzeta_dz, *dzeta_dt; /* [nV*nz] zeta gradients          */
+  const double *Phi;                                    /* [nV*nz] internal heating            */
+  const double *Hi_eff;                                 /* [nV]                                */
+  const double *T_surf;                                 /* [nV] annual mean surface T          */
+  const double *Q_base_grnd;                            /* [nV] frictional + geothermal flux   */
+  const int    *mask;                                   /* [nV] 0 thin / 1 grounded / 2 floating */
+  double dt;
+  double Hi_min_thermo;
+} ufe_heat_inputs;
+
+int ufe_heat_columns(ufe_ctx *ctx, const ufe_heat_inputs *in, double *Ti_tplusdt, int *is_unstable,
+                     int *n_unstable);
+
+/* ---- mesh operator construction (host, once per mesh) -----------------
+ * calc_matrix_operators_mesh_a_b / _b_a / _b_b_2nd_order
+ * (mesh_disc_calc_matrix_operators_2D.f90:198-474,612-764).  Two-call
+ * convention: pass NULL arrays to receive *nnz, then arrays of that size. */
+int ufe_mesh_operators_ab(int nV, int nT, const double *V, const int *Tri, const double *TriGC,
+                          const int *C_ptr, const int *C_ind, int *nnz, int *ptr, int *ind,
+                          double *val_map, double *val_ddx, double *val_ddy);
+int ufe_mesh_operators_ba(int nV, int nT, const double *V, const int *TriC, const double *TriGC,
+                          const int *iTri_ptr, const int *iTri_ind, int *nnz, int *ptr, int *ind,
+                          double *val_map, double *val_ddx, double *val_ddy);
+int ufe_mesh_operators_bb2(int nT, const int *TriC, const double *TriGC, int *nnz, int *ptr, int *ind,
+                           double *val_ddx, double *val_ddy, double *val_d2dx2, double *val_d2dxdy,
+                           double *val_d2dy2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UFE_HIP_H */

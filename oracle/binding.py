@@ -1,0 +1,166 @@
+"""ctypes binding of oracle/_build/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The engine never imports it.
+It reuses the engine package's ctypes structures only as a data layout.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "_build", "liboracle.so")
+
+_o = None
+
+
+def _L():
+    import __graft_entry__ as ge
+    ge.load_package()
+    from ufemism2_amd import _lib
+    return _lib
+
+
+def lib():
+    global _o
+    if _o is not None:
+        return _o
+    if not os.path.exists(SO):
+        subprocess.run(["make", "-C", HERE], check=True, capture_output=True)
+    L = _L()
+    o = C.CDLL(SO)
+    dp, ip = L.dp, L.ip
+    o.oracle_csr_matvec.argtypes = [C.c_int, ip, ip, dp, dp, dp]
+    o.oracle_tridiag.argtypes = [C.c_int, dp, dp, dp, dp, dp]
+    o.oracle_gmres_ilu0.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
+                                    C.c_int, C.c_int, dp]
+    o.oracle_solve_ssa.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), dp, dp, dp, dp, dp,
+                                   ip, ip, dp, dp, dp, dp, C.POINTER(L.SolveInfo)]
+    o.oracle_solve_diva.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), dp, dp, dp, dp, dp,
+                                    ip, ip, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                    C.POINTER(L.SolveInfo)]
+    o.oracle_ssa_residual.restype = C.c_double
+    o.oracle_ssa_residual.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), dp, dp, dp, dp, dp,
+                                      ip, dp, dp, dp, dp, dp, dp]
+    o.oracle_ssa_system.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), dp, dp, dp, dp, dp,
+                                    dp, dp, C.c_int, ip, ip, dp, dp]
+    o.oracle_fraction_gr_b.argtypes = [C.POINTER(L.MeshDesc), dp, dp, dp, dp]
+    o.oracle_heat_columns.argtypes = [C.POINTER(L.HeatInputs), dp, ip, ip]
+    o.oracle_zeta_tridiagonals.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
+    _o = o
+    return o
+
+
+def csr_matvec(ptr, ind, val, x):
+    L = _L()
+    ptr, ind, val, x = L.i32(ptr), L.i32(ind), L.f64(val), L.f64(x)
+    y = np.zeros(ptr.size - 1)
+    lib().oracle_csr_matvec(ptr.size - 1, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val), L.ptr(x), L.ptr(y))
+    return y
+
+
+def tridiag(a, b, c, d):
+    """Thomas solve; a[0] unused, c[n-1] unused (arrays of length n)."""
+    L = _L()
+    a, b, c, d = (L.f64(np.array(t, dtype=np.float64)) for t in (a, b, c, d))
+    x = np.zeros(b.size)
+    lib().oracle_tridiag(b.size, L.ptr(a), L.ptr(b), L.ptr(c), L.ptr(d), L.ptr(x))
+    return x
+
+
+def gmres_ilu0(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, restart=30):
+    L = _L()
+    A = A.tocsr()
+    ptr, ind, val = L.i32(A.indptr), L.i32(A.indices), L.f64(A.data)
+    b = L.f64(b)
+    x = L.f64(np.zeros_like(b) if x0 is None else x0.copy())
+    pres = C.c_double(0)
+    its = lib().oracle_gmres_ilu0(b.size, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val), L.ptr(b),
+                                  L.ptr(x), rtol, abstol, maxit, restart, int(x0 is None), C.byref(pres))
+    return x, its
+
+
+class Oracle:
+    """Oracle solves on one mesh descriptor (MeshDescriptor from the engine package)."""
+
+    def __init__(self, md):
+        self.md = md
+
+    def solve_ssa(self, params, inp, u0=None, v0=None):
+        L = _L()
+        nT = self.md.mesh.nT
+        u = L.f64(np.zeros(nT) if u0 is None else u0).copy()
+        v = L.f64(np.zeros(nT) if v0 is None else v0).copy()
+        info = L.SolveInfo()
+        bcm, bcu, bcv = inp.get("bc_mask"), inp.get("bc_u"), inp.get("bc_v")
+        rc = lib().oracle_solve_ssa(self.md.byref(), C.byref(params), L.ptr(inp["Hi"]), L.ptr(inp["Hs"]),
+                                    L.ptr(inp["A_flow"]), L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]),
+                                    L.ptr(inp.get("mask_grounded"), L.ip), L.ptr(bcm, L.ip), L.ptr(bcu),
+                                    L.ptr(bcv), L.ptr(u), L.ptr(v), C.byref(info))
+        return rc, u, v, info
+
+    def solve_diva(self, params, inp, state=None):
+        L = _L()
+        nT, nz = self.md.mesh.nT, len(self.md.zeta)
+        st = {k: np.zeros(nT) for k in ("u", "v", "ubase", "vbase", "tbx", "tby")}
+        st["eta3b"] = np.zeros(nT * nz)
+        if state:
+            for k, a in state.items():
+                st[k] = L.f64(a).copy()
+        u3 = np.zeros(nT * nz); v3 = np.zeros(nT * nz)
+        info = L.SolveInfo()
+        rc = lib().oracle_solve_diva(self.md.byref(), C.byref(params), L.ptr(inp["Hi"]), L.ptr(inp["Hs"]),
+                                     L.ptr(inp["A_flow"]), L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]),
+                                     L.ptr(inp.get("mask_grounded"), L.ip), L.ptr(inp.get("bc_mask"), L.ip),
+                                     L.ptr(inp.get("bc_u")), L.ptr(inp.get("bc_v")),
+                                     L.ptr(st["u"]), L.ptr(st["v"]), L.ptr(st["ubase"]), L.ptr(st["vbase"]),
+                                     L.ptr(u3), L.ptr(v3), L.ptr(st["eta3b"]), L.ptr(st["tbx"]),
+                                     L.ptr(st["tby"]), C.byref(info))
+        st["u3d"], st["v3d"] = u3, v3
+        return rc, st, info
+
+    def residual(self, params, inp, u, v):
+        L = _L()
+        n = 2 * self.md.mesh.nT
+        r = np.zeros(n); b = np.zeros(n)
+        rel = lib().oracle_ssa_residual(self.md.byref(), C.byref(params), L.ptr(inp["Hi"]), L.ptr(inp["Hs"]),
+                                        L.ptr(inp["A_flow"]), L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]),
+                                        L.ptr(inp.get("bc_mask"), L.ip), L.ptr(inp.get("bc_u")),
+                                        L.ptr(inp.get("bc_v")), L.ptr(L.f64(u)), L.ptr(L.f64(v)),
+                                        L.ptr(r), L.ptr(b))
+        return rel, r, b
+
+    def system(self, params, inp, u, v):
+        import scipy.sparse as sp
+        L = _L()
+        nT = self.md.mesh.nT
+        cap = 4 * (self.md.ops.bb.nnz + 4 * nT)
+        ptr = np.zeros(2 * nT + 1, np.int32); ind = np.zeros(cap, np.int32)
+        val = np.zeros(cap); b = np.zeros(2 * nT)
+        nnz = lib().oracle_ssa_system(self.md.byref(), C.byref(params), L.ptr(inp["Hi"]), L.ptr(inp["Hs"]),
+                                      L.ptr(inp["A_flow"]), L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]),
+                                      L.ptr(L.f64(u)), L.ptr(L.f64(v)), cap, L.ptr(ptr, L.ip),
+                                      L.ptr(ind, L.ip), L.ptr(val), L.ptr(b))
+        assert nnz > 0
+        return sp.csr_matrix((val[:nnz], ind[:nnz], ptr), shape=(2 * nT, 2 * nT)), b
+
+    def fraction_gr_b(self, Hi, Hb, SL):
+        L = _L()
+        out = np.zeros(self.md.mesh.nT)
+        lib().oracle_fraction_gr_b(self.md.byref(), L.ptr(L.f64(Hi)), L.ptr(L.f64(Hb)), L.ptr(L.f64(SL)),
+                                   L.ptr(out))
+        return out
+
+
+def heat_columns(hin):
+    """hin: HeatInputs structure (with its arrays kept alive by the caller)."""
+    L = _L()
+    n = hin.nV * hin.nz
+    T = np.zeros(n); unst = np.zeros(hin.nV, np.int32); nun = C.c_int(0)
+    rc = lib().oracle_heat_columns(C.byref(hin), L.ptr(T), L.ptr(unst, L.ip), C.byref(nun))
+    return rc, T, unst, nun.value

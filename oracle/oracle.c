@@ -1,0 +1,967 @@
+/*
+ * oracle.c -- FP64 CPU restatement of UFEMISM's velocity solve and vertical
+ * heat columns.  TEST INFRASTRUCTURE ONLY: the parity tests, smoke() and the
+ * cpu_baseline leg of bench.py call it as the checker.  The product never
+ * links, loads or calls anything in this directory.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the UFEMISM2.0 checkout).  The Krylov solve restates PETSc's default KSP on
+ * one rank (GMRES(30), left preconditioning with ILU(0), zero initial guess,
+ * convergence on the preconditioned residual norm against
+ * max(rtol*||r0||, abstol)); PETSc is a third-party dependency of the
+ * reference whose version is not pinned (Homebrew latest in CI), so this is a
+ * restatement of its published algorithm, not of its code.
+ *
+ * Pinning (tests/test_oracle_*.py): CSR product against the ut_petsc 6x6 known
+ * answer; Thomas solver against the ut_math_utilities known answer; GMRES
+ * against scipy's direct solve; operators against polynomial exactness;
+ * assembly against the operator expression of the SSA; Picard against the
+ * Schoof (2006) ice-stream closed form; heat columns against the steady
+ * diffusion closed form and dense solves; grounded fractions against the
+ * geometric closed form and Monte-Carlo areas.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../include/ufe_hip.h"
+
+#define ICE_DENSITY 910.0
+#define SEAWATER_DENSITY 1028.0
+#define GRAV 9.81
+#define T0 273.16
+#define CLAUSIUS_CLAPEYRON 8.7e-4
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* ---------------------------------------------------------------------- */
+/* CSR helpers                                                             */
+/* ---------------------------------------------------------------------- */
+
+/* multiply_CSR_matrix_with_vector_1D (CSR_matrix_vector_multiplication.f90:198-334) */
+void oracle_csr_matvec(int m, const int *ptr, const int *ind, const double *val, const double *x,
+                       double *y) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < m; ++i) {
+    double s = 0.0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[ind[k]];
+    y[i] = s;
+  }
+}
+
+static double dot(int n, const double *a, const double *b) {
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Thomas algorithm: solve_tridiagonal_matrix_equation                     */
+/* (src/UPSY/basic/math_utilities/tridiagonal_solver.f90:13-51)           */
+/* a[1..n-1] sub-diagonal (a[0] unused), b diagonal, c[0..n-2] super.     */
+/* a, b, c, d are overwritten like the reference's intent(inout).          */
+/* ---------------------------------------------------------------------- */
+void oracle_tridiag(int n, double *a, double *b, double *c, double *d, double *x) {
+  for (int i = 1; i < n; ++i) {
+    double w = a[i] / b[i - 1];
+    b[i] -= w * c[i - 1];
+    d[i] -= w * d[i - 1];
+  }
+  x[n - 1] = d[n - 1] / b[n - 1];
+  for (int i = n - 2; i >= 0; --i) x[i] = (d[i] - c[i] * x[i + 1]) / b[i];
+}
+
+/* ---------------------------------------------------------------------- */
+/* ILU(0) + GMRES(m), left preconditioned: PETSc's default KSPSolve on one */
+/* rank as called by solve_matrix_equation_PETSc (petsc_basic.f90:66-141).*/
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  int n;
+  int *ptr, *ind, *diag;
+  double *val;  /* factors L (unit, strictly lower) and U in place */
+} ilu0_t;
+
+static int cmp_int_pair(const void *a, const void *b) {
+  const int *x = a, *y = b;
+  return (x[0] > y[0]) - (x[0] < y[0]);
+}
+
+/* Copy A with sorted column indices and factor ILU(0) (IKJ variant). */
+static int ilu0_factor(int n, const int *ptr, const int *ind, const double *val, ilu0_t *f) {
+  int nnz = ptr[n];
+  f->n = n;
+  f->ptr = malloc(sizeof(int) * (n + 1));
+  f->ind = malloc(sizeof(int) * nnz);
+  f->diag = malloc(sizeof(int) * n);
+  f->val = malloc(sizeof(double) * nnz);
+  int *pair = malloc(sizeof(int) * 2 * 64);
+  int cap = 64;
+  memcpy(f->ptr, ptr, sizeof(int) * (n + 1));
+  for (int i = 0; i < n; ++i) {
+    int k0 = ptr[i], len = ptr[i + 1] - k0;
+    if (len > cap) { cap = 2 * len; pair = realloc(pair, sizeof(int) * 2 * cap); }
+    for (int k = 0; k < len; ++k) { pair[2 * k] = ind[k0 + k]; pair[2 * k + 1] = k0 + k; }
+    qsort(pair, len, 2 * sizeof(int), cmp_int_pair);
+    f->diag[i] = -1;
+    for (int k = 0; k < len; ++k) {
+      f->ind[k0 + k] = pair[2 * k];
+      f->val[k0 + k] = val[pair[2 * k + 1]];
+      if (pair[2 * k] == i) f->diag[i] = k0 + k;
+      if (k > 0 && pair[2 * k] == pair[2 * k - 2]) { free(pair); return -1; } /* duplicate */
+    }
+    if (f->diag[i] < 0) { free(pair); return -2; }
+  }
+  free(pair);
+  int *pos = malloc(sizeof(int) * n);
+  for (int i = 0; i < n; ++i) pos[i] = -1;
+  for (int i = 0; i < n; ++i) {
+    for (int k = f->ptr[i]; k < f->ptr[i + 1]; ++k) pos[f->ind[k]] = k;
+    for (int k = f->ptr[i]; k < f->diag[i]; ++k) {
+      int j = f->ind[k];
+      double piv = f->val[f->diag[j]];
+      if (piv == 0.0) { free(pos); return -3; }
+      double lij = f->val[k] / piv;
+      f->val[k] = lij;
+      for (int kk = f->diag[j] + 1; kk < f->ptr[j + 1]; ++kk) {
+        int p = pos[f->ind[kk]];
+        if (p >= 0) f->val[p] -= lij * f->val[kk];
+      }
+    }
+    for (int k = f->ptr[i]; k < f->ptr[i + 1]; ++k) pos[f->ind[k]] = -1;
+    if (f->val[f->diag[i]] == 0.0) { free(pos); return -3; }
+  }
+  free(pos);
+  return 0;
+}
+
+static void ilu0_apply(const ilu0_t *f, const double *r, double *z) {
+  int n = f->n;
+  for (int i = 0; i < n; ++i) {
+    double s = r[i];
+    for (int k = f->ptr[i]; k < f->diag[i]; ++k) s -= f->val[k] * z[f->ind[k]];
+    z[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = z[i];
+    for (int k = f->diag[i] + 1; k < f->ptr[i + 1]; ++k) s -= f->val[k] * z[f->ind[k]];
+    z[i] = s / f->val[f->diag[i]];
+  }
+}
+
+static void ilu0_free(ilu0_t *f) {
+  free(f->ptr); free(f->ind); free(f->diag); free(f->val);
+}
+
+/* GMRES(restart) with modified Gram-Schmidt and Givens rotations, left
+ * preconditioned with ILU(0).  x is overwritten with the solution; when
+ * zero_guess is set it is zeroed first (PETSc's default initial guess).
+ * Returns the iteration count (>= 0) or a negative error. */
+int oracle_gmres_ilu0(int n, const int *ptr, const int *ind, const double *val, const double *b,
+                      double *x, double rtol, double abstol, int maxit, int restart, int zero_guess,
+                      double *final_pres) {
+  ilu0_t f;
+  int rc = ilu0_factor(n, ptr, ind, val, &f);
+  if (rc) return -10 + rc;
+  int m = restart;
+  double *V = malloc(sizeof(double) * (size_t)n * (m + 1));
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+  double *g = malloc(sizeof(double) * (m + 1));
+  double *w = malloc(sizeof(double) * n), *t = malloc(sizeof(double) * n);
+  if (zero_guess) memset(x, 0, sizeof(double) * n);
+  int its = 0;
+  double rnorm0 = -1.0, ttol = 0.0, rnorm = 0.0;
+  int converged = 0;
+  while (!converged && its < maxit) {
+    /* r = M^-1 (b - A x) */
+    oracle_csr_matvec(n, ptr, ind, val, x, t);
+    for (int i = 0; i < n; ++i) t[i] = b[i] - t[i];
+    ilu0_apply(&f, t, V);
+    rnorm = sqrt(dot(n, V, V));
+    if (rnorm0 < 0.0) {
+      rnorm0 = rnorm;
+      ttol = fmax(rtol * rnorm0, abstol);
+    }
+    if (rnorm <= ttol) { converged = 1; break; }
+    for (int i = 0; i < n; ++i) V[i] /= rnorm;
+    memset(g, 0, sizeof(double) * (m + 1));
+    g[0] = rnorm;
+    int k;
+    for (k = 0; k < m && its < maxit; ++k) {
+      ++its;
+      double *vk = V + (size_t)k * n, *vk1 = V + (size_t)(k + 1) * n;
+      oracle_csr_matvec(n, ptr, ind, val, vk, t);
+      ilu0_apply(&f, t, w);
+      for (int j = 0; j <= k; ++j) {
+        double *vj = V + (size_t)j * n;
+        double h = dot(n, w, vj);
+        H[j * m + k] = h;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) w[i] -= h * vj[i];
+      }
+      double hk1 = sqrt(dot(n, w, w));
+      H[(k + 1) * m + k] = hk1;
+      if (hk1 != 0.0)
+        for (int i = 0; i < n; ++i) vk1[i] = w[i] / hk1;
+      for (int j = 0; j < k; ++j) {
+        double a = H[j * m + k], bb = H[(j + 1) * m + k];
+        H[j * m + k] = cs[j] * a + sn[j] * bb;
+        H[(j + 1) * m + k] = -sn[j] * a + cs[j] * bb;
+      }
+      double a = H[k * m + k], bb = H[(k + 1) * m + k];
+      double r = hypot(a, bb);
+      cs[k] = (r == 0.0) ? 1.0 : a / r;
+      sn[k] = (r == 0.0) ? 0.0 : bb / r;
+      H[k * m + k] = r;
+      H[(k + 1) * m + k] = 0.0;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = cs[k] * g[k];
+      rnorm = fabs(g[k + 1]);
+      if (rnorm <= ttol || hk1 == 0.0) { ++k; converged = (rnorm <= ttol); break; }
+    }
+    /* back substitution and update x += V y */
+    int kk = k;
+    double *y = calloc(kk > 0 ? kk : 1, sizeof(double));
+    for (int i = kk - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int j = i + 1; j < kk; ++j) s -= H[i * m + j] * y[j];
+      y[i] = s / H[i * m + i];
+    }
+    for (int j = 0; j < kk; ++j) {
+      double *vj = V + (size_t)j * n;
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n; ++i) x[i] += y[j] * vj[i];
+    }
+    free(y);
+  }
+  if (final_pres) *final_pres = rnorm;
+  free(V); free(H); free(cs); free(sn); free(g); free(w); free(t);
+  ilu0_free(&f);
+  return its;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Mesh-operator applications                                              */
+/* ---------------------------------------------------------------------- */
+
+/* calc_driving_stress (SSA_DIVA_utilities.f90:21-57) */
+static void driving_stress(const ufe_mesh_desc *m, const double *Hi, const double *Hs, double *tdx,
+                           double *tdy) {
+  for (int ti = 0; ti < m->nT; ++ti) {
+    double hb = 0, dx = 0, dy = 0;
+    for (int k = m->ab_ptr[ti]; k < m->ab_ptr[ti + 1]; ++k) {
+      int vi = m->ab_ind[k];
+      hb += m->ab_map[k] * Hi[vi];
+      dx += m->ab_ddx[k] * Hs[vi];
+      dy += m->ab_ddy[k] * Hs[vi];
+    }
+    tdx[ti] = -ICE_DENSITY * GRAV * hb * dx;
+    tdy[ti] = -ICE_DENSITY * GRAV * hb * dy;
+  }
+}
+
+static void apply_csr(int rows, const int *ptr, const int *ind, const double *val, const double *x,
+                      double *y) {
+  oracle_csr_matvec(rows, ptr, ind, val, x, y);
+}
+
+/* vertical_average (mesh_zeta.f90:257-283) */
+static double vertical_average(int nz, const double *zeta, const double *f, int stride) {
+  double s = 0.0;
+  for (int k = 0; k < nz - 1; ++k) s += 0.5 * (f[(k + 1) * stride] + f[k * stride]) * (zeta[k + 1] - zeta[k]);
+  return s;
+}
+
+/* calc_effective_viscosity_Glen_2D (constitutive_equation.f90:25-51) */
+static double glen_2d(double eps0, double ux, double uy, double vx, double vy, double A, double n) {
+  double e2 = ux * ux + vy * vy + ux * vy + 0.25 * (uy + vx) * (uy + vx) + eps0;
+  return 0.5 * pow(A, -1.0 / n) * pow(e2, (1.0 - n) / (2.0 * n));
+}
+
+/* calc_effective_viscosity_Glen_3D_uv_only (constitutive_equation.f90:53-80) */
+static double glen_3d(double eps0, double ux, double uy, double uz, double vx, double vy, double vz,
+                      double A, double n) {
+  double e2 = ux * ux + vy * vy + ux * vy + 0.25 * (uy + vx) * (uy + vx) + 0.25 * (uz * uz + vz * vz) + eps0;
+  return 0.5 * pow(A, -1.0 / n) * pow(e2, (1.0 - n) / (2.0 * n));
+}
+
+/* calc_basal_friction_coefficient + the sliding laws it dispatches to
+ * (sliding_laws.f90:25-81 Weertman :86-130, Budd :200-242, Coulomb :132-198);
+ * do_subgrid_friction_on_A_grid is off by default, so bed roughness is used
+ * as given.  'roughness' is beta_sq (Weertman) or till_yield_stress. */
+static void sliding_beta(const ufe_mesh_desc *m, const ufe_params *p, const double *u_b,
+                         const double *v_b, const double *roughness, double *beta_a) {
+  int nV = m->nV;
+  double *ua = malloc(sizeof(double) * nV), *va = malloc(sizeof(double) * nV);
+  apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_map, u_b, ua);
+  apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_map, v_b, va);
+  for (int vi = 0; vi < nV; ++vi) {
+    double uabs = sqrt(p->slid_delta_v * p->slid_delta_v + ua[vi] * ua[vi] + va[vi] * va[vi]);
+    double b;
+    switch (p->sliding_law) {
+      case UFE_SLIDING_NO_SLIDING: b = 0.0; break;
+      case UFE_SLIDING_WEERTMAN: b = roughness[vi] * pow(uabs, 1.0 / p->slid_weertman_m - 1.0); break;
+      case UFE_SLIDING_BUDD:
+        b = roughness[vi] * pow(uabs, p->slid_budd_q_plastic - 1.0) /
+            pow(p->slid_budd_u_threshold, p->slid_budd_q_plastic);
+        break;
+      case UFE_SLIDING_COULOMB: b = roughness[vi] / uabs; break;
+      default: b = 0.0;
+    }
+    beta_a[vi] = fmin(p->slid_beta_max, b);
+  }
+  free(ua); free(va);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Stiffness matrix (solve_linearised_SSA_DIVA.f90:23-641)                 */
+/* ---------------------------------------------------------------------- */
+
+static int bc_side(int tribi) { return (tribi - 1) / 2; } /* 0 N, 1 E, 2 S, 3 W */
+
+/* Assemble A and b in the reference's row order 2*ti+uv and column order
+ * (stencil entry k: u column then v column; BC rows: neighbours then diagonal). */
+static void assemble(const ufe_mesh_desc *m, const ufe_params *p, const double *N_b,
+                     const double *dNdx, const double *dNdy, const double *beta_b,
+                     const double *tdx, const double *tdy, const int *bc_mask, const double *bc_u,
+                     const double *bc_v, int *ptr, int *ind, double *val, double *bb) {
+  int nnz = 0;
+  ptr[0] = 0;
+  for (int row = 0; row < 2 * m->nT; ++row) {
+    int ti = row / 2, uv = row % 2;
+    if (bc_mask && bc_mask[ti] == 1) {
+      ind[nnz] = row; val[nnz] = 1.0; ++nnz;
+      bb[row] = uv == 0 ? bc_u[ti] : bc_v[ti];
+    } else if (m->TriBI[ti] > 0) {
+      int side = bc_side(m->TriBI[ti]);
+      int choice = uv == 0 ? p->bc_u[side] : p->bc_v[side];
+      if (choice == UFE_BC_INFINITE) {
+        int nn = 0;
+        for (int n = 0; n < 3; ++n) {
+          int tj = m->TriC[3 * ti + n];
+          if (tj < 0) continue;
+          ++nn;
+          ind[nnz] = 2 * tj + uv; val[nnz] = 1.0; ++nnz;
+        }
+        ind[nnz] = row; val[nnz] = -(double)nn; ++nnz;
+        bb[row] = 0.0;
+      } else {
+        ind[nnz] = row; val[nnz] = 1.0; ++nnz;
+        bb[row] = 0.0;
+      }
+    } else if (p->include_crossterms) {
+      /* calc_SSA_DIVA_stiffness_matrix_row_free (:180-329) */
+      double N = N_b[ti], Nx = dNdx[ti], Ny = dNdy[ti], be = beta_b[ti];
+      for (int k = m->bb_ptr[ti]; k < m->bb_ptr[ti + 1]; ++k) {
+        int tj = m->bb_ind[k];
+        double xx = m->bb_d2dx2[k], xy = m->bb_d2dxdy[k], yy = m->bb_d2dy2[k];
+        double dx = m->bb_ddx[k], dy = m->bb_ddy[k];
+        double Au, Av;
+        if (uv == 0) {
+          Au = 4.0 * N * xx + 4.0 * Nx * dx + N * yy + Ny * dy;
+          if (tj == ti) Au -= be;
+          Av = 3.0 * N * xy + 2.0 * Nx * dy + Ny * dx;
+        } else {
+          Av = 4.0 * N * yy + 4.0 * Ny * dy + N * xx + Nx * dx;
+          if (tj == ti) Av -= be;
+          Au = 3.0 * N * xy + 2.0 * Ny * dx + Nx * dy;
+        }
+        ind[nnz] = 2 * tj; val[nnz] = Au; ++nnz;
+        ind[nnz] = 2 * tj + 1; val[nnz] = Av; ++nnz;
+      }
+      bb[row] = uv == 0 ? -tdx[ti] : -tdy[ti];
+    } else {
+      /* calc_SSA_DIVA_sans_stiffness_matrix_row_free (:331-479) */
+      double N = N_b[ti], be = beta_b[ti];
+      for (int k = m->bb_ptr[ti]; k < m->bb_ptr[ti + 1]; ++k) {
+        int tj = m->bb_ind[k];
+        double xx = m->bb_d2dx2[k], xy = m->bb_d2dxdy[k], yy = m->bb_d2dy2[k];
+        double Au, Av;
+        if (uv == 0) {
+          Au = 4.0 * xx + yy;
+          if (tj == ti) Au -= be / N;
+          Av = 3.0 * xy;
+        } else {
+          Av = 4.0 * yy + xx;
+          if (tj == ti) Av -= be / N;
+          Au = 3.0 * xy;
+        }
+        ind[nnz] = 2 * tj; val[nnz] = Au; ++nnz;
+        ind[nnz] = 2 * tj + 1; val[nnz] = Av; ++nnz;
+      }
+      bb[row] = uv == 0 ? -tdx[ti] / N : -tdy[ti] / N;
+    }
+    ptr[row + 1] = nnz;
+  }
+}
+
+static int max_row_nnz(const ufe_mesh_desc *m) {
+  int mx = 0;
+  for (int ti = 0; ti < m->nT; ++ti) {
+    int l = m->bb_ptr[ti + 1] - m->bb_ptr[ti];
+    if (l > mx) mx = l;
+  }
+  return mx;
+}
+
+/* ---------------------------------------------------------------------- */
+/* SSA / DIVA Picard drivers                                               */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+  const ufe_mesh_desc *m;
+  const ufe_params *p;
+  const double *Hi, *Hs, *A_flow, *roughness, *fgr_b;
+  const int *bc_mask;
+  const double *bc_u, *bc_v;
+  int is_diva;
+  /* work */
+  double *tdx, *tdy, *ux, *uy, *vx, *vy, *eta_a, *N_a, *N_b, *dNdx, *dNdy;
+  double *beta_a, *beta_b, *beta_eff_a, *beta_eff_b, *A_vav;
+  double *eta3a, *F1a, *F2a, *F1b, *F2b, *duz_a, *dvz_a, *tmpb, *tmpa;
+  int *Aptr, *Aind;
+  double *Aval, *bb, *xx;
+} picard_t;
+
+static void vec_a_b_3d(const ufe_mesh_desc *m, const double *fa, double *fb) {
+  for (int k = 0; k < m->nz; ++k)
+    apply_csr(m->nT, m->ab_ptr, m->ab_ind, m->ab_map, fa + (size_t)k * m->nV, fb + (size_t)k * m->nT);
+}
+static void vec_b_a_3d(const ufe_mesh_desc *m, const double *fb, double *fa) {
+  for (int k = 0; k < m->nz; ++k)
+    apply_csr(m->nV, m->ba_ptr, m->ba_ind, m->ba_map, fb + (size_t)k * m->nT, fa + (size_t)k * m->nV);
+}
+
+/* One Picard state evaluation at (u, v): everything up to the linear system.
+ * SSA: calc_horizontal_strain_rates, calc_effective_viscosity,
+ *      calc_applied_basal_friction_coefficient (SSA_main.f90:169-175, :339-430).
+ * DIVA: the same plus calc_vertical_shear_strain_rates, 3-D viscosity,
+ *      calc_F_integrals and calc_effective_basal_friction_coefficient
+ *      (DIVA_main.f90:174-186, :375-574), using the DIVA state arrays. */
+static void picard_state(picard_t *s, const double *u, const double *v, double eps0,
+                         double *eta3b, const double *tbx, const double *tby,
+                         const double *ubase, const double *vbase) {
+  const ufe_mesh_desc *m = s->m;
+  const ufe_params *p = s->p;
+  int nV = m->nV, nT = m->nT, nz = m->nz;
+  double n = p->glens_n;
+  double eta_max = 0.5 * pow(1e-18, -1.0 / n) * pow(eps0, (1.0 - n) / (2.0 * n));
+  apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_ddx, u, s->ux);
+  apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_ddy, u, s->uy);
+  apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_ddx, v, s->vx);
+  apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_ddy, v, s->vy);
+  if (!s->is_diva) {
+    for (int vi = 0; vi < nV; ++vi) {
+      double eta = glen_2d(eps0, s->ux[vi], s->uy[vi], s->vx[vi], s->vy[vi], s->A_vav[vi], n);
+      eta = fmin(fmax(eta, p->visc_eff_min), eta_max);
+      s->eta_a[vi] = eta;
+      s->N_a[vi] = eta * fmax(0.1, s->Hi[vi]);
+    }
+  } else {
+    /* vertical shear strain rates, DIVA_main.f90:396-405 */
+    for (int k = 0; k < nz; ++k)
+      for (int ti = 0; ti < nT; ++ti) s->tmpb[(size_t)k * nT + ti] =
+          tbx[ti] * m->zeta[k] / fmax(p->visc_eff_min, eta3b[(size_t)k * nT + ti]);
+    vec_b_a_3d(m, s->tmpb, s->duz_a);
+    for (int k = 0; k < nz; ++k)
+      for (int ti = 0; ti < nT; ++ti) s->tmpb[(size_t)k * nT + ti] =
+          tby[ti] * m->zeta[k] / fmax(p->visc_eff_min, eta3b[(size_t)k * nT + ti]);
+    vec_b_a_3d(m, s->tmpb, s->dvz_a);
+    /* 3-D viscosity, DIVA_main.f90:441-455 */
+    for (int vi = 0; vi < nV; ++vi)
+      for (int k = 0; k < nz; ++k) {
+        size_t i = (size_t)k * nV + vi;
+        double eta = glen_3d(eps0, s->ux[vi], s->uy[vi], s->duz_a[i], s->vx[vi], s->vy[vi], s->dvz_a[i],
+                             s->A_flow[i], n);
+        s->eta3a[i] = fmin(fmax(eta, p->visc_eff_min), eta_max);
+      }
+    vec_a_b_3d(m, s->eta3a, eta3b);
+    /* vertical average and N, DIVA_main.f90:461-469; the reference writes
+     * max(0.1, Hi) with a default-real 0.1 here */
+    for (int vi = 0; vi < nV; ++vi) {
+      double ev = vertical_average(nz, m->zeta, s->eta3a + vi, nV);
+      s->eta_a[vi] = ev;
+      s->N_a[vi] = ev * fmax((double)0.1f, s->Hi[vi]);
+    }
+    /* F-integrals, DIVA_main.f90:497-515 (integrate_from_zeta_is_one_to_zeta_is_zetap) */
+    for (int vi = 0; vi < nV; ++vi) {
+      double h = fmax(0.1, s->Hi[vi]);
+      double I1 = 0.0, I2 = 0.0;
+      s->F1a[(size_t)(nz - 1) * nV + vi] = 0.0;
+      s->F2a[(size_t)(nz - 1) * nV + vi] = 0.0;
+      for (int k = nz - 2; k >= 0; --k) {
+        double z0 = m->zeta[k], z1 = m->zeta[k + 1];
+        double e0 = s->eta3a[(size_t)k * nV + vi], e1 = s->eta3a[(size_t)(k + 1) * nV + vi];
+        I1 = I1 - 0.5 * (z1 / e1 + z0 / e0) * (z1 - z0);
+        I2 = I2 - 0.5 * (z1 * z1 / e1 + z0 * z0 / e0) * (z1 - z0);
+        s->F1a[(size_t)k * nV + vi] = -h * I1;
+        s->F2a[(size_t)k * nV + vi] = -h * I2;
+      }
+    }
+    vec_a_b_3d(m, s->F1a, s->F1b);
+    vec_a_b_3d(m, s->F2a, s->F2b);
+  }
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_map, s->N_a, s->N_b);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddx, s->N_a, s->dNdx);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddy, s->N_a, s->dNdy);
+  if (!s->is_diva) {
+    sliding_beta(m, p, u, v, s->roughness, s->beta_a);
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_map, s->beta_a, s->beta_b);
+    if (p->do_gl_subgrid_friction)
+      for (int ti = 0; ti < nT; ++ti) s->beta_b[ti] *= pow(s->fgr_b[ti], p->subgrid_friction_exponent);
+  } else {
+    sliding_beta(m, p, ubase, vbase, s->roughness, s->beta_a);
+    for (int vi = 0; vi < nV; ++vi) {
+      if (p->sliding_law == UFE_SLIDING_NO_SLIDING) s->beta_eff_a[vi] = 1.0 / s->F2a[vi];
+      else s->beta_eff_a[vi] = s->beta_a[vi] / (1.0 + s->beta_a[vi] * s->F2a[vi]);
+    }
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_map, s->beta_a, s->beta_b);
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_map, s->beta_eff_a, s->beta_eff_b);
+    if (p->do_gl_subgrid_friction)
+      for (int ti = 0; ti < nT; ++ti) s->beta_eff_b[ti] *= pow(s->fgr_b[ti], p->subgrid_friction_exponent);
+  }
+}
+
+static void picard_assemble(picard_t *s) {
+  const double *beta = s->is_diva ? s->beta_eff_b : s->beta_b;
+  assemble(s->m, s->p, s->N_b, s->dNdx, s->dNdy, beta, s->tdx, s->tdy, s->bc_mask, s->bc_u, s->bc_v,
+           s->Aptr, s->Aind, s->Aval, s->bb);
+}
+
+static double rel_residual(picard_t *s, const double *u, const double *v) {
+  int n = 2 * s->m->nT;
+  for (int ti = 0; ti < s->m->nT; ++ti) { s->xx[2 * ti] = u[ti]; s->xx[2 * ti + 1] = v[ti]; }
+  double *r = malloc(sizeof(double) * n);
+  oracle_csr_matvec(n, s->Aptr, s->Aind, s->Aval, s->xx, r);
+  double rr = 0, b2 = 0;
+  for (int i = 0; i < n; ++i) { double d = s->bb[i] - r[i]; rr += d * d; b2 += s->bb[i] * s->bb[i]; }
+  free(r);
+  return b2 > 0 ? sqrt(rr / b2) : sqrt(rr);
+}
+
+static int picard_alloc(picard_t *s) {
+  const ufe_mesh_desc *m = s->m;
+  int nV = m->nV, nT = m->nT, nz = m->nz;
+  size_t nnzA = (size_t)2 * 2 * max_row_nnz(m) * nT + 8 * (size_t)nT;
+#define AL(x, n) s->x = calloc((n), sizeof(*s->x))
+  AL(tdx, nT); AL(tdy, nT); AL(ux, nV); AL(uy, nV); AL(vx, nV); AL(vy, nV); AL(eta_a, nV);
+  AL(N_a, nV); AL(N_b, nT); AL(dNdx, nT); AL(dNdy, nT); AL(beta_a, nV); AL(beta_b, nT);
+  AL(beta_eff_a, nV); AL(beta_eff_b, nT); AL(A_vav, nV);
+  AL(eta3a, (size_t)nV * nz); AL(F1a, (size_t)nV * nz); AL(F2a, (size_t)nV * nz);
+  AL(F1b, (size_t)nT * nz); AL(F2b, (size_t)nT * nz); AL(duz_a, (size_t)nV * nz);
+  AL(dvz_a, (size_t)nV * nz); AL(tmpb, (size_t)nT * nz); AL(tmpa, (size_t)nV * nz);
+  AL(Aptr, 2 * nT + 1); AL(Aind, nnzA); AL(Aval, nnzA); AL(bb, 2 * nT); AL(xx, 2 * nT);
+#undef AL
+  return 0;
+}
+
+static void picard_free(picard_t *s) {
+  void *ps[] = {s->tdx, s->tdy, s->ux, s->uy, s->vx, s->vy, s->eta_a, s->N_a, s->N_b, s->dNdx,
+                s->dNdy, s->beta_a, s->beta_b, s->beta_eff_a, s->beta_eff_b, s->A_vav, s->eta3a,
+                s->F1a, s->F2a, s->F1b, s->F2b, s->duz_a, s->dvz_a, s->tmpb, s->tmpa, s->Aptr,
+                s->Aind, s->Aval, s->bb, s->xx};
+  for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i) free(ps[i]);
+}
+
+/* The shared Picard loop of solve_SSA (SSA_main.f90:149-237) and solve_DIVA
+ * (DIVA_main.f90:154-254). */
+static int picard(picard_t *s, double *u, double *v, double *eta3b, double *tbx, double *tby,
+                  double *ubase, double *vbase, ufe_solve_info *info) {
+  const ufe_mesh_desc *m = s->m;
+  const ufe_params *p = s->p;
+  int nT = m->nT, nV = m->nV, nz = m->nz;
+  double t0 = now_s();
+  picard_alloc(s);
+  driving_stress(m, s->Hi, s->Hs, s->tdx, s->tdy);
+  if (!s->is_diva)
+    for (int vi = 0; vi < nV; ++vi) s->A_vav[vi] = vertical_average(nz, m->zeta, s->A_flow + vi, nV);
+  double L2 = 1e9, L2_prev;
+  int nit_div = 0;
+  double relax = p->visc_it_relax, eps0 = p->epsilon_sq_0;
+  int n_axb = 0, it = 0, converged = 0, rc = 0;
+  double *up = malloc(sizeof(double) * nT), *vp = malloc(sizeof(double) * nT);
+  double resid = -1.0;
+  while (!converged) {
+    ++it;
+    picard_state(s, u, v, eps0, eta3b, tbx, tby, ubase, vbase);
+    picard_assemble(s);
+    if (p->stop_rule == UFE_STOP_RESIDUAL) {
+      resid = rel_residual(s, u, v);
+      if (resid <= p->nonlinear_rtol) { --it; converged = 1; break; }
+    }
+    /* solve_SSA_DIVA_linearised: keep previous solution, solve, split u/v */
+    memcpy(up, u, sizeof(double) * nT);
+    memcpy(vp, v, sizeof(double) * nT);
+    for (int ti = 0; ti < nT; ++ti) { s->xx[2 * ti] = u[ti]; s->xx[2 * ti + 1] = v[ti]; }
+    int its = oracle_gmres_ilu0(2 * nT, s->Aptr, s->Aind, s->Aval, s->bb, s->xx, p->rtol, p->abstol,
+                                p->krylov_maxit, p->gmres_restart, !p->warm_start, NULL);
+    if (its < 0) { rc = UFE_ERR_ARGUMENT; break; }
+    n_axb += its;
+    for (int ti = 0; ti < nT; ++ti) { u[ti] = s->xx[2 * ti]; v[ti] = s->xx[2 * ti + 1]; }
+    /* apply_velocity_limits (SSA_DIVA_utilities.f90:110-141) */
+    for (int ti = 0; ti < nT; ++ti) {
+      double ua = sqrt(u[ti] * u[ti] + v[ti] * v[ti]);
+      if (ua > p->vel_max) { u[ti] = u[ti] * p->vel_max / ua; v[ti] = v[ti] * p->vel_max / ua; }
+    }
+    /* relax_viscosity_iterations (:84-108) */
+    for (int ti = 0; ti < nT; ++ti) {
+      u[ti] = relax * u[ti] + (1.0 - relax) * up[ti];
+      v[ti] = relax * v[ti] + (1.0 - relax) * vp[ti];
+    }
+    if (s->is_diva) {
+      /* calc_basal_velocities, calc_basal_shear_stress (DIVA_main.f90:576-634) */
+      for (int ti = 0; ti < nT; ++ti) {
+        if (p->sliding_law == UFE_SLIDING_NO_SLIDING) { ubase[ti] = 0; vbase[ti] = 0; }
+        else {
+          double d = 1.0 + s->beta_b[ti] * s->F2b[ti];
+          ubase[ti] = u[ti] / d;
+          vbase[ti] = v[ti] / d;
+        }
+        tbx[ti] = u[ti] * s->beta_eff_b[ti];
+        tby[ti] = v[ti] * s->beta_eff_b[ti];
+      }
+    }
+    /* calc_L2_norm_uv (:143-184) */
+    L2_prev = L2;
+    double r1 = 0, r2 = 0;
+    for (int ti = 0; ti < nT; ++ti) {
+      r1 += (u[ti] - up[ti]) * (u[ti] - up[ti]) + (v[ti] - vp[ti]) * (v[ti] - vp[ti]);
+      r2 += (u[ti] + up[ti]) * (u[ti] + up[ti]) + (v[ti] + vp[ti]) * (v[ti] + vp[ti]);
+    }
+    L2 = 2.0 * r1 / fmax(r2, 1e-8);
+    /* adaptive relaxation (SSA_main.f90:196-213) */
+    if (L2 > L2_prev) ++nit_div; else nit_div = 0;
+    if (nit_div > 2) { nit_div = 0; relax *= 0.9; eps0 *= 1.2; }
+    if (relax <= 0.05 || eps0 >= 1e-5) {
+      if (relax < 0.05 || eps0 > 1e-5) { rc = UFE_ERR_DIVERGED; break; }
+    }
+    if (p->stop_rule == UFE_STOP_L2UV && L2 < p->visc_it_norm_dUV_tol) converged = 1;
+    if (it > p->visc_it_nit) { rc = UFE_WARN_NOT_CONVERGED; break; }
+  }
+  if (s->is_diva && rc >= 0) {
+    /* calc_3D_velocities is applied by the caller */
+  }
+  if (info) {
+    if (p->stop_rule != UFE_STOP_RESIDUAL || !converged) {
+      picard_state(s, u, v, eps0, eta3b, tbx, tby, ubase, vbase);
+      picard_assemble(s);
+      resid = rel_residual(s, u, v);
+    }
+    info->n_visc_its = it;
+    info->n_Axb_its = n_axb;
+    info->n_refinements = 0;
+    info->L2_uv = L2;
+    info->rel_residual = resid;
+    info->visc_it_relax_applied = relax;
+    info->epsilon_sq_0_applied = eps0;
+    info->seconds = now_s() - t0;
+  }
+  free(up); free(vp);
+  return rc;
+}
+
+static int any_grounded(int nV, const int *mask) {
+  if (!mask) return 1;
+  for (int vi = 0; vi < nV; ++vi) if (mask[vi]) return 1;
+  return 0;
+}
+
+/* solve_SSA (SSA_main.f90:87-242) */
+int oracle_solve_ssa(const ufe_mesh_desc *m, const ufe_params *p, const double *Hi, const double *Hs,
+                     const double *A_flow, const double *beta_sq, const double *fgr_b,
+                     const int *mask_grounded, const int *bc_mask, const double *bc_u,
+                     const double *bc_v, double *u, double *v, ufe_solve_info *info) {
+  if (!any_grounded(m->nV, mask_grounded) || p->sliding_law == UFE_SLIDING_NO_SLIDING) {
+    memset(u, 0, sizeof(double) * m->nT);
+    memset(v, 0, sizeof(double) * m->nT);
+    if (info) memset(info, 0, sizeof(*info));
+    return 0;
+  }
+  picard_t s;
+  memset(&s, 0, sizeof(s));
+  s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
+  s.bc_mask = bc_mask; s.bc_u = bc_u; s.bc_v = bc_v; s.is_diva = 0;
+  int rc = picard(&s, u, v, NULL, NULL, NULL, NULL, NULL, info);
+  picard_free(&s);
+  return rc;
+}
+
+/* solve_DIVA (DIVA_main.f90:88-262) including calc_3D_velocities (:636-676) */
+int oracle_solve_diva(const ufe_mesh_desc *m, const ufe_params *p, const double *Hi, const double *Hs,
+                      const double *A_flow, const double *beta_sq, const double *fgr_b,
+                      const int *mask_grounded, const int *bc_mask, const double *bc_u,
+                      const double *bc_v, double *u, double *v, double *ubase, double *vbase,
+                      double *u3d, double *v3d, double *eta3b, double *tbx, double *tby,
+                      ufe_solve_info *info) {
+  int nT = m->nT, nz = m->nz;
+  if (!any_grounded(m->nV, mask_grounded)) {
+    memset(u, 0, sizeof(double) * nT); memset(v, 0, sizeof(double) * nT);
+    memset(ubase, 0, sizeof(double) * nT); memset(vbase, 0, sizeof(double) * nT);
+    memset(u3d, 0, sizeof(double) * nT * nz); memset(v3d, 0, sizeof(double) * nT * nz);
+    if (info) memset(info, 0, sizeof(*info));
+    return 0;
+  }
+  picard_t s;
+  memset(&s, 0, sizeof(s));
+  s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
+  s.bc_mask = bc_mask; s.bc_u = bc_u; s.bc_v = bc_v; s.is_diva = 1;
+  int rc = picard(&s, u, v, eta3b, tbx, tby, ubase, vbase, info);
+  if (rc >= 0) {
+    for (int k = 0; k < nz; ++k)
+      for (int ti = 0; ti < nT; ++ti) {
+        size_t i = (size_t)k * nT + ti;
+        if (p->sliding_law == UFE_SLIDING_NO_SLIDING) {
+          u3d[i] = tbx[ti] * s.F1b[i];
+          v3d[i] = tby[ti] * s.F1b[i];
+        } else {
+          u3d[i] = ubase[ti] * (1.0 + s.beta_b[ti] * s.F1b[i]);
+          v3d[i] = vbase[ti] * (1.0 + s.beta_b[ti] * s.F1b[i]);
+        }
+      }
+  }
+  picard_free(&s);
+  return rc;
+}
+
+/* Nonlinear residual of the SSA at (u, v): r = b(u) - A(u) u in the
+ * reference row order 2*ti+uv; returns ||r||/||b||.  Used by the parity tests
+ * to check the whole per-iteration pipeline (strain rates, viscosity,
+ * sliding, a->b maps, assembly, product) at one point. */
+double oracle_ssa_residual(const ufe_mesh_desc *m, const ufe_params *p, const double *Hi,
+                           const double *Hs, const double *A_flow, const double *beta_sq,
+                           const double *fgr_b, const int *bc_mask, const double *bc_u,
+                           const double *bc_v, const double *u, const double *v, double *r_out,
+                           double *b_out) {
+  picard_t s;
+  memset(&s, 0, sizeof(s));
+  s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
+  s.bc_mask = bc_mask; s.bc_u = bc_u; s.bc_v = bc_v; s.is_diva = 0;
+  picard_alloc(&s);
+  driving_stress(m, Hi, Hs, s.tdx, s.tdy);
+  for (int vi = 0; vi < m->nV; ++vi) s.A_vav[vi] = vertical_average(m->nz, m->zeta, A_flow + vi, m->nV);
+  picard_state(&s, u, v, p->epsilon_sq_0, NULL, NULL, NULL, NULL, NULL);
+  picard_assemble(&s);
+  int n = 2 * m->nT;
+  for (int ti = 0; ti < m->nT; ++ti) { s.xx[2 * ti] = u[ti]; s.xx[2 * ti + 1] = v[ti]; }
+  double *r = malloc(sizeof(double) * n);
+  oracle_csr_matvec(n, s.Aptr, s.Aind, s.Aval, s.xx, r);
+  double rr = 0, b2 = 0;
+  for (int i = 0; i < n; ++i) {
+    r[i] = s.bb[i] - r[i];
+    rr += r[i] * r[i];
+    b2 += s.bb[i] * s.bb[i];
+    if (r_out) r_out[i] = r[i];
+    if (b_out) b_out[i] = s.bb[i];
+  }
+  free(r);
+  picard_free(&s);
+  return b2 > 0 ? sqrt(rr / b2) : sqrt(rr);
+}
+
+/* Export the assembled system at (u, v) in CSR (reference ordering); the
+ * caller provides arrays of size 2*nT+1 and nnz_cap. Returns nnz. */
+int oracle_ssa_system(const ufe_mesh_desc *m, const ufe_params *p, const double *Hi, const double *Hs,
+                      const double *A_flow, const double *beta_sq, const double *fgr_b,
+                      const double *u, const double *v, int nnz_cap, int *ptr, int *ind, double *val,
+                      double *b) {
+  picard_t s;
+  memset(&s, 0, sizeof(s));
+  s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
+  picard_alloc(&s);
+  driving_stress(m, Hi, Hs, s.tdx, s.tdy);
+  for (int vi = 0; vi < m->nV; ++vi) s.A_vav[vi] = vertical_average(m->nz, m->zeta, A_flow + vi, m->nV);
+  picard_state(&s, u, v, p->epsilon_sq_0, NULL, NULL, NULL, NULL, NULL);
+  picard_assemble(&s);
+  int n = 2 * m->nT, nnz = s.Aptr[n];
+  if (nnz > nnz_cap) { picard_free(&s); return -nnz; }
+  memcpy(ptr, s.Aptr, sizeof(int) * (n + 1));
+  memcpy(ind, s.Aind, sizeof(int) * nnz);
+  memcpy(val, s.Aval, sizeof(double) * nnz);
+  memcpy(b, s.bb, sizeof(double) * n);
+  picard_free(&s);
+  return nnz;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Grounded fractions (subgrid_grounded_fractions_bilin_TAF.f90:182-370)   */
+/* ---------------------------------------------------------------------- */
+
+static double tri_area(const double *p, const double *q, const double *r) {
+  return fabs((q[0] - p[0]) * (r[1] - p[1]) - (q[1] - p[1]) * (r[0] - p[0])) / 2.0;
+}
+
+static double sub_area(const double *va, const double *vb, const double *vc, double ta, double tb,
+                       double tc) {
+  double lab = ta / (ta - tb), lac = ta / (ta - tc);
+  double pab[2] = {va[0] * (1 - lab) + vb[0] * lab, va[1] * (1 - lab) + vb[1] * lab};
+  double pac[2] = {va[0] * (1 - lac) + vc[0] * lac, va[1] * (1 - lac) + vc[1] * lac};
+  return tri_area(va, pab, pac);
+}
+
+void oracle_fraction_gr_b(const ufe_mesh_desc *m, const double *Hi, const double *Hb,
+                          const double *SL, double *fgr) {
+  const double tol = 1e-9;
+  for (int ti = 0; ti < m->nT; ++ti) {
+    const int *t = m->Tri + 3 * ti;
+    double T[3];
+    for (int n = 0; n < 3; ++n) {
+      int vi = t[n];
+      T[n] = Hi[vi] - fmax(0.0, (SL[vi] - Hb[vi]) * (SEAWATER_DENSITY / ICE_DENSITY));
+    }
+    double tmax = fmax(T[0], fmax(T[1], T[2])), tmin = fmin(T[0], fmin(T[1], T[2]));
+    if (tmin >= 0.0) { fgr[ti] = 1.0; continue; }
+    if (tmax <= 0.0) { fgr[ti] = 0.0; continue; }
+    const double *va = m->V + 2 * t[0], *vb = m->V + 2 * t[1], *vc = m->V + 2 * t[2];
+    double At = tri_area(va, vb, vc), Ag;
+    double a = T[0] >= 0 ? fmax(tol, T[0]) : fmin(-tol, T[0]);
+    double b = T[1] >= 0 ? fmax(tol, T[1]) : fmin(-tol, T[1]);
+    double c = T[2] >= 0 ? fmax(tol, T[2]) : fmin(-tol, T[2]);
+    if (a >= 0 && b >= 0 && c >= 0) Ag = At;
+    else if (a <= 0 && b <= 0 && c <= 0) Ag = 0;
+    else if (a >= 0 && b <= 0 && c <= 0) Ag = sub_area(va, vb, vc, a, b, c);
+    else if (a <= 0 && b >= 0 && c <= 0) Ag = sub_area(vb, vc, va, b, c, a);
+    else if (a <= 0 && b <= 0 && c >= 0) Ag = sub_area(vc, va, vb, c, a, b);
+    else if (a <= 0 && b >= 0 && c >= 0) Ag = At - sub_area(va, vb, vc, a, b, c);
+    else if (a >= 0 && b <= 0 && c >= 0) Ag = At - sub_area(vb, vc, va, b, c, a);
+    else Ag = At - sub_area(vc, va, vb, c, a, b);
+    fgr[ti] = fmin(1.0, fmax(0.0, Ag / At));
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Heat columns (thermodynamics_3D_heat_equation.f90:114-293, :309-479)    */
+/* ---------------------------------------------------------------------- */
+
+/* zeta tridiagonal operators: calc_vertical_operators_reg_1D
+ * (mesh_zeta.f90:287-365) with calc_shape_functions_1D_reg_2nd_order
+ * (shape_functions.f90:17-79), then calc_zeta_operators_tridiagonal (:487-605) */
+void oracle_zeta_tridiagonals(int nz, const double *zeta, double *dl, double *dd, double *du,
+                              double *d2l, double *d2d, double *d2u) {
+  for (int k = 0; k < nz; ++k) {
+    int c[2];
+    if (k == 0) { c[0] = 1; c[1] = 2; }
+    else if (k == nz - 1) { c[0] = nz - 3; c[1] = nz - 2; }
+    else { c[0] = k - 1; c[1] = k + 1; }
+    double dx[2], w[2], A[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 2; ++i) {
+      dx[i] = zeta[c[i]] - zeta[k];
+      w[i] = 1.0 / pow(fabs(dx[i]), 1.5);
+      double w2 = w[i] * w[i];
+      A[0] += w2 * dx[i] * dx[i];
+      A[1] += w2 * dx[i] * 0.5 * dx[i] * dx[i];
+      A[2] += w2 * 0.5 * dx[i] * dx[i] * dx[i];
+      A[3] += w2 * 0.5 * dx[i] * dx[i] * 0.5 * dx[i] * dx[i];
+    }
+    double det = A[0] * A[3] - A[1] * A[2];
+    double M[4] = {A[3] / det, -A[1] / det, -A[2] / det, A[0] / det};
+    double fx[2], fxx[2];
+    for (int i = 0; i < 2; ++i) {
+      double w2 = w[i] * w[i];
+      fx[i] = w2 * (M[0] * dx[i] + M[1] * 0.5 * dx[i] * dx[i]);
+      fxx[i] = w2 * (M[2] * dx[i] + M[3] * 0.5 * dx[i] * dx[i]);
+    }
+    double fxi = -(fx[0] + fx[1]), fxxi = -(fxx[0] + fxx[1]);
+    dd[k] = fxi; d2d[k] = fxxi;
+    if (k > 0) { dl[k - 1] = 0; d2l[k - 1] = 0; }
+    if (k < nz - 1) { du[k] = 0; d2u[k] = 0; }
+    for (int i = 0; i < 2; ++i) {
+      if (c[i] == k - 1) { dl[k - 1] = fx[i]; d2l[k - 1] = fxx[i]; }
+      if (c[i] == k + 1) { du[k] = fx[i]; d2u[k] = fxx[i]; }
+    }
+  }
+}
+
+static void solve_column(int nz, const double *zeta, const double *dl, const double *dd,
+                         const double *du, const double *d2l, const double *d2d, const double *d2u,
+                         const double *Ti, const double *u, const double *v, const double *w,
+                         const double *uadv, const double *vadv, double T_surf, const double *Tpmp,
+                         const double *Ki, const double *Cpi, const double *zx, const double *zy,
+                         const double *zz, const double *zt, const double *Phi, double dt,
+                         int grounded, double Q, double Tbf, double *out) {
+  double a[64], b[64], c[64], d[64];
+  for (int k = 0; k < nz; ++k) { a[k] = 0; b[k] = 0; c[k] = 0; d[k] = 0; }
+  b[0] = 1.0;
+  d[0] = fmin(T_surf, T0);
+  b[nz - 1] = 1.0;
+  if (grounded)
+    d[nz - 1] = fmin(Tpmp[nz - 1], Ti[nz - 2] - (zeta[nz - 1] - zeta[nz - 2]) * Q / (zz[nz - 1] * Ki[nz - 1]));
+  else
+    d[nz - 1] = fmin(Tbf, Tpmp[nz - 1]);
+  for (int k = 1; k < nz - 1; ++k) {
+    double c1 = zt[k] + u[k] * zx[k] + v[k] * zy[k] + w[k] * zz[k];
+    double c2 = -Ki[k] / (ICE_DENSITY * Cpi[k]) * zz[k] * zz[k];
+    a[k] = c1 * dl[k - 1] + c2 * d2l[k - 1];
+    b[k] = 1.0 / dt + c1 * dd[k] + c2 * d2d[k];
+    c[k] = c1 * du[k] + c2 * d2u[k];
+    d[k] = Ti[k] / dt - uadv[k] - vadv[k] + Phi[k] / (ICE_DENSITY * Cpi[k]);
+  }
+  oracle_tridiag(nz, a, b, c, d, out);
+  for (int k = 0; k < nz; ++k) out[k] = fmin(out[k], Tpmp[k]);
+}
+
+int oracle_heat_columns(const ufe_heat_inputs *in, double *Tnew, int *is_unstable, int *n_unstable) {
+  int nV = in->nV, nz = in->nz;
+  if (nz > 64 || nz < 3) return UFE_ERR_ARGUMENT;
+  double dl[64], dd[64], du[64], d2l[64], d2d[64], d2u[64];
+  oracle_zeta_tridiagonals(nz, in->zeta, dl, dd, du, d2l, d2d, d2u);
+  for (int vi = 0; vi < nV; ++vi)
+    if (!(in->Hi_eff[vi] < in->Hi_min_thermo) && in->mask[vi] != 1 && in->mask[vi] != 2)
+      return UFE_ERR_ARGUMENT; /* reference crash: thick ice neither grounded nor floating */
+  int nun = 0;
+#pragma omp parallel for reduction(+ : nun) schedule(static)
+  for (int vi = 0; vi < nV; ++vi) {
+    double col[12][64];
+    double Ti[64], Tn[64];
+    is_unstable[vi] = 0;
+    if (in->Hi_eff[vi] < in->Hi_min_thermo) {
+      for (int k = 0; k < nz; ++k) Tnew[(size_t)k * nV + vi] = in->T_surf[vi];
+      continue;
+    }
+    const double *src[12] = {in->u_3D, in->v_3D, in->w_3D, in->u_dTdxp_upwind, in->v_dTdyp_upwind,
+                             in->dzeta_dx, in->dzeta_dy, in->dzeta_dz, in->dzeta_dt, in->Phi, NULL, NULL};
+    for (int q = 0; q < 10; ++q)
+      for (int k = 0; k < nz; ++k) col[q][k] = src[q][(size_t)k * nV + vi];
+    double Tpmp[64], Ki[64], Cpi[64];
+    for (int k = 0; k < nz; ++k) {
+      double T = in->Ti[(size_t)k * nV + vi];
+      Ti[k] = T;
+      /* calc_pressure_melting_point, calc_thermal_conductivity (Ritz1987),
+       * calc_heat_capacity (Pounder1965): thermodynamics_utilities.f90:130-229 */
+      Tpmp[k] = T0 - CLAUSIUS_CLAPEYRON * in->Hi_eff[vi] * in->zeta[k];
+      Ki[k] = 3.101e8 * exp(-0.0057 * T);
+      Cpi[k] = 2115.3 + 7.79293 * (T - T0);
+    }
+    int grounded = in->mask[vi] == 1;
+    double Tbf = Tpmp[nz - 1];
+    /* The reference does not reset the column between halving attempts: each
+     * attempt starts from the profile the previous attempt left
+     * (thermodynamics_3D_heat_equation.f90:161-239). */
+    int stable = 0, it_dt = 0;
+    double Tcur[64];
+    for (int k = 0; k < nz; ++k) Tcur[k] = Ti[k];
+    while (!stable && it_dt < 10) {
+      ++it_dt;
+      double dta = in->dt * pow(0.5, it_dt - 1);
+      for (int s = 0; s < (1 << (it_dt - 1)); ++s) {
+        solve_column(nz, in->zeta, dl, dd, du, d2l, d2d, d2u, Tcur, col[0], col[1], col[2], col[3], col[4],
+                     in->T_surf[vi], Tpmp, Ki, Cpi, col[5], col[6], col[7], col[8], col[9], dta,
+                     grounded, in->Q_base_grnd[vi], Tbf, Tn);
+        for (int k = 0; k < nz; ++k) Tcur[k] = Tn[k];
+      }
+      stable = 1;
+      for (int k = 0; k < nz; ++k)
+        if (Tcur[k] != Tcur[k] || Tcur[k] < 180.0 || Tcur[k] > T0) stable = 0;
+    }
+    if (!stable) { is_unstable[vi] = 1; ++nun; }
+    for (int k = 0; k < nz; ++k) Tnew[(size_t)k * nV + vi] = Tcur[k];
+  }
+  *n_unstable = nun;
+  return 0;
+}

@@ -1,0 +1,189 @@
+// DIVA kernels (Lipscomb et al. 2019 as implemented in DIVA_main.f90).
+//
+// Device layout of 3-D fields is layer-fastest ([vertex][k], [triangle][k]):
+// one thread walks one column with unit stride and the ζ-integrals run in
+// registers.  The C ABI converts from/to the reference's (n, nz) layout.
+#include "kernels.h"
+
+namespace ufe {
+
+constexpr int kMaxNz = 64;
+
+// One thread per vertex:
+//   strain rates from u_vav (b->a ddx/ddy)            calc_horizontal_strain_rates
+//   du/dz, dv/dz on the b-grid from tau_b and the previous eta_3D_b, mapped
+//   b->a (DIVA_main.f90:396-405), 3-D Glen viscosity with clamp (:441-455),
+//   vertical average and N = eta_vav * max(0.1, Hi) (:461-469; the reference
+//   writes a default-real 0.1 there), F1/F2 integrals from the base
+//   (:497-511, integrate_from_zeta_is_one_to_zeta_is_zetap, mesh_zeta.f90:163-194),
+//   basal friction from u_base (:540) and beta_eff = beta / (1 + beta F2(surface))
+//   (:542-557).
+__global__ void __launch_bounds__(kBlock) k_vertex_diva(
+    int nV, int nz, const int *__restrict__ ptr, const int *__restrict__ ind, const double *__restrict__ wmap,
+    const double *__restrict__ wdx, const double *__restrict__ wdy, const double *__restrict__ zeta,
+    const double *__restrict__ x, const double *__restrict__ xbase, const double *__restrict__ taub,
+    const double *__restrict__ eta3b, const double *__restrict__ A_vk, const double *__restrict__ Hi,
+    const double *__restrict__ rough, PicardParams p, double *__restrict__ N_a, double *__restrict__ beta_a,
+    double *__restrict__ beta_eff_a, double *__restrict__ eta3a, double *__restrict__ F1a,
+    double *__restrict__ F2a) {
+  int vi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (vi >= nV) return;
+  double ux = 0, uy = 0, vx = 0, vy = 0, ub = 0, vb = 0;
+  double duz[kMaxNz], dvz[kMaxNz];
+  for (int k = 0; k < nz; ++k) { duz[k] = 0.0; dvz[k] = 0.0; }
+  for (int q = ptr[vi]; q < ptr[vi + 1]; ++q) {
+    int t = ind[q];
+    double2 uv = reinterpret_cast<const double2 *>(x)[t];
+    double2 bs = reinterpret_cast<const double2 *>(xbase)[t];
+    double2 tb = reinterpret_cast<const double2 *>(taub)[t];
+    double cx = wdx[q], cy = wdy[q], cm = wmap[q];
+    ux += cx * uv.x; uy += cy * uv.x; vx += cx * uv.y; vy += cy * uv.y;
+    ub += cm * bs.x; vb += cm * bs.y;
+    const double *e = eta3b + (size_t)t * nz;
+    for (int k = 0; k < nz; ++k) {
+      double inv = zeta[k] / fmax(p.visc_eff_min, e[k]);
+      duz[k] += cm * tb.x * inv;
+      dvz[k] += cm * tb.y * inv;
+    }
+  }
+  double *eo = eta3a + (size_t)vi * nz;
+  const double *Av = A_vk + (size_t)vi * nz;
+  double eprev = 0.0, evav = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    double e2 = ux * ux + vy * vy + ux * vy + 0.25 * (uy + vx) * (uy + vx) +
+                0.25 * (duz[k] * duz[k] + dvz[k] * dvz[k]) + p.eps0;
+    double eta = 0.5 * pow(Av[k], -1.0 / p.n) * pow(e2, (1.0 - p.n) / (2.0 * p.n));
+    eta = fmin(fmax(eta, p.visc_eff_min), p.eta_max);
+    eo[k] = eta;
+    if (k > 0) evav += 0.5 * (eta + eprev) * (zeta[k] - zeta[k - 1]);
+    eprev = eta;
+  }
+  double h = fmax(0.1, Hi[vi]);
+  N_a[vi] = evav * fmax((double)0.1f, Hi[vi]);
+  double *f1 = F1a + (size_t)vi * nz, *f2 = F2a + (size_t)vi * nz;
+  double I1 = 0.0, I2 = 0.0;
+  f1[nz - 1] = 0.0;
+  f2[nz - 1] = 0.0;
+  for (int k = nz - 2; k >= 0; --k) {
+    double z0 = zeta[k], z1 = zeta[k + 1], e0 = eo[k], e1 = eo[k + 1];
+    I1 = I1 - 0.5 * (z1 / e1 + z0 / e0) * (z1 - z0);
+    I2 = I2 - 0.5 * (z1 * z1 / e1 + z0 * z0 / e0) * (z1 - z0);
+    f1[k] = -h * I1;
+    f2[k] = -h * I2;
+  }
+  double uabs = sqrt(p.delta_v * p.delta_v + ub * ub + vb * vb);
+  double b;
+  switch (p.sliding_law) {
+    case UFE_SLIDING_WEERTMAN: b = rough[vi] * pow(uabs, 1.0 / p.weertman_m - 1.0); break;
+    case UFE_SLIDING_BUDD: b = rough[vi] * pow(uabs, p.budd_q - 1.0) / pow(p.budd_u_th, p.budd_q); break;
+    case UFE_SLIDING_COULOMB: b = rough[vi] / uabs; break;
+    default: b = 0.0;
+  }
+  b = fmin(p.beta_max, b);
+  beta_a[vi] = b;
+  beta_eff_a[vi] = p.no_sliding ? 1.0 / f2[0] : b / (1.0 + b * f2[0]);
+}
+
+// One thread per triangle: eta_3D_b = map_a_b_3D(eta_3D_a) (DIVA_main.f90:458),
+// F2_3D_b at the surface (needed for the basal velocities) and, at the end of
+// the solve, the full F1_3D_b for calc_3D_velocities (:514).
+__global__ void __launch_bounds__(kBlock) k_triangle_3d(int nT, int nz, const int *__restrict__ ptr,
+                                                        const int *__restrict__ ind, const double *__restrict__ wmap,
+                                                        const double *__restrict__ eta3a,
+                                                        const double *__restrict__ F1a,
+                                                        const double *__restrict__ F2a, double *__restrict__ eta3b,
+                                                        double *__restrict__ F1b, double *__restrict__ F2b0,
+                                                        int full_F1) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nT) return;
+  double e[kMaxNz], f[kMaxNz];
+  for (int k = 0; k < nz; ++k) { e[k] = 0.0; f[k] = 0.0; }
+  double f20 = 0.0;
+  for (int q = ptr[ti]; q < ptr[ti + 1]; ++q) {
+    int v = ind[q];
+    double cm = wmap[q];
+    const double *ev = eta3a + (size_t)v * nz;
+    for (int k = 0; k < nz; ++k) e[k] += cm * ev[k];
+    f20 += cm * F2a[(size_t)v * nz];
+    if (full_F1) {
+      const double *fv = F1a + (size_t)v * nz;
+      for (int k = 0; k < nz; ++k) f[k] += cm * fv[k];
+    }
+  }
+  double *eo = eta3b + (size_t)ti * nz;
+  for (int k = 0; k < nz; ++k) eo[k] = e[k];
+  F2b0[ti] = f20;
+  if (full_F1) {
+    double *fo = F1b + (size_t)ti * nz;
+    for (int k = 0; k < nz; ++k) fo[k] = f[k];
+  }
+}
+
+// calc_3D_velocities (DIVA_main.f90:636-676); outputs in the reference's
+// (nT, nz) triangle-fastest layout.
+__global__ void __launch_bounds__(kBlock) k_velocities_3d(int nT, int nz, const double *__restrict__ ubase,
+                                                          const double *__restrict__ beta_b,
+                                                          const double *__restrict__ taub,
+                                                          const double *__restrict__ F1b, int no_sliding,
+                                                          double *__restrict__ u3, double *__restrict__ v3) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nT) return;
+  double2 ub = reinterpret_cast<const double2 *>(ubase)[ti];
+  double2 tb = reinterpret_cast<const double2 *>(taub)[ti];
+  double bb = beta_b[ti];
+  for (int k = 0; k < nz; ++k) {
+    double f = F1b[(size_t)ti * nz + k];
+    size_t o = (size_t)k * nT + ti;
+    if (no_sliding) { u3[o] = tb.x * f; v3[o] = tb.y * f; }
+    else { u3[o] = ub.x * (1.0 + bb * f); v3[o] = ub.y * (1.0 + bb * f); }
+  }
+}
+
+// out[c][r] = in[r][c] for an rows x cols row-major matrix (LDS-tiled).
+__global__ void __launch_bounds__(256) k_transpose(int rows, int cols, const double *__restrict__ in,
+                                                   double *__restrict__ out) {
+  __shared__ double tile[32][33];
+  int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    int r = r0 + j, c = c0 + tx;
+    if (r < rows && c < cols) tile[j][tx] = in[(size_t)r * cols + c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    int c = c0 + j, r = r0 + tx;
+    if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][j];
+  }
+}
+
+void launch_vertex_diva(const DevMesh &m, const double *x, const double *xbase, const double *taub,
+                        const double *eta3b_tk, const double *A_vk, const double *Hi, const double *rough,
+                        const PicardParams &p, double *N_a, double *beta_a, double *beta_eff_a, double *eta3a_vk,
+                        double *F1a_vk, double *F2a_vk, hipStream_t s) {
+  hipLaunchKernelGGL(k_vertex_diva, dim3(blocks_for(m.nV)), dim3(kBlock), 0, s, m.nV, m.nz, m.ba_ptr, m.ba_ind,
+                     m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, A_vk, Hi, rough, p, N_a,
+                     beta_a, beta_eff_a, eta3a_vk, F1a_vk, F2a_vk);
+  UFE_LAUNCH_CHECK();
+}
+
+void launch_triangle_3d(const DevMesh &m, const double *eta3a_vk, const double *F1a_vk, const double *F2a_vk,
+                        double *eta3b_tk, double *F1b_tk, double *F2b0, int full_F1, hipStream_t s) {
+  hipLaunchKernelGGL(k_triangle_3d, dim3(blocks_for(m.nT)), dim3(kBlock), 0, s, m.nT, m.nz, m.ab_ptr, m.ab_ind,
+                     m.ab_map, eta3a_vk, F1a_vk, F2a_vk, eta3b_tk, F1b_tk, F2b0, full_F1);
+  UFE_LAUNCH_CHECK();
+}
+
+void launch_velocities_3d(int nT, int nz, const double *ubase, const double *beta_basal_b, const double *taub,
+                          const double *F1b_tk, int no_sliding, double *u3_out, double *v3_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_velocities_3d, dim3(blocks_for(nT)), dim3(kBlock), 0, s, nT, nz, ubase, beta_basal_b, taub,
+                     F1b_tk, no_sliding, u3_out, v3_out);
+  UFE_LAUNCH_CHECK();
+}
+
+void launch_transpose(int rows, int cols, const double *in, double *out, hipStream_t s) {
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, s, rows, cols, in, out);
+  UFE_LAUNCH_CHECK();
+}
+
+}  // namespace ufe

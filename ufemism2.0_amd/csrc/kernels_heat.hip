@@ -1,0 +1,326 @@
+// Grounded fractions on the triangles and the vertical heat columns, with
+// their C ABI entry points.
+#include <cmath>
+#include <vector>
+
+#include "krylov.h"
+#include "ufe_internal.h"
+
+namespace ufe {
+
+// ---------------------------------------------------------------------------
+// calc_grounded_fractions_bilin_interp_TAF_b (subgrid_grounded_fractions_bilin_TAF.f90:182-250)
+// with calc_grounded_area_triangle (:252-370); TAF per vertex from
+// thickness_above_floatation (ice_geometry_basics.f90:43-56).  FP64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double tri_area(double2 p, double2 q, double2 r) {
+  return fabs((q.x - p.x) * (r.y - p.y) - (q.y - p.y) * (r.x - p.x)) / 2.0;
+}
+
+__device__ __forceinline__ double sub_area(double2 a, double2 b, double2 c, double ta, double tb, double tc) {
+  double lab = ta / (ta - tb), lac = ta / (ta - tc);
+  double2 pab = make_double2(a.x * (1.0 - lab) + b.x * lab, a.y * (1.0 - lab) + b.y * lab);
+  double2 pac = make_double2(a.x * (1.0 - lac) + c.x * lac, a.y * (1.0 - lac) + c.y * lac);
+  return tri_area(a, pab, pac);
+}
+
+__global__ void __launch_bounds__(kBlock) k_fraction_gr_b(int nT, const int *__restrict__ Tri,
+                                                          const double2 *__restrict__ V, const double *__restrict__ Hi,
+                                                          const double *__restrict__ Hb, const double *__restrict__ SL,
+                                                          double *__restrict__ fgr) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nT) return;
+  int vi[3];
+  double T[3];
+  for (int n = 0; n < 3; ++n) {
+    vi[n] = Tri[3 * ti + n];
+    T[n] = Hi[vi[n]] - fmax(0.0, (SL[vi[n]] - Hb[vi[n]]) * (kSeawaterDensity / kIceDensity));
+  }
+  double tmax = fmax(T[0], fmax(T[1], T[2])), tmin = fmin(T[0], fmin(T[1], T[2]));
+  if (tmin >= 0.0) { fgr[ti] = 1.0; return; }
+  if (tmax <= 0.0) { fgr[ti] = 0.0; return; }
+  const double tol = 1e-9;
+  double2 va = V[vi[0]], vb = V[vi[1]], vc = V[vi[2]];
+  double At = tri_area(va, vb, vc), Ag;
+  double a = T[0] >= 0 ? fmax(tol, T[0]) : fmin(-tol, T[0]);
+  double b = T[1] >= 0 ? fmax(tol, T[1]) : fmin(-tol, T[1]);
+  double c = T[2] >= 0 ? fmax(tol, T[2]) : fmin(-tol, T[2]);
+  if (a >= 0 && b >= 0 && c >= 0) Ag = At;
+  else if (a <= 0 && b <= 0 && c <= 0) Ag = 0.0;
+  else if (a >= 0 && b <= 0 && c <= 0) Ag = sub_area(va, vb, vc, a, b, c);
+  else if (a <= 0 && b >= 0 && c <= 0) Ag = sub_area(vb, vc, va, b, c, a);
+  else if (a <= 0 && b <= 0 && c >= 0) Ag = sub_area(vc, va, vb, c, a, b);
+  else if (a <= 0 && b >= 0 && c >= 0) Ag = At - sub_area(va, vb, vc, a, b, c);
+  else if (a >= 0 && b <= 0 && c >= 0) Ag = At - sub_area(vb, vc, va, b, c, a);
+  else Ag = At - sub_area(vc, va, vb, c, a, b);
+  fgr[ti] = fmin(1.0, fmax(0.0, Ag / At));
+}
+
+void launch_fraction_gr_b(const DevMesh &m, const double *Hi, const double *Hb, const double *SL, double *fgr,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_fraction_gr_b, dim3(blocks_for(m.nT)), dim3(kBlock), 0, s, m.nT, m.Tri,
+                     reinterpret_cast<const double2 *>(m.V), Hi, Hb, SL, fgr);
+  UFE_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Heat columns: one thread per vertex column.  Inputs are vertex-fastest
+// (nV, nz), so at each layer k the threads of a wavefront read consecutive
+// addresses.  The column (<= kHeatMaxNz layers) is held in registers.
+//   solve_3D_heat_equation per-vertex loop (thermodynamics_3D_heat_equation.f90:133-252)
+//   solve_1D_heat_equation (:309-479), Thomas (tridiagonal_solver.f90:13-51)
+//   Cp (Pounder 1965), K (Ritz 1987), T_pmp from the input T and H
+//   (thermodynamics_utilities.f90:130-229)
+// ---------------------------------------------------------------------------
+constexpr int kHeatMaxNz = 32;
+
+template <int NZ>
+__device__ __forceinline__ void heat_step(int nz, const double *__restrict__ tri, const double (&Ti)[NZ],
+                                          const double (&c1)[NZ], const double (&c2)[NZ],
+                                          const double (&rhs0)[NZ], double T_surf, const double (&Tpmp)[NZ],
+                                          double Kb, double zzb, double dzb, int grounded, double Q, double Tbf,
+                                          double dt, double (&out)[NZ]) {
+  double bdiag[NZ], cup[NZ], d[NZ];
+  // row 0: surface BC
+  bdiag[0] = 1.0;
+  cup[0] = 0.0;
+  d[0] = fmin(T_surf, kT0);
+  double aprev = 0.0;
+  // forward elimination interleaved with row construction
+  for (int k = 1; k < nz; ++k) {
+    double a, b, c, dk;
+    if (k < nz - 1) {
+      a = c1[k] * tri[0 * kHeatMaxNz + k - 1] + c2[k] * tri[3 * kHeatMaxNz + k - 1];
+      b = 1.0 / dt + c1[k] * tri[1 * kHeatMaxNz + k] + c2[k] * tri[4 * kHeatMaxNz + k];
+      c = c1[k] * tri[2 * kHeatMaxNz + k] + c2[k] * tri[5 * kHeatMaxNz + k];
+      dk = Ti[k] / dt + rhs0[k];
+    } else {
+      a = 0.0;
+      b = 1.0;
+      c = 0.0;
+      dk = grounded ? fmin(Tpmp[nz - 1], Ti[nz - 2] - dzb * Q / (zzb * Kb)) : fmin(Tbf, Tpmp[nz - 1]);
+    }
+    double w = a / bdiag[k - 1];
+    bdiag[k] = b - w * cup[k - 1];
+    d[k] = dk - w * d[k - 1];
+    cup[k] = c;
+    (void)aprev;
+  }
+  out[nz - 1] = d[nz - 1] / bdiag[nz - 1];
+  for (int k = nz - 2; k >= 0; --k) out[k] = (d[k] - cup[k] * out[k + 1]) / bdiag[k];
+  for (int k = 0; k < nz; ++k) out[k] = fmin(out[k], Tpmp[k]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__restrict__ Tout,
+                                                         int *__restrict__ unstable, RedSlot slot,
+                                                         double *__restrict__ count) {
+  __shared__ double tri[6 * kHeatMaxNz];
+  for (int i = threadIdx.x; i < 6 * kHeatMaxNz; i += blockDim.x) tri[i] = h.tri[i];
+  __syncthreads();
+  const int nV = h.nV, nz = h.nz;
+  double nun = 0.0;
+  for (int vi = blockIdx.x * blockDim.x + threadIdx.x; vi < nV; vi += gridDim.x * blockDim.x) {
+    unstable[vi] = 0;
+    if (h.Hi_eff[vi] < h.Hi_min) {
+      double ts = h.T_surf[vi];
+      for (int k = 0; k < nz; ++k) Tout[(size_t)k * nV + vi] = ts;
+      continue;
+    }
+    double Ti[kHeatMaxNz], c1[kHeatMaxNz], c2[kHeatMaxNz], rhs0[kHeatMaxNz], Tpmp[kHeatMaxNz];
+    double Kb = 0.0;
+    for (int k = 0; k < nz; ++k) {
+      size_t o = (size_t)k * nV + vi;
+      double T = h.Ti[o];
+      Ti[k] = T;
+      Tpmp[k] = kT0 - kClausiusClapeyron * h.Hi_eff[vi] * h.zeta[k];
+      double K = 3.101e8 * exp(-0.0057 * T);
+      double Cp = 2115.3 + 7.79293 * (T - kT0);
+      double zz = h.zz[o];
+      c1[k] = h.zt[o] + h.u[o] * h.zx[o] + h.v[o] * h.zy[o] + h.w[o] * zz;
+      c2[k] = -K / (kIceDensity * Cp) * zz * zz;
+      rhs0[k] = -h.uadv[o] - h.vadv[o] + h.Phi[o] / (kIceDensity * Cp);
+      if (k == nz - 1) Kb = K;
+    }
+    double zzb = h.zz[(size_t)(nz - 1) * nV + vi];
+    double dzb = h.zeta[nz - 1] - h.zeta[nz - 2];
+    int grounded = h.mask[vi] == 1;
+    double Q = h.Q[vi], Tbf = Tpmp[nz - 1], ts = h.T_surf[vi];
+    // dt halving without resetting the column between attempts
+    // (thermodynamics_3D_heat_equation.f90:161-239)
+    double Tc[kHeatMaxNz], Tn[kHeatMaxNz];
+    for (int k = 0; k < nz; ++k) Tc[k] = Ti[k];
+    bool stable = false;
+    for (int it = 1; it <= 10 && !stable; ++it) {
+      double dta = h.dt * pow(0.5, (double)(it - 1));
+      int nsub = 1 << (it - 1);
+      for (int sstep = 0; sstep < nsub; ++sstep) {
+        heat_step<kHeatMaxNz>(nz, tri, Tc, c1, c2, rhs0, ts, Tpmp, Kb, zzb, dzb, grounded, Q, Tbf, dta, Tn);
+        for (int k = 0; k < nz; ++k) Tc[k] = Tn[k];
+      }
+      stable = true;
+      for (int k = 0; k < nz; ++k)
+        if (Tc[k] != Tc[k] || Tc[k] < 180.0 || Tc[k] > kT0) stable = false;
+    }
+    if (!stable) { unstable[vi] = 1; nun += 1.0; }
+    for (int k = 0; k < nz; ++k) Tout[(size_t)k * nV + vi] = Tc[k];
+  }
+  double acc[1] = {nun};
+  __shared__ double lds[1][kBlock / 64];
+  block_sum<1>(acc, lds);
+  double tot[1];
+  if (grid_finish<1>(slot, acc, tot)) count[0] = tot[0];
+}
+
+void launch_heat_columns(const HeatDev &h, double *Tout, int *unstable, const RedSlot &slot, double *count,
+                         hipStream_t s) {
+  int nb = std::min(blocks_for(h.nV), kMaxPartials);
+  hipLaunchKernelGGL(k_heat_columns, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
+  UFE_LAUNCH_CHECK();
+}
+
+// zeta tridiagonals: calc_vertical_operators_reg_1D (mesh_zeta.f90:287-365)
+// with 1-D 2nd-order shape functions (shape_functions.f90:17-79), then the
+// tridiagonal extraction of calc_zeta_operators_tridiagonal (:487-605).
+void zeta_tridiagonals(int nz, const double *zeta, double *tri /* [6][kHeatMaxNz] */) {
+  for (int i = 0; i < 6 * kHeatMaxNz; ++i) tri[i] = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    int cidx[2];
+    if (k == 0) { cidx[0] = 1; cidx[1] = 2; }
+    else if (k == nz - 1) { cidx[0] = nz - 3; cidx[1] = nz - 2; }
+    else { cidx[0] = k - 1; cidx[1] = k + 1; }
+    double dx[2], w2[2], A[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 2; ++i) {
+      dx[i] = zeta[cidx[i]] - zeta[k];
+      double w = 1.0 / std::pow(std::fabs(dx[i]), 1.5);
+      w2[i] = w * w;
+      A[0] += w2[i] * dx[i] * dx[i];
+      A[1] += w2[i] * dx[i] * 0.5 * dx[i] * dx[i];
+      A[2] += w2[i] * 0.5 * dx[i] * dx[i] * dx[i];
+      A[3] += w2[i] * 0.5 * dx[i] * dx[i] * 0.5 * dx[i] * dx[i];
+    }
+    double det = A[0] * A[3] - A[1] * A[2];
+    double M[4] = {A[3] / det, -A[1] / det, -A[2] / det, A[0] / det};
+    double fx[2], fxx[2];
+    for (int i = 0; i < 2; ++i) {
+      fx[i] = w2[i] * (M[0] * dx[i] + M[1] * 0.5 * dx[i] * dx[i]);
+      fxx[i] = w2[i] * (M[2] * dx[i] + M[3] * 0.5 * dx[i] * dx[i]);
+    }
+    tri[1 * kHeatMaxNz + k] = -(fx[0] + fx[1]);
+    tri[4 * kHeatMaxNz + k] = -(fxx[0] + fxx[1]);
+    for (int i = 0; i < 2; ++i) {
+      if (cidx[i] == k - 1) { tri[0 * kHeatMaxNz + k - 1] = fx[i]; tri[3 * kHeatMaxNz + k - 1] = fxx[i]; }
+      if (cidx[i] == k + 1) { tri[2 * kHeatMaxNz + k] = fx[i]; tri[5 * kHeatMaxNz + k] = fxx[i]; }
+    }
+  }
+}
+
+}  // namespace ufe
+
+using namespace ufe;
+
+namespace {
+int heat_fail(int code, const std::string &m) {
+  set_global_error(m);
+  return code;
+}
+}  // namespace
+
+// Defined in ufe_ctx.hip
+struct ufe_ctx;
+namespace ufe {
+int ctx_device(const ufe_ctx *c);
+hipStream_t ctx_stream(const ufe_ctx *c);
+const DevMesh *ctx_mesh(const ufe_ctx *c);
+Krylov *ctx_krylov(const ufe_ctx *c);
+int ctx_fail(ufe_ctx *c, int code, const std::string &msg);
+}  // namespace ufe
+
+extern "C" int ufe_fraction_gr_b(ufe_ctx *c, const double *Hi, const double *Hb, const double *SL, double *fgr) {
+  if (!c || !Hi || !Hb || !SL || !fgr) return heat_fail(UFE_ERR_ARGUMENT, "null argument");
+  try {
+    UFE_HIP(hipSetDevice(ctx_device(c)));
+    const DevMesh &m = *ctx_mesh(c);
+    hipStream_t s = ctx_stream(c);
+    DBuf dHi(sizeof(double) * m.nV), dHb(sizeof(double) * m.nV), dSL(sizeof(double) * m.nV),
+        dF(sizeof(double) * m.nT);
+    UFE_HIP(hipMemcpyAsync(dHi.p, Hi, dHi.bytes, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dHb.p, Hb, dHb.bytes, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dSL.p, SL, dSL.bytes, hipMemcpyHostToDevice, s));
+    launch_fraction_gr_b(m, dHi.as<double>(), dHb.as<double>(), dSL.as<double>(), dF.as<double>(), s);
+    UFE_HIP(hipMemcpyAsync(fgr, dF.p, dF.bytes, hipMemcpyDeviceToHost, s));
+    UFE_HIP(hipStreamSynchronize(s));
+    return UFE_OK;
+  } catch (const std::exception &e) {
+    return ctx_fail(c, UFE_ERR_DEVICE, e.what());
+  }
+}
+
+extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *is_unstable,
+                                int *n_unstable) {
+  if (!in || !Tnew || !is_unstable || !n_unstable) return heat_fail(UFE_ERR_ARGUMENT, "null argument");
+  if (in->nz < 3 || in->nz > kHeatMaxNz || in->nV < 0) return heat_fail(UFE_ERR_ARGUMENT, "unsupported nz");
+  const double *fields[] = {in->zeta, in->Ti, in->u_3D, in->v_3D, in->w_3D, in->u_dTdxp_upwind, in->v_dTdyp_upwind,
+                            in->dzeta_dx, in->dzeta_dy, in->dzeta_dz, in->dzeta_dt, in->Phi, in->Hi_eff,
+                            in->T_surf, in->Q_base_grnd};
+  for (const double *f : fields)
+    if (!f) return heat_fail(UFE_ERR_ARGUMENT, "null input field");
+  if (!in->mask) return heat_fail(UFE_ERR_ARGUMENT, "null mask");
+  int nV = in->nV, nz = in->nz;
+  for (int vi = 0; vi < nV; ++vi)
+    if (!(in->Hi_eff[vi] < in->Hi_min_thermo) && in->mask[vi] != 1 && in->mask[vi] != 2)
+      return heat_fail(UFE_ERR_ARGUMENT,
+                       "Hi_eff > Hi_min_thermo, but mask_grounded_ice and mask_floating_ice are both false");
+  try {
+    int dev = c ? ctx_device(c) : 0;
+    UFE_HIP(hipSetDevice(dev));
+    hipStream_t s;
+    UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    size_t n3 = sizeof(double) * (size_t)nV * nz, n1 = sizeof(double) * (size_t)nV;
+    std::vector<std::unique_ptr<DBuf>> keep;
+    auto up = [&](const void *h, size_t bytes) {
+      keep.push_back(std::make_unique<DBuf>(std::max<size_t>(bytes, 8)));
+      if (bytes) UFE_HIP(hipMemcpyAsync(keep.back()->p, h, bytes, hipMemcpyHostToDevice, s));
+      return keep.back()->p;
+    };
+    double tri[6 * kHeatMaxNz];
+    zeta_tridiagonals(nz, in->zeta, tri);
+    HeatDev h;
+    h.nV = nV; h.nz = nz;
+    h.zeta = (const double *)up(in->zeta, sizeof(double) * nz);
+    h.tri = (const double *)up(tri, sizeof(tri));
+    h.Ti = (const double *)up(in->Ti, n3);
+    h.u = (const double *)up(in->u_3D, n3);
+    h.v = (const double *)up(in->v_3D, n3);
+    h.w = (const double *)up(in->w_3D, n3);
+    h.uadv = (const double *)up(in->u_dTdxp_upwind, n3);
+    h.vadv = (const double *)up(in->v_dTdyp_upwind, n3);
+    h.zx = (const double *)up(in->dzeta_dx, n3);
+    h.zy = (const double *)up(in->dzeta_dy, n3);
+    h.zz = (const double *)up(in->dzeta_dz, n3);
+    h.zt = (const double *)up(in->dzeta_dt, n3);
+    h.Phi = (const double *)up(in->Phi, n3);
+    h.Hi_eff = (const double *)up(in->Hi_eff, n1);
+    h.T_surf = (const double *)up(in->T_surf, n1);
+    h.Q = (const double *)up(in->Q_base_grnd, n1);
+    h.mask = (const int *)up(in->mask, sizeof(int) * nV);
+    h.dt = in->dt;
+    h.Hi_min = in->Hi_min_thermo;
+    DBuf Tout(std::max<size_t>(n3, 8)), unst(sizeof(int) * std::max(nV, 1)), cnt(sizeof(double) * 2);
+    DBuf part(sizeof(double) * kMaxPartials), tick(sizeof(unsigned int));
+    UFE_HIP(hipMemsetAsync(tick.p, 0, sizeof(unsigned int), s));
+    RedSlot slot{part.as<double>(), tick.as<unsigned int>(), cnt.as<double>()};
+    if (nV > 0) launch_heat_columns(h, Tout.as<double>(), unst.as<int>(), slot, cnt.as<double>(), s);
+    double hc = 0.0;
+    if (nV > 0) {
+      UFE_HIP(hipMemcpyAsync(Tnew, Tout.p, n3, hipMemcpyDeviceToHost, s));
+      UFE_HIP(hipMemcpyAsync(is_unstable, unst.p, sizeof(int) * nV, hipMemcpyDeviceToHost, s));
+      UFE_HIP(hipMemcpyAsync(&hc, cnt.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    UFE_HIP(hipStreamSynchronize(s));
+    UFE_HIP(hipStreamDestroy(s));
+    *n_unstable = (int)hc;
+    return UFE_OK;
+  } catch (const std::exception &e) {
+    return heat_fail(UFE_ERR_DEVICE, e.what());
+  }
+}

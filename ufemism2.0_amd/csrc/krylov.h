@@ -1,0 +1,148 @@
+// Krylov solvers and preconditioners on the device.
+//
+// The system matrix is BSR with 2x2 blocks (one block row per triangle,
+// unknowns 2*ti+uv).  BiCGStab runs entirely on the device: its scalars live
+// in device memory, every kernel checks a device "done" flag and returns
+// early once the solve has converged, and the host only polls the flag every
+// `check_every` iterations.
+//
+// Preconditioners
+//   PC_NONE / PC_JACOBI / PC_BLOCK_JACOBI : fused into the BiCGStab vector
+//       updates (block-local, no extra pass over memory)
+//   multilevel aggregation (Amg) : fixed hierarchy per mesh built on the host
+//       (greedy aggregation of the triangle adjacency, repeated on the
+//       aggregate graph); per Picard iteration only the Galerkin values are
+//       recomputed on the device, as atomic-free gather sums of fine blocks
+//       into coarse blocks.  One V-cycle: damped block-Jacobi pre/post
+//       smoothing, piecewise-constant prolongation, damped block-Jacobi
+//       sweeps on the coarsest level in a single workgroup.  It is a fixed
+//       linear operator, so plain BiCGStab applies.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "kernels.h"
+
+namespace ufe {
+
+template <typename T> struct VecT;
+template <> struct VecT<double> { using V2 = double2; using V4 = double4; };
+template <> struct VecT<float> { using V2 = float2; using V4 = float4; };
+
+template <typename T>
+struct BsrT {
+  int nb = 0, nnzb = 0;
+  const int *ptr = nullptr, *col = nullptr;
+  const typename VecT<T>::V4 *val = nullptr;
+  const typename VecT<T>::V4 *dinv = nullptr;  // 2x2 inverse of each diagonal block
+};
+
+// Device buffer with RAII (hipMalloc/hipFree).
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  explicit DBuf(size_t b) { alloc(b); }
+  void alloc(size_t b);
+  void free();
+  ~DBuf() { free(); }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  template <typename U> U *as() const { return static_cast<U *>(p); }
+};
+
+// Aggregation hierarchy (pattern only; values live in AmgValues<T>).
+struct AmgLevelPattern {
+  int nb = 0, nnzb = 0;             // this level's matrix (level 0 = the system)
+  // levels >= 1: BSR pattern, row of each entry, and the Galerkin gather
+  // (entry c sums the previous level's entries glist[gptr[c] .. gptr[c+1]))
+  DBuf ptr, col, crow, gptr, glist;
+  // levels < last: aggregate of each block row and members of each aggregate
+  DBuf agg, mem_ptr, mem_ind;
+};
+
+struct AmgHierarchy {
+  std::vector<std::unique_ptr<AmgLevelPattern>> levels;  // levels[0] has nb/nnzb of the system
+  int n_levels() const { return (int)levels.size(); }
+};
+
+// Build the hierarchy on the host from the block pattern of the system and
+// the triangle adjacency (TriC).  coarse_target: stop when a level has at
+// most this many block rows.
+std::unique_ptr<AmgHierarchy> build_amg_hierarchy(int nT, const int *TriC, const int *ptr0, const int *col0,
+                                                  int coarse_target, int max_levels);
+
+template <typename T>
+struct AmgValues {
+  std::vector<std::unique_ptr<DBuf>> val, dinv;  // levels >= 1 (level 0 values are the system's)
+  std::vector<std::unique_ptr<DBuf>> r, z, zt;   // per-level vectors (levels >= 1 for r, all for z/zt)
+};
+
+struct SolveStats {
+  int its = 0;
+  int status = 0;       // 1 converged, 2 breakdown, 3 maxit
+  double rel = 0.0;     // ||r|| / ||r0|| at exit (recurrence)
+  double r0 = 0.0;
+};
+
+class Krylov {
+ public:
+  Krylov(int nb, hipStream_t s);
+  ~Krylov();
+
+  // Preconditioner choice for subsequent solves.
+  void set_precond(int pc) { pc_ = pc; }
+  void set_diag(const int *diag) { A_diag_ = diag; }
+  int precond() const { return pc_; }
+  void attach_amg(const AmgHierarchy *h) { amg_ = h; }
+  // Recompute Galerkin values on all levels from the level-0 matrix.
+  void amg_setup64(const BsrT<double> &A0);
+  void amg_setup32(const BsrT<float> &A0);
+  void set_amg_options(double omega, int coarse_sweeps, double coarse_scale) {
+    amg_omega_ = omega; amg_coarse_sweeps_ = coarse_sweeps; amg_coarse_scale_ = coarse_scale;
+  }
+
+  // BiCGStab: x in (initial guess unless zero_guess) / out.  Stops when
+  // ||r|| <= max(rtol*||r0||, abstol).
+  SolveStats bicgstab64(const BsrT<double> &A, const double *b, double *x, double rtol, double abstol, int maxit,
+                        bool zero_guess, int check_every);
+  SolveStats bicgstab32(const BsrT<float> &A, const float *b, float *x, double rtol, double abstol, int maxit,
+                        bool zero_guess, int check_every);
+
+  // FP64 residual r = b - A x with ||r||^2 and ||b||^2 (blocking read).
+  void residual64(const BsrT<double> &A, const double *x, const double *b, double *r, double *rr, double *bb);
+
+  hipStream_t stream() const { return s_; }
+  RedSlot slot(int i) const;
+
+  // scratch vectors usable by callers between solves
+  double *tmp64(int i);
+  float *tmp32(int i);
+
+ private:
+  template <typename T> SolveStats bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, double abstol,
+                                            int maxit, bool zero_guess, int check_every);
+  template <typename T> void precond_apply(const BsrT<T> &A, const T *r, T *z);
+  template <typename T> void amg_setup(const BsrT<T> &A0);
+  template <typename T> AmgValues<T> &amg_vals();
+
+  int nb_;
+  hipStream_t s_;
+  int pc_ = UFE_PC_BLOCK_JACOBI;
+  const AmgHierarchy *amg_ = nullptr;
+  const int *A_diag_ = nullptr;
+  double amg_omega_ = 0.7;
+  int amg_coarse_sweeps_ = 20;
+  double amg_coarse_scale_ = 1.0;
+  AmgValues<double> amg64_;
+  AmgValues<float> amg32_;
+  std::vector<BsrT<double>> lev64_;
+  std::vector<BsrT<float>> lev32_;
+  DBuf vec64_[10], vec32_[10];
+  DBuf scal_;            // KrylovScalars
+  DBuf red_part_, red_tick_, red_res_;
+  KrylovScalars *h_scal_ = nullptr;  // pinned host mirror
+};
+
+}  // namespace ufe

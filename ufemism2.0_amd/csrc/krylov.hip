@@ -1,0 +1,842 @@
+// Device Krylov solver (BiCGStab), BSR 2x2 SpMV and the aggregation
+// multigrid preconditioner.  See krylov.h for the design.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "krylov.h"
+
+namespace ufe {
+
+void DBuf::alloc(size_t b) {
+  free();
+  if (b == 0) return;
+  UFE_HIP(hipMalloc(&p, b));
+  bytes = b;
+}
+
+void DBuf::free() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ===========================================================================
+// BSR SpMV with fused epilogues.  LPR lanes cooperate on one block row: lane
+// k of a row reads blocks ptr[row]+k, +LPR, ... so consecutive lanes read
+// consecutive blocks (coalesced), then the partial sums are combined with
+// cross-lane shuffles.  Reductions are accumulated in FP64.
+// ===========================================================================
+enum SpmvOp { OP_PLAIN = 0, OP_RESID = 1, OP_DOT1 = 2, OP_DOT2 = 3 };
+
+constexpr int kLpr = 4;
+
+template <typename T, int OP>
+__global__ void __launch_bounds__(kBlock) k_bsr_spmv(
+    int nb, const int *__restrict__ ptr, const int *__restrict__ col,
+    const typename VecT<T>::V4 *__restrict__ val, const typename VecT<T>::V2 *__restrict__ x,
+    typename VecT<T>::V2 *__restrict__ y, const typename VecT<T>::V2 *__restrict__ b,
+    const typename VecT<T>::V2 *__restrict__ w, RedSlot slot, double *__restrict__ out,
+    KrylovScalars *__restrict__ sc) {
+  using V2 = typename VecT<T>::V2;
+  using V4 = typename VecT<T>::V4;
+  if (sc && sc->done) return;
+  const int lane = threadIdx.x % kLpr;
+  const int groups = (gridDim.x * blockDim.x) / kLpr;
+  double acc[2] = {0.0, 0.0};
+  for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr; row < nb; row += groups) {
+    T y0 = 0, y1 = 0;
+    if (x) {
+      const int k1 = ptr[row + 1];
+      for (int k = ptr[row] + lane; k < k1; k += kLpr) {
+        V4 a = val[k];
+        V2 xv = x[col[k]];
+        y0 += a.x * xv.x + a.y * xv.y;
+        y1 += a.z * xv.x + a.w * xv.y;
+      }
+#pragma unroll
+      for (int off = kLpr / 2; off > 0; off >>= 1) {
+        y0 += __shfl_down(y0, off, kLpr);
+        y1 += __shfl_down(y1, off, kLpr);
+      }
+    }
+    if (lane == 0) {
+      if (OP == OP_RESID) {
+        V2 bb = b[row];
+        T r0 = bb.x - y0, r1 = bb.y - y1;
+        V2 r;
+        r.x = r0; r.y = r1;
+        y[row] = r;
+        acc[0] += (double)r0 * r0 + (double)r1 * r1;
+        acc[1] += (double)bb.x * bb.x + (double)bb.y * bb.y;
+      } else {
+        V2 r;
+        r.x = y0; r.y = y1;
+        y[row] = r;
+        if (OP == OP_DOT1) {
+          V2 ww = w[row];
+          acc[0] += (double)ww.x * y0 + (double)ww.y * y1;
+        } else if (OP == OP_DOT2) {
+          V2 ss = w[row];
+          acc[0] += (double)y0 * ss.x + (double)y1 * ss.y;
+          acc[1] += (double)y0 * y0 + (double)y1 * y1;
+        }
+      }
+    }
+  }
+  if (OP == OP_PLAIN) return;
+  __shared__ double lds[2][kBlock / 64];
+  block_sum<2>(acc, lds);
+  double tot[2];
+  if (grid_finish<2>(slot, acc, tot)) {
+    if (OP == OP_RESID) {
+      out[0] = tot[0];
+      out[1] = tot[1];
+    } else if (OP == OP_DOT1) {
+      // alpha = rho / (rhat, v)
+      if (tot[0] == 0.0 || !std::isfinite(tot[0])) sc->done = 2;
+      else sc->alpha = sc->rho / tot[0];
+    } else {
+      // omega = (t, s) / (t, t)
+      sc->omega = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+    }
+  }
+}
+
+template <typename T, int OP>
+static void spmv_launch(const BsrT<T> &A, const T *x, T *y, const T *b, const T *w, const RedSlot &slot,
+                        double *out, KrylovScalars *sc, hipStream_t s) {
+  using V2 = typename VecT<T>::V2;
+  long threads = (long)A.nb * kLpr;
+  int nblk = blocks_for(threads);
+  if (OP != OP_PLAIN && nblk > kMaxPartials) nblk = kMaxPartials;
+  hipLaunchKernelGGL((k_bsr_spmv<T, OP>), dim3(nblk), dim3(kBlock), 0, s, A.nb, A.ptr, A.col, A.val,
+                     reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y), reinterpret_cast<const V2 *>(b),
+                     reinterpret_cast<const V2 *>(w), slot, out, sc);
+  UFE_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// Block-local preconditioners, fused into vector updates
+// ===========================================================================
+template <typename T>
+__device__ __forceinline__ typename VecT<T>::V2 pc_local(int pc, const typename VecT<T>::V4 *__restrict__ dinv,
+                                                         const typename VecT<T>::V4 *__restrict__ aval,
+                                                         const int *__restrict__ diag, int i,
+                                                         typename VecT<T>::V2 q) {
+  typename VecT<T>::V2 z;
+  if (pc == UFE_PC_BLOCK_JACOBI) {
+    typename VecT<T>::V4 d = dinv[i];
+    z.x = d.x * q.x + d.y * q.y;
+    z.y = d.z * q.x + d.w * q.y;
+  } else if (pc == UFE_PC_JACOBI) {
+    typename VecT<T>::V4 a = aval[diag[i]];
+    z.x = q.x / a.x;
+    z.y = q.y / a.w;
+  } else {
+    z = q;
+  }
+  return z;
+}
+
+// p = r + beta (p - omega v); phat = M p   (beta from the device scalars)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bicg_p(int nb, const typename VecT<T>::V2 *__restrict__ r,
+                                                   typename VecT<T>::V2 *__restrict__ p,
+                                                   const typename VecT<T>::V2 *__restrict__ v,
+                                                   typename VecT<T>::V2 *__restrict__ phat, int pc,
+                                                   const typename VecT<T>::V4 *__restrict__ dinv,
+                                                   const typename VecT<T>::V4 *__restrict__ aval,
+                                                   const int *__restrict__ diag, const KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  T beta = (T)((sc->rho / sc->rho_old) * (sc->alpha / sc->omega));
+  T om = (T)sc->omega;
+  typename VecT<T>::V2 rr = r[i], pp = p[i], vv = v[i];
+  pp.x = rr.x + beta * (pp.x - om * vv.x);
+  pp.y = rr.y + beta * (pp.y - om * vv.y);
+  p[i] = pp;
+  if (phat) phat[i] = pc_local<T>(pc, dinv, aval, diag, i, pp);
+}
+
+// s = r - alpha v; shat = M s
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bicg_s(int nb, const typename VecT<T>::V2 *__restrict__ r,
+                                                   const typename VecT<T>::V2 *__restrict__ v,
+                                                   typename VecT<T>::V2 *__restrict__ s,
+                                                   typename VecT<T>::V2 *__restrict__ shat, int pc,
+                                                   const typename VecT<T>::V4 *__restrict__ dinv,
+                                                   const typename VecT<T>::V4 *__restrict__ aval,
+                                                   const int *__restrict__ diag, const KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  T al = (T)sc->alpha;
+  typename VecT<T>::V2 rr = r[i], vv = v[i], ss;
+  ss.x = rr.x - al * vv.x;
+  ss.y = rr.y - al * vv.y;
+  s[i] = ss;
+  if (shat) shat[i] = pc_local<T>(pc, dinv, aval, diag, i, ss);
+}
+
+// x += alpha phat + omega shat; r = s - omega t; rho_new = (rhat, r); ||r||^2
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bicg_x(int nb, typename VecT<T>::V2 *__restrict__ x,
+                                                   const typename VecT<T>::V2 *__restrict__ phat,
+                                                   const typename VecT<T>::V2 *__restrict__ shat,
+                                                   const typename VecT<T>::V2 *__restrict__ s,
+                                                   const typename VecT<T>::V2 *__restrict__ t,
+                                                   typename VecT<T>::V2 *__restrict__ r,
+                                                   const typename VecT<T>::V2 *__restrict__ rhat, RedSlot slot,
+                                                   KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  T al = (T)sc->alpha, om = (T)sc->omega;
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
+    typename VecT<T>::V2 xx = x[i], ph = phat[i], sh = shat[i], ss = s[i], tt = t[i], rh = rhat[i], rr;
+    xx.x += al * ph.x + om * sh.x;
+    xx.y += al * ph.y + om * sh.y;
+    x[i] = xx;
+    rr.x = ss.x - om * tt.x;
+    rr.y = ss.y - om * tt.y;
+    r[i] = rr;
+    acc[0] += (double)rh.x * rr.x + (double)rh.y * rr.y;
+    acc[1] += (double)rr.x * rr.x + (double)rr.y * rr.y;
+  }
+  __shared__ double lds[2][kBlock / 64];
+  block_sum<2>(acc, lds);
+  double tot[2];
+  if (grid_finish<2>(slot, acc, tot)) {
+    sc->its += 1;
+    sc->rho_old = sc->rho;
+    sc->rho = tot[0];
+    sc->rr = tot[1];
+    if (!(tot[1] <= sc->tol2) && !std::isfinite(tot[1])) sc->done = 2;
+    else if (tot[1] <= sc->tol2) sc->done = 1;
+    else if (sc->omega == 0.0 || tot[0] == 0.0) sc->done = 2;
+    else if (sc->its >= sc->maxit) sc->done = 3;
+  }
+}
+
+// rhat = r, p = v = 0, scalars from ||r0||^2 (rr_in)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bicg_init(int nb, const typename VecT<T>::V2 *__restrict__ r,
+                                                      typename VecT<T>::V2 *__restrict__ rhat,
+                                                      typename VecT<T>::V2 *__restrict__ p,
+                                                      typename VecT<T>::V2 *__restrict__ v,
+                                                      const double *__restrict__ rr_in, double rtol, double abstol,
+                                                      int maxit, KrylovScalars *__restrict__ sc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) {
+    rhat[i] = r[i];
+    typename VecT<T>::V2 z;
+    z.x = 0; z.y = 0;
+    p[i] = z;
+    v[i] = z;
+  }
+  if (i == 0) {
+    double rr = rr_in[0];
+    sc->rho = rr;
+    sc->rho_old = 1.0;
+    sc->alpha = 1.0;
+    sc->omega = 1.0;
+    sc->rr = rr;
+    sc->bb = rr_in[1];
+    double tol = fmax(rtol * sqrt(rr), abstol);
+    sc->tol2 = tol * tol;
+    sc->its = 0;
+    sc->maxit = maxit;
+    sc->done = (rr <= sc->tol2) ? 1 : 0;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_zero2(int nb, typename VecT<T>::V2 *__restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) {
+    typename VecT<T>::V2 z;
+    z.x = 0; z.y = 0;
+    x[i] = z;
+  }
+}
+
+// ===========================================================================
+// Aggregation multigrid kernels
+// ===========================================================================
+
+// Galerkin values of level l+1: entry c = sum of level-l blocks glist[gptr[c]..].
+// The thread owning a diagonal entry writes that block's 2x2 inverse.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_galerkin(int nnzc, const int *__restrict__ gptr,
+                                                     const int *__restrict__ glist,
+                                                     const typename VecT<T>::V4 *__restrict__ fine,
+                                                     const int *__restrict__ crow, const int *__restrict__ ccol,
+                                                     typename VecT<T>::V4 *__restrict__ coarse,
+                                                     typename VecT<T>::V4 *__restrict__ cdinv) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nnzc) return;
+  double a = 0, b = 0, cc = 0, d = 0;
+  for (int k = gptr[c]; k < gptr[c + 1]; ++k) {
+    typename VecT<T>::V4 f = fine[glist[k]];
+    a += f.x; b += f.y; cc += f.z; d += f.w;
+  }
+  typename VecT<T>::V4 o;
+  o.x = (T)a; o.y = (T)b; o.z = (T)cc; o.w = (T)d;
+  coarse[c] = o;
+  int r = crow[c];
+  if (ccol[c] == r) {
+    double det = a * d - b * cc, id = 1.0 / det;
+    typename VecT<T>::V4 inv;
+    inv.x = (T)(d * id); inv.y = (T)(-b * id); inv.z = (T)(-cc * id); inv.w = (T)(a * id);
+    cdinv[r] = inv;
+  }
+}
+
+// z0 = omega D0^-1 r0  (pre-smoothing from a zero guess on the finest level)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_amg_presmooth(int nb, const typename VecT<T>::V2 *__restrict__ r,
+                                                          const typename VecT<T>::V4 *__restrict__ dinv, T omega,
+                                                          typename VecT<T>::V2 *__restrict__ z,
+                                                          const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  typename VecT<T>::V2 q = r[i], o;
+  typename VecT<T>::V4 d = dinv[i];
+  o.x = omega * (d.x * q.x + d.y * q.y);
+  o.y = omega * (d.z * q.x + d.w * q.y);
+  z[i] = o;
+}
+
+// For each next-level aggregate I: rc[I] = sum over members i of (r - A z)_i,
+// and zc[I] = omega Dc^-1 rc[I] (the next level's pre-smoothing from zero).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_amg_restrict(
+    int nc, const int *__restrict__ mem_ptr, const int *__restrict__ mem_ind, const int *__restrict__ ptr,
+    const int *__restrict__ col, const typename VecT<T>::V4 *__restrict__ val,
+    const typename VecT<T>::V2 *__restrict__ r, const typename VecT<T>::V2 *__restrict__ z,
+    typename VecT<T>::V2 *__restrict__ rc, const typename VecT<T>::V4 *__restrict__ cdinv, T omega,
+    typename VecT<T>::V2 *__restrict__ zc, const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  double s0 = 0, s1 = 0;
+  for (int m = mem_ptr[I]; m < mem_ptr[I + 1]; ++m) {
+    int i = mem_ind[m];
+    typename VecT<T>::V2 ri = r[i];
+    double q0 = ri.x, q1 = ri.y;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+      typename VecT<T>::V4 a = val[k];
+      typename VecT<T>::V2 zj = z[col[k]];
+      q0 -= (double)a.x * zj.x + (double)a.y * zj.y;
+      q1 -= (double)a.z * zj.x + (double)a.w * zj.y;
+    }
+    s0 += q0;
+    s1 += q1;
+  }
+  typename VecT<T>::V2 o;
+  o.x = (T)s0; o.y = (T)s1;
+  rc[I] = o;
+  if (zc) {
+    typename VecT<T>::V4 d = cdinv[I];
+    typename VecT<T>::V2 zz;
+    zz.x = omega * (T)(d.x * s0 + d.y * s1);
+    zz.y = omega * (T)(d.z * s0 + d.w * s1);
+    zc[I] = zz;
+  }
+}
+
+// zout_i = zn_i + omega D^-1 (r - A zn)_i with zn = z + scale * P ec (prolongation
+// fused into the post-smoothing sweep).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
+    int nb, const int *__restrict__ ptr, const int *__restrict__ col, const typename VecT<T>::V4 *__restrict__ val,
+    const typename VecT<T>::V4 *__restrict__ dinv, const int *__restrict__ agg,
+    const typename VecT<T>::V2 *__restrict__ r, const typename VecT<T>::V2 *__restrict__ z,
+    const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega, typename VecT<T>::V2 *__restrict__ zout,
+    const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  typename VecT<T>::V2 ri = r[i];
+  T q0 = ri.x, q1 = ri.y;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    int j = col[k];
+    typename VecT<T>::V4 a = val[k];
+    typename VecT<T>::V2 zj = z[j], ej = ec[agg[j]];
+    T z0 = zj.x + scale * ej.x, z1 = zj.y + scale * ej.y;
+    q0 -= a.x * z0 + a.y * z1;
+    q1 -= a.z * z0 + a.w * z1;
+  }
+  typename VecT<T>::V2 zi = z[i], ei = ec[agg[i]];
+  typename VecT<T>::V4 d = dinv[i];
+  typename VecT<T>::V2 o;
+  o.x = zi.x + scale * ei.x + omega * (d.x * q0 + d.y * q1);
+  o.y = zi.y + scale * ei.y + omega * (d.z * q0 + d.w * q1);
+  zout[i] = o;
+}
+
+// Coarsest level: `sweeps` damped block-Jacobi sweeps from zero, in one
+// workgroup with the iterate in LDS.  nb <= kCoarseMax.
+constexpr int kCoarseMax = 1024;
+constexpr int kCoarseThreads = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads) k_amg_coarse(int nb, const int *__restrict__ ptr,
+                                                              const int *__restrict__ col,
+                                                              const typename VecT<T>::V4 *__restrict__ val,
+                                                              const typename VecT<T>::V4 *__restrict__ dinv,
+                                                              const typename VecT<T>::V2 *__restrict__ r, T omega,
+                                                              int sweeps, typename VecT<T>::V2 *__restrict__ z,
+                                                              const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  __shared__ T za[2][2 * kCoarseMax];
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    za[0][2 * i] = 0;
+    za[0][2 * i + 1] = 0;
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int it = 0; it < sweeps; ++it) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      typename VecT<T>::V2 ri = r[i];
+      T q0 = ri.x, q1 = ri.y;
+      for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+        int j = col[k];
+        typename VecT<T>::V4 a = val[k];
+        T z0 = za[cur][2 * j], z1 = za[cur][2 * j + 1];
+        q0 -= a.x * z0 + a.y * z1;
+        q1 -= a.z * z0 + a.w * z1;
+      }
+      typename VecT<T>::V4 d = dinv[i];
+      za[1 - cur][2 * i] = za[cur][2 * i] + omega * (d.x * q0 + d.y * q1);
+      za[1 - cur][2 * i + 1] = za[cur][2 * i + 1] + omega * (d.z * q0 + d.w * q1);
+    }
+    __syncthreads();
+    cur = 1 - cur;
+  }
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    typename VecT<T>::V2 o;
+    o.x = za[cur][2 * i];
+    o.y = za[cur][2 * i + 1];
+    z[i] = o;
+  }
+}
+
+// FP32 <-> FP64 conversions and FP64 axpy used by the refinement loop
+__global__ void __launch_bounds__(kBlock) k_f64_to_f32(int n, const double *__restrict__ a, float *__restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+static std::vector<int> greedy_aggregate(int n, const std::vector<int> &aptr, const std::vector<int> &aind,
+                                         int *n_agg) {
+  std::vector<int> agg(n, -1);
+  int na = 0;
+  // pass 1: a root whose neighbours are all free takes them
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    bool free_nb = true;
+    for (int k = aptr[i]; k < aptr[i + 1]; ++k)
+      if (agg[aind[k]] >= 0) { free_nb = false; break; }
+    if (!free_nb) continue;
+    agg[i] = na;
+    for (int k = aptr[i]; k < aptr[i + 1]; ++k) agg[aind[k]] = na;
+    ++na;
+  }
+  // pass 2: leftovers join a neighbouring aggregate (or form a singleton)
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    for (int k = aptr[i]; k < aptr[i + 1]; ++k)
+      if (agg[aind[k]] >= 0) { agg[i] = agg[aind[k]]; break; }
+    if (agg[i] < 0) agg[i] = na++;
+  }
+  *n_agg = na;
+  return agg;
+}
+
+template <typename U>
+static void upload(DBuf &d, const std::vector<U> &h) {
+  d.alloc(std::max<size_t>(h.size(), 1) * sizeof(U));
+  if (!h.empty()) UFE_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
+}
+
+std::unique_ptr<AmgHierarchy> build_amg_hierarchy(int nT, const int *TriC, const int *ptr0, const int *col0,
+                                                  int coarse_target, int max_levels) {
+  auto H = std::make_unique<AmgHierarchy>();
+  coarse_target = std::min(coarse_target, kCoarseMax);
+  // level-0 aggregation graph: triangle neighbours across edges
+  std::vector<int> aptr(nT + 1, 0), aind;
+  aind.reserve(3 * (size_t)nT);
+  for (int t = 0; t < nT; ++t) {
+    for (int q = 0; q < 3; ++q)
+      if (TriC[3 * t + q] >= 0) aind.push_back(TriC[3 * t + q]);
+    aptr[t + 1] = (int)aind.size();
+  }
+  std::vector<int> mptr(ptr0, ptr0 + nT + 1), mcol(col0, col0 + ptr0[nT]);
+  std::vector<int> gptr, glist;  // gather lists building the current level
+  int n = nT;
+  for (int lev = 0; lev < max_levels; ++lev) {
+    auto L = std::make_unique<AmgLevelPattern>();
+    L->nb = n;
+    L->nnzb = mptr[n];
+    if (lev > 0) {
+      std::vector<int> crow(mptr[n]);
+      for (int i = 0; i < n; ++i)
+        for (int k = mptr[i]; k < mptr[i + 1]; ++k) crow[k] = i;
+      upload(L->ptr, mptr);
+      upload(L->col, mcol);
+      upload(L->crow, crow);
+      upload(L->gptr, gptr);
+      upload(L->glist, glist);
+    }
+    bool last = (n <= coarse_target) || (lev == max_levels - 1);
+    int na = 0;
+    std::vector<int> agg;
+    if (!last) {
+      agg = greedy_aggregate(n, aptr, aind, &na);
+      if (na >= n || na < 1) last = true;
+    }
+    if (last) {
+      if (n > kCoarseMax) return nullptr;  // coarsest level must fit the single-workgroup solve
+      H->levels.push_back(std::move(L));
+      break;
+    }
+    // members of each aggregate
+    std::vector<int> cnt(na + 1, 0);
+    for (int i = 0; i < n; ++i) ++cnt[agg[i] + 1];
+    std::partial_sum(cnt.begin(), cnt.end(), cnt.begin());
+    std::vector<int> mem(n), fill(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n; ++i) mem[fill[agg[i]]++] = i;
+    upload(L->agg, agg);
+    upload(L->mem_ptr, cnt);
+    upload(L->mem_ind, mem);
+    // next-level pattern and gather lists
+    std::vector<std::vector<std::pair<int, int>>> rows(na);  // (coarse col, fine entry)
+    for (int i = 0; i < n; ++i)
+      for (int k = mptr[i]; k < mptr[i + 1]; ++k) rows[agg[i]].push_back({agg[mcol[k]], k});
+    std::vector<int> cptr(na + 1, 0), ccol;
+    gptr.assign(1, 0);
+    glist.clear();
+    for (int I = 0; I < na; ++I) {
+      auto &rw = rows[I];
+      std::sort(rw.begin(), rw.end());
+      for (size_t q = 0; q < rw.size();) {
+        size_t q2 = q;
+        while (q2 < rw.size() && rw[q2].first == rw[q].first) glist.push_back(rw[q2++].second);
+        ccol.push_back(rw[q].first);
+        gptr.push_back((int)glist.size());
+        q = q2;
+      }
+      cptr[I + 1] = (int)ccol.size();
+      std::vector<std::pair<int, int>>().swap(rw);
+    }
+    // next-level aggregation graph: aggregates adjacent if any members are
+    std::vector<std::vector<int>> nbrs(na);
+    for (int i = 0; i < n; ++i)
+      for (int k = aptr[i]; k < aptr[i + 1]; ++k) {
+        int a = agg[i], b = agg[aind[k]];
+        if (a != b) nbrs[a].push_back(b);
+      }
+    std::vector<int> nptr(na + 1, 0), nind;
+    for (int I = 0; I < na; ++I) {
+      auto &v = nbrs[I];
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      nind.insert(nind.end(), v.begin(), v.end());
+      nptr[I + 1] = (int)nind.size();
+    }
+    H->levels.push_back(std::move(L));
+    aptr.swap(nptr);
+    aind.swap(nind);
+    mptr.swap(cptr);
+    mcol.swap(ccol);
+    n = na;
+  }
+  return H;
+}
+
+Krylov::Krylov(int nb, hipStream_t s) : nb_(nb), s_(s) {
+  for (auto &v : vec64_) v.alloc(sizeof(double) * 2 * (size_t)nb);
+  for (auto &v : vec32_) v.alloc(sizeof(float) * 2 * (size_t)nb);
+  scal_.alloc(sizeof(KrylovScalars));
+  red_part_.alloc(sizeof(double) * kMaxPartials * 4 * 4);
+  red_tick_.alloc(sizeof(unsigned int) * 4);
+  red_res_.alloc(sizeof(double) * 4 * 4);
+  UFE_HIP(hipMemset(red_tick_.p, 0, sizeof(unsigned int) * 4));
+  UFE_HIP(hipMemset(scal_.p, 0, sizeof(KrylovScalars)));
+  UFE_HIP(hipHostMalloc((void **)&h_scal_, sizeof(KrylovScalars), hipHostMallocDefault));
+}
+
+Krylov::~Krylov() {
+  if (h_scal_) (void)hipHostFree(h_scal_);
+}
+
+RedSlot Krylov::slot(int i) const {
+  RedSlot r;
+  r.partials = red_part_.as<double>() + (size_t)i * kMaxPartials * 4;
+  r.ticket = red_tick_.as<unsigned int>() + i;
+  r.result = red_res_.as<double>() + 4 * i;
+  return r;
+}
+
+double *Krylov::tmp64(int i) { return vec64_[i].as<double>(); }
+float *Krylov::tmp32(int i) { return vec32_[i].as<float>(); }
+
+void Krylov::residual64(const BsrT<double> &A, const double *x, const double *b, double *r, double *rr,
+                        double *bb) {
+  RedSlot sl = slot(3);
+  spmv_launch<double, OP_RESID>(A, x, r, b, nullptr, sl, sl.result, nullptr, s_);
+  double h[2];
+  UFE_HIP(hipMemcpyAsync(h, sl.result, sizeof(h), hipMemcpyDeviceToHost, s_));
+  UFE_HIP(hipStreamSynchronize(s_));
+  *rr = h[0];
+  *bb = h[1];
+}
+
+template <> AmgValues<double> &Krylov::amg_vals<double>() { return amg64_; }
+template <> AmgValues<float> &Krylov::amg_vals<float>() { return amg32_; }
+
+template <typename T>
+void Krylov::amg_setup(const BsrT<T> &A0) {
+  using V2 = typename VecT<T>::V2;
+  using V4 = typename VecT<T>::V4;
+  if (!amg_) return;
+  AmgValues<T> &av = amg_vals<T>();
+  std::vector<BsrT<T>> &lev = std::is_same<T, double>::value ? *reinterpret_cast<std::vector<BsrT<T>> *>(&lev64_)
+                                                             : *reinterpret_cast<std::vector<BsrT<T>> *>(&lev32_);
+  int L = amg_->n_levels();
+  if ((int)av.val.size() != L) {
+    av.val.clear(); av.dinv.clear(); av.r.clear(); av.z.clear(); av.zt.clear();
+    for (int l = 0; l < L; ++l) {
+      const AmgLevelPattern &P = *amg_->levels[l];
+      av.val.push_back(std::make_unique<DBuf>(l > 0 ? sizeof(V4) * (size_t)P.nnzb : 0));
+      av.dinv.push_back(std::make_unique<DBuf>(l > 0 ? sizeof(V4) * (size_t)P.nb : 0));
+      av.r.push_back(std::make_unique<DBuf>(sizeof(V2) * (size_t)P.nb));
+      av.z.push_back(std::make_unique<DBuf>(sizeof(V2) * (size_t)P.nb));
+      av.zt.push_back(std::make_unique<DBuf>(sizeof(V2) * (size_t)P.nb));
+    }
+  }
+  lev.assign(L, BsrT<T>());
+  lev[0] = A0;
+  for (int l = 1; l < L; ++l) {
+    const AmgLevelPattern &P = *amg_->levels[l];
+    BsrT<T> B;
+    B.nb = P.nb;
+    B.nnzb = P.nnzb;
+    B.ptr = P.ptr.as<int>();
+    B.col = P.col.as<int>();
+    B.val = av.val[l]->template as<V4>();
+    B.dinv = av.dinv[l]->template as<V4>();
+    hipLaunchKernelGGL(k_galerkin<T>, dim3(blocks_for(P.nnzb)), dim3(kBlock), 0, s_, P.nnzb, P.gptr.as<int>(),
+                       P.glist.as<int>(), lev[l - 1].val, P.crow.as<int>(), P.col.as<int>(),
+                       av.val[l]->template as<V4>(), av.dinv[l]->template as<V4>());
+    UFE_LAUNCH_CHECK();
+    lev[l] = B;
+  }
+}
+
+void Krylov::amg_setup64(const BsrT<double> &A0) { amg_setup<double>(A0); }
+void Krylov::amg_setup32(const BsrT<float> &A0) { amg_setup<float>(A0); }
+
+template <typename T>
+void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
+  using V2 = typename VecT<T>::V2;
+  AmgValues<T> &av = amg_vals<T>();
+  std::vector<BsrT<T>> &lev = std::is_same<T, double>::value ? *reinterpret_cast<std::vector<BsrT<T>> *>(&lev64_)
+                                                             : *reinterpret_cast<std::vector<BsrT<T>> *>(&lev32_);
+  KrylovScalars *sc = scal_.as<KrylovScalars>();
+  int L = (int)lev.size();
+  T om = (T)amg_omega_, cs = (T)amg_coarse_scale_;
+  // level vectors: r_l (l>=1 in av.r), z_l in av.z; level 0 input r, output z
+  const V2 *rl = reinterpret_cast<const V2 *>(r);
+  V2 *z0 = av.z[0]->template as<V2>();
+  if (L == 1) {
+    hipLaunchKernelGGL(k_amg_coarse<T>, dim3(1), dim3(kCoarseThreads), 0, s_, lev[0].nb, lev[0].ptr, lev[0].col,
+                       lev[0].val, lev[0].dinv, rl, om, amg_coarse_sweeps_, reinterpret_cast<V2 *>(z), sc);
+    UFE_LAUNCH_CHECK();
+    return;
+  }
+  hipLaunchKernelGGL(k_amg_presmooth<T>, dim3(blocks_for(lev[0].nb)), dim3(kBlock), 0, s_, lev[0].nb, rl,
+                     lev[0].dinv, om, z0, sc);
+  UFE_LAUNCH_CHECK();
+  for (int l = 0; l < L - 1; ++l) {
+    const AmgLevelPattern &P = *amg_->levels[l];
+    const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
+    bool next_coarsest = (l + 1 == L - 1);
+    int nc = lev[l + 1].nb;
+    hipLaunchKernelGGL(k_amg_restrict<T>, dim3(blocks_for(nc)), dim3(kBlock), 0, s_, nc, P.mem_ptr.as<int>(),
+                       P.mem_ind.as<int>(), lev[l].ptr, lev[l].col, lev[l].val, rin, av.z[l]->template as<V2>(),
+                       av.r[l + 1]->template as<V2>(), lev[l + 1].dinv, om,
+                       next_coarsest ? (V2 *)nullptr : av.z[l + 1]->template as<V2>(), sc);
+    UFE_LAUNCH_CHECK();
+  }
+  {
+    int c = L - 1;
+    hipLaunchKernelGGL(k_amg_coarse<T>, dim3(1), dim3(kCoarseThreads), 0, s_, lev[c].nb, lev[c].ptr, lev[c].col,
+                       lev[c].val, lev[c].dinv, av.r[c]->template as<V2>(), om, amg_coarse_sweeps_,
+                       av.z[c]->template as<V2>(), sc);
+    UFE_LAUNCH_CHECK();
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    const AmgLevelPattern &P = *amg_->levels[l];
+    const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
+    V2 *zout = l == 0 ? reinterpret_cast<V2 *>(z) : av.zt[l]->template as<V2>();
+    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(blocks_for(lev[l].nb)), dim3(kBlock), 0, s_, lev[l].nb,
+                       lev[l].ptr, lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin,
+                       av.z[l]->template as<V2>(), av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
+    UFE_LAUNCH_CHECK();
+    if (l > 0) std::swap(av.z[l], av.zt[l]);
+  }
+}
+
+template <typename T>
+SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, double abstol, int maxit,
+                            bool zero_guess, int check_every) {
+  using V2 = typename VecT<T>::V2;
+  T *r, *rhat, *p, *v, *phat, *s, *shat, *t;
+  if (std::is_same<T, double>::value) {
+    r = (T *)tmp64(0); rhat = (T *)tmp64(1); p = (T *)tmp64(2); v = (T *)tmp64(3);
+    phat = (T *)tmp64(4); s = (T *)tmp64(5); shat = (T *)tmp64(6); t = (T *)tmp64(7);
+  } else {
+    r = (T *)tmp32(0); rhat = (T *)tmp32(1); p = (T *)tmp32(2); v = (T *)tmp32(3);
+    phat = (T *)tmp32(4); s = (T *)tmp32(5); shat = (T *)tmp32(6); t = (T *)tmp32(7);
+  }
+  KrylovScalars *sc = scal_.as<KrylovScalars>();
+  const bool amg = (pc_ == UFE_PC_AMG) && amg_;
+  const int pcl = amg ? UFE_PC_NONE : pc_;
+  const int nb = A.nb;
+  SolveStats st;
+  int total_its = 0;
+  double tol_abs = 0.0;
+  int nbx = std::min(blocks_for(nb), kMaxPartials);
+  for (int restart = 0; restart < 4; ++restart) {
+    RedSlot sl0 = slot(0), sl1 = slot(1), sl2 = slot(2);
+    if (zero_guess && restart == 0) {
+      hipLaunchKernelGGL(k_zero2<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x));
+      UFE_LAUNCH_CHECK();
+    }
+    spmv_launch<T, OP_RESID>(A, x, r, b, nullptr, sl0, sl0.result, nullptr, s_);
+    double h0[2];
+    UFE_HIP(hipMemcpyAsync(h0, sl0.result, sizeof(h0), hipMemcpyDeviceToHost, s_));
+    UFE_HIP(hipStreamSynchronize(s_));
+    double rn = std::sqrt(h0[0]);
+    if (restart == 0) {
+      st.r0 = rn;
+      tol_abs = std::max(rtol * rn, abstol);
+    }
+    st.rel = rn;
+    if (rn <= tol_abs || !std::isfinite(rn)) {
+      st.status = std::isfinite(rn) ? 1 : 2;
+      break;
+    }
+    hipLaunchKernelGGL(k_bicg_init<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(r),
+                       reinterpret_cast<V2 *>(rhat), reinterpret_cast<V2 *>(p), reinterpret_cast<V2 *>(v),
+                       sl0.result, 0.0, tol_abs, maxit - total_its, sc);
+    UFE_LAUNCH_CHECK();
+    bool finished = false;
+    while (!finished) {
+      for (int k = 0; k < check_every; ++k) {
+        hipLaunchKernelGGL(k_bicg_p<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
+                           reinterpret_cast<const V2 *>(r), reinterpret_cast<V2 *>(p), reinterpret_cast<const V2 *>(v),
+                           reinterpret_cast<V2 *>(amg ? (T *)nullptr : phat), pcl, A.dinv, A.val, A_diag_, sc);
+        UFE_LAUNCH_CHECK();
+        if (amg) precond_apply<T>(A, p, phat);
+        spmv_launch<T, OP_DOT1>(A, phat, v, nullptr, rhat, sl1, nullptr, sc, s_);
+        hipLaunchKernelGGL(k_bicg_s<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
+                           reinterpret_cast<const V2 *>(r), reinterpret_cast<const V2 *>(v), reinterpret_cast<V2 *>(s),
+                           reinterpret_cast<V2 *>(amg ? (T *)nullptr : shat), pcl, A.dinv, A.val, A_diag_, sc);
+        UFE_LAUNCH_CHECK();
+        if (amg) precond_apply<T>(A, s, shat);
+        spmv_launch<T, OP_DOT2>(A, shat, t, nullptr, s, sl2, nullptr, sc, s_);
+        hipLaunchKernelGGL(k_bicg_x<T>, dim3(nbx), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x),
+                           reinterpret_cast<const V2 *>(phat), reinterpret_cast<const V2 *>(shat),
+                           reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
+                           reinterpret_cast<const V2 *>(rhat), sl0, sc);
+        UFE_LAUNCH_CHECK();
+      }
+      UFE_HIP(hipMemcpyAsync(h_scal_, sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s_));
+      UFE_HIP(hipStreamSynchronize(s_));
+      if (h_scal_->done) finished = true;
+    }
+    total_its += h_scal_->its;
+    st.status = h_scal_->done;
+    // verify the true residual (recurrence drift, breakdown) and restart if needed
+    RedSlot sl3 = slot(3);
+    spmv_launch<T, OP_RESID>(A, x, t, b, nullptr, sl3, sl3.result, nullptr, s_);
+    double hr[2];
+    UFE_HIP(hipMemcpyAsync(hr, sl3.result, sizeof(hr), hipMemcpyDeviceToHost, s_));
+    UFE_HIP(hipStreamSynchronize(s_));
+    st.rel = std::sqrt(hr[0]);
+    if (st.rel <= tol_abs * 1.0001) { st.status = 1; break; }
+    if (total_its >= maxit) { st.status = 3; break; }
+    zero_guess = false;
+  }
+  st.rel = st.r0 > 0 ? st.rel / st.r0 : st.rel;
+  st.its = total_its;
+  return st;
+}
+
+SolveStats Krylov::bicgstab64(const BsrT<double> &A, const double *b, double *x, double rtol, double abstol,
+                              int maxit, bool zero_guess, int check_every) {
+  return bicgstab<double>(A, b, x, rtol, abstol, maxit, zero_guess, check_every);
+}
+
+SolveStats Krylov::bicgstab32(const BsrT<float> &A, const float *b, float *x, double rtol, double abstol,
+                              int maxit, bool zero_guess, int check_every) {
+  return bicgstab<float>(A, b, x, rtol, abstol, maxit, zero_guess, check_every);
+}
+
+// Generic wrappers declared in kernels.h
+void launch_bsr_spmv64(const Bsr64 &A, const double *x, double *y, hipStream_t s) {
+  BsrT<double> B;
+  B.nb = A.nb; B.nnzb = A.nnzb; B.ptr = A.ptr; B.col = A.col; B.val = A.val;
+  RedSlot dummy{};
+  spmv_launch<double, OP_PLAIN>(B, x, y, nullptr, nullptr, dummy, nullptr, nullptr, s);
+}
+
+void launch_bsr_spmv32(const Bsr32 &A, const float *x, float *y, hipStream_t s) {
+  BsrT<float> B;
+  B.nb = A.nb; B.nnzb = A.nnzb; B.ptr = A.ptr; B.col = A.col; B.val = A.val;
+  RedSlot dummy{};
+  spmv_launch<float, OP_PLAIN>(B, x, y, nullptr, nullptr, dummy, nullptr, nullptr, s);
+}
+
+void launch_bsr_residual64(const Bsr64 &A, const double *x, const double *b, double *r, const RedSlot *slot,
+                           double *out2, hipStream_t s) {
+  BsrT<double> B;
+  B.nb = A.nb; B.nnzb = A.nnzb; B.ptr = A.ptr; B.col = A.col; B.val = A.val;
+  spmv_launch<double, OP_RESID>(B, x, r, b, nullptr, *slot, out2, nullptr, s);
+}
+
+void launch_f64_to_f32(int n, const double *a, float *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, a, b);
+  UFE_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Scalar CSR product (generic boundary, MatMult)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_csr_spmv(int m, const int *__restrict__ ptr, const int *__restrict__ ind,
+                                                     const double *__restrict__ val, const double *__restrict__ x,
+                                                     double *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[ind[k]];
+  y[i] = s;
+}
+
+void launch_csr_spmv64(int m, const int *ptr, const int *ind, const double *val, const double *x, double *y,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_csr_spmv, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, ptr, ind, val, x, y);
+  UFE_LAUNCH_CHECK();
+}
+
+}  // namespace ufe

@@ -1,0 +1,784 @@
+// Context, Picard driver and C ABI of the engine.
+//
+// The Picard loop is host code that only launches kernels and reads a few
+// scalars per iteration (convergence norms); every array stays in HBM.  The
+// loop restates solve_SSA (SSA_main.f90:149-237) and solve_DIVA
+// (DIVA_main.f90:154-254): strain rates -> viscosity -> friction -> linear
+// solve -> velocity limits -> relaxation -> L2_uv, with the adaptive
+// relaxation / regularisation rule of SSA_main.f90:196-213.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "krylov.h"
+#include "ufe_internal.h"
+
+using namespace ufe;
+
+struct ufe_ctx {
+  int device = 0;
+  hipStream_t s = nullptr;
+  ufe_params prm{};
+  DevMesh m;
+  std::vector<std::unique_ptr<DBuf>> own;  // mesh/operator storage
+  std::vector<int> h_TriBI, h_TriC;
+  // system
+  DBuf A64, A32, Dinv64, Dinv32;
+  // inputs
+  DBuf Hi, Hs, A_vk_vf, A_vk_lf, A_vav, rough, fgr, bcu, bcv, rowtype;
+  std::vector<signed char> h_rowtype;
+  bool grounded_exists = true;
+  bool inputs_ready = false;
+  // iterate and work arrays
+  DBuf u, uprev, x, rhs, r64, tdx, tdy, N_a, beta_a, beta_eff_a, tri_coef, beta_basal_b, out2;
+  // DIVA
+  DBuf eta3b, eta3a, F1a, F2a, F1b, F2b0, ubase, taub, tmp3;
+  bool diva_state_ready = false;
+  std::unique_ptr<Krylov> kry;
+  std::unique_ptr<AmgHierarchy> amg;
+  std::string err;
+  double timings[5] = {0, 0, 0, 0, 0};
+  hipEvent_t ev[8];
+};
+
+namespace {
+
+int fail(ufe_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  set_global_error(msg);
+  return code;
+}
+
+template <typename F>
+int guarded(ufe_ctx *c, F &&f) {
+  try {
+    return f();
+  } catch (const HipError &e) {
+    return fail(c, UFE_ERR_DEVICE, e.what());
+  } catch (const std::exception &e) {
+    return fail(c, UFE_ERR_ARGUMENT, e.what());
+  }
+}
+
+template <typename U>
+U *dev_copy(ufe_ctx *c, const U *h, size_t n) {
+  auto b = std::make_unique<DBuf>(std::max<size_t>(n, 1) * sizeof(U));
+  if (n) UFE_HIP(hipMemcpy(b->p, h, n * sizeof(U), hipMemcpyHostToDevice));
+  U *p = b->as<U>();
+  c->own.push_back(std::move(b));
+  return p;
+}
+
+void h2d(DBuf &d, const void *h, size_t bytes, hipStream_t s) {
+  if (d.bytes < bytes) d.alloc(bytes);
+  UFE_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+}
+
+BsrT<double> sys64(ufe_ctx *c) {
+  BsrT<double> A;
+  A.nb = c->m.nT; A.nnzb = c->m.bb_nnz; A.ptr = c->m.bb_ptr; A.col = c->m.bb_ind;
+  A.val = c->A64.as<double4>(); A.dinv = c->Dinv64.as<double4>();
+  return A;
+}
+
+BsrT<float> sys32(ufe_ctx *c) {
+  BsrT<float> A;
+  A.nb = c->m.nT; A.nnzb = c->m.bb_nnz; A.ptr = c->m.bb_ptr; A.col = c->m.bb_ind;
+  A.val = c->A32.as<float4>(); A.dinv = c->Dinv32.as<float4>();
+  return A;
+}
+
+// Row types from border indices, BC choices and the prescribed-velocity mask.
+void build_rowtypes(ufe_ctx *c, const int *bc_mask) {
+  int nT = c->m.nT;
+  c->h_rowtype.assign(2 * (size_t)nT, ROW_FREE);
+  for (int ti = 0; ti < nT; ++ti) {
+    for (int uv = 0; uv < 2; ++uv) {
+      signed char t = ROW_FREE;
+      if (bc_mask && bc_mask[ti] == 1) t = ROW_DIRICHLET;
+      else if (c->h_TriBI[ti] > 0) {
+        int side = (c->h_TriBI[ti] - 1) / 2;  // 1,2 N; 3,4 E; 5,6 S; 7,8 W
+        int choice = uv == 0 ? c->prm.bc_u[side] : c->prm.bc_v[side];
+        t = choice == UFE_BC_ZERO ? ROW_ZERO : ROW_INFINITE;
+      }
+      c->h_rowtype[2 * ti + uv] = t;
+    }
+  }
+  h2d(c->rowtype, c->h_rowtype.data(), c->h_rowtype.size(), c->s);
+}
+
+PicardParams picard_params(const ufe_params &q, double eps0) {
+  PicardParams p;
+  p.n = q.glens_n;
+  p.eps0 = eps0;
+  p.visc_eff_min = q.visc_eff_min;
+  p.eta_max = 0.5 * std::pow(1e-18, -1.0 / q.glens_n) * std::pow(eps0, (1.0 - q.glens_n) / (2.0 * q.glens_n));
+  p.delta_v = q.slid_delta_v;
+  p.beta_max = q.slid_beta_max;
+  p.weertman_m = q.slid_weertman_m;
+  p.budd_q = q.slid_budd_q_plastic;
+  p.budd_u_th = q.slid_budd_u_threshold;
+  p.subgrid_exp = q.subgrid_friction_exponent;
+  p.sliding_law = q.sliding_law;
+  p.do_subgrid = q.do_gl_subgrid_friction;
+  p.crossterms = q.include_crossterms;
+  p.no_sliding = q.sliding_law == UFE_SLIDING_NO_SLIDING;
+  return p;
+}
+
+}  // namespace
+
+// FP32 correction added to the FP64 iterate (refinement step).
+__global__ void __launch_bounds__(kBlock) k_add_f32(int n, double *__restrict__ x, const float *__restrict__ d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += (double)d[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_or_zero(int n, const double *__restrict__ a, double *__restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a ? a[i] : 0.0;
+}
+
+namespace ufe {
+void launch_f64_to_f32(int n, const double *a, float *b, hipStream_t s);
+}
+
+// ---------------------------------------------------------------------------
+// Linear solve of one Picard iteration: x := solution of A x = rhs.
+// ---------------------------------------------------------------------------
+static void linear_solve(ufe_ctx *c, int *its, int *nref) {
+  const ufe_params &q = c->prm;
+  int n = 2 * c->m.nT;
+  Krylov &K = *c->kry;
+  bool warm = q.warm_start != 0;
+  hipLaunchKernelGGL(k_copy_or_zero, dim3(blocks_for(n)), dim3(kBlock), 0, c->s, n,
+                     warm ? c->u.as<double>() : (const double *)nullptr, c->x.as<double>());
+  UFE_LAUNCH_CHECK();
+  *its = 0;
+  *nref = 0;
+  if (q.precision == UFE_PREC_FP64) {
+    BsrT<double> A = sys64(c);
+    if (q.precond == UFE_PC_AMG) K.amg_setup64(A);
+    SolveStats st = K.bicgstab64(A, c->rhs.as<double>(), c->x.as<double>(), q.rtol, q.abstol, q.krylov_maxit,
+                                 !warm, q.check_every);
+    *its = st.its;
+    return;
+  }
+  // mixed: FP64 residual, FP32 corrections
+  BsrT<double> A = sys64(c);
+  BsrT<float> A32 = sys32(c);
+  if (q.precond == UFE_PC_AMG) K.amg_setup32(A32);
+  double rr, bb;
+  K.residual64(A, c->x.as<double>(), c->rhs.as<double>(), c->r64.as<double>(), &rr, &bb);
+  double r0 = std::sqrt(rr);
+  double target = std::max(q.rtol * r0, q.abstol);
+  float *r32 = K.tmp32(8), *d32 = K.tmp32(9);
+  for (int ref = 0; ref < 20 && std::sqrt(rr) > target; ++ref) {
+    launch_f64_to_f32(n, c->r64.as<double>(), r32, c->s);
+    double rn = std::sqrt(rr);
+    // inner tolerance: enough to reach the target this step, at least inner_rtol
+    double inner = std::max(std::min(q.inner_rtol, 0.5 * target / rn), 1e-6);
+    SolveStats st = K.bicgstab32(A32, r32, d32, inner, 0.0, q.krylov_maxit - *its, true, q.check_every);
+    *its += st.its;
+    hipLaunchKernelGGL(k_add_f32, dim3(blocks_for(n)), dim3(kBlock), 0, c->s, n, c->x.as<double>(), d32);
+    UFE_LAUNCH_CHECK();
+    K.residual64(A, c->x.as<double>(), c->rhs.as<double>(), c->r64.as<double>(), &rr, &bb);
+    ++*nref;
+    if (*its >= q.krylov_maxit) break;
+  }
+}
+
+static double rel_residual(ufe_ctx *c) {
+  double rr, bb;
+  c->kry->residual64(sys64(c), c->u.as<double>(), c->rhs.as<double>(), c->r64.as<double>(), &rr, &bb);
+  return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
+// Everything up to the assembled linear system at the current iterate.
+static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
+  const ufe_params &q = c->prm;
+  PicardParams p = picard_params(q, eps0);
+  hipStream_t s = c->s;
+  UFE_HIP(hipEventRecord(c->ev[0], s));
+  if (!diva) {
+    launch_vertex_ssa(c->m, c->u.as<double>(), c->A_vav.as<double>(), c->Hi.as<double>(), c->rough.as<double>(), p,
+                      c->N_a.as<double>(), c->beta_a.as<double>(), s);
+  } else {
+    launch_vertex_diva(c->m, c->u.as<double>(), c->ubase.as<double>(), c->taub.as<double>(), c->eta3b.as<double>(),
+                       c->A_vk_lf.as<double>(), c->Hi.as<double>(), c->rough.as<double>(), p, c->N_a.as<double>(),
+                       c->beta_a.as<double>(), c->beta_eff_a.as<double>(), c->eta3a.as<double>(),
+                       c->F1a.as<double>(), c->F2a.as<double>(), s);
+    launch_triangle_3d(c->m, c->eta3a.as<double>(), c->F1a.as<double>(), c->F2a.as<double>(), c->eta3b.as<double>(),
+                       c->F1b.as<double>(), c->F2b0.as<double>(), need_F1 ? 1 : 0, s);
+  }
+  UFE_HIP(hipEventRecord(c->ev[1], s));
+  launch_triangle(c->m, c->N_a.as<double>(), c->beta_a.as<double>(), diva ? c->beta_eff_a.as<double>() : nullptr,
+                  c->fgr.as<double>(), c->tdx.as<double>(), c->tdy.as<double>(), c->rowtype.as<signed char>(),
+                  c->bcu.as<double>(), c->bcv.as<double>(), p, c->tri_coef.as<double4>(),
+                  c->beta_basal_b.as<double>(), c->rhs.as<double>(), s);
+  UFE_HIP(hipEventRecord(c->ev[2], s));
+  bool mirror = q.precision == UFE_PREC_MIXED;
+  launch_assemble(c->m, c->rowtype.as<signed char>(), c->tri_coef.as<double4>(), q.include_crossterms,
+                  c->A64.as<double4>(), mirror ? c->A32.as<float4>() : nullptr, c->Dinv64.as<double4>(),
+                  mirror ? c->Dinv32.as<float4>() : nullptr, s);
+  UFE_HIP(hipEventRecord(c->ev[3], s));
+}
+
+static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
+  const ufe_params &q = c->prm;
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = c->s;
+  int nT = c->m.nT;
+  ufe_solve_info inf{};
+  if (!c->inputs_ready) return fail(c, UFE_ERR_ARGUMENT, "inputs not uploaded");
+  if (!c->grounded_exists || (!diva && q.sliding_law == UFE_SLIDING_NO_SLIDING)) {
+    // solve_SSA / solve_DIVA early exit: no grounded ice -> zero velocities
+    UFE_HIP(hipMemsetAsync(c->u.p, 0, c->u.bytes, s));
+    if (diva) {
+      UFE_HIP(hipMemsetAsync(c->ubase.p, 0, c->ubase.bytes, s));
+      UFE_HIP(hipMemsetAsync(c->taub.p, 0, c->taub.bytes, s));
+      UFE_HIP(hipMemsetAsync(c->F1b.p, 0, c->F1b.bytes, s));
+    }
+    UFE_HIP(hipStreamSynchronize(s));
+    if (info) *info = inf;
+    return UFE_OK;
+  }
+  c->kry->set_precond(q.precond);
+  launch_driving_stress(c->m, c->Hi.as<double>(), c->Hs.as<double>(), c->tdx.as<double>(), c->tdy.as<double>(), s);
+  if (!diva)
+    launch_vertical_average(c->m.nV, c->m.nz, c->m.zeta, c->A_vk_vf.as<double>(), c->A_vav.as<double>(), s);
+  if (diva && !c->diva_state_ready) {
+    UFE_HIP(hipMemsetAsync(c->eta3b.p, 0, c->eta3b.bytes, s));
+    UFE_HIP(hipMemsetAsync(c->ubase.p, 0, c->ubase.bytes, s));
+    UFE_HIP(hipMemsetAsync(c->taub.p, 0, c->taub.bytes, s));
+    c->diva_state_ready = true;
+  }
+  double L2 = 1e9, L2_prev;
+  int nit_div = 0, it = 0, n_axb = 0, n_ref = 0, rc = UFE_OK;
+  double relax = q.visc_it_relax, eps0 = q.epsilon_sq_0;
+  double resid = -1.0;
+  bool converged = false;
+  for (double &t : c->timings) t = 0.0;
+  while (!converged) {
+    ++it;
+    picard_state(c, diva, eps0, false);
+    if (q.stop_rule == UFE_STOP_RESIDUAL) {
+      resid = rel_residual(c);
+      if (resid <= q.nonlinear_rtol) { --it; converged = true; break; }
+    }
+    int its = 0, nref = 0;
+    linear_solve(c, &its, &nref);
+    n_axb += its;
+    n_ref += nref;
+    RedSlot sl = c->kry->slot(3);
+    launch_update(nT, c->x.as<double>(), c->u.as<double>(), c->uprev.as<double>(), q.vel_max, relax, sl,
+                  c->out2.as<double>(), c->beta_basal_b.as<double>(), c->tri_coef.as<double4>(),
+                  diva ? c->F2b0.as<double>() : nullptr, q.sliding_law == UFE_SLIDING_NO_SLIDING,
+                  diva ? c->ubase.as<double>() : nullptr, diva ? c->taub.as<double>() : nullptr, s);
+    double h2[2];
+    UFE_HIP(hipMemcpyAsync(h2, c->out2.p, sizeof(h2), hipMemcpyDeviceToHost, s));
+    UFE_HIP(hipStreamSynchronize(s));
+    float ms;
+    for (int k = 0; k < 3; ++k) {
+      UFE_HIP(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+      c->timings[k] += ms;
+    }
+    L2_prev = L2;
+    L2 = 2.0 * h2[0] / std::max(h2[1], 1e-8);
+    if (L2 > L2_prev) ++nit_div; else nit_div = 0;
+    if (nit_div > 2) { nit_div = 0; relax *= 0.9; eps0 *= 1.2; }
+    if (relax <= 0.05 || eps0 >= 1e-5) {
+      if (relax < 0.05 || eps0 > 1e-5) {
+        rc = fail(c, UFE_ERR_DIVERGED, "viscosity iteration still diverges (relaxation or regularisation limit)");
+        break;
+      }
+    }
+    if (q.stop_rule == UFE_STOP_L2UV && L2 < q.visc_it_norm_dUV_tol) converged = true;
+    if (it > q.visc_it_nit) { rc = UFE_WARN_NOT_CONVERGED; break; }
+  }
+  if (rc >= 0 && (q.stop_rule != UFE_STOP_RESIDUAL || !converged)) {
+    // state at the returned iterate (for the residual and, for DIVA, eta_3D_b/F)
+    picard_state(c, diva, eps0, false);
+    resid = rel_residual(c);
+  }
+  if (diva && rc >= 0) {
+    // F1 on the b-grid at the returned state for calc_3D_velocities
+    launch_triangle_3d(c->m, c->eta3a.as<double>(), c->F1a.as<double>(), c->F2a.as<double>(),
+                       c->tmp3.as<double>(), c->F1b.as<double>(), c->F2b0.as<double>(), 1, s);
+  }
+  UFE_HIP(hipStreamSynchronize(s));
+  inf.n_visc_its = it;
+  inf.n_Axb_its = n_axb;
+  inf.n_refinements = n_ref;
+  inf.L2_uv = L2;
+  inf.rel_residual = resid;
+  inf.visc_it_relax_applied = relax;
+  inf.epsilon_sq_0_applied = eps0;
+  inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (info) *info = inf;
+  return rc;
+}
+
+// Accessors for the other translation units (grounded fractions, heat columns).
+namespace ufe {
+int ctx_device(const ufe_ctx *c) { return c->device; }
+hipStream_t ctx_stream(const ufe_ctx *c) { return c->s; }
+const DevMesh *ctx_mesh(const ufe_ctx *c) { return &c->m; }
+Krylov *ctx_krylov(const ufe_ctx *c) { return c->kry.get(); }
+int ctx_fail(ufe_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
+}  // namespace ufe
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" const char *ufe_last_error(const ufe_ctx *ctx) { return ctx ? ctx->err.c_str() : global_error(); }
+
+extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md) {
+  if (!out || !md) return fail(nullptr, UFE_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  auto c = std::make_unique<ufe_ctx>();
+  int rc = guarded(c.get(), [&]() -> int {
+    int ndev = 0;
+    UFE_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(nullptr, UFE_ERR_DEVICE, "no such HIP device");
+    if (md->nV <= 0 || md->nT <= 0 || md->nz < 2 || md->nz > 64)
+      return fail(nullptr, UFE_ERR_ARGUMENT, "invalid mesh sizes");
+    c->device = device;
+    UFE_HIP(hipSetDevice(device));
+    UFE_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    for (auto &e : c->ev) UFE_HIP(hipEventCreate(&e));
+    ufe_params_default(&c->prm);
+    DevMesh &m = c->m;
+    m.nV = md->nV; m.nT = md->nT; m.nz = md->nz;
+    int nV = m.nV, nT = m.nT, nz = m.nz;
+    // validate index ranges on the host before anything reaches a kernel
+    auto check_csr = [&](int rows, int ncols, const int *ptr, const int *ind, int nnz, const char *name) {
+      if (ptr[0] != 0 || ptr[rows] != nnz) throw std::runtime_error(std::string(name) + ": bad row pointer");
+      for (int i = 0; i < rows; ++i)
+        if (ptr[i + 1] < ptr[i]) throw std::runtime_error(std::string(name) + ": row pointer not monotone");
+      for (int k = 0; k < nnz; ++k)
+        if (ind[k] < 0 || ind[k] >= ncols) throw std::runtime_error(std::string(name) + ": column out of range");
+    };
+    check_csr(nT, nV, md->ab_ptr, md->ab_ind, md->ab_nnz, "a->b operators");
+    check_csr(nV, nT, md->ba_ptr, md->ba_ind, md->ba_nnz, "b->a operators");
+    check_csr(nT, nT, md->bb_ptr, md->bb_ind, md->bb_nnz, "b->b operators");
+    for (int k = 0; k < 3 * nT; ++k) {
+      if (md->TriC[k] < -1 || md->TriC[k] >= nT) throw std::runtime_error("TriC out of range");
+      if (md->Tri[k] < 0 || md->Tri[k] >= nV) throw std::runtime_error("Tri out of range");
+    }
+    c->h_TriBI.assign(md->TriBI, md->TriBI + nT);
+    c->h_TriC.assign(md->TriC, md->TriC + 3 * nT);
+    m.ab_nnz = md->ab_nnz; m.ba_nnz = md->ba_nnz; m.bb_nnz = md->bb_nnz;
+    m.ab_ptr = dev_copy(c.get(), md->ab_ptr, nT + 1);
+    m.ab_ind = dev_copy(c.get(), md->ab_ind, md->ab_nnz);
+    m.ab_map = dev_copy(c.get(), md->ab_map, md->ab_nnz);
+    m.ab_ddx = dev_copy(c.get(), md->ab_ddx, md->ab_nnz);
+    m.ab_ddy = dev_copy(c.get(), md->ab_ddy, md->ab_nnz);
+    m.ba_ptr = dev_copy(c.get(), md->ba_ptr, nV + 1);
+    m.ba_ind = dev_copy(c.get(), md->ba_ind, md->ba_nnz);
+    m.ba_map = dev_copy(c.get(), md->ba_map, md->ba_nnz);
+    m.ba_ddx = dev_copy(c.get(), md->ba_ddx, md->ba_nnz);
+    m.ba_ddy = dev_copy(c.get(), md->ba_ddy, md->ba_nnz);
+    // M2 rows sorted by column: the block pattern of the system matrix
+    int nnz = md->bb_nnz;
+    std::vector<int> ind(nnz), row(nnz), diag(nT, -1), perm(nnz);
+    std::vector<double> v[5];
+    for (auto &a : v) a.resize(nnz);
+    const double *src[5] = {md->bb_ddx, md->bb_ddy, md->bb_d2dx2, md->bb_d2dxdy, md->bb_d2dy2};
+    for (int ti = 0; ti < nT; ++ti) {
+      int k0 = md->bb_ptr[ti], k1 = md->bb_ptr[ti + 1];
+      for (int k = k0; k < k1; ++k) perm[k] = k;
+      std::sort(perm.begin() + k0, perm.begin() + k1,
+                [&](int a, int b) { return md->bb_ind[a] < md->bb_ind[b]; });
+      for (int k = k0; k < k1; ++k) {
+        ind[k] = md->bb_ind[perm[k]];
+        row[k] = ti;
+        if (k > k0 && ind[k] == ind[k - 1]) throw std::runtime_error("duplicate column in b->b operator row");
+        if (ind[k] == ti) diag[ti] = k;
+        for (int q = 0; q < 5; ++q) v[q][k] = src[q][perm[k]];
+      }
+      if (diag[ti] < 0) throw std::runtime_error("b->b operator row without diagonal");
+      // the 'infinite' boundary rows use the TriC neighbours; they must lie in the stencil
+      for (int q = 0; q < 3; ++q) {
+        int tj = md->TriC[3 * ti + q];
+        if (tj < 0) continue;
+        if (!std::binary_search(ind.begin() + k0, ind.begin() + k1, tj))
+          throw std::runtime_error("TriC neighbour missing from the b->b stencil");
+      }
+    }
+    m.bb_ptr = dev_copy(c.get(), md->bb_ptr, nT + 1);
+    m.bb_ind = dev_copy(c.get(), ind.data(), nnz);
+    m.bb_row = dev_copy(c.get(), row.data(), nnz);
+    m.bb_diag = dev_copy(c.get(), diag.data(), nT);
+    m.bb_ddx = dev_copy(c.get(), v[0].data(), nnz);
+    m.bb_ddy = dev_copy(c.get(), v[1].data(), nnz);
+    m.bb_d2dx2 = dev_copy(c.get(), v[2].data(), nnz);
+    m.bb_d2dxdy = dev_copy(c.get(), v[3].data(), nnz);
+    m.bb_d2dy2 = dev_copy(c.get(), v[4].data(), nnz);
+    m.TriC = dev_copy(c.get(), md->TriC, 3 * (size_t)nT);
+    m.Tri = dev_copy(c.get(), md->Tri, 3 * (size_t)nT);
+    m.V = dev_copy(c.get(), md->V, 2 * (size_t)nV);
+    m.zeta = dev_copy(c.get(), md->zeta, nz);
+    // system and work arrays
+    c->A64.alloc(sizeof(double4) * (size_t)nnz);
+    c->A32.alloc(sizeof(float4) * (size_t)nnz);
+    c->Dinv64.alloc(sizeof(double4) * (size_t)nT);
+    c->Dinv32.alloc(sizeof(float4) * (size_t)nT);
+    size_t v2 = sizeof(double) * 2 * (size_t)nT;
+    for (DBuf *b : {&c->u, &c->uprev, &c->x, &c->rhs, &c->r64, &c->ubase, &c->taub}) b->alloc(v2);
+    for (DBuf *b : {&c->tdx, &c->tdy, &c->beta_basal_b, &c->F2b0, &c->fgr, &c->bcu, &c->bcv})
+      b->alloc(sizeof(double) * (size_t)nT);
+    c->tri_coef.alloc(sizeof(double4) * (size_t)nT);
+    for (DBuf *b : {&c->N_a, &c->beta_a, &c->beta_eff_a, &c->A_vav, &c->Hi, &c->Hs, &c->rough})
+      b->alloc(sizeof(double) * (size_t)nV);
+    for (DBuf *b : {&c->eta3a, &c->F1a, &c->F2a, &c->A_vk_vf, &c->A_vk_lf})
+      b->alloc(sizeof(double) * (size_t)nV * nz);
+    for (DBuf *b : {&c->eta3b, &c->F1b, &c->tmp3}) b->alloc(sizeof(double) * (size_t)nT * nz);
+    c->out2.alloc(sizeof(double) * 4);
+    UFE_HIP(hipMemset(c->u.p, 0, v2));
+    c->kry = std::make_unique<Krylov>(nT, c->s);
+    c->kry->set_diag(m.bb_diag);
+    c->amg = build_amg_hierarchy(nT, md->TriC, md->bb_ptr, ind.data(), 1024, 12);
+    if (c->amg) c->kry->attach_amg(c->amg.get());
+    build_rowtypes(c.get(), nullptr);
+    UFE_HIP(hipStreamSynchronize(c->s));
+    return UFE_OK;
+  });
+  if (rc != UFE_OK) {
+    set_global_error(c->err.empty() ? std::string(global_error()) : c->err);
+    return rc;
+  }
+  *out = c.release();
+  return UFE_OK;
+}
+
+extern "C" int ufe_ctx_destroy(ufe_ctx *c) {
+  if (!c) return UFE_OK;
+  (void)hipSetDevice(c->device);
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  for (auto &e : c->ev) (void)hipEventDestroy(e);
+  hipStream_t s = c->s;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+  return UFE_OK;
+}
+
+extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
+  if (!c || !p) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  return guarded(c, [&]() -> int {
+    if (p->precond == UFE_PC_AMG && !c->amg) return fail(c, UFE_ERR_UNSUPPORTED, "no aggregation hierarchy for this mesh");
+    if (p->krylov != UFE_KRYLOV_BICGSTAB) return fail(c, UFE_ERR_UNSUPPORTED, "only BiCGStab is implemented");
+    if (p->check_every < 1 || p->krylov_maxit < 1) return fail(c, UFE_ERR_ARGUMENT, "invalid Krylov controls");
+    c->prm = *p;
+    UFE_HIP(hipSetDevice(c->device));
+    // border row types depend on the BC choices; keep any Dirichlet rows
+    std::vector<int> mask(c->m.nT, 0);
+    for (int ti = 0; ti < c->m.nT; ++ti) mask[ti] = c->h_rowtype.size() ? (c->h_rowtype[2 * ti] == ROW_DIRICHLET) : 0;
+    build_rowtypes(c, mask.data());
+    UFE_HIP(hipStreamSynchronize(c->s));
+    return UFE_OK;
+  });
+}
+
+__global__ void k_transpose(int rows, int cols, const double *__restrict__ in, double *__restrict__ out);
+
+extern "C" int ufe_upload_inputs(ufe_ctx *c, const double *Hi, const double *Hs, const double *A_flow,
+                                 const double *beta_sq, const double *fgr, const int *mask_grounded,
+                                 const int *bc_mask, const double *bc_u, const double *bc_v) {
+  if (!c || !Hi || !Hs || !A_flow || !beta_sq || !fgr) return fail(c, UFE_ERR_ARGUMENT, "null input array");
+  if ((bc_mask || bc_u || bc_v) && !(bc_mask && bc_u && bc_v))
+    return fail(c, UFE_ERR_ARGUMENT, "need to provide prescribed u,v fields and mask!");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    int nV = c->m.nV, nT = c->m.nT, nz = c->m.nz;
+    hipStream_t s = c->s;
+    h2d(c->Hi, Hi, sizeof(double) * nV, s);
+    h2d(c->Hs, Hs, sizeof(double) * nV, s);
+    h2d(c->rough, beta_sq, sizeof(double) * nV, s);
+    h2d(c->fgr, fgr, sizeof(double) * nT, s);
+    h2d(c->A_vk_vf, A_flow, sizeof(double) * (size_t)nV * nz, s);
+    launch_transpose(nz, nV, c->A_vk_vf.as<double>(), c->A_vk_lf.as<double>(), s);
+    c->grounded_exists = true;
+    if (mask_grounded) {
+      c->grounded_exists = false;
+      for (int vi = 0; vi < nV; ++vi) if (mask_grounded[vi]) { c->grounded_exists = true; break; }
+    }
+    std::vector<double> zero(nT, 0.0);
+    h2d(c->bcu, bc_u ? bc_u : zero.data(), sizeof(double) * nT, s);
+    h2d(c->bcv, bc_v ? bc_v : zero.data(), sizeof(double) * nT, s);
+    build_rowtypes(c, bc_mask);
+    UFE_HIP(hipStreamSynchronize(s));
+    c->inputs_ready = true;
+    return UFE_OK;
+  });
+}
+
+static void interleave_up(ufe_ctx *c, DBuf &dst, const double *a, const double *b) {
+  int nT = c->m.nT;
+  std::vector<double> h(2 * (size_t)nT);
+  for (int i = 0; i < nT; ++i) { h[2 * i] = a ? a[i] : 0.0; h[2 * i + 1] = b ? b[i] : 0.0; }
+  UFE_HIP(hipMemcpy(dst.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+static void interleave_down(ufe_ctx *c, const DBuf &src, double *a, double *b) {
+  int nT = c->m.nT;
+  std::vector<double> h(2 * (size_t)nT);
+  UFE_HIP(hipMemcpy(h.data(), src.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nT; ++i) { if (a) a[i] = h[2 * i]; if (b) b[i] = h[2 * i + 1]; }
+}
+
+extern "C" int ufe_upload_velocity(ufe_ctx *c, const double *u_b, const double *v_b) {
+  if (!c || !u_b || !v_b) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    interleave_up(c, c->u, u_b, v_b);
+    return UFE_OK;
+  });
+}
+
+extern "C" int ufe_download_velocity(ufe_ctx *c, double *u_b, double *v_b) {
+  if (!c || !u_b || !v_b) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    UFE_HIP(hipStreamSynchronize(c->s));
+    interleave_down(c, c->u, u_b, v_b);
+    return UFE_OK;
+  });
+}
+
+extern "C" int ufe_upload_diva_state(ufe_ctx *c, const double *eta_3D_b, const double *tau_bx_b,
+                                     const double *tau_by_b, const double *u_base_b, const double *v_base_b) {
+  if (!c) return fail(c, UFE_ERR_ARGUMENT, "null context");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    int nT = c->m.nT, nz = c->m.nz;
+    if (eta_3D_b) {
+      h2d(c->tmp3, eta_3D_b, sizeof(double) * (size_t)nT * nz, c->s);
+      launch_transpose(nz, nT, c->tmp3.as<double>(), c->eta3b.as<double>(), c->s);
+    } else {
+      UFE_HIP(hipMemsetAsync(c->eta3b.p, 0, c->eta3b.bytes, c->s));
+    }
+    UFE_HIP(hipStreamSynchronize(c->s));
+    interleave_up(c, c->taub, tau_bx_b, tau_by_b);
+    interleave_up(c, c->ubase, u_base_b, v_base_b);
+    c->diva_state_ready = true;
+    return UFE_OK;
+  });
+}
+
+extern "C" int ufe_download_diva(ufe_ctx *c, double *u_base_b, double *v_base_b, double *u_3D_b, double *v_3D_b,
+                                 double *eta_3D_b, double *tau_bx_b, double *tau_by_b) {
+  if (!c) return fail(c, UFE_ERR_ARGUMENT, "null context");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    int nT = c->m.nT, nz = c->m.nz;
+    hipStream_t s = c->s;
+    UFE_HIP(hipStreamSynchronize(s));
+    if (u_base_b || v_base_b) interleave_down(c, c->ubase, u_base_b, v_base_b);
+    if (tau_bx_b || tau_by_b) interleave_down(c, c->taub, tau_bx_b, tau_by_b);
+    if (u_3D_b || v_3D_b) {
+      DBuf u3(sizeof(double) * (size_t)nT * nz), v3(sizeof(double) * (size_t)nT * nz);
+      launch_velocities_3d(nT, nz, c->ubase.as<double>(), c->beta_basal_b.as<double>(), c->taub.as<double>(),
+                           c->F1b.as<double>(), c->prm.sliding_law == UFE_SLIDING_NO_SLIDING, u3.as<double>(),
+                           v3.as<double>(), s);
+      UFE_HIP(hipStreamSynchronize(s));
+      if (u_3D_b) UFE_HIP(hipMemcpy(u_3D_b, u3.p, u3.bytes, hipMemcpyDeviceToHost));
+      if (v_3D_b) UFE_HIP(hipMemcpy(v_3D_b, v3.p, v3.bytes, hipMemcpyDeviceToHost));
+    }
+    if (eta_3D_b) {
+      launch_transpose(nT, nz, c->eta3b.as<double>(), c->tmp3.as<double>(), s);
+      UFE_HIP(hipStreamSynchronize(s));
+      UFE_HIP(hipMemcpy(eta_3D_b, c->tmp3.p, sizeof(double) * (size_t)nT * nz, hipMemcpyDeviceToHost));
+    }
+    return UFE_OK;
+  });
+}
+
+extern "C" int ufe_run(ufe_ctx *c, int stress_balance, ufe_solve_info *info) {
+  if (!c) return fail(c, UFE_ERR_ARGUMENT, "null context");
+  if (stress_balance != UFE_STRESS_SSA && stress_balance != UFE_STRESS_DIVA)
+    return fail(c, UFE_ERR_UNSUPPORTED, "stress balance must be SSA or DIVA");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    return run_picard(c, stress_balance == UFE_STRESS_DIVA, info);
+  });
+}
+
+extern "C" int ufe_solve_ssa(ufe_ctx *c, const double *Hi, const double *Hs, const double *A_flow,
+                             const double *beta_sq, const double *fgr, const int *mask_grounded, const int *bc_mask,
+                             const double *bc_u, const double *bc_v, double *u_b, double *v_b, ufe_solve_info *info) {
+  int rc = ufe_upload_inputs(c, Hi, Hs, A_flow, beta_sq, fgr, mask_grounded, bc_mask, bc_u, bc_v);
+  if (rc < 0) return rc;
+  rc = ufe_upload_velocity(c, u_b, v_b);
+  if (rc < 0) return rc;
+  int rr = ufe_run(c, UFE_STRESS_SSA, info);
+  if (rr < 0) return rr;
+  rc = ufe_download_velocity(c, u_b, v_b);
+  return rc < 0 ? rc : rr;
+}
+
+extern "C" int ufe_solve_diva(ufe_ctx *c, const double *Hi, const double *Hs, const double *A_flow,
+                              const double *beta_sq, const double *fgr, const int *mask_grounded, const int *bc_mask,
+                              const double *bc_u, const double *bc_v, double *u_vav_b, double *v_vav_b,
+                              double *u_base_b, double *v_base_b, double *u_3D_b, double *v_3D_b, double *eta_3D_b,
+                              double *tau_bx_b, double *tau_by_b, ufe_solve_info *info) {
+  int rc = ufe_upload_inputs(c, Hi, Hs, A_flow, beta_sq, fgr, mask_grounded, bc_mask, bc_u, bc_v);
+  if (rc < 0) return rc;
+  rc = ufe_upload_velocity(c, u_vav_b, v_vav_b);
+  if (rc < 0) return rc;
+  rc = ufe_upload_diva_state(c, eta_3D_b, tau_bx_b, tau_by_b, u_base_b, v_base_b);
+  if (rc < 0) return rc;
+  int rr = ufe_run(c, UFE_STRESS_DIVA, info);
+  if (rr < 0) return rr;
+  rc = ufe_download_velocity(c, u_vav_b, v_vav_b);
+  if (rc < 0) return rc;
+  rc = ufe_download_diva(c, u_base_b, v_base_b, u_3D_b, v_3D_b, eta_3D_b, tau_bx_b, tau_by_b);
+  return rc < 0 ? rc : rr;
+}
+
+extern "C" int ufe_last_timings(const ufe_ctx *c, double *ms5) {
+  if (!c || !ms5) return UFE_ERR_ARGUMENT;
+  for (int i = 0; i < 5; ++i) ms5[i] = c->timings[i];
+  return UFE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Generic CSR boundary
+// ---------------------------------------------------------------------------
+extern "C" int ufe_csr_matvec(int m, int n, int nnz, const int *ptr, const int *ind, const double *val,
+                              const double *x, double *y) {
+  if (m < 0 || n < 0 || nnz < 0 || !ptr || (nnz && (!ind || !val)) || !x || !y)
+    return fail(nullptr, UFE_ERR_ARGUMENT, "invalid CSR arguments");
+  if (ptr[0] != 0 || ptr[m] != nnz) return fail(nullptr, UFE_ERR_ARGUMENT, "bad row pointer");
+  for (int i = 0; i < m; ++i)
+    if (ptr[i + 1] < ptr[i]) return fail(nullptr, UFE_ERR_ARGUMENT, "row pointer not monotone");
+  for (int k = 0; k < nnz; ++k)
+    if (ind[k] < 0 || ind[k] >= n) return fail(nullptr, UFE_ERR_ARGUMENT, "column index out of range");
+  return guarded(nullptr, [&]() -> int {
+    hipStream_t s;
+    UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DBuf dp(sizeof(int) * (m + 1)), di(sizeof(int) * std::max(nnz, 1)), dv(sizeof(double) * std::max(nnz, 1)),
+        dx(sizeof(double) * std::max(n, 1)), dy(sizeof(double) * std::max(m, 1));
+    UFE_HIP(hipMemcpy(dp.p, ptr, sizeof(int) * (m + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+      UFE_HIP(hipMemcpy(di.p, ind, sizeof(int) * nnz, hipMemcpyHostToDevice));
+      UFE_HIP(hipMemcpy(dv.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    if (n) UFE_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (m) launch_csr_spmv64(m, dp.as<int>(), di.as<int>(), dv.as<double>(), dx.as<double>(), dy.as<double>(), s);
+    UFE_HIP(hipStreamSynchronize(s));
+    if (m) UFE_HIP(hipMemcpy(y, dy.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    UFE_HIP(hipStreamDestroy(s));
+    return UFE_OK;
+  });
+}
+
+// Solve A x = b for a general square CSR matrix: rows/columns are paired into
+// 2x2 blocks (explicit zeros where needed) and the BSR BiCGStab is used.
+extern "C" int ufe_csr_solve(int n, int nnz, const int *ptr, const int *ind, const double *val, const double *b,
+                             double *x, double rtol, double abstol, int maxit, int precond, int *n_its,
+                             double *rel_residual) {
+  if (n <= 0 || nnz < 0 || !ptr || !ind || !val || !b || !x) return fail(nullptr, UFE_ERR_ARGUMENT, "invalid CSR arguments");
+  if (ptr[0] != 0 || ptr[n] != nnz) return fail(nullptr, UFE_ERR_ARGUMENT, "bad row pointer");
+  for (int i = 0; i < n; ++i)
+    if (ptr[i + 1] < ptr[i]) return fail(nullptr, UFE_ERR_ARGUMENT, "row pointer not monotone");
+  for (int k = 0; k < nnz; ++k)
+    if (ind[k] < 0 || ind[k] >= n) return fail(nullptr, UFE_ERR_ARGUMENT, "column index out of range");
+  if (precond == UFE_PC_AMG) return fail(nullptr, UFE_ERR_UNSUPPORTED, "AMG needs the mesh context");
+  if (maxit < 1) return fail(nullptr, UFE_ERR_ARGUMENT, "maxit must be positive");
+  return guarded(nullptr, [&]() -> int {
+    int nb = (n + 1) / 2;
+    std::vector<std::vector<std::pair<int, std::array<double, 4>>>> rows(nb);
+    auto slot = [&](int I, int J) -> std::array<double, 4> & {
+      for (auto &e : rows[I]) if (e.first == J) return e.second;
+      rows[I].push_back({J, {0.0, 0.0, 0.0, 0.0}});
+      return rows[I].back().second;
+    };
+    for (int i = 0; i < n; ++i)
+      for (int k = ptr[i]; k < ptr[i + 1]; ++k) slot(i / 2, ind[k] / 2)[2 * (i % 2) + ind[k] % 2] += val[k];
+    std::vector<int> bptr(nb + 1, 0), bcol, bdiag(nb, -1);
+    std::vector<double> bval, dinv(4 * (size_t)nb, 0.0);
+    for (int I = 0; I < nb; ++I) {
+      auto &rw = rows[I];
+      slot(I, I);  // diagonal block always present
+      std::sort(rw.begin(), rw.end(), [](const auto &a, const auto &c) { return a.first < c.first; });
+      if (n % 2 == 1 && I == nb - 1) slot(I, I)[3] = 1.0;  // padding unknown: identity row
+      for (auto &e : rw) {
+        if (e.first == I) bdiag[I] = (int)bcol.size();
+        bcol.push_back(e.first);
+        bval.insert(bval.end(), e.second.begin(), e.second.end());
+      }
+      bptr[I + 1] = (int)bcol.size();
+      const double *d = &bval[4 * (size_t)bdiag[I]];
+      double det = d[0] * d[3] - d[1] * d[2];
+      if (precond == UFE_PC_BLOCK_JACOBI && det == 0.0)
+        throw std::runtime_error("singular diagonal block; use a different preconditioner");
+      if (precond == UFE_PC_JACOBI && (d[0] == 0.0 || d[3] == 0.0))
+        throw std::runtime_error("zero diagonal entry; use a different preconditioner");
+      if (det != 0.0) {
+        dinv[4 * I + 0] = d[3] / det; dinv[4 * I + 1] = -d[1] / det;
+        dinv[4 * I + 2] = -d[2] / det; dinv[4 * I + 3] = d[0] / det;
+      }
+    }
+    hipStream_t s;
+    UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<double> bb(2 * (size_t)nb, 0.0), xx(2 * (size_t)nb, 0.0);
+    std::copy(b, b + n, bb.begin());
+    std::copy(x, x + n, xx.begin());
+    int out_rc = UFE_OK;
+    {
+      DBuf dptr(sizeof(int) * bptr.size()), dcol(sizeof(int) * std::max<size_t>(bcol.size(), 1)),
+          ddiag(sizeof(int) * nb), dval(sizeof(double) * std::max<size_t>(bval.size(), 4)),
+          ddinv(sizeof(double) * dinv.size()), db(sizeof(double) * bb.size()), dx(sizeof(double) * xx.size());
+      UFE_HIP(hipMemcpy(dptr.p, bptr.data(), dptr.bytes, hipMemcpyHostToDevice));
+      UFE_HIP(hipMemcpy(dcol.p, bcol.data(), sizeof(int) * bcol.size(), hipMemcpyHostToDevice));
+      UFE_HIP(hipMemcpy(ddiag.p, bdiag.data(), ddiag.bytes, hipMemcpyHostToDevice));
+      UFE_HIP(hipMemcpy(dval.p, bval.data(), sizeof(double) * bval.size(), hipMemcpyHostToDevice));
+      UFE_HIP(hipMemcpy(ddinv.p, dinv.data(), ddinv.bytes, hipMemcpyHostToDevice));
+      UFE_HIP(hipMemcpy(db.p, bb.data(), db.bytes, hipMemcpyHostToDevice));
+      UFE_HIP(hipMemcpy(dx.p, xx.data(), dx.bytes, hipMemcpyHostToDevice));
+      Krylov K(nb, s);
+      K.set_precond(precond);
+      K.set_diag(ddiag.as<int>());
+      BsrT<double> A;
+      A.nb = nb; A.nnzb = (int)bcol.size(); A.ptr = dptr.as<int>(); A.col = dcol.as<int>();
+      A.val = dval.as<double4>(); A.dinv = ddinv.as<double4>();
+      SolveStats st = K.bicgstab64(A, db.as<double>(), dx.as<double>(), rtol, abstol, maxit, false, 8);
+      UFE_HIP(hipMemcpy(xx.data(), dx.p, dx.bytes, hipMemcpyDeviceToHost));
+      if (n_its) *n_its = st.its;
+      if (rel_residual) *rel_residual = st.rel;
+      if (st.status != 1) out_rc = UFE_WARN_NOT_CONVERGED;
+    }
+    std::copy(xx.begin(), xx.begin() + n, x);
+    UFE_HIP(hipStreamDestroy(s));
+    return out_rc;
+  });
+}
+
+// Nonlinear SSA residual at a given velocity: r = b(u) - A(u) u with the
+// resident inputs (the linearised system of solve_linearised_SSA_DIVA.f90
+// assembled at u), interleaved rows 2*ti+uv.
+extern "C" int ufe_ssa_residual(ufe_ctx *c, const double *u_b, const double *v_b, double *r, double *b,
+                                double *rel_out) {
+  if (!c || !u_b || !v_b) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    if (!c->inputs_ready) return fail(c, UFE_ERR_ARGUMENT, "inputs not uploaded");
+    interleave_up(c, c->u, u_b, v_b);
+    launch_driving_stress(c->m, c->Hi.as<double>(), c->Hs.as<double>(), c->tdx.as<double>(), c->tdy.as<double>(),
+                          c->s);
+    launch_vertical_average(c->m.nV, c->m.nz, c->m.zeta, c->A_vk_vf.as<double>(), c->A_vav.as<double>(), c->s);
+    picard_state(c, false, c->prm.epsilon_sq_0, false);
+    double rel = rel_residual(c);
+    if (rel_out) *rel_out = rel;
+    size_t n = 2 * (size_t)c->m.nT;
+    if (r) UFE_HIP(hipMemcpy(r, c->r64.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (b) UFE_HIP(hipMemcpy(b, c->rhs.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    return UFE_OK;
+  });
+}

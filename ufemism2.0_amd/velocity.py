@@ -1,0 +1,231 @@
+"""Host-side mirror of UFEMISM's ice-velocity module interface.
+
+``VelocitySolver.solve_SSA`` / ``solve_DIVA`` keep the argument meaning and
+error behaviour of solve_SSA (SSA_main.f90:87-242) and solve_DIVA
+(DIVA_main.f90:88-262): velocities are inout (initial guess in, solution
+out), the optional prescribed-velocity boundary condition needs all three of
+mask, u and v, the Picard iteration returns n_visc_its and n_Axb_its, hitting
+visc_it_nit is a warning, a diverging iteration is an error.  All compute runs
+in libufe_hip.so; there is no CPU path here.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import numpy as np
+
+from . import _lib as L
+from .operators import MeshDescriptor
+
+
+class ViscosityIterationWarning(UserWarning):
+    pass
+
+
+class VelocitySolverError(RuntimeError):
+    pass
+
+
+_PARAM_NAMES = [f for f, _ in L.Params._fields_]
+
+
+class VelocitySolver:
+    """One device context: mesh, operators and solver state in HBM."""
+
+    def __init__(self, md: MeshDescriptor, device: int = 0, **params):
+        self.md = md
+        self.lib = L.lib()
+        self._ctx = C.c_void_p()
+        rc = self.lib.ufe_ctx_create(C.byref(self._ctx), device, md.byref())
+        if rc != 0:
+            raise VelocitySolverError(f"ufe_ctx_create failed ({rc}): {self.lib.ufe_last_error(None).decode()}")
+        self.params = L.default_params()
+        self.set_params(**params)
+
+    # -- configuration -------------------------------------------------------
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if k not in _PARAM_NAMES:
+                raise KeyError(f"unknown parameter {k}")
+            if k in ("bc_u", "bc_v"):
+                arr = getattr(self.params, k)
+                for i, x in enumerate(v):
+                    arr[i] = int(x)
+            else:
+                setattr(self.params, k, v)
+        self._check(self.lib.ufe_set_params(self._ctx, C.byref(self.params)))
+
+    def _check(self, rc):
+        if rc < 0:
+            raise VelocitySolverError(f"libufe_hip error {rc}: {self.lib.ufe_last_error(self._ctx).decode()}")
+        return rc
+
+    @staticmethod
+    def _bc(bc_mask, bc_u, bc_v):
+        given = [a is not None for a in (bc_mask, bc_u, bc_v)]
+        if any(given) and not all(given):
+            raise VelocitySolverError("need to provide prescribed u,v fields and mask!")
+        if not any(given):
+            return None, None, None
+        return L.i32(bc_mask), L.f64(bc_u), L.f64(bc_v)
+
+    # -- device-resident form --------------------------------------------------
+    def upload(self, Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice=None,
+               BC_prescr_mask_b=None, BC_prescr_u_b=None, BC_prescr_v_b=None):
+        bm, bu, bv = self._bc(BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b)
+        self._keep = [L.f64(Hi), L.f64(Hs), L.f64(A_flow), L.f64(beta_sq), L.f64(fraction_gr_b),
+                      None if mask_grounded_ice is None else L.i32(mask_grounded_ice), bm, bu, bv]
+        k = self._keep
+        self._check(self.lib.ufe_upload_inputs(self._ctx, L.ptr(k[0]), L.ptr(k[1]), L.ptr(k[2]), L.ptr(k[3]),
+                                               L.ptr(k[4]), L.ptr(k[5], L.ip), L.ptr(k[6], L.ip), L.ptr(k[7]),
+                                               L.ptr(k[8])))
+
+    def set_velocity(self, u_b, v_b):
+        self._check(self.lib.ufe_upload_velocity(self._ctx, L.ptr(L.f64(u_b)), L.ptr(L.f64(v_b))))
+
+    def get_velocity(self):
+        nT = self.md.mesh.nT
+        u = np.zeros(nT)
+        v = np.zeros(nT)
+        self._check(self.lib.ufe_download_velocity(self._ctx, L.ptr(u), L.ptr(v)))
+        return u, v
+
+    def run(self, stress_balance="SSA"):
+        info = L.SolveInfo()
+        sb = L.STRESS_DIVA if stress_balance.upper() == "DIVA" else L.STRESS_SSA
+        rc = self._check(self.lib.ufe_run(self._ctx, sb, C.byref(info)))
+        if rc == L.UFE_WARN_NOT_CONVERGED:
+            warnings.warn(f"viscosity iteration failed to converge within {self.params.visc_it_nit} iterations!",
+                          ViscosityIterationWarning)
+        return info
+
+    def timings(self):
+        t = np.zeros(5)
+        self.lib.ufe_last_timings(self._ctx, L.ptr(t))
+        return {"vertex": t[0], "triangle": t[1], "assembly": t[2]}
+
+    def residual(self, u_b, v_b):
+        """FP64 nonlinear SSA residual r = b(u) - A(u) u at (u_b, v_b)."""
+        n = 2 * self.md.mesh.nT
+        r = np.zeros(n)
+        b = np.zeros(n)
+        rel = C.c_double()
+        self._check(self.lib.ufe_ssa_residual(self._ctx, L.ptr(L.f64(u_b)), L.ptr(L.f64(v_b)), L.ptr(r),
+                                              L.ptr(b), C.byref(rel)))
+        return rel.value, r, b
+
+    # -- drop-in entry points ---------------------------------------------------
+    def solve_SSA(self, Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice=None, u_b=None, v_b=None,
+                  BC_prescr_mask_b=None, BC_prescr_u_b=None, BC_prescr_v_b=None):
+        """Returns (u_b, v_b, n_visc_its, n_Axb_its, info)."""
+        nT = self.md.mesh.nT
+        u = L.f64(np.zeros(nT) if u_b is None else u_b).copy()
+        v = L.f64(np.zeros(nT) if v_b is None else v_b).copy()
+        self.upload(Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice,
+                    BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b)
+        self.set_velocity(u, v)
+        info = self.run("SSA")
+        u, v = self.get_velocity()
+        return u, v, info.n_visc_its, info.n_Axb_its, info
+
+    def solve_DIVA(self, Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice=None, state=None,
+                   BC_prescr_mask_b=None, BC_prescr_u_b=None, BC_prescr_v_b=None):
+        """state: dict of the DIVA solver fields carried between calls
+        (u_vav_b, v_vav_b, u_base_b, v_base_b, eta_3D_b, tau_bx_b, tau_by_b);
+        returns (state, n_visc_its, n_Axb_its, info) with u_3D_b, v_3D_b added."""
+        nT, nz = self.md.mesh.nT, len(self.md.zeta)
+        st = {k: np.zeros(nT) for k in ("u_vav_b", "v_vav_b", "u_base_b", "v_base_b", "tau_bx_b", "tau_by_b")}
+        st["eta_3D_b"] = np.zeros(nT * nz)
+        for k, a in (state or {}).items():
+            if k in st:
+                st[k] = L.f64(a).copy()
+        self.upload(Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice,
+                    BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b)
+        self.set_velocity(st["u_vav_b"], st["v_vav_b"])
+        self._check(self.lib.ufe_upload_diva_state(self._ctx, L.ptr(st["eta_3D_b"]), L.ptr(st["tau_bx_b"]),
+                                                   L.ptr(st["tau_by_b"]), L.ptr(st["u_base_b"]),
+                                                   L.ptr(st["v_base_b"])))
+        info = self.run("DIVA")
+        st["u_vav_b"], st["v_vav_b"] = self.get_velocity()
+        st["u_3D_b"] = np.zeros(nT * nz)
+        st["v_3D_b"] = np.zeros(nT * nz)
+        self._check(self.lib.ufe_download_diva(self._ctx, L.ptr(st["u_base_b"]), L.ptr(st["v_base_b"]),
+                                               L.ptr(st["u_3D_b"]), L.ptr(st["v_3D_b"]), L.ptr(st["eta_3D_b"]),
+                                               L.ptr(st["tau_bx_b"]), L.ptr(st["tau_by_b"])))
+        return st, info.n_visc_its, info.n_Axb_its, info
+
+    def fraction_gr_b(self, Hi, Hb, SL):
+        out = np.zeros(self.md.mesh.nT)
+        self._check(self.lib.ufe_fraction_gr_b(self._ctx, L.ptr(L.f64(Hi)), L.ptr(L.f64(Hb)), L.ptr(L.f64(SL)),
+                                               L.ptr(out)))
+        return out
+
+    def close(self):
+        if self._ctx:
+            self.lib.ufe_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_heat_inputs(nV, nz, zeta, fields: dict, dt: float, Hi_min_thermo: float = 10.0):
+    """HeatInputs with its arrays kept alive in the returned tuple."""
+    keep = {k: (L.i32(v) if k == "mask" else L.f64(v)) for k, v in fields.items()}
+    keep["zeta"] = L.f64(zeta)
+    h = L.HeatInputs()
+    h.nV, h.nz = nV, nz
+    for name in ("zeta", "Ti", "u_3D", "v_3D", "w_3D", "u_dTdxp_upwind", "v_dTdyp_upwind", "dzeta_dx", "dzeta_dy",
+                 "dzeta_dz", "dzeta_dt", "Phi", "Hi_eff", "T_surf", "Q_base_grnd"):
+        setattr(h, name, L.ptr(keep[name]))
+    h.mask = L.ptr(keep["mask"], L.ip)
+    h.dt = dt
+    h.Hi_min_thermo = Hi_min_thermo
+    return h, keep
+
+
+def heat_columns(hin, solver: VelocitySolver | None = None):
+    """Vertical heat columns on the device; returns (Ti_next, is_unstable, n_unstable)."""
+    lib = L.lib()
+    n = hin.nV * hin.nz
+    T = np.zeros(n)
+    unst = np.zeros(hin.nV, np.int32)
+    nun = C.c_int(0)
+    ctx = solver._ctx if solver is not None else None
+    rc = lib.ufe_heat_columns(ctx, C.byref(hin), L.ptr(T), L.ptr(unst, L.ip), C.byref(nun))
+    if rc < 0:
+        raise VelocitySolverError(f"ufe_heat_columns failed ({rc}): {lib.ufe_last_error(None).decode()}")
+    return T, unst, nun.value
+
+
+def csr_matvec(A, x):
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    lib = L.lib()
+    y = np.zeros(A.shape[0])
+    ptr, ind, val = L.i32(A.indptr), L.i32(A.indices), L.f64(A.data)
+    rc = lib.ufe_csr_matvec(A.shape[0], A.shape[1], A.nnz, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val),
+                            L.ptr(L.f64(x)), L.ptr(y))
+    if rc < 0:
+        raise VelocitySolverError(lib.ufe_last_error(None).decode())
+    return y
+
+
+def csr_solve(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, precond=L.PC_BLOCK_JACOBI):
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    lib = L.lib()
+    x = L.f64(np.zeros(A.shape[0]) if x0 is None else x0).copy()
+    ptr, ind, val = L.i32(A.indptr), L.i32(A.indices), L.f64(A.data)
+    its = C.c_int(0)
+    rel = C.c_double(0)
+    rc = lib.ufe_csr_solve(A.shape[0], A.nnz, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val), L.ptr(L.f64(b)),
+                           L.ptr(x), rtol, abstol, maxit, precond, C.byref(its), C.byref(rel))
+    if rc < 0:
+        raise VelocitySolverError(lib.ufe_last_error(None).decode())
+    return x, its.value, rel.value

@@ -1,0 +1,262 @@
+"""Seeded synthetic workloads for the velocity solve and the heat columns.
+
+Each builder returns a ``Workload`` with the mesh, its operators, the zeta
+grid and the per-vertex / per-triangle inputs of solve_SSA / solve_DIVA
+(Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask) in the engine's layout.
+
+Workloads (sizes are the targets; the seed and mesh digest identify a run):
+  dome        parabolic dome, 2.5k vertices, SSA, Weertman m=3, beta_sq=1e4
+  channel     marine channel 640x80 km, bed -100 -> -700 m, H 1500 -> 200 m
+              plus a Gaussian random field, ~200k vertices, DIVA nz=10
+  continental fractal bed, plastic ice profile, ~400k vertices, nz=12
+  ensemble    members of the channel geometry with A and beta_sq drawn
+              log-uniformly, uneven resolutions
+
+Geometry relations follow ice_geometry_basics.f90 (is_floating :12-26,
+ice_surface_elevation :28-41, thickness_above_floatation :43-56) with
+rho_i = 910, rho_sw = 1028 (parameters.f90:23-25).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .mesh import Mesh, generate_mesh, uniform_mesh
+from .operators import MeshDescriptor, MeshOperators, build_operators
+
+RHO_I = 910.0
+RHO_SW = 1028.0
+SEC_PER_YEAR = 31556943.36
+
+
+def zeta_regular(nz: int) -> np.ndarray:
+    """initialise_scaled_vertical_coordinate_regular (mesh_zeta.f90:58-84)."""
+    return np.arange(nz, dtype=np.float64) / (nz - 1)
+
+
+def surface_elevation(Hi, Hb, SL):
+    return Hi + np.maximum(SL - RHO_I / RHO_SW * Hi, Hb)
+
+
+def thickness_above_floatation(Hi, Hb, SL):
+    return Hi - np.maximum(0.0, (SL - Hb) * (RHO_SW / RHO_I))
+
+
+def is_floating(Hi, Hb, SL):
+    return Hi < (SL - Hb) * RHO_SW / RHO_I
+
+
+def grounded_fraction_b(mesh: Mesh, TAF: np.ndarray) -> np.ndarray:
+    """Sub-grid grounded fraction per triangle from vertex TAF (input preparation;
+    same relations as subgrid_grounded_fractions_bilin_TAF.f90:182-370)."""
+    T = TAF[mesh.Tri]
+    P = mesh.V[mesh.Tri]
+    tol = 1e-9
+    Tc = np.where(T >= 0, np.maximum(tol, T), np.minimum(-tol, T))
+
+    def area(p, q, r):
+        return np.abs((q[..., 0] - p[..., 0]) * (r[..., 1] - p[..., 1]) -
+                      (q[..., 1] - p[..., 1]) * (r[..., 0] - p[..., 0])) / 2.0
+
+    At = area(P[:, 0], P[:, 1], P[:, 2])
+    out = np.where(T.min(axis=1) >= 0, 1.0, 0.0)
+    mixed = (T.min(axis=1) < 0) & (T.max(axis=1) > 0)
+    for i in np.nonzero(mixed)[0]:
+        a, b, c = Tc[i]
+        va, vb, vc = P[i]
+        grd = np.array([a, b, c]) >= 0
+        # rotate so that the odd one out is first
+        odd = int(np.nonzero(grd != (grd.sum() >= 2))[0][0])
+        idx = [(odd + k) % 3 for k in range(3)]
+        t = Tc[i][idx]
+        p = P[i][idx]
+        lab = t[0] / (t[0] - t[1])
+        lac = t[0] / (t[0] - t[2])
+        pab = p[0] * (1 - lab) + p[1] * lab
+        pac = p[0] * (1 - lac) + p[2] * lac
+        sub = area(p[0], pab, pac)
+        Ag = sub if grd[odd] else At[i] - sub
+        out[i] = min(1.0, max(0.0, Ag / At[i]))
+    return out
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    mesh: Mesh
+    ops: MeshOperators
+    zeta: np.ndarray
+    Hi: np.ndarray
+    Hb: np.ndarray
+    SL: np.ndarray
+    A_flow: np.ndarray        # (nz*nV,) vertex fastest
+    beta_sq: np.ndarray
+    seed: int
+    stress_balance: str = "SSA"
+    params: dict = dataclasses.field(default_factory=dict)
+    _md: MeshDescriptor = None
+
+    @property
+    def Hs(self):
+        return surface_elevation(self.Hi, self.Hb, self.SL)
+
+    @property
+    def fgr_b(self):
+        return grounded_fraction_b(self.mesh, thickness_above_floatation(self.Hi, self.Hb, self.SL))
+
+    @property
+    def mask_grounded(self):
+        return ((self.Hi > 0) & ~is_floating(self.Hi, self.Hb, self.SL)).astype(np.int32)
+
+    @property
+    def nz(self):
+        return int(len(self.zeta))
+
+    def descriptor(self) -> MeshDescriptor:
+        if self._md is None:
+            self._md = MeshDescriptor(self.mesh, self.ops, self.zeta)
+        return self._md
+
+    def inputs(self) -> dict:
+        """Arrays in the engine/oracle layout, all contiguous."""
+        d = {"Hi": np.ascontiguousarray(self.Hi), "Hs": np.ascontiguousarray(self.Hs),
+             "A_flow": np.ascontiguousarray(self.A_flow), "beta_sq": np.ascontiguousarray(self.beta_sq),
+             "fgr_b": np.ascontiguousarray(self.fgr_b), "mask_grounded": self.mask_grounded}
+        return d
+
+    def describe(self) -> dict:
+        return {"workload": self.name, "nV": self.mesh.nV, "nTri": self.mesh.nT, "nz": self.nz,
+                "unknowns": 2 * self.mesh.nT, "seed": self.seed, "mesh": self.mesh.digest(),
+                "stress_balance": self.stress_balance}
+
+
+def _grf(rng, xmin, xmax, ymin, ymax, corr, std, dx):
+    """Gaussian random field with Gaussian correlation length `corr`, sampled on a grid."""
+    nx = int(np.ceil((xmax - xmin) / dx)) + 1
+    ny = int(np.ceil((ymax - ymin) / dx)) + 1
+    px, py = 2 * nx, 2 * ny  # pad to limit wrap-around correlation
+    w = rng.standard_normal((px, py))
+    kx = np.fft.fftfreq(px, d=dx) * 2 * np.pi
+    ky = np.fft.fftfreq(py, d=dx) * 2 * np.pi
+    K2 = kx[:, None] ** 2 + ky[None, :] ** 2
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-K2 * corr ** 2 / 8.0)))[:nx, :ny]
+    f = (f - f.mean()) / f.std() * std
+    gx = xmin + dx * np.arange(nx)
+    gy = ymin + dx * np.arange(ny)
+    return gx, gy, f
+
+
+def _sample(gx, gy, f, x, y):
+    from scipy.interpolate import RegularGridInterpolator
+    return RegularGridInterpolator((gx, gy), f, bounds_error=False, fill_value=None)(np.stack([x, y], 1))
+
+
+def dome(seed: int = 1, h: float = 36e3, nz: int = 10) -> Workload:
+    """Parabolic dome H = 3000 sqrt(1 - (r/750 km)^2) on a flat bed at 0 m."""
+    mesh = uniform_mesh(-900e3, 900e3, -900e3, 900e3, h, seed=seed)
+    r = np.hypot(mesh.V[:, 0], mesh.V[:, 1])
+    Hi = 3000.0 * np.sqrt(np.clip(1.0 - (r / 750e3) ** 2, 0.0, None))
+    nV = mesh.nV
+    return Workload("dome_ssa", mesh, build_operators(mesh), zeta_regular(nz), Hi, np.zeros(nV),
+                    np.zeros(nV), np.full(nz * nV, 1e-16), np.full(nV, 1e4), seed, "SSA",
+                    {"visc_it_norm_dUV_tol": 1e-3, "rtol": 1e-8})
+
+
+def channel_geometry(x, y, rng_field):
+    gx, gy, f = rng_field
+    Hb = -100.0 - 600.0 * x / 640e3
+    Hi = 1500.0 - 1300.0 * x / 640e3 + _sample(gx, gy, f, x, y)
+    return np.maximum(Hi, 0.0), Hb
+
+
+def channel_gl_x():
+    """Grounding-line position of the mean channel profile (no random field)."""
+    x = np.linspace(0, 640e3, 64001)
+    H = 1500.0 - 1300.0 * x / 640e3
+    b = -100.0 - 600.0 * x / 640e3
+    fl = H < -b * RHO_SW / RHO_I
+    return float(x[np.argmax(fl)])
+
+
+def channel(seed: int = 2, target_nV: int = 200_000, nz: int = 10, A: float = 1e-16,
+            beta_sq: float = 1e4, stress_balance: str = "DIVA") -> Workload:
+    """Marine channel: 640 x 80 km, refined towards the grounding line.
+
+    The size field grows linearly away from the grounding line with slope 0.05
+    between h_min and h_max = 2.5 h_min; h_min is scaled so that the mesh has
+    about target_nV vertices (a 16 -> 2 km field on this domain gives only
+    ~15k vertices, so the resolution is scaled to reach the stated size).
+    """
+    rng = np.random.default_rng(seed)
+    field = _grf(rng, 0, 640e3, 0, 80e3, 20e3, 50.0, 1e3)
+    x_gl = channel_gl_x()
+    # vertices ~ area / h^2 for the square-cell quadtree; h_max dominates
+    h_max = np.sqrt(640e3 * 80e3 / target_nV) * 1.08
+    h_min = h_max / 2.5
+
+    def size(x, y):
+        return np.minimum(h_max, h_min + 0.05 * np.abs(x - x_gl))
+
+    mesh = generate_mesh(0.0, 640e3, 0.0, 80e3, size, seed=seed)
+    Hi, Hb = channel_geometry(mesh.V[:, 0], mesh.V[:, 1], field)
+    nV = mesh.nV
+    return Workload(f"channel_{stress_balance.lower()}", mesh, build_operators(mesh), zeta_regular(nz),
+                    Hi, Hb, np.zeros(nV), np.full(nz * nV, A), np.full(nV, beta_sq), seed, stress_balance,
+                    {"visc_it_norm_dUV_tol": 1e-3})
+
+
+def continental(seed: int = 3, target_nV: int = 400_000, nz: int = 12) -> Workload:
+    """Fractal bed (spectral slope -2) rescaled to [-2000, 2000] m and a
+    perfectly plastic ice profile (yield stress 100 kPa) clipped to the bed."""
+    rng = np.random.default_rng(seed)
+    L = 2000e3
+    n = 512
+    dx = 2 * L / (n - 1)
+    kx = np.fft.fftfreq(n, d=dx)
+    K = np.hypot(kx[:, None], kx[None, :])
+    K[0, 0] = np.inf
+    amp = K ** (-1.0)  # power ~ k^-2
+    spec = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * amp
+    bed = np.real(np.fft.ifft2(spec))
+    bed = -2000.0 + 4000.0 * (bed - bed.min()) / (bed.max() - bed.min())
+    g = -L + dx * np.arange(n)
+    R = 1500e3
+    tau_y = 1e5
+    h_fine = np.sqrt((2 * L) ** 2 / target_nV) * 0.55
+    h_coarse = 4.0 * h_fine
+
+    def size(x, y):
+        d = np.abs(R - np.hypot(x, y))
+        return np.minimum(h_coarse, h_fine + 0.08 * d)
+
+    mesh = generate_mesh(-L, L, -L, L, size, seed=seed)
+    x, y = mesh.V[:, 0], mesh.V[:, 1]
+    Hb = _sample(g, g, bed, x, y)
+    d = np.clip(R - np.hypot(x, y), 0.0, None)
+    Hs_pl = np.sqrt(2.0 * tau_y / (RHO_I * 9.81) * d)
+    Hi = np.where(d > 0, np.maximum(0.0, Hs_pl - np.maximum(Hb, 0.0)), 0.0)
+    nV = mesh.nV
+    return Workload("continental_diva", mesh, build_operators(mesh), zeta_regular(nz), Hi, Hb,
+                    np.zeros(nV), np.full(nz * nV, 1e-16), np.full(nV, 1e4), seed, "DIVA",
+                    {"visc_it_norm_dUV_tol": 1e-3})
+
+
+def ensemble_member(k: int, seed: int = 4, nV_range=(20_000, 120_000), nz: int = 10,
+                    stress_balance: str = "DIVA") -> Workload:
+    """Member k of the ensemble: channel geometry, uneven resolution, A and
+    beta_sq drawn log-uniformly from [1e-17, 1e-16] and [1e3, 1e5]."""
+    rng = np.random.default_rng([seed, k])
+    nV = int(rng.integers(nV_range[0], nV_range[1] + 1))
+    A = float(10 ** rng.uniform(-17, -16))
+    beta = float(10 ** rng.uniform(3, 5))
+    w = channel(seed=int(rng.integers(1 << 30)), target_nV=nV, nz=nz, A=A, beta_sq=beta,
+                stress_balance=stress_balance)
+    w.name = f"ensemble_member_{k}"
+    return w
+
+
+def channel_small(seed: int = 2, target_nV: int = 6000, nz: int = 10, stress_balance="SSA"):
+    """A few-thousand-vertex channel for tests the oracle finishes in seconds."""
+    return channel(seed=seed, target_nV=target_nV, nz=nz, stress_balance=stress_balance)
