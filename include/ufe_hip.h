@@ -51,7 +51,7 @@ extern "C" {
 #define UFE_PC_AMG                 3  /* aggregation multigrid V-cycle (coarse correction + block-Jacobi smoothing) */
 
 #define UFE_STOP_L2UV              0  /* reference stop test: L2_uv < visc_it_norm_dUV_tol */
-#define UFE_STOP_RESIDUAL          1  /* FP64 nonlinear residual ||b(u)-A(u)u||/||b(u)|| <= nonlinear_rtol */
+#define UFE_STOP_RESIDUAL          1  /* FP64 block-scaled nonlinear residual (see rel_residual) <= nonlinear_rtol */
 
 #define UFE_STRESS_SSA             0
 #define UFE_STRESS_DIVA            1
@@ -90,7 +90,7 @@ typedef struct ufe_params {
   double visc_it_relax;              /* visc_it_relax                    0.2    */
   double visc_it_norm_dUV_tol;       /* visc_it_norm_dUV_tol             5e-5   */
   double rtol;                       /* stress_balance_PETSc_rtol        1e-7   */
-  double abstol;                     /* stress_balance_PETSc_abstol      1e-5   */
+  double abstol;                     /* stress_balance_PETSc_abstol      1e-5   (block-scaled residual, m/yr) */
   int    include_crossterms;         /* do_include_SSADIVA_crossterms    1      */
   int    sliding_law;                /* choice_sliding_law               WEERTMAN */
   double slid_weertman_m;            /* slid_Weertman_m                  3      */
@@ -109,7 +109,7 @@ typedef struct ufe_params {
   int    gmres_restart;              /*                                  30     */
   int    krylov_maxit;               /* per linear solve                 10000  */
   int    warm_start;                 /* 0: zero initial guess (PETSc default), 1: previous iterate */
-  double inner_rtol;                 /* MIXED: inner FP32 solve tolerance relative to the FP64 residual */
+  double inner_rtol;                 /* MIXED: smallest residual reduction asked of one FP32 inner solve 1e-3 */
   int    stop_rule;                  /* UFE_STOP_*                       L2UV   */
   double nonlinear_rtol;             /* for UFE_STOP_RESIDUAL            1e-6   */
   int    check_every;                /* host polls Krylov convergence every k iterations */
@@ -121,10 +121,14 @@ typedef struct ufe_solve_info {
   int    n_Axb_its;                  /* Krylov iterations summed over Picard iterations     */
   int    n_refinements;              /* FP64 refinement residuals evaluated (MIXED)          */
   double L2_uv;                      /* last reference convergence norm                     */
-  double rel_residual;               /* FP64 ||b(u)-A(u)u||/||b(u)|| at the returned u       */
+  double rel_residual;               /* FP64 ||D^-1(b(u)-A(u)u)|| / ||D^-1 b(u)|| at the returned u,
+                                        D = 2x2 diagonal blocks (rows in velocity units), over
+                                        rows where the velocity limit vel_max is not active      */
   double visc_it_relax_applied;
   double epsilon_sq_0_applied;
   double seconds;                    /* wall time of the Picard loop                        */
+  int    n_clamped;                  /* triangles at the velocity limit (left out of rel_residual) */
+  int    n_fallbacks;                /* MIXED: linear solves redone in FP64 after FP32 refinement stalled */
 } ufe_solve_info;
 
 typedef struct ufe_ctx ufe_ctx;

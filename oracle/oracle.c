@@ -535,15 +535,42 @@ static void picard_assemble(picard_t *s) {
            s->Aptr, s->Aind, s->Aval, s->bb);
 }
 
-static double rel_residual(picard_t *s, const double *u, const double *v) {
-  int n = 2 * s->m->nT;
-  for (int ti = 0; ti < s->m->nT; ++ti) { s->xx[2 * ti] = u[ti]; s->xx[2 * ti + 1] = v[ti]; }
+/* Block-scaled nonlinear residual ||D^-1 (b(u) - A(u) u)|| / ||D^-1 b(u)||, D the
+ * 2x2 diagonal block of each triangle's (u, v) rows, over the rows of triangles
+ * where the velocity limit (apply_velocity_limits, SSA_DIVA_utilities.f90:110-141)
+ * is not active.  Scaling each block row by its diagonal block makes every row a
+ * velocity equation, so momentum rows and boundary rows are on one scale. */
+static double scaled_rel_residual(int nT, const int *Aptr, const int *Aind, const double *Aval,
+                                  const double *bvec, const double *x, double vel_max) {
+  int n = 2 * nT;
   double *r = malloc(sizeof(double) * n);
-  oracle_csr_matvec(n, s->Aptr, s->Aind, s->Aval, s->xx, r);
+  oracle_csr_matvec(n, Aptr, Aind, Aval, x, r);
   double rr = 0, b2 = 0;
-  for (int i = 0; i < n; ++i) { double d = s->bb[i] - r[i]; rr += d * d; b2 += s->bb[i] * s->bb[i]; }
+  for (int ti = 0; ti < nT; ++ti) {
+    double u = x[2 * ti], v = x[2 * ti + 1];
+    if (vel_max > 0 && sqrt(u * u + v * v) >= vel_max * (1.0 - 1e-9)) continue;
+    double D[4] = {0, 0, 0, 0};
+    for (int q = 0; q < 2; ++q)
+      for (int k = Aptr[2 * ti + q]; k < Aptr[2 * ti + q + 1]; ++k) {
+        int j = Aind[k];
+        if (j == 2 * ti) D[2 * q] += Aval[k];
+        if (j == 2 * ti + 1) D[2 * q + 1] += Aval[k];
+      }
+    double det = D[0] * D[3] - D[1] * D[2];
+    double i0 = D[3] / det, i1 = -D[1] / det, i2 = -D[2] / det, i3 = D[0] / det;
+    double r0 = bvec[2 * ti] - r[2 * ti], r1 = bvec[2 * ti + 1] - r[2 * ti + 1];
+    double s0 = i0 * r0 + i1 * r1, s1 = i2 * r0 + i3 * r1;
+    double c0 = i0 * bvec[2 * ti] + i1 * bvec[2 * ti + 1], c1 = i2 * bvec[2 * ti] + i3 * bvec[2 * ti + 1];
+    rr += s0 * s0 + s1 * s1;
+    b2 += c0 * c0 + c1 * c1;
+  }
   free(r);
   return b2 > 0 ? sqrt(rr / b2) : sqrt(rr);
+}
+
+static double rel_residual(picard_t *s, const double *u, const double *v) {
+  for (int ti = 0; ti < s->m->nT; ++ti) { s->xx[2 * ti] = u[ti]; s->xx[2 * ti + 1] = v[ti]; }
+  return scaled_rel_residual(s->m->nT, s->Aptr, s->Aind, s->Aval, s->bb, s->xx, s->p->vel_max);
 }
 
 static int picard_alloc(picard_t *s) {
@@ -761,8 +788,10 @@ double oracle_ssa_residual(const ufe_mesh_desc *m, const ufe_params *p, const do
     if (b_out) b_out[i] = s.bb[i];
   }
   free(r);
+  double rel = scaled_rel_residual(m->nT, s.Aptr, s.Aind, s.Aval, s.bb, s.xx, p->vel_max);
+  (void)rr; (void)b2;
   picard_free(&s);
-  return b2 > 0 ? sqrt(rr / b2) : sqrt(rr);
+  return rel;
 }
 
 /* Export the assembled system at (u, v) in CSR (reference ordering); the

@@ -68,7 +68,7 @@ class SolveInfo(C.Structure):
         ("n_visc_its", C.c_int), ("n_Axb_its", C.c_int), ("n_refinements", C.c_int),
         ("L2_uv", C.c_double), ("rel_residual", C.c_double),
         ("visc_it_relax_applied", C.c_double), ("epsilon_sq_0_applied", C.c_double),
-        ("seconds", C.c_double),
+        ("seconds", C.c_double), ("n_clamped", C.c_int), ("n_fallbacks", C.c_int),
     ]
 
     def as_dict(self):

@@ -48,8 +48,11 @@ void launch_triangle(const DevMesh &m, const double *N_a, const double *beta_a, 
                      const double *fgr, const double *tdx, const double *tdy, const signed char *rowtype,
                      const double *bc_u, const double *bc_v, const PicardParams &p, double4 *tri_coef,
                      double *beta_basal_b, double *rhs, hipStream_t s);
+// Assembles S = D^-1 A (FP64, and FP32 if S32), the diagonal blocks D, identity
+// Dinv (the scaled system's block-Jacobi inverse) and b_s = D^-1 rhs.
 void launch_assemble(const DevMesh &m, const signed char *rowtype, const double4 *tri_coef, int crossterms,
-                     double4 *A, float4 *A32, double4 *Dinv, float4 *Dinv32, hipStream_t s);
+                     const double *rhs, double4 *S, float4 *S32, double4 *Dblk, float4 *Dblk32, double4 *Dinv,
+                     float4 *Dinv32, double *rhs_s, hipStream_t s);
 void launch_driving_stress(const DevMesh &m, const double *Hi, const double *Hs, double *tdx, double *tdy,
                            hipStream_t s);
 void launch_vertical_average(int nV, int nz, const double *zeta, const double *f_vk, double *out, hipStream_t s);

@@ -233,6 +233,8 @@ hipStream_t ctx_stream(const ufe_ctx *c);
 const DevMesh *ctx_mesh(const ufe_ctx *c);
 Krylov *ctx_krylov(const ufe_ctx *c);
 int ctx_fail(ufe_ctx *c, int code, const std::string &msg);
+const std::vector<int> &ctx_vperm(const ufe_ctx *c);
+const std::vector<int> &ctx_tperm(const ufe_ctx *c);
 }  // namespace ufe
 
 extern "C" int ufe_fraction_gr_b(ufe_ctx *c, const double *Hi, const double *Hb, const double *SL, double *fgr) {
@@ -243,12 +245,16 @@ extern "C" int ufe_fraction_gr_b(ufe_ctx *c, const double *Hi, const double *Hb,
     hipStream_t s = ctx_stream(c);
     DBuf dHi(sizeof(double) * m.nV), dHb(sizeof(double) * m.nV), dSL(sizeof(double) * m.nV),
         dF(sizeof(double) * m.nT);
-    UFE_HIP(hipMemcpyAsync(dHi.p, Hi, dHi.bytes, hipMemcpyHostToDevice, s));
-    UFE_HIP(hipMemcpyAsync(dHb.p, Hb, dHb.bytes, hipMemcpyHostToDevice, s));
-    UFE_HIP(hipMemcpyAsync(dSL.p, SL, dSL.bytes, hipMemcpyHostToDevice, s));
+    const std::vector<int> &vp = ctx_vperm(c), &tp = ctx_tperm(c);
+    std::vector<double> h1(m.nV), h2(m.nV), h3(m.nV), hf(m.nT);
+    for (int i = 0; i < m.nV; ++i) { h1[i] = Hi[vp[i]]; h2[i] = Hb[vp[i]]; h3[i] = SL[vp[i]]; }
+    UFE_HIP(hipMemcpyAsync(dHi.p, h1.data(), dHi.bytes, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dHb.p, h2.data(), dHb.bytes, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dSL.p, h3.data(), dSL.bytes, hipMemcpyHostToDevice, s));
     launch_fraction_gr_b(m, dHi.as<double>(), dHb.as<double>(), dSL.as<double>(), dF.as<double>(), s);
-    UFE_HIP(hipMemcpyAsync(fgr, dF.p, dF.bytes, hipMemcpyDeviceToHost, s));
+    UFE_HIP(hipMemcpyAsync(hf.data(), dF.p, dF.bytes, hipMemcpyDeviceToHost, s));
     UFE_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < m.nT; ++i) fgr[tp[i]] = hf[i];
     return UFE_OK;
   } catch (const std::exception &e) {
     return ctx_fail(c, UFE_ERR_DEVICE, e.what());

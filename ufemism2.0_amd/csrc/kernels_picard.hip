@@ -108,26 +108,20 @@ __global__ void __launch_bounds__(kBlock) k_triangle(
 // Assembly: one thread per block entry, atomic-free (each block has one
 // writer).  Rows follow calc_SSA_DIVA_stiffness_matrix_row_free (:180-329),
 // the "sans" variant (:331-479) and calc_SSA_DIVA_stiffness_matrix_row_BC
-// (:481-641, 'infinite' and 'zero'), Dirichlet rows (:94-107).  The thread
-// that owns the diagonal block also writes its 2x2 inverse for the
-// block-Jacobi preconditioner.
+// (:481-641, 'infinite' and 'zero'), Dirichlet rows (:94-107).
+//
+// The system is stored left-scaled by the inverse of its diagonal blocks,
+// S = D^-1 A and b_s = D^-1 b (block-Jacobi row scaling): every block row then
+// has the identity on its diagonal, whatever the physical scale of the row
+// (momentum rows ~ N/h^2, boundary rows ~ 1), which is what the aggregation
+// coarse correction needs.  D itself is kept so that residuals are reported
+// in the unscaled norm (r = D r_s).  The solution is unchanged by the scaling.
 // ---------------------------------------------------------------------------
-template <bool kMirror32>
-__global__ void __launch_bounds__(kBlock) k_assemble(
-    int nnzb, const int *__restrict__ row, const int *__restrict__ col,
-    const double *__restrict__ Mx, const double *__restrict__ My, const double *__restrict__ Mxx,
-    const double *__restrict__ Mxy, const double *__restrict__ Myy,
-    const signed char *__restrict__ rowtype, const int *__restrict__ TriC,
-    const double4 *__restrict__ tri_coef, int crossterms, double4 *__restrict__ A,
-    float4 *__restrict__ A32, double4 *__restrict__ Dinv, float4 *__restrict__ Dinv32) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nnzb) return;
-  int ti = row[e], tj = col[e];
-  bool diag = (ti == tj);
-  double4 c = tri_coef[ti];
-  double N = c.x, Nx = c.y, Ny = c.z, be = c.w;
-  double xx = Mxx[e], xy = Mxy[e], yy = Myy[e], dx = Mx[e], dy = My[e];
-  double blk[4];
+__device__ __forceinline__ void system_block(int ti, int tj, const signed char *__restrict__ rowtype,
+                                             const int *__restrict__ TriC, double4 c, int crossterms, double xx,
+                                             double xy, double yy, double dx, double dy, double blk[4]) {
+  const bool diag = (ti == tj);
+  const double N = c.x, Nx = c.y, Ny = c.z, be = c.w;
   for (int uv = 0; uv < 2; ++uv) {
     int rt = rowtype[2 * ti + uv];
     double a_same, a_other;  // coefficient on the same / the other component of tj
@@ -162,14 +156,41 @@ __global__ void __launch_bounds__(kBlock) k_assemble(
     if (uv == 0) { blk[0] = a_same; blk[1] = a_other; }
     else { blk[2] = a_other; blk[3] = a_same; }
   }
-  A[e] = make_double4(blk[0], blk[1], blk[2], blk[3]);
-  if (kMirror32) A32[e] = make_float4((float)blk[0], (float)blk[1], (float)blk[2], (float)blk[3]);
-  if (diag) {
-    double det = blk[0] * blk[3] - blk[1] * blk[2];
-    double id = 1.0 / det;
-    double4 inv = make_double4(blk[3] * id, -blk[1] * id, -blk[2] * id, blk[0] * id);
-    Dinv[ti] = inv;
-    if (kMirror32) Dinv32[ti] = make_float4((float)inv.x, (float)inv.y, (float)inv.z, (float)inv.w);
+}
+
+template <bool kMirror32>
+__global__ void __launch_bounds__(kBlock) k_assemble(
+    int nnzb, const int *__restrict__ row, const int *__restrict__ col, const int *__restrict__ diagpos,
+    const double *__restrict__ Mx, const double *__restrict__ My, const double *__restrict__ Mxx,
+    const double *__restrict__ Mxy, const double *__restrict__ Myy,
+    const signed char *__restrict__ rowtype, const int *__restrict__ TriC,
+    const double4 *__restrict__ tri_coef, int crossterms, const double2 *__restrict__ rhs,
+    double4 *__restrict__ S, float4 *__restrict__ S32, double4 *__restrict__ Dblk, float4 *__restrict__ Dblk32,
+    double4 *__restrict__ Dinv, float4 *__restrict__ Dinv32, double2 *__restrict__ rhs_s) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnzb) return;
+  int ti = row[e], tj = col[e];
+  double4 c = tri_coef[ti];
+  double blk[4], dg[4];
+  system_block(ti, tj, rowtype, TriC, c, crossterms, Mxx[e], Mxy[e], Myy[e], Mx[e], My[e], blk);
+  int d = diagpos[ti];
+  system_block(ti, ti, rowtype, TriC, c, crossterms, Mxx[d], Mxy[d], Myy[d], Mx[d], My[d], dg);
+  double det = dg[0] * dg[3] - dg[1] * dg[2], id = 1.0 / det;
+  double i0 = dg[3] * id, i1 = -dg[1] * id, i2 = -dg[2] * id, i3 = dg[0] * id;
+  double s0 = i0 * blk[0] + i1 * blk[2], s1 = i0 * blk[1] + i1 * blk[3];
+  double s2 = i2 * blk[0] + i3 * blk[2], s3 = i2 * blk[1] + i3 * blk[3];
+  if (ti == tj) { s0 = 1.0; s1 = 0.0; s2 = 0.0; s3 = 1.0; }
+  S[e] = make_double4(s0, s1, s2, s3);
+  if (kMirror32) S32[e] = make_float4((float)s0, (float)s1, (float)s2, (float)s3);
+  if (ti == tj) {
+    Dblk[ti] = make_double4(dg[0], dg[1], dg[2], dg[3]);
+    Dinv[ti] = make_double4(1.0, 0.0, 0.0, 1.0);
+    if (kMirror32) {
+      Dinv32[ti] = make_float4(1.0f, 0.0f, 0.0f, 1.0f);
+      Dblk32[ti] = make_float4((float)dg[0], (float)dg[1], (float)dg[2], (float)dg[3]);
+    }
+    double2 b = rhs[ti];
+    rhs_s[ti] = make_double2(i0 * b.x + i1 * b.y, i2 * b.x + i3 * b.y);
   }
 }
 
@@ -273,15 +294,18 @@ void launch_triangle(const DevMesh &m, const double *N_a, const double *beta_a, 
 }
 
 void launch_assemble(const DevMesh &m, const signed char *rowtype, const double4 *tri_coef, int crossterms,
-                     double4 *A, float4 *A32, double4 *Dinv, float4 *Dinv32, hipStream_t s) {
-  if (A32)
+                     const double *rhs, double4 *S, float4 *S32, double4 *Dblk, float4 *Dblk32, double4 *Dinv,
+                     float4 *Dinv32, double *rhs_s, hipStream_t s) {
+  const double2 *r2 = reinterpret_cast<const double2 *>(rhs);
+  double2 *rs2 = reinterpret_cast<double2 *>(rhs_s);
+  if (S32)
     hipLaunchKernelGGL(k_assemble<true>, dim3(blocks_for(m.bb_nnz)), dim3(kBlock), 0, s, m.bb_nnz, m.bb_row,
-                       m.bb_ind, m.bb_ddx, m.bb_ddy, m.bb_d2dx2, m.bb_d2dxdy, m.bb_d2dy2, rowtype, m.TriC,
-                       tri_coef, crossterms, A, A32, Dinv, Dinv32);
+                       m.bb_ind, m.bb_diag, m.bb_ddx, m.bb_ddy, m.bb_d2dx2, m.bb_d2dxdy, m.bb_d2dy2, rowtype, m.TriC,
+                       tri_coef, crossterms, r2, S, S32, Dblk, Dblk32, Dinv, Dinv32, rs2);
   else
     hipLaunchKernelGGL(k_assemble<false>, dim3(blocks_for(m.bb_nnz)), dim3(kBlock), 0, s, m.bb_nnz, m.bb_row,
-                       m.bb_ind, m.bb_ddx, m.bb_ddy, m.bb_d2dx2, m.bb_d2dxdy, m.bb_d2dy2, rowtype, m.TriC,
-                       tri_coef, crossterms, A, A32, Dinv, Dinv32);
+                       m.bb_ind, m.bb_diag, m.bb_ddx, m.bb_ddy, m.bb_d2dx2, m.bb_d2dxdy, m.bb_d2dy2, rowtype, m.TriC,
+                       tri_coef, crossterms, r2, S, S32, Dblk, Dblk32, Dinv, Dinv32, rs2);
   UFE_LAUNCH_CHECK();
 }
 

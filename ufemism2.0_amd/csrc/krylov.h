@@ -14,9 +14,9 @@
 //       aggregate graph); per Picard iteration only the Galerkin values are
 //       recomputed on the device, as atomic-free gather sums of fine blocks
 //       into coarse blocks.  One V-cycle: damped block-Jacobi pre/post
-//       smoothing, piecewise-constant prolongation, damped block-Jacobi
-//       sweeps on the coarsest level in a single workgroup.  It is a fixed
-//       linear operator, so plain BiCGStab applies.
+//       smoothing, piecewise-constant prolongation, exact solve on the
+//       coarsest level (dense inverse formed in LDS once per Picard
+//       iteration).  It is a fixed linear operator, so plain BiCGStab applies.
 #pragma once
 
 #include <memory>
@@ -94,6 +94,9 @@ class Krylov {
   // Preconditioner choice for subsequent solves.
   void set_precond(int pc) { pc_ = pc; }
   void set_diag(const int *diag) { A_diag_ = diag; }
+  // Left scaling of the systems solved next (their diagonal blocks D): the
+  // stopping tests then use ||D r_s|| instead of ||r_s||.  null = none.
+  void set_scale(const double4 *d64, const float4 *d32) { dscale64_ = d64; dscale32_ = d32; }
   int precond() const { return pc_; }
   void attach_amg(const AmgHierarchy *h) { amg_ = h; }
   // Recompute Galerkin values on all levels from the level-0 matrix.
@@ -111,7 +114,14 @@ class Krylov {
                         bool zero_guess, int check_every);
 
   // FP64 residual r = b - A x with ||r||^2 and ||b||^2 (blocking read).
-  void residual64(const BsrT<double> &A, const double *x, const double *b, double *r, double *rr, double *bb);
+  // vmax > 0: rows whose block of x has magnitude >= vmax are left out of both
+  // sums and counted in *n_clamped.
+  // Norms are those of the system given (for the left-scaled system
+  // D^-1 A, D^-1 b: the block-Jacobi-scaled residual, in velocity units).
+  // With dblk and write_unscaled, r receives the unscaled residual D r_s.
+  void residual64(const BsrT<double> &A, const double *x, const double *b, double *r, double *rr, double *bb,
+                  double vmax = 0.0, int *n_clamped = nullptr, const double4 *dblk = nullptr,
+                  int write_unscaled = 0);
 
   hipStream_t stream() const { return s_; }
   RedSlot slot(int i) const;
@@ -126,6 +136,9 @@ class Krylov {
   template <typename T> void precond_apply(const BsrT<T> &A, const T *r, T *z);
   template <typename T> void amg_setup(const BsrT<T> &A0);
   template <typename T> AmgValues<T> &amg_vals();
+  template <typename T> const void *dscale() const;
+  const double4 *dscale64_ = nullptr;
+  const float4 *dscale32_ = nullptr;
 
   int nb_;
   hipStream_t s_;
@@ -140,9 +153,11 @@ class Krylov {
   std::vector<BsrT<double>> lev64_;
   std::vector<BsrT<float>> lev32_;
   DBuf vec64_[10], vec32_[10];
+  DBuf coarse_inv_;      // dense inverse of the coarsest level (column-major)
   DBuf scal_;            // KrylovScalars
   DBuf red_part_, red_tick_, red_res_;
   KrylovScalars *h_scal_ = nullptr;  // pinned host mirror
+  int verbose_ = 0;
 };
 
 }  // namespace ufe

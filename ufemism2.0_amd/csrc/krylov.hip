@@ -2,6 +2,8 @@
 // multigrid preconditioner.  See krylov.h for the design.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -38,13 +40,14 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
     const typename VecT<T>::V4 *__restrict__ val, const typename VecT<T>::V2 *__restrict__ x,
     typename VecT<T>::V2 *__restrict__ y, const typename VecT<T>::V2 *__restrict__ b,
     const typename VecT<T>::V2 *__restrict__ w, RedSlot slot, double *__restrict__ out,
-    KrylovScalars *__restrict__ sc) {
+    KrylovScalars *__restrict__ sc, double vmax, const typename VecT<T>::V4 *__restrict__ dblk,
+    int write_unscaled) {
   using V2 = typename VecT<T>::V2;
   using V4 = typename VecT<T>::V4;
   if (sc && sc->done) return;
   const int lane = threadIdx.x % kLpr;
   const int groups = (gridDim.x * blockDim.x) / kLpr;
-  double acc[2] = {0.0, 0.0};
+  double acc[3] = {0.0, 0.0, 0.0};
   for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr; row < nb; row += groups) {
     T y0 = 0, y1 = 0;
     if (x) {
@@ -67,9 +70,24 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
         T r0 = bb.x - y0, r1 = bb.y - y1;
         V2 r;
         r.x = r0; r.y = r1;
+        if (dblk && write_unscaled) {
+          // write the residual of the unscaled system, r = D r_s (norms stay scaled)
+          V4 d = dblk[row];
+          r.x = d.x * r0 + d.y * r1;
+          r.y = d.z * r0 + d.w * r1;
+        }
         y[row] = r;
-        acc[0] += (double)r0 * r0 + (double)r1 * r1;
-        acc[1] += (double)bb.x * bb.x + (double)bb.y * bb.y;
+        // rows where the velocity limit is active are left out of the norms
+        bool clamped = false;
+        if (vmax > 0.0) {
+          V2 xv = x[row];
+          clamped = sqrt((double)xv.x * xv.x + (double)xv.y * xv.y) >= vmax * (1.0 - 1e-9);
+        }
+        if (clamped) acc[2] += 1.0;
+        else {
+          acc[0] += (double)r0 * r0 + (double)r1 * r1;
+          acc[1] += (double)bb.x * bb.x + (double)bb.y * bb.y;
+        }
       } else {
         V2 r;
         r.x = y0; r.y = y1;
@@ -86,13 +104,14 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
     }
   }
   if (OP == OP_PLAIN) return;
-  __shared__ double lds[2][kBlock / 64];
-  block_sum<2>(acc, lds);
-  double tot[2];
-  if (grid_finish<2>(slot, acc, tot)) {
+  __shared__ double lds[3][kBlock / 64];
+  block_sum<3>(acc, lds);
+  double tot[3];
+  if (grid_finish<3>(slot, acc, tot)) {
     if (OP == OP_RESID) {
       out[0] = tot[0];
       out[1] = tot[1];
+      out[2] = tot[2];
     } else if (OP == OP_DOT1) {
       // alpha = rho / (rhat, v)
       if (tot[0] == 0.0 || !std::isfinite(tot[0])) sc->done = 2;
@@ -106,14 +125,15 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
 
 template <typename T, int OP>
 static void spmv_launch(const BsrT<T> &A, const T *x, T *y, const T *b, const T *w, const RedSlot &slot,
-                        double *out, KrylovScalars *sc, hipStream_t s) {
+                        double *out, KrylovScalars *sc, hipStream_t s, double vmax = 0.0,
+                        const typename VecT<T>::V4 *dblk = nullptr, int write_unscaled = 0) {
   using V2 = typename VecT<T>::V2;
   long threads = (long)A.nb * kLpr;
   int nblk = blocks_for(threads);
   if (OP != OP_PLAIN && nblk > kMaxPartials) nblk = kMaxPartials;
   hipLaunchKernelGGL((k_bsr_spmv<T, OP>), dim3(nblk), dim3(kBlock), 0, s, A.nb, A.ptr, A.col, A.val,
                      reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y), reinterpret_cast<const V2 *>(b),
-                     reinterpret_cast<const V2 *>(w), slot, out, sc);
+                     reinterpret_cast<const V2 *>(w), slot, out, sc, vmax, dblk, write_unscaled);
   UFE_LAUNCH_CHECK();
 }
 
@@ -190,7 +210,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(int nb, typename VecT<T>::V2 
                                                    const typename VecT<T>::V2 *__restrict__ t,
                                                    typename VecT<T>::V2 *__restrict__ r,
                                                    const typename VecT<T>::V2 *__restrict__ rhat, RedSlot slot,
-                                                   KrylovScalars *__restrict__ sc) {
+                                                   KrylovScalars *__restrict__ sc,
+                                                   const typename VecT<T>::V4 *__restrict__ dblk) {
   if (sc->done) return;
   T al = (T)sc->alpha, om = (T)sc->omega;
   double acc[2] = {0.0, 0.0};
@@ -203,7 +224,14 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(int nb, typename VecT<T>::V2 
     rr.y = ss.y - om * tt.y;
     r[i] = rr;
     acc[0] += (double)rh.x * rr.x + (double)rh.y * rr.y;
-    acc[1] += (double)rr.x * rr.x + (double)rr.y * rr.y;
+    if (dblk) {
+      // stopping norm of a left-scaled system measured unscaled: ||D r_s||
+      typename VecT<T>::V4 d = dblk[i];
+      double u0 = (double)d.x * rr.x + (double)d.y * rr.y, u1 = (double)d.z * rr.x + (double)d.w * rr.y;
+      acc[1] += u0 * u0 + u1 * u1;
+    } else {
+      acc[1] += (double)rr.x * rr.x + (double)rr.y * rr.y;
+    }
   }
   __shared__ double lds[2][kBlock / 64];
   block_sum<2>(acc, lds);
@@ -310,31 +338,56 @@ __global__ void __launch_bounds__(kBlock) k_amg_presmooth(int nb, const typename
   z[i] = o;
 }
 
-// For each next-level aggregate I: rc[I] = sum over members i of (r - A z)_i,
-// and zc[I] = omega Dc^-1 rc[I] (the next level's pre-smoothing from zero).
+// q = r - A z, kLpr lanes per block row (same access pattern as the SpMV).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_amg_restrict(
-    int nc, const int *__restrict__ mem_ptr, const int *__restrict__ mem_ind, const int *__restrict__ ptr,
-    const int *__restrict__ col, const typename VecT<T>::V4 *__restrict__ val,
-    const typename VecT<T>::V2 *__restrict__ r, const typename VecT<T>::V2 *__restrict__ z,
-    typename VecT<T>::V2 *__restrict__ rc, const typename VecT<T>::V4 *__restrict__ cdinv, T omega,
-    typename VecT<T>::V2 *__restrict__ zc, const KrylovScalars *__restrict__ sc) {
+__global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restrict__ ptr, const int *__restrict__ col,
+                                                      const typename VecT<T>::V4 *__restrict__ val,
+                                                      const typename VecT<T>::V2 *__restrict__ r,
+                                                      const typename VecT<T>::V2 *__restrict__ z,
+                                                      typename VecT<T>::V2 *__restrict__ q,
+                                                      const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  const int lane = threadIdx.x % kLpr;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr;
+  if (row >= nb) return;
+  T y0 = 0, y1 = 0;
+  for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLpr) {
+    typename VecT<T>::V4 a = val[k];
+    typename VecT<T>::V2 zj = z[col[k]];
+    y0 += a.x * zj.x + a.y * zj.y;
+    y1 += a.z * zj.x + a.w * zj.y;
+  }
+#pragma unroll
+  for (int off = kLpr / 2; off > 0; off >>= 1) {
+    y0 += __shfl_down(y0, off, kLpr);
+    y1 += __shfl_down(y1, off, kLpr);
+  }
+  if (lane == 0) {
+    typename VecT<T>::V2 ri = r[row], o;
+    o.x = ri.x - y0;
+    o.y = ri.y - y1;
+    q[row] = o;
+  }
+}
+
+// rc[I] = sum of q over the members of aggregate I, and zc[I] = omega Dc^-1 rc[I]
+// (the next level's pre-smoothing from zero).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_amg_restrict(int nc, const int *__restrict__ mem_ptr,
+                                                         const int *__restrict__ mem_ind,
+                                                         const typename VecT<T>::V2 *__restrict__ q,
+                                                         typename VecT<T>::V2 *__restrict__ rc,
+                                                         const typename VecT<T>::V4 *__restrict__ cdinv, T omega,
+                                                         typename VecT<T>::V2 *__restrict__ zc,
+                                                         const KrylovScalars *__restrict__ sc) {
   if (sc && sc->done) return;
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nc) return;
   double s0 = 0, s1 = 0;
   for (int m = mem_ptr[I]; m < mem_ptr[I + 1]; ++m) {
-    int i = mem_ind[m];
-    typename VecT<T>::V2 ri = r[i];
-    double q0 = ri.x, q1 = ri.y;
-    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
-      typename VecT<T>::V4 a = val[k];
-      typename VecT<T>::V2 zj = z[col[k]];
-      q0 -= (double)a.x * zj.x + (double)a.y * zj.y;
-      q1 -= (double)a.z * zj.x + (double)a.w * zj.y;
-    }
-    s0 += q0;
-    s1 += q1;
+    typename VecT<T>::V2 qi = q[mem_ind[m]];
+    s0 += qi.x;
+    s1 += qi.y;
   }
   typename VecT<T>::V2 o;
   o.x = (T)s0; o.y = (T)s1;
@@ -348,8 +401,8 @@ __global__ void __launch_bounds__(kBlock) k_amg_restrict(
   }
 }
 
-// zout_i = zn_i + omega D^-1 (r - A zn)_i with zn = z + scale * P ec (prolongation
-// fused into the post-smoothing sweep).
+// zout_i = zn_i + omega D^-1 (r - A zn)_i with zn = z + scale * P ec: the
+// prolongation fused into the post-smoothing sweep, kLpr lanes per block row.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
     int nb, const int *__restrict__ ptr, const int *__restrict__ col, const typename VecT<T>::V4 *__restrict__ val,
@@ -358,70 +411,111 @@ __global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
     const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega, typename VecT<T>::V2 *__restrict__ zout,
     const KrylovScalars *__restrict__ sc) {
   if (sc && sc->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x % kLpr;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr;
   if (i >= nb) return;
-  typename VecT<T>::V2 ri = r[i];
-  T q0 = ri.x, q1 = ri.y;
-  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+  T y0 = 0, y1 = 0;
+  for (int k = ptr[i] + lane; k < ptr[i + 1]; k += kLpr) {
     int j = col[k];
     typename VecT<T>::V4 a = val[k];
     typename VecT<T>::V2 zj = z[j], ej = ec[agg[j]];
     T z0 = zj.x + scale * ej.x, z1 = zj.y + scale * ej.y;
-    q0 -= a.x * z0 + a.y * z1;
-    q1 -= a.z * z0 + a.w * z1;
+    y0 += a.x * z0 + a.y * z1;
+    y1 += a.z * z0 + a.w * z1;
   }
-  typename VecT<T>::V2 zi = z[i], ei = ec[agg[i]];
-  typename VecT<T>::V4 d = dinv[i];
-  typename VecT<T>::V2 o;
-  o.x = zi.x + scale * ei.x + omega * (d.x * q0 + d.y * q1);
-  o.y = zi.y + scale * ei.y + omega * (d.z * q0 + d.w * q1);
-  zout[i] = o;
+#pragma unroll
+  for (int off = kLpr / 2; off > 0; off >>= 1) {
+    y0 += __shfl_down(y0, off, kLpr);
+    y1 += __shfl_down(y1, off, kLpr);
+  }
+  if (lane == 0) {
+    typename VecT<T>::V2 ri = r[i], zi = z[i], ei = ec[agg[i]];
+    T q0 = ri.x - y0, q1 = ri.y - y1;
+    typename VecT<T>::V4 d = dinv[i];
+    typename VecT<T>::V2 o;
+    o.x = zi.x + scale * ei.x + omega * (d.x * q0 + d.y * q1);
+    o.y = zi.y + scale * ei.y + omega * (d.z * q0 + d.w * q1);
+    zout[i] = o;
+  }
 }
 
-// Coarsest level: `sweeps` damped block-Jacobi sweeps from zero, in one
-// workgroup with the iterate in LDS.  nb <= kCoarseMax.
-constexpr int kCoarseMax = 1024;
+// Coarsest level: an exact solve with the dense inverse of the coarsest
+// Galerkin matrix (nb <= kCoarseMax blocks), formed once per Picard iteration
+// by Gauss-Jordan elimination with partial pivoting in LDS.
+constexpr int kCoarseMax = 24;
+constexpr int kCoarseN = 2 * kCoarseMax;
 constexpr int kCoarseThreads = 1024;
 
 template <typename T>
-__global__ void __launch_bounds__(kCoarseThreads) k_amg_coarse(int nb, const int *__restrict__ ptr,
-                                                              const int *__restrict__ col,
-                                                              const typename VecT<T>::V4 *__restrict__ val,
-                                                              const typename VecT<T>::V4 *__restrict__ dinv,
-                                                              const typename VecT<T>::V2 *__restrict__ r, T omega,
-                                                              int sweeps, typename VecT<T>::V2 *__restrict__ z,
-                                                              const KrylovScalars *__restrict__ sc) {
-  if (sc && sc->done) return;
-  __shared__ T za[2][2 * kCoarseMax];
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    za[0][2 * i] = 0;
-    za[0][2 * i + 1] = 0;
+__global__ void __launch_bounds__(256) k_coarse_invert(int nb, const int *__restrict__ ptr, const int *__restrict__ col,
+                                                       const typename VecT<T>::V4 *__restrict__ val,
+                                                       double *__restrict__ inv_cm) {
+  __shared__ double a[kCoarseN][2 * kCoarseN + 1];
+  __shared__ int piv;
+  const int n = 2 * nb, tid = threadIdx.x;
+  for (int i = tid; i < kCoarseN * (2 * kCoarseN + 1); i += blockDim.x) (&a[0][0])[i] = 0.0;
+  __syncthreads();
+  for (int i = tid; i < nb; i += blockDim.x) {
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+      int j = col[k];
+      typename VecT<T>::V4 v = val[k];
+      a[2 * i][2 * j] = v.x; a[2 * i][2 * j + 1] = v.y;
+      a[2 * i + 1][2 * j] = v.z; a[2 * i + 1][2 * j + 1] = v.w;
+    }
+    a[2 * i][n + 2 * i] = 1.0;
+    a[2 * i + 1][n + 2 * i + 1] = 1.0;
   }
   __syncthreads();
-  int cur = 0;
-  for (int it = 0; it < sweeps; ++it) {
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      typename VecT<T>::V2 ri = r[i];
-      T q0 = ri.x, q1 = ri.y;
-      for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
-        int j = col[k];
-        typename VecT<T>::V4 a = val[k];
-        T z0 = za[cur][2 * j], z1 = za[cur][2 * j + 1];
-        q0 -= a.x * z0 + a.y * z1;
-        q1 -= a.z * z0 + a.w * z1;
-      }
-      typename VecT<T>::V4 d = dinv[i];
-      za[1 - cur][2 * i] = za[cur][2 * i] + omega * (d.x * q0 + d.y * q1);
-      za[1 - cur][2 * i + 1] = za[cur][2 * i + 1] + omega * (d.z * q0 + d.w * q1);
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      int p = k;
+      double best = fabs(a[k][k]);
+      for (int i = k + 1; i < n; ++i)
+        if (fabs(a[i][k]) > best) { best = fabs(a[i][k]); p = i; }
+      piv = p;
     }
     __syncthreads();
-    cur = 1 - cur;
+    int p = piv;
+    if (p != k)
+      for (int j = tid; j < 2 * n; j += blockDim.x) { double t = a[k][j]; a[k][j] = a[p][j]; a[p][j] = t; }
+    __syncthreads();
+    double d = a[k][k];
+    double rd = d != 0.0 ? 1.0 / d : 0.0;
+    __syncthreads();
+    for (int j = tid; j < 2 * n; j += blockDim.x) a[k][j] *= rd;
+    __syncthreads();
+    for (int e = tid; e < n * 2 * n; e += blockDim.x) {
+      int i = e / (2 * n), j = e % (2 * n);
+      if (i == k || j == k) continue;
+      a[i][j] -= a[i][k] * a[k][j];
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x)
+      if (i != k) a[i][k] = 0.0;
+    __syncthreads();
   }
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    typename VecT<T>::V2 o;
-    o.x = za[cur][2 * i];
-    o.y = za[cur][2 * i + 1];
-    z[i] = o;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    int i = e % n, j = e / n;
+    inv_cm[(size_t)j * n + i] = a[i][n + j];  // column-major
+  }
+}
+
+// z = Ainv r on the coarsest level
+template <typename T>
+__global__ void __launch_bounds__(64) k_coarse_apply(int nb, const double *__restrict__ inv_cm,
+                                                     const typename VecT<T>::V2 *__restrict__ r,
+                                                     typename VecT<T>::V2 *__restrict__ z,
+                                                     const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  __shared__ double rs[kCoarseN];
+  const int n = 2 * nb, i = threadIdx.x;
+  if (i < nb) { rs[2 * i] = r[i].x; rs[2 * i + 1] = r[i].y; }
+  __syncthreads();
+  if (i < n) {
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += inv_cm[(size_t)j * n + i] * rs[j];
+    T *zz = reinterpret_cast<T *>(z);
+    zz[i] = (T)s;
   }
 }
 
@@ -563,6 +657,7 @@ std::unique_ptr<AmgHierarchy> build_amg_hierarchy(int nT, const int *TriC, const
 }
 
 Krylov::Krylov(int nb, hipStream_t s) : nb_(nb), s_(s) {
+  if (const char *e = std::getenv("UFE_VERBOSE")) verbose_ = std::atoi(e);
   for (auto &v : vec64_) v.alloc(sizeof(double) * 2 * (size_t)nb);
   for (auto &v : vec32_) v.alloc(sizeof(float) * 2 * (size_t)nb);
   scal_.alloc(sizeof(KrylovScalars));
@@ -590,17 +685,20 @@ double *Krylov::tmp64(int i) { return vec64_[i].as<double>(); }
 float *Krylov::tmp32(int i) { return vec32_[i].as<float>(); }
 
 void Krylov::residual64(const BsrT<double> &A, const double *x, const double *b, double *r, double *rr,
-                        double *bb) {
+                        double *bb, double vmax, int *n_clamped, const double4 *dblk, int write_unscaled) {
   RedSlot sl = slot(3);
-  spmv_launch<double, OP_RESID>(A, x, r, b, nullptr, sl, sl.result, nullptr, s_);
-  double h[2];
+  spmv_launch<double, OP_RESID>(A, x, r, b, nullptr, sl, sl.result, nullptr, s_, vmax, dblk, write_unscaled);
+  double h[3];
   UFE_HIP(hipMemcpyAsync(h, sl.result, sizeof(h), hipMemcpyDeviceToHost, s_));
   UFE_HIP(hipStreamSynchronize(s_));
   *rr = h[0];
   *bb = h[1];
+  if (n_clamped) *n_clamped = (int)h[2];
 }
 
 template <> AmgValues<double> &Krylov::amg_vals<double>() { return amg64_; }
+template <> const void *Krylov::dscale<double>() const { return dscale64_; }
+template <> const void *Krylov::dscale<float>() const { return dscale32_; }
 template <> AmgValues<float> &Krylov::amg_vals<float>() { return amg32_; }
 
 template <typename T>
@@ -640,6 +738,11 @@ void Krylov::amg_setup(const BsrT<T> &A0) {
     UFE_LAUNCH_CHECK();
     lev[l] = B;
   }
+  const BsrT<T> &Cc = lev[L - 1];
+  if (!coarse_inv_.p) coarse_inv_.alloc(sizeof(double) * kCoarseN * kCoarseN);
+  hipLaunchKernelGGL(k_coarse_invert<T>, dim3(1), dim3(256), 0, s_, Cc.nb, Cc.ptr, Cc.col, Cc.val,
+                     coarse_inv_.as<double>());
+  UFE_LAUNCH_CHECK();
 }
 
 void Krylov::amg_setup64(const BsrT<double> &A0) { amg_setup<double>(A0); }
@@ -658,8 +761,8 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
   const V2 *rl = reinterpret_cast<const V2 *>(r);
   V2 *z0 = av.z[0]->template as<V2>();
   if (L == 1) {
-    hipLaunchKernelGGL(k_amg_coarse<T>, dim3(1), dim3(kCoarseThreads), 0, s_, lev[0].nb, lev[0].ptr, lev[0].col,
-                       lev[0].val, lev[0].dinv, rl, om, amg_coarse_sweeps_, reinterpret_cast<V2 *>(z), sc);
+    hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(64), 0, s_, lev[0].nb, coarse_inv_.as<double>(), rl,
+                       reinterpret_cast<V2 *>(z), sc);
     UFE_LAUNCH_CHECK();
     return;
   }
@@ -671,24 +774,26 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     bool next_coarsest = (l + 1 == L - 1);
     int nc = lev[l + 1].nb;
+hipLaunchKernelGGL(k_amg_resid<T>, dim3(blocks_for((long)lev[l].nb * kLpr)), dim3(kBlock), 0, s_, lev[l].nb,
+                       lev[l].ptr, lev[l].col, lev[l].val, rin, av.z[l]->template as<V2>(),
+                       av.zt[l]->template as<V2>(), sc);
+    UFE_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_amg_restrict<T>, dim3(blocks_for(nc)), dim3(kBlock), 0, s_, nc, P.mem_ptr.as<int>(),
-                       P.mem_ind.as<int>(), lev[l].ptr, lev[l].col, lev[l].val, rin, av.z[l]->template as<V2>(),
-                       av.r[l + 1]->template as<V2>(), lev[l + 1].dinv, om,
-                       next_coarsest ? (V2 *)nullptr : av.z[l + 1]->template as<V2>(), sc);
+                       P.mem_ind.as<int>(), av.zt[l]->template as<V2>(), av.r[l + 1]->template as<V2>(),
+                       lev[l + 1].dinv, om, next_coarsest ? (V2 *)nullptr : av.z[l + 1]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
   }
   {
     int c = L - 1;
-    hipLaunchKernelGGL(k_amg_coarse<T>, dim3(1), dim3(kCoarseThreads), 0, s_, lev[c].nb, lev[c].ptr, lev[c].col,
-                       lev[c].val, lev[c].dinv, av.r[c]->template as<V2>(), om, amg_coarse_sweeps_,
-                       av.z[c]->template as<V2>(), sc);
+    hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(64), 0, s_, lev[c].nb, coarse_inv_.as<double>(),
+                       av.r[c]->template as<V2>(), av.z[c]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
   }
   for (int l = L - 2; l >= 0; --l) {
     const AmgLevelPattern &P = *amg_->levels[l];
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     V2 *zout = l == 0 ? reinterpret_cast<V2 *>(z) : av.zt[l]->template as<V2>();
-    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(blocks_for(lev[l].nb)), dim3(kBlock), 0, s_, lev[l].nb,
+    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(blocks_for((long)lev[l].nb * kLpr)), dim3(kBlock), 0, s_, lev[l].nb,
                        lev[l].ptr, lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin,
                        av.z[l]->template as<V2>(), av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
     UFE_LAUNCH_CHECK();
@@ -722,8 +827,10 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
       hipLaunchKernelGGL(k_zero2<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x));
       UFE_LAUNCH_CHECK();
     }
-    spmv_launch<T, OP_RESID>(A, x, r, b, nullptr, sl0, sl0.result, nullptr, s_);
-    double h0[2];
+    // r (scaled) for the iteration; its unscaled norm for the stopping test
+    spmv_launch<T, OP_RESID>(A, x, r, b, nullptr, sl0, sl0.result, nullptr, s_, 0.0,
+                             reinterpret_cast<const typename VecT<T>::V4 *>(dscale<T>()));
+    double h0[3];
     UFE_HIP(hipMemcpyAsync(h0, sl0.result, sizeof(h0), hipMemcpyDeviceToHost, s_));
     UFE_HIP(hipStreamSynchronize(s_));
     double rn = std::sqrt(h0[0]);
@@ -758,19 +865,25 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
         hipLaunchKernelGGL(k_bicg_x<T>, dim3(nbx), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x),
                            reinterpret_cast<const V2 *>(phat), reinterpret_cast<const V2 *>(shat),
                            reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
-                           reinterpret_cast<const V2 *>(rhat), sl0, sc);
+                           reinterpret_cast<const V2 *>(rhat), sl0, sc,
+                           reinterpret_cast<const typename VecT<T>::V4 *>(dscale<T>()));
         UFE_LAUNCH_CHECK();
       }
       UFE_HIP(hipMemcpyAsync(h_scal_, sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s_));
       UFE_HIP(hipStreamSynchronize(s_));
+      if (verbose_ > 1)
+        std::fprintf(stderr, "[ufe:krylov%s] its %d |r| %.3e tol %.3e rho %.3e omega %.3e done %d\n",
+                     sizeof(T) == 4 ? "32" : "64", h_scal_->its, std::sqrt(h_scal_->rr), std::sqrt(h_scal_->tol2),
+                     h_scal_->rho, h_scal_->omega, h_scal_->done);
       if (h_scal_->done) finished = true;
     }
     total_its += h_scal_->its;
     st.status = h_scal_->done;
     // verify the true residual (recurrence drift, breakdown) and restart if needed
     RedSlot sl3 = slot(3);
-    spmv_launch<T, OP_RESID>(A, x, t, b, nullptr, sl3, sl3.result, nullptr, s_);
-    double hr[2];
+    spmv_launch<T, OP_RESID>(A, x, t, b, nullptr, sl3, sl3.result, nullptr, s_, 0.0,
+                             reinterpret_cast<const typename VecT<T>::V4 *>(dscale<T>()));
+    double hr[3];
     UFE_HIP(hipMemcpyAsync(hr, sl3.result, sizeof(hr), hipMemcpyDeviceToHost, s_));
     UFE_HIP(hipStreamSynchronize(s_));
     st.rel = std::sqrt(hr[0]);

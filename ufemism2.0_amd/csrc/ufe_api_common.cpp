@@ -55,7 +55,7 @@ extern "C" void ufe_params_default(ufe_params *p) {
   p->gmres_restart = 30;
   p->krylov_maxit = 10000;
   p->warm_start = 0;
-  p->inner_rtol = 1e-2;
+  p->inner_rtol = 1e-3;
   p->stop_rule = UFE_STOP_L2UV;
   p->nonlinear_rtol = 1e-6;
   p->check_every = 8;

@@ -10,6 +10,9 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -27,8 +30,10 @@ struct ufe_ctx {
   DevMesh m;
   std::vector<std::unique_ptr<DBuf>> own;  // mesh/operator storage
   std::vector<int> h_TriBI, h_TriC;
+  // locality ordering: device index -> caller index
+  std::vector<int> vperm, tperm;
   // system
-  DBuf A64, A32, Dinv64, Dinv32;
+  DBuf A64, A32, Dinv64, Dinv32, Dblk, Dblk32, rhs_s;  // left-scaled system S = D^-1 A, b_s = D^-1 b
   // inputs
   DBuf Hi, Hs, A_vk_vf, A_vk_lf, A_vav, rough, fgr, bcu, bcv, rowtype;
   std::vector<signed char> h_rowtype;
@@ -43,6 +48,7 @@ struct ufe_ctx {
   std::unique_ptr<AmgHierarchy> amg;
   std::string err;
   double timings[5] = {0, 0, 0, 0, 0};
+  int n_fallbacks = 0;
   hipEvent_t ev[8];
 };
 
@@ -161,10 +167,13 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
   UFE_LAUNCH_CHECK();
   *its = 0;
   *nref = 0;
+  // All stopping tests use the block-Jacobi-scaled residual (velocity units),
+  // the norm PETSc's left-preconditioned KSP test measures.
+  K.set_scale(nullptr, nullptr);
   if (q.precision == UFE_PREC_FP64) {
     BsrT<double> A = sys64(c);
     if (q.precond == UFE_PC_AMG) K.amg_setup64(A);
-    SolveStats st = K.bicgstab64(A, c->rhs.as<double>(), c->x.as<double>(), q.rtol, q.abstol, q.krylov_maxit,
+    SolveStats st = K.bicgstab64(A, c->rhs_s.as<double>(), c->x.as<double>(), q.rtol, q.abstol, q.krylov_maxit,
                                  !warm, q.check_every);
     *its = st.its;
     return;
@@ -174,28 +183,50 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
   BsrT<float> A32 = sys32(c);
   if (q.precond == UFE_PC_AMG) K.amg_setup32(A32);
   double rr, bb;
-  K.residual64(A, c->x.as<double>(), c->rhs.as<double>(), c->r64.as<double>(), &rr, &bb);
+  K.residual64(A, c->x.as<double>(), c->rhs_s.as<double>(), c->r64.as<double>(), &rr, &bb);
   double r0 = std::sqrt(rr);
   double target = std::max(q.rtol * r0, q.abstol);
   float *r32 = K.tmp32(8), *d32 = K.tmp32(9);
+  const double rr_start = rr;
   for (int ref = 0; ref < 20 && std::sqrt(rr) > target; ++ref) {
     launch_f64_to_f32(n, c->r64.as<double>(), r32, c->s);
     double rn = std::sqrt(rr);
-    // inner tolerance: enough to reach the target this step, at least inner_rtol
-    double inner = std::max(std::min(q.inner_rtol, 0.5 * target / rn), 1e-6);
+    // each FP32 solve reduces the residual by at most inner_rtol (well above
+    // FP32 round-off); the FP64 refinement loop takes as many steps as needed
+    double inner = std::max(0.5 * target / rn, q.inner_rtol);
     SolveStats st = K.bicgstab32(A32, r32, d32, inner, 0.0, q.krylov_maxit - *its, true, q.check_every);
     *its += st.its;
     hipLaunchKernelGGL(k_add_f32, dim3(blocks_for(n)), dim3(kBlock), 0, c->s, n, c->x.as<double>(), d32);
     UFE_LAUNCH_CHECK();
-    K.residual64(A, c->x.as<double>(), c->rhs.as<double>(), c->r64.as<double>(), &rr, &bb);
+    K.residual64(A, c->x.as<double>(), c->rhs_s.as<double>(), c->r64.as<double>(), &rr, &bb);
     ++*nref;
     if (*its >= q.krylov_maxit) break;
   }
+  if (!(rr <= rr_start) || !(std::sqrt(rr) <= std::max(target, 0.5 * r0))) {
+    // refinement did not make progress (FP32 stagnation): restart from the
+    // initial guess with the FP64 solver
+    ++c->n_fallbacks;
+    hipLaunchKernelGGL(k_copy_or_zero, dim3(blocks_for(n)), dim3(kBlock), 0, c->s, n,
+                       warm ? c->u.as<double>() : (const double *)nullptr, c->x.as<double>());
+    UFE_LAUNCH_CHECK();
+    if (q.precond == UFE_PC_AMG) K.amg_setup64(A);
+    SolveStats st = K.bicgstab64(A, c->rhs_s.as<double>(), c->x.as<double>(), q.rtol, q.abstol, q.krylov_maxit,
+                                 !warm, q.check_every);
+    *its += st.its;
+  }
 }
 
-static double rel_residual(ufe_ctx *c) {
+// Nonlinear residual at the iterate, ||D^-1 (b - A u)|| / ||D^-1 b|| with D the
+// 2x2 diagonal blocks: every block row becomes a velocity equation (m/yr), so
+// momentum rows and boundary rows count on the same scale.  Rows of triangles
+// where the velocity limit of apply_velocity_limits
+// (SSA_DIVA_utilities.f90:110-141) is active are left out: there the
+// converged iterate satisfies the limit, not the momentum balance.  The
+// unscaled residual b - A u is left in r64.
+static double rel_residual(ufe_ctx *c, int *n_clamped = nullptr) {
   double rr, bb;
-  c->kry->residual64(sys64(c), c->u.as<double>(), c->rhs.as<double>(), c->r64.as<double>(), &rr, &bb);
+  c->kry->residual64(sys64(c), c->u.as<double>(), c->rhs_s.as<double>(), c->r64.as<double>(), &rr, &bb,
+                     c->prm.vel_max, n_clamped, c->Dblk.as<double4>(), 1);
   return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
@@ -224,8 +255,9 @@ static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
   UFE_HIP(hipEventRecord(c->ev[2], s));
   bool mirror = q.precision == UFE_PREC_MIXED;
   launch_assemble(c->m, c->rowtype.as<signed char>(), c->tri_coef.as<double4>(), q.include_crossterms,
-                  c->A64.as<double4>(), mirror ? c->A32.as<float4>() : nullptr, c->Dinv64.as<double4>(),
-                  mirror ? c->Dinv32.as<float4>() : nullptr, s);
+                  c->rhs.as<double>(), c->A64.as<double4>(), mirror ? c->A32.as<float4>() : nullptr,
+                  c->Dblk.as<double4>(), mirror ? c->Dblk32.as<float4>() : nullptr, c->Dinv64.as<double4>(),
+                  mirror ? c->Dinv32.as<float4>() : nullptr, c->rhs_s.as<double>(), s);
   UFE_HIP(hipEventRecord(c->ev[3], s));
 }
 
@@ -259,16 +291,18 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
     c->diva_state_ready = true;
   }
   double L2 = 1e9, L2_prev;
-  int nit_div = 0, it = 0, n_axb = 0, n_ref = 0, rc = UFE_OK;
+  int nit_div = 0, it = 0, n_axb = 0, n_ref = 0, rc = UFE_OK, n_clamped = 0;
   double relax = q.visc_it_relax, eps0 = q.epsilon_sq_0;
   double resid = -1.0;
   bool converged = false;
+  const bool verbose = std::getenv("UFE_VERBOSE") != nullptr;
   for (double &t : c->timings) t = 0.0;
+  c->n_fallbacks = 0;
   while (!converged) {
     ++it;
     picard_state(c, diva, eps0, false);
     if (q.stop_rule == UFE_STOP_RESIDUAL) {
-      resid = rel_residual(c);
+      resid = rel_residual(c, &n_clamped);
       if (resid <= q.nonlinear_rtol) { --it; converged = true; break; }
     }
     int its = 0, nref = 0;
@@ -298,13 +332,16 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
         break;
       }
     }
+    if (verbose)
+      std::fprintf(stderr, "[ufe] it %d resid %.3e L2 %.3e relax %.3f eps0 %.2e krylov %d refs %d clamped %d\n", it,
+                   resid, L2, relax, eps0, its, nref, n_clamped);
     if (q.stop_rule == UFE_STOP_L2UV && L2 < q.visc_it_norm_dUV_tol) converged = true;
     if (it > q.visc_it_nit) { rc = UFE_WARN_NOT_CONVERGED; break; }
   }
   if (rc >= 0 && (q.stop_rule != UFE_STOP_RESIDUAL || !converged)) {
     // state at the returned iterate (for the residual and, for DIVA, eta_3D_b/F)
     picard_state(c, diva, eps0, false);
-    resid = rel_residual(c);
+    resid = rel_residual(c, &n_clamped);
   }
   if (diva && rc >= 0) {
     // F1 on the b-grid at the returned state for calc_3D_velocities
@@ -320,8 +357,127 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
   inf.visc_it_relax_applied = relax;
   inf.epsilon_sq_0_applied = eps0;
   inf.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  inf.n_clamped = n_clamped;
+  inf.n_fallbacks = c->n_fallbacks;
   if (info) *info = inf;
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Locality ordering.  Triangles and vertices are renumbered along a Morton
+// (Z-order) curve of their positions, so neighbours in the mesh are close in
+// memory: the gathers of the operator products and of the SpMV hit the same
+// cache lines, and the greedy aggregation builds compact aggregates whose
+// members have nearby indices.  The caller's numbering is restored at the
+// boundary.
+// ---------------------------------------------------------------------------
+static uint32_t morton2(uint32_t x, uint32_t y) {
+  auto spread = [](uint32_t v) {
+    v &= 0xffff;
+    v = (v | (v << 8)) & 0x00ff00ff;
+    v = (v | (v << 4)) & 0x0f0f0f0f;
+    v = (v | (v << 2)) & 0x33333333;
+    v = (v | (v << 1)) & 0x55555555;
+    return v;
+  };
+  return spread(x) | (spread(y) << 1);
+}
+
+static std::vector<int> morton_order(int n, const double *xy) {
+  double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+  for (int i = 0; i < n; ++i) {
+    x0 = std::min(x0, xy[2 * i]); x1 = std::max(x1, xy[2 * i]);
+    y0 = std::min(y0, xy[2 * i + 1]); y1 = std::max(y1, xy[2 * i + 1]);
+  }
+  double sx = 65535.0 / std::max(x1 - x0, 1e-30), sy = 65535.0 / std::max(y1 - y0, 1e-30);
+  double sc = std::min(sx, sy);
+  std::vector<uint32_t> key(n);
+  for (int i = 0; i < n; ++i)
+    key[i] = morton2((uint32_t)((xy[2 * i] - x0) * sc), (uint32_t)((xy[2 * i + 1] - y0) * sc));
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  return order;
+}
+
+struct PermutedMesh {
+  ufe_mesh_desc desc{};
+  std::vector<int> t_n2o, t_o2n, v_n2o, v_o2n;
+  std::vector<double> V, TriGC, ab_v[3], ba_v[3], bb_v[5];
+  std::vector<int> Tri, TriC, TriBI, ab_ptr, ab_ind, ba_ptr, ba_ind, bb_ptr, bb_ind;
+};
+
+static void permute_csr(int rows, const int *ptr, const int *ind, const double *const *vals, int nv,
+                        const std::vector<int> &row_n2o, const std::vector<int> &col_o2n, std::vector<int> &optr,
+                        std::vector<int> &oind, std::vector<double> *ovals) {
+  optr.assign(rows + 1, 0);
+  oind.clear();
+  for (int q = 0; q < nv; ++q) ovals[q].clear();
+  for (int r = 0; r < rows; ++r) {
+    int o = row_n2o[r];
+    for (int k = ptr[o]; k < ptr[o + 1]; ++k) {
+      oind.push_back(col_o2n[ind[k]]);
+      for (int q = 0; q < nv; ++q) ovals[q].push_back(vals[q][k]);
+    }
+    optr[r + 1] = (int)oind.size();
+  }
+}
+
+static void permute_mesh(const ufe_mesh_desc &m, PermutedMesh &p) {
+  int nV = m.nV, nT = m.nT;
+  p.v_n2o = morton_order(nV, m.V);
+  p.t_n2o = morton_order(nT, m.TriGC);
+  p.v_o2n.assign(nV, 0);
+  p.t_o2n.assign(nT, 0);
+  for (int i = 0; i < nV; ++i) p.v_o2n[p.v_n2o[i]] = i;
+  for (int i = 0; i < nT; ++i) p.t_o2n[p.t_n2o[i]] = i;
+  p.V.resize(2 * (size_t)nV);
+  for (int i = 0; i < nV; ++i) { p.V[2 * i] = m.V[2 * p.v_n2o[i]]; p.V[2 * i + 1] = m.V[2 * p.v_n2o[i] + 1]; }
+  p.Tri.resize(3 * (size_t)nT); p.TriC.resize(3 * (size_t)nT); p.TriBI.resize(nT); p.TriGC.resize(2 * (size_t)nT);
+  for (int i = 0; i < nT; ++i) {
+    int o = p.t_n2o[i];
+    for (int q = 0; q < 3; ++q) {
+      p.Tri[3 * i + q] = p.v_o2n[m.Tri[3 * o + q]];
+      int tc = m.TriC[3 * o + q];
+      p.TriC[3 * i + q] = tc < 0 ? -1 : p.t_o2n[tc];
+    }
+    p.TriBI[i] = m.TriBI[o];
+    p.TriGC[2 * i] = m.TriGC[2 * o];
+    p.TriGC[2 * i + 1] = m.TriGC[2 * o + 1];
+  }
+  const double *ab[3] = {m.ab_map, m.ab_ddx, m.ab_ddy};
+  const double *ba[3] = {m.ba_map, m.ba_ddx, m.ba_ddy};
+  const double *bb[5] = {m.bb_ddx, m.bb_ddy, m.bb_d2dx2, m.bb_d2dxdy, m.bb_d2dy2};
+  permute_csr(nT, m.ab_ptr, m.ab_ind, ab, 3, p.t_n2o, p.v_o2n, p.ab_ptr, p.ab_ind, p.ab_v);
+  permute_csr(nV, m.ba_ptr, m.ba_ind, ba, 3, p.v_n2o, p.t_o2n, p.ba_ptr, p.ba_ind, p.ba_v);
+  permute_csr(nT, m.bb_ptr, m.bb_ind, bb, 5, p.t_n2o, p.t_o2n, p.bb_ptr, p.bb_ind, p.bb_v);
+  ufe_mesh_desc &d = p.desc;
+  d = m;
+  d.V = p.V.data(); d.Tri = p.Tri.data(); d.TriC = p.TriC.data(); d.TriBI = p.TriBI.data(); d.TriGC = p.TriGC.data();
+  d.ab_ptr = p.ab_ptr.data(); d.ab_ind = p.ab_ind.data();
+  d.ab_map = p.ab_v[0].data(); d.ab_ddx = p.ab_v[1].data(); d.ab_ddy = p.ab_v[2].data();
+  d.ba_ptr = p.ba_ptr.data(); d.ba_ind = p.ba_ind.data();
+  d.ba_map = p.ba_v[0].data(); d.ba_ddx = p.ba_v[1].data(); d.ba_ddy = p.ba_v[2].data();
+  d.bb_ptr = p.bb_ptr.data(); d.bb_ind = p.bb_ind.data();
+  d.bb_ddx = p.bb_v[0].data(); d.bb_ddy = p.bb_v[1].data(); d.bb_d2dx2 = p.bb_v[2].data();
+  d.bb_d2dxdy = p.bb_v[3].data(); d.bb_d2dy2 = p.bb_v[4].data();
+}
+
+// caller layout (n per layer, `layers` layers) -> device numbering
+template <typename U>
+static std::vector<U> to_dev(const std::vector<int> &n2o, const U *src, int layers = 1) {
+  size_t n = n2o.size();
+  std::vector<U> out(n * layers);
+  for (int k = 0; k < layers; ++k)
+    for (size_t i = 0; i < n; ++i) out[k * n + i] = src[k * n + n2o[i]];
+  return out;
+}
+
+template <typename U>
+static void from_dev(const std::vector<int> &n2o, const U *src, U *dst, int layers = 1) {
+  size_t n = n2o.size();
+  for (int k = 0; k < layers; ++k)
+    for (size_t i = 0; i < n; ++i) dst[k * n + n2o[i]] = src[k * n + i];
 }
 
 // Accessors for the other translation units (grounded fractions, heat columns).
@@ -331,6 +487,8 @@ hipStream_t ctx_stream(const ufe_ctx *c) { return c->s; }
 const DevMesh *ctx_mesh(const ufe_ctx *c) { return &c->m; }
 Krylov *ctx_krylov(const ufe_ctx *c) { return c->kry.get(); }
 int ctx_fail(ufe_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
+const std::vector<int> &ctx_vperm(const ufe_ctx *c) { return c->vperm; }
+const std::vector<int> &ctx_tperm(const ufe_ctx *c) { return c->tperm; }
 }  // namespace ufe
 
 // ===========================================================================
@@ -371,6 +529,11 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
       if (md->TriC[k] < -1 || md->TriC[k] >= nT) throw std::runtime_error("TriC out of range");
       if (md->Tri[k] < 0 || md->Tri[k] >= nV) throw std::runtime_error("Tri out of range");
     }
+    PermutedMesh pm;
+    permute_mesh(*md, pm);
+    c->vperm = pm.v_n2o;
+    c->tperm = pm.t_n2o;
+    md = &pm.desc;
     c->h_TriBI.assign(md->TriBI, md->TriBI + nT);
     c->h_TriC.assign(md->TriC, md->TriC + 3 * nT);
     m.ab_nnz = md->ab_nnz; m.ba_nnz = md->ba_nnz; m.bb_nnz = md->bb_nnz;
@@ -429,6 +592,9 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
     c->A32.alloc(sizeof(float4) * (size_t)nnz);
     c->Dinv64.alloc(sizeof(double4) * (size_t)nT);
     c->Dinv32.alloc(sizeof(float4) * (size_t)nT);
+    c->Dblk.alloc(sizeof(double4) * (size_t)nT);
+    c->Dblk32.alloc(sizeof(float4) * (size_t)nT);
+    c->rhs_s.alloc(sizeof(double) * 2 * (size_t)nT);
     size_t v2 = sizeof(double) * 2 * (size_t)nT;
     for (DBuf *b : {&c->u, &c->uprev, &c->x, &c->rhs, &c->r64, &c->ubase, &c->taub}) b->alloc(v2);
     for (DBuf *b : {&c->tdx, &c->tdy, &c->beta_basal_b, &c->F2b0, &c->fgr, &c->bcu, &c->bcv})
@@ -442,8 +608,14 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
     c->out2.alloc(sizeof(double) * 4);
     UFE_HIP(hipMemset(c->u.p, 0, v2));
     c->kry = std::make_unique<Krylov>(nT, c->s);
+    {
+      const char *e;
+      double om = (e = std::getenv("UFE_AMG_OMEGA")) ? std::atof(e) : 0.7;
+      double cs = (e = std::getenv("UFE_AMG_SCALE")) ? std::atof(e) : 1.0;
+      c->kry->set_amg_options(om, 0, cs);
+    }
     c->kry->set_diag(m.bb_diag);
-    c->amg = build_amg_hierarchy(nT, md->TriC, md->bb_ptr, ind.data(), 1024, 12);
+    c->amg = build_amg_hierarchy(nT, md->TriC, md->bb_ptr, ind.data(), 24, 16);
     if (c->amg) c->kry->attach_amg(c->amg.get());
     build_rowtypes(c.get(), nullptr);
     UFE_HIP(hipStreamSynchronize(c->s));
@@ -497,11 +669,13 @@ extern "C" int ufe_upload_inputs(ufe_ctx *c, const double *Hi, const double *Hs,
     UFE_HIP(hipSetDevice(c->device));
     int nV = c->m.nV, nT = c->m.nT, nz = c->m.nz;
     hipStream_t s = c->s;
-    h2d(c->Hi, Hi, sizeof(double) * nV, s);
-    h2d(c->Hs, Hs, sizeof(double) * nV, s);
-    h2d(c->rough, beta_sq, sizeof(double) * nV, s);
-    h2d(c->fgr, fgr, sizeof(double) * nT, s);
-    h2d(c->A_vk_vf, A_flow, sizeof(double) * (size_t)nV * nz, s);
+    auto pHi = to_dev(c->vperm, Hi), pHs = to_dev(c->vperm, Hs), pB = to_dev(c->vperm, beta_sq);
+    auto pF = to_dev(c->tperm, fgr), pA = to_dev(c->vperm, A_flow, nz);
+    h2d(c->Hi, pHi.data(), sizeof(double) * nV, s);
+    h2d(c->Hs, pHs.data(), sizeof(double) * nV, s);
+    h2d(c->rough, pB.data(), sizeof(double) * nV, s);
+    h2d(c->fgr, pF.data(), sizeof(double) * nT, s);
+    h2d(c->A_vk_vf, pA.data(), sizeof(double) * (size_t)nV * nz, s);
     launch_transpose(nz, nV, c->A_vk_vf.as<double>(), c->A_vk_lf.as<double>(), s);
     c->grounded_exists = true;
     if (mask_grounded) {
@@ -509,9 +683,12 @@ extern "C" int ufe_upload_inputs(ufe_ctx *c, const double *Hi, const double *Hs,
       for (int vi = 0; vi < nV; ++vi) if (mask_grounded[vi]) { c->grounded_exists = true; break; }
     }
     std::vector<double> zero(nT, 0.0);
-    h2d(c->bcu, bc_u ? bc_u : zero.data(), sizeof(double) * nT, s);
-    h2d(c->bcv, bc_v ? bc_v : zero.data(), sizeof(double) * nT, s);
-    build_rowtypes(c, bc_mask);
+    auto pu = bc_u ? to_dev(c->tperm, bc_u) : zero, pv = bc_v ? to_dev(c->tperm, bc_v) : zero;
+    h2d(c->bcu, pu.data(), sizeof(double) * nT, s);
+    h2d(c->bcv, pv.data(), sizeof(double) * nT, s);
+    std::vector<int> pm;
+    if (bc_mask) pm = to_dev(c->tperm, bc_mask);
+    build_rowtypes(c, bc_mask ? pm.data() : nullptr);
     UFE_HIP(hipStreamSynchronize(s));
     c->inputs_ready = true;
     return UFE_OK;
@@ -521,7 +698,11 @@ extern "C" int ufe_upload_inputs(ufe_ctx *c, const double *Hi, const double *Hs,
 static void interleave_up(ufe_ctx *c, DBuf &dst, const double *a, const double *b) {
   int nT = c->m.nT;
   std::vector<double> h(2 * (size_t)nT);
-  for (int i = 0; i < nT; ++i) { h[2 * i] = a ? a[i] : 0.0; h[2 * i + 1] = b ? b[i] : 0.0; }
+  for (int i = 0; i < nT; ++i) {
+    int o = c->tperm[i];
+    h[2 * i] = a ? a[o] : 0.0;
+    h[2 * i + 1] = b ? b[o] : 0.0;
+  }
   UFE_HIP(hipMemcpy(dst.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
@@ -529,7 +710,11 @@ static void interleave_down(ufe_ctx *c, const DBuf &src, double *a, double *b) {
   int nT = c->m.nT;
   std::vector<double> h(2 * (size_t)nT);
   UFE_HIP(hipMemcpy(h.data(), src.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int i = 0; i < nT; ++i) { if (a) a[i] = h[2 * i]; if (b) b[i] = h[2 * i + 1]; }
+  for (int i = 0; i < nT; ++i) {
+    int o = c->tperm[i];
+    if (a) a[o] = h[2 * i];
+    if (b) b[o] = h[2 * i + 1];
+  }
 }
 
 extern "C" int ufe_upload_velocity(ufe_ctx *c, const double *u_b, const double *v_b) {
@@ -557,8 +742,10 @@ extern "C" int ufe_upload_diva_state(ufe_ctx *c, const double *eta_3D_b, const d
   return guarded(c, [&]() -> int {
     UFE_HIP(hipSetDevice(c->device));
     int nT = c->m.nT, nz = c->m.nz;
+    std::vector<double> pe;
     if (eta_3D_b) {
-      h2d(c->tmp3, eta_3D_b, sizeof(double) * (size_t)nT * nz, c->s);
+      pe = to_dev(c->tperm, eta_3D_b, nz);
+      h2d(c->tmp3, pe.data(), sizeof(double) * (size_t)nT * nz, c->s);
       launch_transpose(nz, nT, c->tmp3.as<double>(), c->eta3b.as<double>(), c->s);
     } else {
       UFE_HIP(hipMemsetAsync(c->eta3b.p, 0, c->eta3b.bytes, c->s));
@@ -587,13 +774,22 @@ extern "C" int ufe_download_diva(ufe_ctx *c, double *u_base_b, double *v_base_b,
                            c->F1b.as<double>(), c->prm.sliding_law == UFE_SLIDING_NO_SLIDING, u3.as<double>(),
                            v3.as<double>(), s);
       UFE_HIP(hipStreamSynchronize(s));
-      if (u_3D_b) UFE_HIP(hipMemcpy(u_3D_b, u3.p, u3.bytes, hipMemcpyDeviceToHost));
-      if (v_3D_b) UFE_HIP(hipMemcpy(v_3D_b, v3.p, v3.bytes, hipMemcpyDeviceToHost));
+      std::vector<double> h((size_t)nT * nz);
+      if (u_3D_b) {
+        UFE_HIP(hipMemcpy(h.data(), u3.p, u3.bytes, hipMemcpyDeviceToHost));
+        from_dev(c->tperm, h.data(), u_3D_b, nz);
+      }
+      if (v_3D_b) {
+        UFE_HIP(hipMemcpy(h.data(), v3.p, v3.bytes, hipMemcpyDeviceToHost));
+        from_dev(c->tperm, h.data(), v_3D_b, nz);
+      }
     }
     if (eta_3D_b) {
       launch_transpose(nT, nz, c->eta3b.as<double>(), c->tmp3.as<double>(), s);
       UFE_HIP(hipStreamSynchronize(s));
-      UFE_HIP(hipMemcpy(eta_3D_b, c->tmp3.p, sizeof(double) * (size_t)nT * nz, hipMemcpyDeviceToHost));
+      std::vector<double> h((size_t)nT * nz);
+      UFE_HIP(hipMemcpy(h.data(), c->tmp3.p, sizeof(double) * (size_t)nT * nz, hipMemcpyDeviceToHost));
+      from_dev(c->tperm, h.data(), eta_3D_b, nz);
     }
     return UFE_OK;
   });
@@ -776,9 +972,17 @@ extern "C" int ufe_ssa_residual(ufe_ctx *c, const double *u_b, const double *v_b
     picard_state(c, false, c->prm.epsilon_sq_0, false);
     double rel = rel_residual(c);
     if (rel_out) *rel_out = rel;
-    size_t n = 2 * (size_t)c->m.nT;
-    if (r) UFE_HIP(hipMemcpy(r, c->r64.p, n * sizeof(double), hipMemcpyDeviceToHost));
-    if (b) UFE_HIP(hipMemcpy(b, c->rhs.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    int nT = c->m.nT;
+    std::vector<double> h(2 * (size_t)nT);
+    auto down = [&](const DBuf &src, double *dst) {
+      UFE_HIP(hipMemcpy(h.data(), src.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < nT; ++i) {
+        dst[2 * c->tperm[i]] = h[2 * i];
+        dst[2 * c->tperm[i] + 1] = h[2 * i + 1];
+      }
+    };
+    if (r) down(c->r64, r);
+    if (b) down(c->rhs, b);
     return UFE_OK;
   });
 }

@@ -193,7 +193,7 @@ def channel(seed: int = 2, target_nV: int = 200_000, nz: int = 10, A: float = 1e
     field = _grf(rng, 0, 640e3, 0, 80e3, 20e3, 50.0, 1e3)
     x_gl = channel_gl_x()
     # vertices ~ area / h^2 for the square-cell quadtree; h_max dominates
-    h_max = np.sqrt(640e3 * 80e3 / target_nV) * 1.08
+    h_max = np.sqrt(640e3 * 80e3 / target_nV) * 1.03
     h_min = h_max / 2.5
 
     def size(x, y):
@@ -260,3 +260,26 @@ def ensemble_member(k: int, seed: int = 4, nV_range=(20_000, 120_000), nz: int =
 def channel_small(seed: int = 2, target_nV: int = 6000, nz: int = 10, stress_balance="SSA"):
     """A few-thousand-vertex channel for tests the oracle finishes in seconds."""
     return channel(seed=seed, target_nV=target_nV, nz=nz, stress_balance=stress_balance)
+
+
+def schoof_icestream(h: float = 16e3, seed: int = 5, nz: int = 10, A: float = 1e-18, H: float = 2000.0,
+                     dhdx: float = -3e-4, L: float = 150e3, m: float = 1.0) -> Workload:
+    """The reference's SSA_icestream integrated test: a 2 km slab on a bed
+    inclined along x over [-400, 400] km^2, Glen A = 1e-18, and a till yield
+    stress tau_y = f |y/L|^m with f = -rho_i g H dhdx
+    (idealised_geometries.f90:186-212, sliding_laws.f90:450-480,
+    SSA_icestream/config_0*.cfg).  beta_sq carries tau_y; with the Coulomb
+    law (beta = tau_y / sqrt(delta_v^2 + |u|^2)) this is the reference's
+    'idealised' sliding law for this experiment.  The ice is grounded
+    everywhere (sea level -10 km)."""
+    mesh = uniform_mesh(-400e3, 400e3, -400e3, 400e3, h, seed=seed)
+    nV = mesh.nV
+    x, y = mesh.V[:, 0], mesh.V[:, 1]
+    f = -RHO_I * 9.81 * H * dhdx
+    tau_y = f * np.abs(y / L) ** m
+    # u: 'zero' north/south, 'infinite' east/west (du/dx = 0); v: 'zero'
+    params = {"sliding_law": 3, "visc_it_relax": 0.3, "slid_delta_v": 1e-3, "epsilon_sq_0": 1e-10,
+              "bc_u": [1, 0, 1, 0], "bc_v": [1, 1, 1, 1]}
+    return Workload(f"schoof_icestream_{int(h / 1e3)}km", mesh, build_operators(mesh), zeta_regular(nz),
+                    np.full(nV, H), dhdx * x, np.full(nV, -10000.0), np.full(nz * nV, A), tau_y, seed, "SSA",
+                    params)
