@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark: converged velocity solves per second at 1e-6 relative residual,
+plain vs chosen form, on one GPU.
+
+Workload: the marine channel (640 x 80 km, bed -100 -> -700 m, H 1500 -> 200 m
+plus a seeded Gaussian random field, mesh refined 16 km -> 2 km at the grounding
+line, ~200k vertices / ~394k triangles, DIVA with 10 layers, Glen n = 3,
+A = 1e-16, Weertman m = 3, beta_sq = 1e4), synthetic and seeded.
+
+One step = one converged DIVA velocity solve from a zero initial guess (zero
+velocity and zero DIVA state, set on the device) until the FP64 block-scaled
+nonlinear residual ||D^-1(b(u) - A(u)u)|| / ||D^-1 b(u)|| <= 1e-6, A and b
+assembled at the final u.  Inputs are uploaded to HBM before the timed region.
+
+  chosen form: Anderson-mixed Picard (depth 2, mixing 0.6), warm-started FP64
+               BiCGStab with the aggregation-AMG preconditioner, rtol 0.1
+  plain form:  the reference iteration (relaxation 0.2 with the adaptive
+               relaxation rule, zero Krylov initial guess, rtol 1e-7), FP64
+               BiCGStab with the same preconditioner
+
+`value` is the chosen form's solves/s; the plain form is timed in the same run
+(`plain`), and both are checked against each other (max relative velocity
+difference).  `roofline` is the FP64 block product (the kernel that streams the
+fine-level matrix, the dominant memory traffic of every Krylov iteration),
+timed with HIP events on the solver's stream after the timed solves.
+`cpu_baseline` is the oracle's FP64 restatement of the reference iteration
+(GMRES(30) + ILU(0)) on the host cores, timed on a bounded sample of Picard
+iterations of the same workload and scaled to solves/s with the plain form's
+Picard count.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BASE = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-6, visc_it_nit=1000,
+            krylov_maxit=20000)
+CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.1, accel_depth=2, accel_mixing=0.6)
+PLAIN = dict(precision=0, precond=3, warm_start=0, rtol=1e-7)
+METRIC = "converged velocity solves/s at 1e-6 rel. residual, plain vs chosen form, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch of the block product from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", "bsr_spmv_channel200k_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def solve_once(s, info_check=True):
+    s.set_velocity(None, None)
+    s.reset_diva_state()
+    info = s.run("DIVA")
+    if info_check and not info.rel_residual <= BASE["nonlinear_rtol"]:
+        raise RuntimeError(f"solve did not converge: residual {info.rel_residual}")
+    return info
+
+
+def cpu_baseline(w, seconds, n_picard_plain):
+    """Oracle (CPU restatement of the reference's linear solve, GMRES(30) with
+    left ILU(0), one rank) on a bounded sample of the same workload.  A full
+    CPU solve of this size takes hours, so the sample is the first Picard
+    iteration's linear system (assembled at zero velocity): two truncated
+    GMRES runs give the time per iteration and the residual reduction rate,
+    extrapolated to the reference tolerance (rtol 1e-7) and multiplied by
+    the plain form's Picard count."""
+    import math
+    import numpy as np
+    from ufemism2_amd import _lib as L
+    from oracle import binding as OB
+    o = OB.Oracle(w.descriptor())
+    inp = w.inputs()
+    p = L.default_params()
+    for k, v in BASE.items():
+        setattr(p, k, v)
+    nT = w.mesh.nT
+    A, b = o.system(p, inp, np.zeros(nT), np.zeros(nT))
+    A.sort_indices()
+    t0 = time.perf_counter()
+    OB.gmres_ilu0_stats(A, b, rtol=1e-30, maxit=30)
+    t_it = (time.perf_counter() - t0) / 30
+    m1 = max(30, int(seconds / 3.0 / t_it / 30) * 30)
+    t0 = time.perf_counter()
+    _, i1, r1 = OB.gmres_ilu0_stats(A, b, rtol=1e-30, maxit=m1)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, i2, r2 = OB.gmres_ilu0_stats(A, b, rtol=1e-30, maxit=2 * m1)
+    t2 = time.perf_counter() - t0
+    per_it = t2 / i2
+    rate = (r2 / r1) ** (1.0 / max(i2 - i1, 1))
+    r0 = r1 / rate ** i1
+    its_needed = math.log(1e-7) / math.log(rate) if rate < 1.0 else float("inf")
+    t_solve = per_it * its_needed * n_picard_plain
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"oracle GMRES(30)+ILU(0) on the first Picard system ({2 * nT} unknowns): {i1}+{i2} "
+                      f"iterations in {t1 + t2:.1f} s, {per_it * 1e3:.1f} ms/iteration, residual rate "
+                      f"{rate:.5f}/iteration -> {its_needed:.0f} iterations to rtol 1e-7 (r0 {r0:.3g}), times "
+                      f"{n_picard_plain} Picard iterations of the plain form (estimate, not a full solve)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed chosen-form solves")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed plain-form solves")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample length")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--nv", type=int, default=200_000, help="target vertices of the channel mesh")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the velocity solve runs on a single MI355X here; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import __graft_entry__ as ge
+    ge.load_package()
+    from ufemism2_amd import velocity as V, workloads as W
+
+    t0 = time.perf_counter()
+    w = W.channel(target_nV=args.nv, stress_balance="DIVA")
+    md, inp = w.descriptor(), w.inputs()
+    log(f"[bench] workload {w.describe()} built in {time.perf_counter() - t0:.1f}s")
+    args_up = (inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+
+    s = V.VelocitySolver(md, 0, **BASE, **CHOSEN)
+    s.upload(*args_up)
+    for i in range(args.warmup):
+        info = solve_once(s)
+        log(f"[bench] warmup {i}: picard {info.n_visc_its} krylov {info.n_Axb_its} "
+            f"residual {info.rel_residual:.2e} {info.seconds:.2f}s")
+    infos = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        infos.append(solve_once(s))
+        log(f"[bench] step {i}: picard {infos[-1].n_visc_its} krylov {infos[-1].n_Axb_its} "
+            f"residual {infos[-1].rel_residual:.2e} {infos[-1].seconds:.2f}s")
+    t_chosen = time.perf_counter() - t0
+    u_c, v_c = s.get_velocity()
+    spmv_ms, spmv_bytes = s.profile_spmv(args.spmv_reps)
+    s.close()
+
+    p = V.VelocitySolver(md, 0, **BASE, **PLAIN)
+    p.upload(*args_up)
+    pinfos = []
+    t0 = time.perf_counter()
+    for i in range(args.plain_steps):
+        pinfos.append(solve_once(p))
+        log(f"[bench] plain {i}: picard {pinfos[-1].n_visc_its} krylov {pinfos[-1].n_Axb_its} "
+            f"residual {pinfos[-1].rel_residual:.2e} {pinfos[-1].seconds:.2f}s")
+    t_plain = time.perf_counter() - t0
+    u_p, v_p = p.get_velocity()
+    p.close()
+    scale = max(np.abs(u_p).max(), np.abs(v_p).max())
+    diff = max(np.abs(u_c - u_p).max(), np.abs(v_c - v_p).max()) / scale
+
+    value = args.steps / t_chosen
+    plain_value = args.plain_steps / t_plain if args.plain_steps else None
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_chosen / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "channel_diva_200k", "nV": w.mesh.nV, "nTri": w.mesh.nT, "nz": w.nz,
+                   "unknowns": 2 * w.mesh.nT, "seed": w.seed, "mesh": w.mesh.digest(),
+                   "stop": "block-scaled nonlinear residual <= 1e-6",
+                   "chosen": "Anderson(2, 0.6) Picard, FP64 AMG-BiCGStab warm, rtol 0.1",
+                   "plain": "relaxed Picard (reference rule), FP64 AMG-BiCGStab zero guess, rtol 1e-7"},
+        "chosen": {"picard": [i.n_visc_its for i in infos], "krylov": [i.n_Axb_its for i in infos],
+                   "residual": [i.rel_residual for i in infos]},
+        "plain": {"value": plain_value, "unit": "solves/s",
+                  "ms_per_step": 1e3 * t_plain / max(args.plain_steps, 1),
+                  "picard": [i.n_visc_its for i in pinfos], "krylov": [i.n_Axb_its for i in pinfos],
+                  "residual": [i.rel_residual for i in pinfos]},
+        "speedup_vs_plain": value / plain_value if plain_value else None,
+        "max_rel_vel_diff_chosen_vs_plain": diff,
+        "roofline": {"kernel": "k_bsr_spmv<double, OP_PLAIN>", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "avg_ms": spmv_ms, "algorithmic_bytes": spmv_bytes},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        n_plain = pinfos[0].n_visc_its if pinfos else 231
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, n_plain)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -113,7 +113,19 @@ typedef struct ufe_params {
   int    stop_rule;                  /* UFE_STOP_*                       L2UV   */
   double nonlinear_rtol;             /* for UFE_STOP_RESIDUAL            1e-6   */
   int    check_every;                /* host polls Krylov convergence every k iterations */
+  /* Nonlinear acceleration.  accel_depth = 0 keeps the reference's relaxed
+   * Picard iteration with its adaptive relaxation rule.  accel_depth = m > 0
+   * replaces the relaxation step by Anderson mixing over the last m Picard
+   * residuals f = G(u) - u (G = one linearised solve plus velocity limits)
+   * with mixing factor accel_mixing; it needs stop_rule = UFE_STOP_RESIDUAL,
+   * and the history restarts when the nonlinear residual exceeds
+   * accel_restart times the best one seen.  m <= UFE_ACCEL_MAX_DEPTH. */
+  int    accel_depth;                /*                                  0      */
+  double accel_mixing;               /*                                  0.5    */
+  double accel_restart;              /*                                  10     */
 } ufe_params;
+
+#define UFE_ACCEL_MAX_DEPTH 8
 
 /* Per-solve diagnostics. */
 typedef struct ufe_solve_info {
@@ -173,10 +185,12 @@ int ufe_solve_diva(ufe_ctx *ctx, const double *Hi, const double *Hs, const doubl
 int ufe_upload_inputs(ufe_ctx *ctx, const double *Hi, const double *Hs, const double *A_flow,
                       const double *beta_sq, const double *fraction_gr_b, const int *mask_grounded_ice,
                       const int *BC_prescr_mask_b, const double *BC_prescr_u_b, const double *BC_prescr_v_b);
+/* u_b, v_b both NULL: zero velocity set on the device (no host transfer) */
 int ufe_upload_velocity(ufe_ctx *ctx, const double *u_b, const double *v_b);
 int ufe_run(ufe_ctx *ctx, int stress_balance, ufe_solve_info *info);
 int ufe_download_velocity(ufe_ctx *ctx, double *u_b, double *v_b);
-/* DIVA state up/download (eta_3D_b, tau_b, u_base_b) and 3-D outputs */
+/* DIVA state up/download (eta_3D_b, tau_b, u_base_b) and 3-D outputs;
+ * NULL fields upload as zero */
 int ufe_upload_diva_state(ufe_ctx *ctx, const double *eta_3D_b, const double *tau_bx_b,
                           const double *tau_by_b, const double *u_base_b, const double *v_base_b);
 int ufe_download_diva(ufe_ctx *ctx, double *u_base_b, double *v_base_b, double *u_3D_b,
@@ -190,6 +204,11 @@ int ufe_ssa_residual(ufe_ctx *ctx, const double *u_b, const double *v_b, double 
 /* Time of the last ufe_run spent per stage, from HIP events, in ms:
  * [0] vertex/viscosity, [1] triangle/a->b, [2] assembly, [3] SpMV, [4] Krylov vector ops */
 int ufe_last_timings(const ufe_ctx *ctx, double *ms5);
+/* Measurement hook (no reference counterpart): average duration in ms, from
+ * HIP events on the context's stream, of `reps` launches of the FP64 block
+ * product y = A x with the system assembled last, and its algorithmic HBM
+ * bytes per launch (blocks + column indices + row pointers + x + y). */
+int ufe_profile_spmv(ufe_ctx *ctx, int reps, double *avg_ms, double *bytes);
 
 /* ---- linear-algebra boundary ------------------------------------------ */
 

@@ -74,6 +74,12 @@ def tridiag(a, b, c, d):
 
 
 def gmres_ilu0(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, restart=30):
+    x, its, _ = gmres_ilu0_stats(A, b, x0, rtol, abstol, maxit, restart)
+    return x, its
+
+
+def gmres_ilu0_stats(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, restart=30):
+    """(x, iterations, final preconditioned residual norm)."""
     L = _L()
     A = A.tocsr()
     ptr, ind, val = L.i32(A.indptr), L.i32(A.indices), L.f64(A.data)
@@ -82,7 +88,7 @@ def gmres_ilu0(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, restart=30):
     pres = C.c_double(0)
     its = lib().oracle_gmres_ilu0(b.size, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val), L.ptr(b),
                                   L.ptr(x), rtol, abstol, maxit, restart, int(x0 is None), C.byref(pres))
-    return x, its
+    return x, its, pres.value
 
 
 class Oracle:

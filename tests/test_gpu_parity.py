@@ -1,0 +1,261 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle, the golden
+fixtures and closed forms.  Bars: bit-exact or round-off for single
+assemblies and products; <= 1e-5 max relative velocity difference for
+converged solves (the converged-solve accuracy bar of the velocity solve)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import manufactured as MF
+from conftest import max_rel, params
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
+TIGHT = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-8, visc_it_nit=800,
+             krylov_maxit=20000)
+CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.1, accel_depth=2, accel_mixing=0.6)
+PLAIN = dict(precision=0, precond=2, warm_start=0, rtol=1e-10)
+
+
+def solve(V, w, stress=None, **kw):
+    s = V.VelocitySolver(w.descriptor(), **kw)
+    inp = w.inputs()
+    args = (inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    try:
+        if (stress or w.stress_balance) == "DIVA":
+            st, nv, na, info = s.solve_DIVA(*args)
+            return st, info
+        u, v, nv, na, info = s.solve_SSA(*args)
+        return {"u_vav_b": u, "v_vav_b": v}, info
+    finally:
+        s.close()
+
+
+# -- products and single assemblies ------------------------------------------
+
+def test_csr_matvec_ut_petsc(V):
+    from test_oracle_linear import petsc_6x6
+    A, x, y = petsc_6x6()
+    assert np.array_equal(V.csr_matvec(A, x), y)
+
+
+def test_csr_matvec_large_random(V, OB):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    n = 400_000
+    A = sp.random(n, n, density=12 / n, random_state=6, format="csr")
+    x = rng.normal(size=n)
+    assert max_rel(V.csr_matvec(A, x), OB.csr_matvec(A.indptr, A.indices, A.data, x)) < 1e-14
+
+
+def test_csr_solve_vs_direct(V, OB, L, dome):
+    import scipy.sparse.linalg as spl
+    rng = np.random.default_rng(3)
+    u = rng.normal(0, 50, dome.mesh.nT); v = rng.normal(0, 50, dome.mesh.nT)
+    A, b = OB.Oracle(dome.descriptor()).system(params(L), dome.inputs(), u, v)
+    x, its, rel = V.csr_solve(A, b, rtol=1e-12, maxit=20000)
+    xd = spl.spsolve(A.tocsc(), b)
+    assert its > 0 and max_rel(x, xd) < 1e-8
+
+
+@pytest.mark.parametrize("cross", [1, 0])
+@pytest.mark.parametrize("case", ["dome", "chan"])
+def test_ssa_residual_parity(V, OB, L, request, case, cross):
+    w = request.getfixturevalue(case)
+    inp = w.inputs()
+    s = V.VelocitySolver(w.descriptor(), include_crossterms=cross)
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    rng = np.random.default_rng(1)
+    u = rng.normal(0, 80, w.mesh.nT); v = rng.normal(0, 80, w.mesh.nT)
+    rel_g, r_g, b_g = s.residual(u, v)
+    rel_o, r_o, b_o = OB.Oracle(w.descriptor()).residual(params(L, include_crossterms=cross), inp, u, v)
+    s.close()
+    assert max_rel(b_g, b_o) < 1e-12
+    assert max_rel(r_g, r_o) < 1e-11
+    assert abs(rel_g - rel_o) < 1e-11 * rel_o
+
+
+def test_prescribed_velocity_rows(V, OB, L, dome):
+    inp = dome.inputs()
+    nT = dome.mesh.nT
+    mask = np.zeros(nT, np.int32); mask[::7] = 1
+    bu = np.linspace(-30, 30, nT); bv = np.linspace(10, -10, nT)
+    s = V.VelocitySolver(dome.descriptor())
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"], mask, bu, bv)
+    rng = np.random.default_rng(2)
+    u = rng.normal(0, 80, nT); v = rng.normal(0, 80, nT)
+    _, r_g, b_g = s.residual(u, v)
+    s.close()
+    o_inp = dict(inp, bc_mask=mask, bc_u=bu, bc_v=bv)
+    _, r_o, b_o = OB.Oracle(dome.descriptor()).residual(params(L), o_inp, u, v)
+    assert max_rel(r_g, r_o) < 1e-11
+    sel = np.nonzero(mask)[0]
+    assert np.array_equal(b_g[2 * sel], bu[sel]) and np.array_equal(r_g[2 * sel], bu[sel] - u[sel])
+
+
+@pytest.mark.parametrize("cross", [1, 0])
+def test_manufactured_rows_on_gpu(V, L, W, cross):
+    w = MF.linear_case(W)
+    u, v, ax, ay, bx, by = MF.expected_rows(w, 1e-16, 1000.0, 1e3, crossterms=bool(cross))
+    inp = w.inputs()
+    s = V.VelocitySolver(w.descriptor(), glens_n=1.0, sliding_law=1, slid_weertman_m=1.0, include_crossterms=cross)
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    _, r, b = s.residual(u, v)
+    s.close()
+    Ax = b - r
+    fr = np.nonzero(w.mesh.TriBI == 0)[0]
+    sc = np.abs(ax[fr]).max()
+    assert np.abs(Ax[2 * fr] - ax[fr]).max() < 1e-8 * sc
+    assert np.abs(Ax[2 * fr + 1] - ay[fr]).max() < 1e-8 * sc
+    assert np.allclose(b[2 * fr], bx[fr], rtol=1e-10)
+
+
+def test_full_size_residual_property(V, OB, L, W):
+    """At the benchmark size (~200k vertices) one assembly + product on the GPU
+    equals the oracle's (size-independent check of every per-row kernel)."""
+    w = W.channel(target_nV=200_000, stress_balance="SSA")
+    inp = w.inputs()
+    s = V.VelocitySolver(w.descriptor(), include_crossterms=0, vel_max=1e5)
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    rng = np.random.default_rng(4)
+    u = rng.normal(100, 50, w.mesh.nT); v = rng.normal(0, 20, w.mesh.nT)
+    rel_g, r_g, b_g = s.residual(u, v)
+    s.close()
+    rel_o, r_o, b_o = OB.Oracle(w.descriptor()).residual(params(L, include_crossterms=0, vel_max=1e5), inp, u, v)
+    assert max_rel(b_g, b_o) < 1e-12 and max_rel(r_g, r_o) < 1e-10
+
+
+# -- converged solves ----------------------------------------------------------
+
+def test_dome_ssa_reference_rule_matches_oracle(V, OB, L, dome):
+    """Reference stopping rule (L2_uv) and relaxation: the same iteration on
+    both sides, so the iterates agree to the linear-solve tolerance."""
+    kw = dict(visc_it_norm_dUV_tol=1e-6, rtol=1e-11, abstol=0.0, visc_it_nit=300)
+    st, info = solve(V, dome, **kw)
+    rc, uo, vo, io = OB.Oracle(dome.descriptor()).solve_ssa(params(L, **kw), dome.inputs())
+    assert rc == 0 and info.n_visc_its == io.n_visc_its
+    assert max_rel(st["u_vav_b"], uo) < 1e-6 and max_rel(st["v_vav_b"], vo) < 1e-6
+
+
+@pytest.fixture(scope="module")
+def channel_fixtures(W):
+    out = {}
+    for sb in ("SSA", "DIVA"):
+        f = np.load(os.path.join(HERE, "golden", f"oracle_channel_{sb.lower()}.npz"))
+        w = W.channel_small(target_nV=1500, stress_balance=sb)
+        assert str(f["digest"]) == w.mesh.digest()
+        out[sb] = (w, f)
+    return out
+
+
+@pytest.mark.parametrize("form", ["plain", "chosen"])
+@pytest.mark.parametrize("sb", ["SSA", "DIVA"])
+def test_channel_converged_solve_matches_oracle(V, channel_fixtures, sb, form):
+    w, f = channel_fixtures[sb]
+    st, info = solve(V, w, **TIGHT, **(PLAIN if form == "plain" else CHOSEN))
+    assert info.rel_residual <= 1e-8
+    sc = max(np.abs(f["u"]).max(), np.abs(f["v"]).max())
+    err = max(np.abs(st["u_vav_b"] - f["u"]).max(), np.abs(st["v_vav_b"] - f["v"]).max()) / sc
+    assert err < 1e-5, err
+    if sb == "DIVA":
+        assert max_rel(st["u_base_b"], f["u_base"]) < 1e-5
+        assert max_rel(st["tau_bx_b"], f["tau_bx"]) < 1e-4
+        nT, nz = w.mesh.nT, w.nz
+        u3 = st["u_3D_b"].reshape(nz, nT)
+        assert max_rel(u3[0], f["u3d_surface"]) < 1e-5 and max_rel(u3[-1], f["u3d_base"]) < 1e-5
+
+
+def test_chosen_form_is_deterministic(V, channel_fixtures):
+    w, _ = channel_fixtures["DIVA"]
+    a, ia = solve(V, w, **TIGHT, **CHOSEN)
+    b, ib = solve(V, w, **TIGHT, **CHOSEN)
+    assert ia.n_visc_its == ib.n_visc_its and ia.n_Axb_its == ib.n_Axb_its
+    assert np.array_equal(a["u_vav_b"], b["u_vav_b"]) and np.array_equal(a["v_vav_b"], b["v_vav_b"])
+
+
+def test_mixed_precision_matches_fp64(V, dome):
+    kw = dict(stop_rule=1, nonlinear_rtol=1e-8, visc_it_nit=300, abstol=0.0, rtol=1e-10)
+    a, _ = solve(V, dome, precision=0, **kw)
+    b, ib = solve(V, dome, precision=1, inner_rtol=0.1, **kw)
+    assert ib.n_refinements > 0
+    assert max_rel(b["u_vav_b"], a["u_vav_b"]) < 1e-5 and max_rel(b["v_vav_b"], a["v_vav_b"]) < 1e-5
+
+
+def test_schoof_icestream_gpu(V, L, W):
+    """The reference's SSA_icestream check on the GPU with its own settings
+    (L2 stop 5e-8, relaxation 0.3): RMSE against the closed form."""
+    from test_oracle_picard import schoof_u
+    g = GOLD["schoof_icestream"]
+    w = W.schoof_icestream(h=16e3)
+    kw = dict(w.params, visc_it_norm_dUV_tol=g["visc_it_norm_dUV_tol"], visc_it_nit=g["visc_it_nit"])
+    st, info = solve(V, w, **kw)
+    ua = schoof_u(w.mesh.TriGC[:, 1], g["A"], g["H"], g["dhdx"], g["L"], g["m"])
+    rmse = np.sqrt(np.mean((st["u_vav_b"] - ua) ** 2))
+    assert rmse < 2.0 * g["reference_rmse"]["32km"], rmse
+
+
+# -- edge cases ----------------------------------------------------------------
+
+def test_no_grounded_ice_gives_zero_velocity(V, dome):
+    inp = dome.inputs()
+    s = V.VelocitySolver(dome.descriptor())
+    u, v, nv, na, info = s.solve_SSA(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"],
+                                     np.zeros(dome.mesh.nV, np.int32), u_b=np.ones(dome.mesh.nT))
+    s.close()
+    assert nv == 0 and not u.any() and not v.any()
+
+
+def test_argument_errors(V, L, dome):
+    s = V.VelocitySolver(dome.descriptor())
+    inp = dome.inputs()
+    with pytest.raises(V.VelocitySolverError):
+        s.solve_SSA(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], None,
+                    BC_prescr_mask_b=np.zeros(dome.mesh.nT, np.int32))
+    with pytest.raises(V.VelocitySolverError):
+        s.set_params(accel_depth=3, stop_rule=0)
+    with pytest.raises(V.VelocitySolverError):
+        s.set_params(accel_depth=9, stop_rule=1)
+    s.close()
+
+
+def test_not_converged_is_a_warning(V, dome):
+    with pytest.warns(V.ViscosityIterationWarning):
+        st, info = solve(V, dome, visc_it_nit=2, visc_it_norm_dUV_tol=1e-12)
+    assert info.n_visc_its == 3
+
+
+# -- grounded fractions and heat columns -----------------------------------------
+
+def test_fraction_gr_b_parity(V, OB, dome):
+    rng = np.random.default_rng(11)
+    nV = dome.mesh.nV
+    Hi, Hb, SL = rng.uniform(0, 800, nV), rng.uniform(-900, 100, nV), np.zeros(nV)
+    s = V.VelocitySolver(dome.descriptor())
+    g = s.fraction_gr_b(Hi, Hb, SL)
+    s.close()
+    o = OB.Oracle(dome.descriptor()).fraction_gr_b(Hi, Hb, SL)
+    assert np.abs(g - o).max() < 1e-12
+
+
+@pytest.mark.parametrize("nV,nz", [(1000, 10), (200_000, 12), (3, 30)])
+def test_heat_columns_parity(V, OB, W, nV, nz):
+    c = W.heat_columns_inputs(nV, nz, seed=nV)
+    f = dict(c["fields"])
+    hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], f, c["dt"])
+    T, unst, nun = V.heat_columns(hin)
+    rc, To, uo, no = OB.heat_columns(hin)
+    assert rc == 0
+    assert np.array_equal(unst, uo) and nun == no
+    assert np.abs(T - To).max() < 1e-9
+
+
+def test_heat_steady_profile_fixed_point_gpu(V):
+    from test_oracle_fgr_heat import steady_case
+    nV, nz, f, Ti = steady_case()
+    hin, keep = V.make_heat_inputs(nV, nz, f["zeta"], {k: v for k, v in f.items() if k != "zeta"}, 100.0)
+    T, unst, nun = V.heat_columns(hin)
+    assert nun == 0 and np.abs(T.reshape(nz, nV) - Ti).max() < 1e-9 * 273

@@ -1,0 +1,143 @@
+"""Grounded fractions and heat columns of the oracle, pinned by geometry and
+by closed forms."""
+
+import numpy as np
+import pytest
+
+from conftest import max_rel
+
+
+def clip_area(P, t):
+    """Area of {x in triangle P : linear interpolant of t >= 0} by clipping the
+    triangle polygon with the half-plane (Sutherland-Hodgman, one plane)."""
+    poly, val = list(P), list(t)
+    out = []
+    for i in range(3):
+        a, b = poly[i], poly[(i + 1) % 3]
+        fa, fb = val[i], val[(i + 1) % 3]
+        if fa >= 0:
+            out.append(a)
+        if (fa >= 0) != (fb >= 0):
+            s = fa / (fa - fb)
+            out.append(a + s * (b - a))
+    if len(out) < 3:
+        return 0.0
+    q = np.array(out)
+    x, y = q[:, 0], q[:, 1]
+    return 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+
+
+@pytest.fixture(scope="module")
+def fgr_case(W):
+    w = W.dome(h=60e3)
+    rng = np.random.default_rng(11)
+    nV = w.mesh.nV
+    Hi = rng.uniform(0, 800, nV)
+    Hb = rng.uniform(-900, 100, nV)
+    SL = np.zeros(nV)
+    return w, Hi, Hb, SL
+
+
+def test_fraction_gr_b_half_plane_clip(OB, W, fgr_case):
+    w, Hi, Hb, SL = fgr_case
+    got = OB.Oracle(w.descriptor()).fraction_gr_b(Hi, Hb, SL)
+    TAF = W.thickness_above_floatation(Hi, Hb, SL)
+    want = np.empty(w.mesh.nT)
+    for ti, t in enumerate(w.mesh.Tri):
+        P = w.mesh.V[t]
+        At = clip_area(P, np.ones(3))
+        want[ti] = clip_area(P, TAF[t]) / At
+    assert np.abs(got - want).max() < 1e-9
+    assert ((got > 0) & (got < 1)).sum() > 50
+
+
+def test_fraction_gr_b_monte_carlo(OB, W, fgr_case):
+    w, Hi, Hb, SL = fgr_case
+    got = OB.Oracle(w.descriptor()).fraction_gr_b(Hi, Hb, SL)
+    TAF = W.thickness_above_floatation(Hi, Hb, SL)
+    rng = np.random.default_rng(0)
+    mixed = np.nonzero((got > 0.05) & (got < 0.95))[0][:20]
+    r = rng.random((20000, 2))
+    r = np.where(r.sum(1, keepdims=True) > 1, 1 - r, r)
+    bary = np.c_[1 - r.sum(1), r]
+    for ti in mixed:
+        frac = ((bary @ TAF[w.mesh.Tri[ti]]) >= 0).mean()
+        assert abs(frac - got[ti]) < 0.015
+
+
+def test_zeta_operators_exact_on_quadratics(OB, W):
+    for nz in (5, 12, 30):
+        zeta = np.sort(np.r_[0.0, 1.0, np.random.default_rng(nz).uniform(0.02, 0.98, nz - 2)])
+        L = OB._L()
+        dl, dd, du, d2l, d2d, d2u = (np.zeros(nz) for _ in range(6))
+        OB.lib().oracle_zeta_tridiagonals(nz, L.ptr(L.f64(zeta)), *(L.ptr(a) for a in (dl, dd, du, d2l, d2d, d2u)))
+        f = 3 * zeta ** 2 - 2 * zeta + 1
+        D1 = np.diag(dd) + np.diag(dl[:-1], -1) + np.diag(du[:-1], 1)
+        D2 = np.diag(d2d) + np.diag(d2l[:-1], -1) + np.diag(d2u[:-1], 1)
+        # interior rows use the two neighbours; end rows their two one-sided neighbours
+        # which the tridiagonal keeps only in part, so compare interior rows
+        assert np.allclose((D1 @ f)[1:-1], (6 * zeta - 2)[1:-1], rtol=1e-10, atol=1e-10)
+        assert np.allclose((D2 @ f)[1:-1], 6.0, rtol=1e-9)
+
+
+def heat_inputs(L, nV, nz, fields, dt):
+    from ufemism2_amd.velocity import make_heat_inputs
+    return make_heat_inputs(nV, nz, fields["zeta"], {k: v for k, v in fields.items() if k != "zeta"}, dt)
+
+
+def steady_case(nV=64, nz=15, seed=0):
+    """Linear profiles T = T_s + s zeta with the basal gradient of the flux
+    condition, s = Q H / K(T_base), no advection or heating."""
+    rng = np.random.default_rng(seed)
+    zeta = np.linspace(0, 1, nz)
+    H = rng.uniform(300, 1500, nV)
+    Ts = rng.uniform(220, 240, nV)
+    Q = rng.uniform(0.6e6, 0.9e6, nV)
+    s = np.full(nV, 10.0)
+    for _ in range(200):  # fixed point for K evaluated at the basal temperature
+        s = Q * H / (3.101e8 * np.exp(-0.0057 * (Ts + s)))
+    assert (Ts + s < 273.16 - 8.7e-4 * H).all()
+    Ti = (Ts[None, :] + s[None, :] * zeta[:, None])
+    z0 = np.zeros((nz, nV))
+    f = {"zeta": zeta, "Ti": Ti.ravel(), "Hi_eff": H, "T_surf": Ts, "Q_base_grnd": Q,
+         "dzeta_dz": np.broadcast_to(-1.0 / H, (nz, nV)).ravel().copy(), "mask": np.ones(nV, np.int32)}
+    for k in ("u_3D", "v_3D", "w_3D", "u_dTdxp_upwind", "v_dTdyp_upwind", "dzeta_dx", "dzeta_dy", "dzeta_dt", "Phi"):
+        f[k] = z0.ravel().copy()
+    return nV, nz, f, Ti
+
+
+@pytest.mark.parametrize("dt", [1.0, 100.0, 1e6])
+def test_heat_steady_geothermal_profile_is_fixed_point(L, OB, dt):
+    nV, nz, f, Ti = steady_case()
+    hin, keep = heat_inputs(L, nV, nz, f, dt)
+    rc, T, unst, nun = OB.heat_columns(hin)
+    assert rc == 0 and nun == 0
+    assert np.abs(T.reshape(nz, nV) - Ti).max() < 1e-9 * 273
+
+
+def test_heat_relaxes_to_steady_profile(L, OB):
+    nV, nz, f, Ti = steady_case(seed=1)
+    f["Ti"] = np.broadcast_to(f["T_surf"], (nz, nV)).ravel().copy()
+    for _ in range(1500):
+        hin, keep = heat_inputs(L, nV, nz, f, 1e5)
+        rc, T, unst, nun = OB.heat_columns(hin)
+        f["Ti"] = T
+    assert max_rel(T.reshape(nz, nV), Ti) < 1e-6
+
+
+def test_heat_thin_ice_unstable_and_errors(L, OB, W):
+    c = W.heat_columns_inputs(500, 10, seed=3)
+    f = dict(c["fields"]); f["zeta"] = c["zeta"]
+    f["T_surf"] = f["T_surf"].copy(); f["T_surf"][7] = 150.0  # column pinned below 180 K
+    f["Hi_eff"] = f["Hi_eff"].copy(); f["Hi_eff"][7] = 1500.0
+    hin, keep = heat_inputs(L, 500, 10, f, 10.0)
+    rc, T, unst, nun = OB.heat_columns(hin)
+    assert rc == 0
+    assert unst[7] == 1 and nun >= 1
+    thin = np.nonzero(f["Hi_eff"] < 10.0)[0]
+    assert thin.size > 0
+    assert np.array_equal(T.reshape(10, 500)[:, thin], np.broadcast_to(f["T_surf"][thin], (10, thin.size)))
+    f["mask"] = f["mask"].copy(); f["mask"][np.nonzero(f["Hi_eff"] >= 10)[0][0]] = 0
+    hin, keep = heat_inputs(L, 500, 10, f, 10.0)
+    rc, *_ = OB.heat_columns(hin)
+    assert rc < 0

@@ -60,6 +60,9 @@ class Params(C.Structure):
         ("gmres_restart", C.c_int), ("krylov_maxit", C.c_int), ("warm_start", C.c_int),
         ("inner_rtol", C.c_double), ("stop_rule", C.c_int), ("nonlinear_rtol", C.c_double),
         ("check_every", C.c_int),
+        ("accel_depth", C.c_int),
+        ("accel_mixing", C.c_double),
+        ("accel_restart", C.c_double),
     ]
 
 
@@ -141,6 +144,7 @@ def lib():
     _sig(L, 'ufe_upload_diva_state', argtypes=[C.c_void_p, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_download_diva', argtypes=[C.c_void_p, dp, dp, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_last_timings', argtypes=[C.c_void_p, dp])
+    _sig(L, 'ufe_profile_spmv', argtypes=[C.c_void_p, C.c_int, dp, dp])
     _sig(L, 'ufe_ssa_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_csr_matvec', argtypes=[C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_csr_solve', argtypes=[C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double,
@@ -158,7 +162,7 @@ EXPORTED = [
     "ufe_params_default", "ufe_last_error", "ufe_version", "ufe_ctx_create", "ufe_ctx_destroy",
     "ufe_set_params", "ufe_solve_ssa", "ufe_solve_diva", "ufe_upload_inputs", "ufe_upload_velocity",
     "ufe_run", "ufe_download_velocity", "ufe_upload_diva_state", "ufe_download_diva",
-    "ufe_last_timings", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns",
+    "ufe_last_timings", "ufe_profile_spmv", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
 ]
 

@@ -60,6 +60,17 @@ void launch_update(int nT, const double *x, double *u, double *uprev, double vel
                    const RedSlot &slot, double *out2, const double *beta_basal_b, const double4 *tri_coef,
                    const double *F2b0, int no_sliding, double *ubase, double *taub, hipStream_t s);
 
+// Anderson mixing (kernels_anderson.hip)
+constexpr int kAAMaxDepth = UFE_ACCEL_MAX_DEPTH;
+struct AAGamma {
+  double g[kAAMaxDepth];
+};
+void launch_aa_columns(int nT, double *x, const double *u, double *fprev, double *gprev, double *dF, double *dG,
+                       long stride, int p, int have_prev, int ncols, double vel_max, const RedSlot &slot,
+                       double *out, hipStream_t s);
+void launch_aa_mix(int n, double *x, const double *u, const double *dF, const double *dG, long stride, int ncols,
+                   const AAGamma &gam, double mix, hipStream_t s);
+
 // DIVA kernels (kernels_diva.hip)
 void launch_vertex_diva(const DevMesh &m, const double *x, const double *xbase, const double *taub,
                         const double *eta3b_tk, const double *A_vk, const double *Hi, const double *rough,

@@ -123,6 +123,12 @@ class Krylov {
                   double vmax = 0.0, int *n_clamped = nullptr, const double4 *dblk = nullptr,
                   int write_unscaled = 0);
 
+  // Average duration (ms, HIP events on this solver's stream) of `reps`
+  // back-to-back launches of the FP64 block product y = A x, and the
+  // algorithmic HBM bytes of one launch (matrix blocks and column indices,
+  // row pointers, x and y once each).
+  double time_spmv64(const BsrT<double> &A, const double *x, double *y, int reps, double *bytes);
+
   hipStream_t stream() const { return s_; }
   RedSlot slot(int i) const;
 

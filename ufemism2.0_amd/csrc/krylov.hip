@@ -673,6 +673,26 @@ Krylov::~Krylov() {
   if (h_scal_) (void)hipHostFree(h_scal_);
 }
 
+double Krylov::time_spmv64(const BsrT<double> &A, const double *x, double *y, int reps, double *bytes) {
+  hipEvent_t e0, e1;
+  UFE_HIP(hipEventCreate(&e0));
+  UFE_HIP(hipEventCreate(&e1));
+  RedSlot sl = slot(3);
+  spmv_launch<double, OP_PLAIN>(A, x, y, nullptr, nullptr, sl, nullptr, nullptr, s_);  // warm caches/clocks
+  UFE_HIP(hipEventRecord(e0, s_));
+  for (int i = 0; i < reps; ++i) spmv_launch<double, OP_PLAIN>(A, x, y, nullptr, nullptr, sl, nullptr, nullptr, s_);
+  UFE_HIP(hipEventRecord(e1, s_));
+  UFE_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  UFE_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (bytes)
+    *bytes = (double)A.nnzb * (sizeof(double4) + sizeof(int)) + (double)(A.nb + 1) * sizeof(int) +
+             2.0 * A.nb * sizeof(double2);
+  return reps > 0 ? ms / reps : 0.0;
+}
+
 RedSlot Krylov::slot(int i) const {
   RedSlot r;
   r.partials = red_part_.as<double>() + (size_t)i * kMaxPartials * 4;

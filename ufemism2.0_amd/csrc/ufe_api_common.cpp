@@ -58,5 +58,8 @@ extern "C" void ufe_params_default(ufe_params *p) {
   p->inner_rtol = 1e-3;
   p->stop_rule = UFE_STOP_L2UV;
   p->nonlinear_rtol = 1e-6;
+  p->accel_depth = 0;
+  p->accel_mixing = 0.5;
+  p->accel_restart = 10.0;
   p->check_every = 8;
 }

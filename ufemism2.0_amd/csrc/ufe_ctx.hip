@@ -44,6 +44,9 @@ struct ufe_ctx {
   // DIVA
   DBuf eta3b, eta3a, F1a, F2a, F1b, F2b0, ubase, taub, tmp3;
   bool diva_state_ready = false;
+  // Anderson mixing: ring of m difference columns (dF, dG), previous f and G,
+  // reduction slot of width 2 kAAMaxDepth + 1
+  DBuf aa_dF, aa_dG, aa_fprev, aa_gprev, aa_part, aa_tick, aa_out;
   std::unique_ptr<Krylov> kry;
   std::unique_ptr<AmgHierarchy> amg;
   std::string err;
@@ -261,6 +264,109 @@ static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
   UFE_HIP(hipEventRecord(c->ev[3], s));
 }
 
+// ---------------------------------------------------------------------------
+// Anderson mixing step (accel_depth > 0): x holds G(u_k) from the linear
+// solve on entry and the mixed next iterate on exit.  The normal equations
+// (dF^T dF) gamma = dF^T f_k are at most 8x8 and solved on the host.
+// ---------------------------------------------------------------------------
+struct AndersonState {
+  int cols = 0, pos = 0;
+  bool have_prev = false;
+  double best = 1e300;
+  double H[kAAMaxDepth][kAAMaxDepth] = {};
+  int n_restarts = 0;
+};
+
+static void aa_prepare(ufe_ctx *c) {
+  size_t n = 2 * (size_t)c->m.nT;
+  size_t cols = (size_t)kAAMaxDepth * n * sizeof(double);
+  if (c->aa_dF.bytes < cols) {
+    c->aa_dF.alloc(cols);
+    c->aa_dG.alloc(cols);
+    c->aa_fprev.alloc(n * sizeof(double));
+    c->aa_gprev.alloc(n * sizeof(double));
+    c->aa_part.alloc(sizeof(double) * kMaxPartials * (2 * kAAMaxDepth + 1));
+    c->aa_tick.alloc(sizeof(unsigned int));
+    c->aa_out.alloc(sizeof(double) * (2 * kAAMaxDepth + 1));
+    UFE_HIP(hipMemsetAsync(c->aa_tick.p, 0, sizeof(unsigned int), c->s));
+  }
+}
+
+// Solves (H + lambda I) g = b by Gaussian elimination with partial pivoting;
+// false if the system is numerically singular.
+static bool small_solve(int m, const double (&H)[kAAMaxDepth][kAAMaxDepth], const double *b, double *g) {
+  double a[kAAMaxDepth][kAAMaxDepth + 1];
+  double tr = 0.0;
+  for (int i = 0; i < m; ++i) tr = std::max(tr, H[i][i]);
+  if (!(tr > 0.0)) return false;
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < m; ++j) a[i][j] = H[i][j] + (i == j ? 1e-12 * tr : 0.0);
+    a[i][m] = b[i];
+  }
+  for (int k = 0; k < m; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < m; ++i)
+      if (std::fabs(a[i][k]) > std::fabs(a[piv][k])) piv = i;
+    if (!(std::fabs(a[piv][k]) > 1e-14 * tr)) return false;
+    if (piv != k)
+      for (int j = 0; j <= m; ++j) std::swap(a[k][j], a[piv][j]);
+    for (int i = k + 1; i < m; ++i) {
+      double f = a[i][k] / a[k][k];
+      for (int j = k; j <= m; ++j) a[i][j] -= f * a[k][j];
+    }
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double t = a[i][m];
+    for (int j = i + 1; j < m; ++j) t -= a[i][j] * g[j];
+    g[i] = t / a[i][i];
+  }
+  return true;
+}
+
+static void aa_step(ufe_ctx *c, AndersonState &aa, double resid) {
+  const ufe_params &q = c->prm;
+  const int m = q.accel_depth;
+  const int nT = c->m.nT;
+  const long stride = 2L * nT;
+  if (resid > q.accel_restart * aa.best) {
+    aa.cols = 0;
+    aa.pos = 0;
+    ++aa.n_restarts;
+  }
+  aa.best = std::min(aa.best, resid);
+  int p = -1;
+  if (aa.have_prev) {
+    p = aa.pos;
+    aa.pos = (aa.pos + 1) % m;
+    aa.cols = std::min(aa.cols + 1, m);
+  }
+  RedSlot sl;
+  sl.partials = c->aa_part.as<double>();
+  sl.ticket = c->aa_tick.as<unsigned int>();
+  sl.result = c->aa_out.as<double>();
+  launch_aa_columns(nT, c->x.as<double>(), c->u.as<double>(), c->aa_fprev.as<double>(), c->aa_gprev.as<double>(),
+                    c->aa_dF.as<double>(), c->aa_dG.as<double>(), stride, p < 0 ? 0 : p, aa.have_prev ? 1 : 0,
+                    aa.cols, q.vel_max, sl, c->aa_out.as<double>(), c->s);
+  double h[2 * kAAMaxDepth + 1];
+  UFE_HIP(hipMemcpyAsync(h, c->aa_out.p, sizeof(h), hipMemcpyDeviceToHost, c->s));
+  UFE_HIP(hipStreamSynchronize(c->s));
+  aa.have_prev = true;
+  AAGamma gam{};
+  int ncols = aa.cols;
+  if (p >= 0) {
+    for (int j = 0; j < aa.cols; ++j) aa.H[p][j] = aa.H[j][p] = h[j];
+    if (!small_solve(aa.cols, aa.H, h + kAAMaxDepth, gam.g)) {
+      // degenerate history: plain mixing step, history restarts
+      ncols = 0;
+      aa.cols = 0;
+      aa.pos = 0;
+      ++aa.n_restarts;
+    }
+  }
+  launch_aa_mix((int)stride, c->x.as<double>(), c->u.as<double>(), c->aa_dF.as<double>(), c->aa_dG.as<double>(),
+                stride, ncols, gam, q.accel_mixing, c->s);
+}
+
 static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
   const ufe_params &q = c->prm;
   auto t0 = std::chrono::steady_clock::now();
@@ -295,6 +401,8 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
   double relax = q.visc_it_relax, eps0 = q.epsilon_sq_0;
   double resid = -1.0;
   bool converged = false;
+  AndersonState aa;
+  if (q.accel_depth > 0) aa_prepare(c);
   const bool verbose = std::getenv("UFE_VERBOSE") != nullptr;
   for (double &t : c->timings) t = 0.0;
   c->n_fallbacks = 0;
@@ -309,8 +417,10 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
     linear_solve(c, &its, &nref);
     n_axb += its;
     n_ref += nref;
+    if (q.accel_depth > 0) aa_step(c, aa, resid);
     RedSlot sl = c->kry->slot(3);
-    launch_update(nT, c->x.as<double>(), c->u.as<double>(), c->uprev.as<double>(), q.vel_max, relax, sl,
+    launch_update(nT, c->x.as<double>(), c->u.as<double>(), c->uprev.as<double>(), q.vel_max,
+                  q.accel_depth > 0 ? 1.0 : relax, sl,
                   c->out2.as<double>(), c->beta_basal_b.as<double>(), c->tri_coef.as<double4>(),
                   diva ? c->F2b0.as<double>() : nullptr, q.sliding_law == UFE_SLIDING_NO_SLIDING,
                   diva ? c->ubase.as<double>() : nullptr, diva ? c->taub.as<double>() : nullptr, s);
@@ -324,7 +434,8 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
     }
     L2_prev = L2;
     L2 = 2.0 * h2[0] / std::max(h2[1], 1e-8);
-    if (L2 > L2_prev) ++nit_div; else nit_div = 0;
+    // adaptive relaxation / regularisation rule of the relaxed iteration
+    if (q.accel_depth == 0 && L2 > L2_prev) ++nit_div; else nit_div = 0;
     if (nit_div > 2) { nit_div = 0; relax *= 0.9; eps0 *= 1.2; }
     if (relax <= 0.05 || eps0 >= 1e-5) {
       if (relax < 0.05 || eps0 > 1e-5) {
@@ -646,6 +757,12 @@ extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
     if (p->precond == UFE_PC_AMG && !c->amg) return fail(c, UFE_ERR_UNSUPPORTED, "no aggregation hierarchy for this mesh");
     if (p->krylov != UFE_KRYLOV_BICGSTAB) return fail(c, UFE_ERR_UNSUPPORTED, "only BiCGStab is implemented");
     if (p->check_every < 1 || p->krylov_maxit < 1) return fail(c, UFE_ERR_ARGUMENT, "invalid Krylov controls");
+    if (p->accel_depth < 0 || p->accel_depth > UFE_ACCEL_MAX_DEPTH)
+      return fail(c, UFE_ERR_ARGUMENT, "accel_depth out of range");
+    if (p->accel_depth > 0 && (p->stop_rule != UFE_STOP_RESIDUAL || !(p->accel_mixing > 0.0) ||
+                               !(p->accel_mixing <= 1.0) || !(p->accel_restart > 1.0)))
+      return fail(c, UFE_ERR_ARGUMENT,
+                  "Anderson acceleration needs stop_rule = RESIDUAL, 0 < accel_mixing <= 1, accel_restart > 1");
     c->prm = *p;
     UFE_HIP(hipSetDevice(c->device));
     // border row types depend on the BC choices; keep any Dirichlet rows
@@ -697,6 +814,11 @@ extern "C" int ufe_upload_inputs(ufe_ctx *c, const double *Hi, const double *Hs,
 
 static void interleave_up(ufe_ctx *c, DBuf &dst, const double *a, const double *b) {
   int nT = c->m.nT;
+  if (!a && !b) {  // zero field, set on the device
+    UFE_HIP(hipMemsetAsync(dst.p, 0, 2 * (size_t)nT * sizeof(double), c->s));
+    UFE_HIP(hipStreamSynchronize(c->s));
+    return;
+  }
   std::vector<double> h(2 * (size_t)nT);
   for (int i = 0; i < nT; ++i) {
     int o = c->tperm[i];
@@ -718,7 +840,7 @@ static void interleave_down(ufe_ctx *c, const DBuf &src, double *a, double *b) {
 }
 
 extern "C" int ufe_upload_velocity(ufe_ctx *c, const double *u_b, const double *v_b) {
-  if (!c || !u_b || !v_b) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  if (!c || (!u_b) != (!v_b)) return fail(c, UFE_ERR_ARGUMENT, "null argument");
   return guarded(c, [&]() -> int {
     UFE_HIP(hipSetDevice(c->device));
     interleave_up(c, c->u, u_b, v_b);
@@ -835,6 +957,16 @@ extern "C" int ufe_solve_diva(ufe_ctx *c, const double *Hi, const double *Hs, co
   if (rc < 0) return rc;
   rc = ufe_download_diva(c, u_base_b, v_base_b, u_3D_b, v_3D_b, eta_3D_b, tau_bx_b, tau_by_b);
   return rc < 0 ? rc : rr;
+}
+
+extern "C" int ufe_profile_spmv(ufe_ctx *c, int reps, double *avg_ms, double *bytes) {
+  if (!c || !avg_ms || reps < 1) return fail(c, UFE_ERR_ARGUMENT, "invalid argument");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    // product with the current iterate, into a scratch vector
+    *avg_ms = c->kry->time_spmv64(sys64(c), c->u.as<double>(), c->kry->tmp64(7), reps, bytes);
+    return UFE_OK;
+  });
 }
 
 extern "C" int ufe_last_timings(const ufe_ctx *c, double *ms5) {

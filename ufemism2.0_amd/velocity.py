@@ -83,7 +83,15 @@ class VelocitySolver:
                                                L.ptr(k[8])))
 
     def set_velocity(self, u_b, v_b):
+        """Initial guess; u_b = v_b = None sets zero velocity on the device."""
+        if u_b is None and v_b is None:
+            self._check(self.lib.ufe_upload_velocity(self._ctx, None, None))
+            return
         self._check(self.lib.ufe_upload_velocity(self._ctx, L.ptr(L.f64(u_b)), L.ptr(L.f64(v_b))))
+
+    def reset_diva_state(self):
+        """Zero DIVA solver state (eta_3D_b, tau_b, u_base_b) on the device."""
+        self._check(self.lib.ufe_upload_diva_state(self._ctx, None, None, None, None, None))
 
     def get_velocity(self):
         nT = self.md.mesh.nT
@@ -105,6 +113,13 @@ class VelocitySolver:
         t = np.zeros(5)
         self.lib.ufe_last_timings(self._ctx, L.ptr(t))
         return {"vertex": t[0], "triangle": t[1], "assembly": t[2]}
+
+    def profile_spmv(self, reps=200):
+        """(average ms, algorithmic bytes) of the FP64 block product on the
+        system assembled last, timed with HIP events on the solver stream."""
+        ms, by = C.c_double(0), C.c_double(0)
+        self._check(self.lib.ufe_profile_spmv(self._ctx, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def residual(self, u_b, v_b):
         """FP64 nonlinear SSA residual r = b(u) - A(u) u at (u_b, v_b)."""

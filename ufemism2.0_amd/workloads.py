@@ -283,3 +283,46 @@ def schoof_icestream(h: float = 16e3, seed: int = 5, nz: int = 10, A: float = 1e
     return Workload(f"schoof_icestream_{int(h / 1e3)}km", mesh, build_operators(mesh), zeta_regular(nz),
                     np.full(nV, H), dhdx * x, np.full(nV, -10000.0), np.full(nz * nV, A), tau_y, seed, "SSA",
                     params)
+
+
+def heat_columns_inputs(nV: int, nz: int = 12, seed: int = 6, dt: float = 10.0) -> dict:
+    """Synthetic inputs of the vertical heat columns for nV vertices, in the
+    reference's [k][vi] layer-major layout (vertex fastest): a cold surface,
+    a near-steady geothermal profile, horizontal velocities decaying with
+    depth, upwind advection terms, strain heating, 10 % floating columns and
+    5 % thin-ice columns (thermodynamics_3D_heat_equation.f90:47-160 inputs).
+    Units: K, m/yr, J/m^2/yr (Q), J/m^3/yr (Phi)."""
+    rng = np.random.default_rng(seed)
+    zeta = zeta_regular(nz)
+    H = rng.uniform(200.0, 3500.0, nV)
+    thin = rng.random(nV) < 0.05
+    H[thin] = rng.uniform(0.0, 9.0, thin.sum())
+    Ts = rng.uniform(225.0, 262.0, nV)
+    Q = rng.uniform(1.2e6, 2.2e6, nV)
+    K_s = 3.101e8 * np.exp(-0.0057 * Ts)
+    Tpmp_b = 273.16 - 8.7e-4 * H
+    Tb = np.minimum(Ts + Q * H / K_s * 0.8, Tpmp_b - 0.5)
+    z = zeta[:, None]
+    Ti = Ts[None, :] + (Tb - Ts)[None, :] * z ** 1.2
+    Ti = np.minimum(Ti, 273.16 - 8.7e-4 * H[None, :] * z)
+    us = rng.uniform(0.0, 400.0, nV)
+    u3 = us[None, :] * (1.0 - 0.6 * z ** 4) * np.cos(rng.uniform(0, 2 * np.pi, nV))[None, :]
+    v3 = us[None, :] * (1.0 - 0.6 * z ** 4) * np.sin(rng.uniform(0, 2 * np.pi, nV))[None, :]
+    w3 = -rng.uniform(0.0, 0.3, nV)[None, :] * (1.0 - z)
+    Hs = np.maximum(H, 1.0)
+    f = {
+        "zeta": zeta, "Ti": Ti, "u_3D": u3, "v_3D": v3, "w_3D": w3,
+        "u_dTdxp_upwind": 1e-3 * rng.normal(size=(nz, nV)),
+        "v_dTdyp_upwind": 1e-3 * rng.normal(size=(nz, nV)),
+        "dzeta_dx": np.broadcast_to(rng.normal(0, 1e-3, nV) / Hs, (nz, nV)),
+        "dzeta_dy": np.broadcast_to(rng.normal(0, 1e-3, nV) / Hs, (nz, nV)),
+        "dzeta_dz": np.broadcast_to(-1.0 / Hs, (nz, nV)),
+        "dzeta_dt": np.broadcast_to(rng.normal(0, 1e-5, nV), (nz, nV)),
+        "Phi": rng.uniform(0.0, 2e4, (nz, nV)) * z ** 3,
+        "Hi_eff": H, "T_surf": Ts, "Q_base_grnd": Q,
+        "mask": np.where(rng.random(nV) < 0.1, 2, 1).astype(np.int32),
+    }
+    out = {k: np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1)) for k, v in f.items()
+           if k not in ("mask", "zeta")}
+    out["mask"] = f["mask"]
+    return {"nV": nV, "nz": nz, "zeta": zeta, "fields": out, "dt": dt}
