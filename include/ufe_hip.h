@@ -119,10 +119,15 @@ typedef struct ufe_params {
    * residuals f = G(u) - u (G = one linearised solve plus velocity limits)
    * with mixing factor accel_mixing; it needs stop_rule = UFE_STOP_RESIDUAL,
    * and the history restarts when the nonlinear residual exceeds
-   * accel_restart times the best one seen.  m <= UFE_ACCEL_MAX_DEPTH. */
+   * accel_restart times the best one seen.  m <= UFE_ACCEL_MAX_DEPTH.
+   * accel_continuation = c > 0 (with accel_depth > 0) starts the
+   * regularisation at epsilon_sq_0 * 10^c and lowers it tenfold per Picard
+   * iteration to epsilon_sq_0 (the mixing history restarts there); the stop
+   * test applies from then on. */
   int    accel_depth;                /*                                  0      */
   double accel_mixing;               /*                                  0.5    */
   double accel_restart;              /*                                  10     */
+  int    accel_continuation;         /*                                  0      */
 } ufe_params;
 
 #define UFE_ACCEL_MAX_DEPTH 8

@@ -63,6 +63,7 @@ class Params(C.Structure):
         ("accel_depth", C.c_int),
         ("accel_mixing", C.c_double),
         ("accel_restart", C.c_double),
+        ("accel_continuation", C.c_int),
     ]
 
 

@@ -61,5 +61,6 @@ extern "C" void ufe_params_default(ufe_params *p) {
   p->accel_depth = 0;
   p->accel_mixing = 0.5;
   p->accel_restart = 10.0;
+  p->accel_continuation = 0;
   p->check_every = 8;
 }

@@ -406,12 +406,17 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
   const bool verbose = std::getenv("UFE_VERBOSE") != nullptr;
   for (double &t : c->timings) t = 0.0;
   c->n_fallbacks = 0;
+  const int n_cont = q.accel_depth > 0 ? q.accel_continuation : 0;
   while (!converged) {
     ++it;
-    picard_state(c, diva, eps0, false);
+    // regularisation continuation: epsilon_sq_0 * 10^(n_cont - it + 1) for it <= n_cont
+    const bool in_cont = it <= n_cont;
+    const double eps_it = in_cont ? eps0 * std::pow(10.0, n_cont - it + 1) : eps0;
+    if (q.accel_depth > 0 && it == n_cont + 1 && n_cont > 0) aa = AndersonState();
+    picard_state(c, diva, eps_it, false);
     if (q.stop_rule == UFE_STOP_RESIDUAL) {
       resid = rel_residual(c, &n_clamped);
-      if (resid <= q.nonlinear_rtol) { --it; converged = true; break; }
+      if (!in_cont && resid <= q.nonlinear_rtol) { --it; converged = true; break; }
     }
     int its = 0, nref = 0;
     linear_solve(c, &its, &nref);
@@ -759,6 +764,9 @@ extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
     if (p->check_every < 1 || p->krylov_maxit < 1) return fail(c, UFE_ERR_ARGUMENT, "invalid Krylov controls");
     if (p->accel_depth < 0 || p->accel_depth > UFE_ACCEL_MAX_DEPTH)
       return fail(c, UFE_ERR_ARGUMENT, "accel_depth out of range");
+    if (p->accel_continuation < 0 || p->accel_continuation > 12 ||
+        (p->accel_continuation > 0 && p->accel_depth == 0))
+      return fail(c, UFE_ERR_ARGUMENT, "accel_continuation needs accel_depth > 0 and at most 12 decades");
     if (p->accel_depth > 0 && (p->stop_rule != UFE_STOP_RESIDUAL || !(p->accel_mixing > 0.0) ||
                                !(p->accel_mixing <= 1.0) || !(p->accel_restart > 1.0)))
       return fail(c, UFE_ERR_ARGUMENT,

@@ -1,0 +1,78 @@
+! ufe_fortran_driver -- runs one velocity solve through ufe_velocity_gpu from a
+! case file (raw little-endian stream written by the Python host, see
+! ufemism2.0_amd/fortran_case.py) and writes u_b, v_b and the iteration
+! counts.  Used by the tests to show the Fortran interface end to end.
+!
+!   ufe_fortran_driver CASE_FILE OUT_FILE
+program ufe_fortran_driver
+
+  use iso_c_binding
+  use ufe_velocity_gpu
+  implicit none
+
+  character(len=4096) :: case_file, out_file
+  integer :: u_in, u_out
+  integer(c_int) :: hdr(7), ip(6)
+  real(c_double) :: rp(5)
+  integer :: nV, nT, nz, ab_nnz, ba_nnz, bb_nnz, stress
+  real(c_double), allocatable :: V(:, :), TriGC(:, :), zeta(:)
+  integer(c_int), allocatable :: Tri(:, :), TriC(:, :), TriBI(:)
+  integer(c_int), allocatable :: ab_ptr(:), ab_ind(:), ba_ptr(:), ba_ind(:), bb_ptr(:), bb_ind(:)
+  real(c_double), allocatable :: ab_map(:), ab_ddx(:), ab_ddy(:), ba_map(:), ba_ddx(:), ba_ddy(:)
+  real(c_double), allocatable :: bb_ddx(:), bb_ddy(:), bb_d2dx2(:), bb_d2dxdy(:), bb_d2dy2(:)
+  real(c_double), allocatable :: Hi(:), Hs(:), A_flow(:, :), beta_sq(:), fgr(:)
+  integer(c_int), allocatable :: mask(:)
+  real(c_double), allocatable :: u_b(:), v_b(:), ub(:), vb(:), u3(:, :), v3(:, :), eta3(:, :), tbx(:), tby(:)
+  type(ufe_gpu_context) :: g
+  type(ufe_gpu_params) :: p
+  type(ufe_gpu_info) :: info
+  integer :: n_visc_its, n_Axb_its
+
+  call get_command_argument(1, case_file)
+  call get_command_argument(2, out_file)
+  open(newunit=u_in, file=trim(case_file), access='stream', form='unformatted', status='old', action='read')
+  read(u_in) hdr
+  nV = hdr(1); nT = hdr(2); nz = hdr(3); ab_nnz = hdr(4); ba_nnz = hdr(5); bb_nnz = hdr(6); stress = hdr(7)
+  allocate(V(nV, 2), Tri(nT, 3), TriC(nT, 3), TriBI(nT), TriGC(nT, 2), zeta(nz))
+  read(u_in) V, Tri, TriC, TriBI, TriGC, zeta
+  allocate(ab_ptr(nT + 1), ab_ind(ab_nnz), ab_map(ab_nnz), ab_ddx(ab_nnz), ab_ddy(ab_nnz))
+  read(u_in) ab_ptr, ab_ind, ab_map, ab_ddx, ab_ddy
+  allocate(ba_ptr(nV + 1), ba_ind(ba_nnz), ba_map(ba_nnz), ba_ddx(ba_nnz), ba_ddy(ba_nnz))
+  read(u_in) ba_ptr, ba_ind, ba_map, ba_ddx, ba_ddy
+  allocate(bb_ptr(nT + 1), bb_ind(bb_nnz), bb_ddx(bb_nnz), bb_ddy(bb_nnz), bb_d2dx2(bb_nnz), &
+           bb_d2dxdy(bb_nnz), bb_d2dy2(bb_nnz))
+  read(u_in) bb_ptr, bb_ind, bb_ddx, bb_ddy, bb_d2dx2, bb_d2dxdy, bb_d2dy2
+  read(u_in) ip, rp
+  allocate(Hi(nV), Hs(nV), A_flow(nV, nz), beta_sq(nV), fgr(nT), mask(nV))
+  read(u_in) Hi, Hs, A_flow, beta_sq, fgr, mask
+  close(u_in)
+
+  call ufe_gpu_init(g, 0, V, Tri, TriC, TriBI, TriGC, zeta, ab_ptr, ab_ind, ab_map, ab_ddx, ab_ddy, &
+                    ba_ptr, ba_ind, ba_map, ba_ddx, ba_ddy, bb_ptr, bb_ind, bb_ddx, bb_ddy, bb_d2dx2, &
+                    bb_d2dxdy, bb_d2dy2)
+  p = ufe_gpu_default_params()
+  p%stop_rule = ip(1); p%visc_it_nit = ip(2); p%include_crossterms = ip(3)
+  p%precond = ip(4); p%warm_start = ip(5); p%accel_depth = ip(6)
+  p%nonlinear_rtol = rp(1); p%vel_max = rp(2); p%abstol = rp(3); p%rtol = rp(4); p%accel_mixing = rp(5)
+  call ufe_gpu_set_params(g, p)
+
+  allocate(u_b(nT), v_b(nT))
+  u_b = 0.0_c_double
+  v_b = 0.0_c_double
+  if (stress == 0) then
+    call solve_SSA_gpu(g, Hi, Hs, A_flow, beta_sq, fgr, mask /= 0, u_b, v_b, n_visc_its, n_Axb_its, info=info)
+  else
+    allocate(ub(nT), vb(nT), u3(nT, nz), v3(nT, nz), eta3(nT, nz), tbx(nT), tby(nT))
+    ub = 0.0_c_double; vb = 0.0_c_double; eta3 = 0.0_c_double; tbx = 0.0_c_double; tby = 0.0_c_double
+    call solve_DIVA_gpu(g, Hi, Hs, A_flow, beta_sq, fgr, mask /= 0, u_b, v_b, ub, vb, u3, v3, eta3, tbx, tby, &
+                        n_visc_its, n_Axb_its, info=info)
+  end if
+  call ufe_gpu_finalise(g)
+
+  open(newunit=u_out, file=trim(out_file), access='stream', form='unformatted', status='replace', action='write')
+  write(u_out) u_b, v_b, int(n_visc_its, c_int), int(n_Axb_its, c_int), info%rel_residual
+  close(u_out)
+  write(*, '(a,i0,a,i0,a,es10.3)') 'solve done: picard ', n_visc_its, ' krylov ', n_Axb_its, ' residual ', &
+    info%rel_residual
+
+end program ufe_fortran_driver

@@ -1,0 +1,352 @@
+! ufe_velocity_gpu -- Fortran host interface of libufe_hip.so.
+!
+! ISO_C_BINDING interfaces to the C ABI (include/ufe_hip.h) and thin wrappers
+! that keep the argument meaning of the reference's ice-velocity module:
+!   solve_SSA_gpu  <-> solve_SSA  (SSA_main.f90:87-242)
+!   solve_DIVA_gpu <-> solve_DIVA (DIVA_main.f90:88-262)
+! Arrays are the caller's 1-based Fortran arrays; (n, nz) fields are passed
+! as they lie in memory (first index fastest).  Mesh connectivity and operator
+! indices are converted to 0-based once, in ufe_gpu_init.  A failing call
+! stops the program with the library's message (the reference's crash); a
+! Picard loop that hits visc_it_nit prints a warning and returns, as the
+! reference does.
+module ufe_velocity_gpu
+
+  use iso_c_binding
+  implicit none
+
+  private
+  public :: ufe_gpu_context, ufe_gpu_params, ufe_gpu_info, ufe_gpu_init, ufe_gpu_finalise, &
+            ufe_gpu_set_params, ufe_gpu_default_params, solve_SSA_gpu, solve_DIVA_gpu, &
+            ufe_gpu_fraction_gr_b
+
+  integer(c_int), parameter, public :: UFE_OK = 0, UFE_WARN_NOT_CONVERGED = 1
+
+  ! struct ufe_mesh_desc
+  type, bind(C) :: ufe_mesh_desc
+    integer(c_int) :: nV, nT, nz
+    type(c_ptr)    :: V, Tri, TriC, TriBI, TriGC, zeta
+    integer(c_int) :: ab_nnz
+    type(c_ptr)    :: ab_ptr, ab_ind, ab_map, ab_ddx, ab_ddy
+    integer(c_int) :: ba_nnz
+    type(c_ptr)    :: ba_ptr, ba_ind, ba_map, ba_ddx, ba_ddy
+    integer(c_int) :: bb_nnz
+    type(c_ptr)    :: bb_ptr, bb_ind, bb_ddx, bb_ddy, bb_d2dx2, bb_d2dxdy, bb_d2dy2
+  end type ufe_mesh_desc
+
+  ! struct ufe_params (field order and types as in the C header)
+  type, bind(C) :: ufe_gpu_params
+    real(c_double) :: glens_n, epsilon_sq_0, visc_eff_min, vel_max
+    integer(c_int) :: visc_it_nit
+    real(c_double) :: visc_it_relax, visc_it_norm_dUV_tol, rtol, abstol
+    integer(c_int) :: include_crossterms, sliding_law
+    real(c_double) :: slid_weertman_m, slid_budd_q_plastic, slid_budd_u_threshold, slid_delta_v, slid_beta_max
+    integer(c_int) :: do_gl_subgrid_friction
+    real(c_double) :: subgrid_friction_exponent
+    integer(c_int) :: bc_u(4), bc_v(4)
+    integer(c_int) :: precision, krylov, precond, gmres_restart, krylov_maxit, warm_start
+    real(c_double) :: inner_rtol
+    integer(c_int) :: stop_rule
+    real(c_double) :: nonlinear_rtol
+    integer(c_int) :: check_every
+    integer(c_int) :: accel_depth
+    real(c_double) :: accel_mixing, accel_restart
+    integer(c_int) :: accel_continuation
+  end type ufe_gpu_params
+
+  ! struct ufe_solve_info
+  type, bind(C) :: ufe_gpu_info
+    integer(c_int) :: n_visc_its, n_Axb_its, n_refinements
+    real(c_double) :: L2_uv, rel_residual, visc_it_relax_applied, epsilon_sq_0_applied, seconds
+    integer(c_int) :: n_clamped, n_fallbacks
+  end type ufe_gpu_info
+
+  ! A device context plus the 0-based copies of the mesh arrays it was built from.
+  type :: ufe_gpu_context
+    type(c_ptr) :: ctx = c_null_ptr
+    integer     :: nV = 0, nT = 0, nz = 0
+  end type ufe_gpu_context
+
+  interface
+    subroutine c_ufe_params_default(p) bind(C, name='ufe_params_default')
+      import :: ufe_gpu_params
+      type(ufe_gpu_params), intent(out) :: p
+    end subroutine
+    function c_ufe_last_error(ctx) bind(C, name='ufe_last_error') result(msg)
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr) :: msg
+    end function
+    function c_ufe_ctx_create(ctx, device, mesh) bind(C, name='ufe_ctx_create') result(rc)
+      import :: c_ptr, c_int, ufe_mesh_desc
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int), value :: device
+      type(ufe_mesh_desc), intent(in) :: mesh
+      integer(c_int) :: rc
+    end function
+    function c_ufe_ctx_destroy(ctx) bind(C, name='ufe_ctx_destroy') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function c_ufe_set_params(ctx, p) bind(C, name='ufe_set_params') result(rc)
+      import :: c_ptr, c_int, ufe_gpu_params
+      type(c_ptr), value :: ctx
+      type(ufe_gpu_params), intent(in) :: p
+      integer(c_int) :: rc
+    end function
+    function c_ufe_solve_ssa(ctx, Hi, Hs, A_flow, beta_sq, fgr, mask, bcm, bcu, bcv, u_b, v_b, info) &
+        bind(C, name='ufe_solve_ssa') result(rc)
+      import :: c_ptr, c_int, ufe_gpu_info
+      type(c_ptr), value :: ctx, Hi, Hs, A_flow, beta_sq, fgr, mask, bcm, bcu, bcv, u_b, v_b
+      type(ufe_gpu_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+    function c_ufe_solve_diva(ctx, Hi, Hs, A_flow, beta_sq, fgr, mask, bcm, bcu, bcv, u, v, ubase, vbase, &
+        u3, v3, eta3, tbx, tby, info) bind(C, name='ufe_solve_diva') result(rc)
+      import :: c_ptr, c_int, ufe_gpu_info
+      type(c_ptr), value :: ctx, Hi, Hs, A_flow, beta_sq, fgr, mask, bcm, bcu, bcv, u, v, ubase, vbase, &
+                            u3, v3, eta3, tbx, tby
+      type(ufe_gpu_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+    function c_ufe_fraction_gr_b(ctx, Hi, Hb, SL, fgr) bind(C, name='ufe_fraction_gr_b') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, Hi, Hb, SL, fgr
+      integer(c_int) :: rc
+    end function
+  end interface
+
+contains
+
+  ! ---- helpers ---------------------------------------------------------------
+
+  subroutine crash_on(rc, ctx, where)
+    integer(c_int), intent(in) :: rc
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: where
+    if (rc < 0) then
+      write(0, '(a)') 'ufe_velocity_gpu: ' // where // ' failed: ' // c_string(c_ufe_last_error(ctx))
+      error stop 1
+    end if
+  end subroutine crash_on
+
+  function c_string(p) result(s)
+    type(c_ptr), intent(in) :: p
+    character(len=:), allocatable :: s
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n
+    interface
+      function c_strlen(q) bind(C, name='strlen') result(l)
+        import :: c_ptr, c_size_t
+        type(c_ptr), value :: q
+        integer(c_size_t) :: l
+      end function
+    end interface
+    if (.not. c_associated(p)) then
+      s = ''
+      return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: s)
+    s = transfer(chars(1:n), s)
+  end function c_string
+
+  function ufe_gpu_default_params() result(p)
+    type(ufe_gpu_params) :: p
+    call c_ufe_params_default(p)
+  end function ufe_gpu_default_params
+
+  ! ---- context ---------------------------------------------------------------
+
+  ! Mesh in the reference's layout (mesh_types.f90): V(nV,2), Tri/TriC(nT,3)
+  ! 1-based with TriC = 0 for no neighbour, TriBI(nT), TriGC(nT,2), zeta(nz);
+  ! operators as 1-based CSR (ptr(n+1), ind(nnz), val(nnz)) of
+  ! M_{map,ddx,ddy}_a_b, M_{map,ddx,ddy}_b_a (one pattern each) and
+  ! M2_{ddx,ddy,d2dx2,d2dxdy,d2dy2}_b_b (one pattern).
+  subroutine ufe_gpu_init(g, device, V, Tri, TriC, TriBI, TriGC, zeta, &
+      ab_ptr, ab_ind, ab_map, ab_ddx, ab_ddy, ba_ptr, ba_ind, ba_map, ba_ddx, ba_ddy, &
+      bb_ptr, bb_ind, bb_ddx, bb_ddy, bb_d2dx2, bb_d2dxdy, bb_d2dy2)
+    type(ufe_gpu_context), intent(out) :: g
+    integer, intent(in) :: device
+    real(c_double), intent(in) :: V(:, :), TriGC(:, :), zeta(:)
+    integer(c_int), intent(in) :: Tri(:, :), TriC(:, :), TriBI(:)
+    integer(c_int), intent(in) :: ab_ptr(:), ab_ind(:), ba_ptr(:), ba_ind(:), bb_ptr(:), bb_ind(:)
+    real(c_double), intent(in), target :: ab_map(:), ab_ddx(:), ab_ddy(:), ba_map(:), ba_ddx(:), ba_ddy(:)
+    real(c_double), intent(in), target :: bb_ddx(:), bb_ddy(:), bb_d2dx2(:), bb_d2dxdy(:), bb_d2dy2(:)
+    type(ufe_mesh_desc) :: d
+    real(c_double), allocatable, target :: Vi(:), GCi(:), z(:)
+    integer(c_int), allocatable, target :: Tri0(:), TriC0(:), TriBI0(:)
+    integer(c_int), allocatable, target :: abp(:), abi(:), bap(:), bai(:), bbp(:), bbi(:)
+    integer :: nV, nT, i, n
+    integer(c_int) :: rc
+
+    nV = size(V, 1)
+    nT = size(Tri, 1)
+    g%nV = nV
+    g%nT = nT
+    g%nz = size(zeta)
+    allocate(Vi(2 * nV), GCi(2 * nT), Tri0(3 * nT), TriC0(3 * nT), TriBI0(nT), z(size(zeta)))
+    do i = 1, nV
+      Vi(2 * i - 1) = V(i, 1)
+      Vi(2 * i)     = V(i, 2)
+    end do
+    do i = 1, nT
+      GCi(2 * i - 1) = TriGC(i, 1)
+      GCi(2 * i)     = TriGC(i, 2)
+      do n = 1, 3
+        Tri0(3 * (i - 1) + n)  = Tri(i, n) - 1
+        TriC0(3 * (i - 1) + n) = TriC(i, n) - 1
+      end do
+    end do
+    TriBI0 = TriBI
+    z = zeta
+    abp = ab_ptr - 1
+    abi = ab_ind - 1
+    bap = ba_ptr - 1
+    bai = ba_ind - 1
+    bbp = bb_ptr - 1
+    bbi = bb_ind - 1
+
+    d%nV = nV
+    d%nT = nT
+    d%nz = size(zeta)
+    d%V = c_loc(Vi)
+    d%Tri = c_loc(Tri0)
+    d%TriC = c_loc(TriC0)
+    d%TriBI = c_loc(TriBI0)
+    d%TriGC = c_loc(GCi)
+    d%zeta = c_loc(z)
+    d%ab_nnz = size(ab_ind)
+    d%ab_ptr = c_loc(abp)
+    d%ab_ind = c_loc(abi)
+    d%ab_map = c_loc(ab_map)
+    d%ab_ddx = c_loc(ab_ddx)
+    d%ab_ddy = c_loc(ab_ddy)
+    d%ba_nnz = size(ba_ind)
+    d%ba_ptr = c_loc(bap)
+    d%ba_ind = c_loc(bai)
+    d%ba_map = c_loc(ba_map)
+    d%ba_ddx = c_loc(ba_ddx)
+    d%ba_ddy = c_loc(ba_ddy)
+    d%bb_nnz = size(bb_ind)
+    d%bb_ptr = c_loc(bbp)
+    d%bb_ind = c_loc(bbi)
+    d%bb_ddx = c_loc(bb_ddx)
+    d%bb_ddy = c_loc(bb_ddy)
+    d%bb_d2dx2 = c_loc(bb_d2dx2)
+    d%bb_d2dxdy = c_loc(bb_d2dxdy)
+    d%bb_d2dy2 = c_loc(bb_d2dy2)
+    ! the library copies everything to the device during the call
+    rc = c_ufe_ctx_create(g%ctx, int(device, c_int), d)
+    call crash_on(rc, c_null_ptr, 'ufe_ctx_create')
+  end subroutine ufe_gpu_init
+
+  subroutine ufe_gpu_set_params(g, p)
+    type(ufe_gpu_context), intent(in) :: g
+    type(ufe_gpu_params), intent(in) :: p
+    call crash_on(c_ufe_set_params(g%ctx, p), g%ctx, 'ufe_set_params')
+  end subroutine ufe_gpu_set_params
+
+  subroutine ufe_gpu_finalise(g)
+    type(ufe_gpu_context), intent(inout) :: g
+    integer(c_int) :: rc
+    if (c_associated(g%ctx)) rc = c_ufe_ctx_destroy(g%ctx)
+    g%ctx = c_null_ptr
+  end subroutine ufe_gpu_finalise
+
+  ! ---- drop-in entry points -----------------------------------------------------
+
+  ! solve_SSA (SSA_main.f90:87-242): u_b, v_b are the initial guess on entry
+  ! and the solution on exit; the prescribed-velocity boundary condition needs
+  ! all three of BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b.
+  subroutine solve_SSA_gpu(g, Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice, u_b, v_b, &
+      n_visc_its, n_Axb_its, BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b, info)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Hi(:), Hs(:), A_flow(:, :), beta_sq(:), fraction_gr_b(:)
+    logical, intent(in) :: mask_grounded_ice(:)
+    real(c_double), intent(inout), target :: u_b(:), v_b(:)
+    integer, intent(out) :: n_visc_its, n_Axb_its
+    integer, intent(in), optional :: BC_prescr_mask_b(:)
+    real(c_double), intent(in), optional, target :: BC_prescr_u_b(:), BC_prescr_v_b(:)
+    type(ufe_gpu_info), intent(out), optional :: info
+    integer(c_int), allocatable, target :: mask(:), bcm(:)
+    type(c_ptr) :: pbm, pbu, pbv
+    type(ufe_gpu_info) :: inf
+    integer(c_int) :: rc
+
+    call bc_pointers(BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b, bcm, pbm, pbu, pbv)
+    mask = merge(1_c_int, 0_c_int, mask_grounded_ice)
+    rc = c_ufe_solve_ssa(g%ctx, c_loc(Hi), c_loc(Hs), c_loc(A_flow), c_loc(beta_sq), c_loc(fraction_gr_b), &
+                         c_loc(mask), pbm, pbu, pbv, c_loc(u_b), c_loc(v_b), inf)
+    call crash_on(rc, g%ctx, 'solve_SSA_gpu')
+    if (rc == UFE_WARN_NOT_CONVERGED) write(0, '(a)') 'WARNING: viscosity iteration failed to converge!'
+    n_visc_its = inf%n_visc_its
+    n_Axb_its = inf%n_Axb_its
+    if (present(info)) info = inf
+  end subroutine solve_SSA_gpu
+
+  ! solve_DIVA (DIVA_main.f90:88-262): u_vav_b, v_vav_b and the DIVA state
+  ! (u_base_b, v_base_b, eta_3D_b, tau_bx_b, tau_by_b) are inout across calls;
+  ! u_3D_b, v_3D_b(nT, nz) are outputs (calc_3D_velocities).
+  subroutine solve_DIVA_gpu(g, Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice, u_vav_b, v_vav_b, &
+      u_base_b, v_base_b, u_3D_b, v_3D_b, eta_3D_b, tau_bx_b, tau_by_b, n_visc_its, n_Axb_its, &
+      BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b, info)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Hi(:), Hs(:), A_flow(:, :), beta_sq(:), fraction_gr_b(:)
+    logical, intent(in) :: mask_grounded_ice(:)
+    real(c_double), intent(inout), target :: u_vav_b(:), v_vav_b(:), u_base_b(:), v_base_b(:)
+    real(c_double), intent(inout), target :: u_3D_b(:, :), v_3D_b(:, :), eta_3D_b(:, :), tau_bx_b(:), tau_by_b(:)
+    integer, intent(out) :: n_visc_its, n_Axb_its
+    integer, intent(in), optional :: BC_prescr_mask_b(:)
+    real(c_double), intent(in), optional, target :: BC_prescr_u_b(:), BC_prescr_v_b(:)
+    type(ufe_gpu_info), intent(out), optional :: info
+    integer(c_int), allocatable, target :: mask(:), bcm(:)
+    type(c_ptr) :: pbm, pbu, pbv
+    type(ufe_gpu_info) :: inf
+    integer(c_int) :: rc
+
+    call bc_pointers(BC_prescr_mask_b, BC_prescr_u_b, BC_prescr_v_b, bcm, pbm, pbu, pbv)
+    mask = merge(1_c_int, 0_c_int, mask_grounded_ice)
+    rc = c_ufe_solve_diva(g%ctx, c_loc(Hi), c_loc(Hs), c_loc(A_flow), c_loc(beta_sq), c_loc(fraction_gr_b), &
+                          c_loc(mask), pbm, pbu, pbv, c_loc(u_vav_b), c_loc(v_vav_b), c_loc(u_base_b), &
+                          c_loc(v_base_b), c_loc(u_3D_b), c_loc(v_3D_b), c_loc(eta_3D_b), c_loc(tau_bx_b), &
+                          c_loc(tau_by_b), inf)
+    call crash_on(rc, g%ctx, 'solve_DIVA_gpu')
+    if (rc == UFE_WARN_NOT_CONVERGED) write(0, '(a)') 'WARNING: viscosity iteration failed to converge!'
+    n_visc_its = inf%n_visc_its
+    n_Axb_its = inf%n_Axb_its
+    if (present(info)) info = inf
+  end subroutine solve_DIVA_gpu
+
+  ! calc_grounded_fractions_bilin_interp_TAF_b (subgrid_grounded_fractions_bilin_TAF.f90:182-250)
+  subroutine ufe_gpu_fraction_gr_b(g, Hi, Hb, SL, fraction_gr_b)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Hi(:), Hb(:), SL(:)
+    real(c_double), intent(out), target :: fraction_gr_b(:)
+    call crash_on(c_ufe_fraction_gr_b(g%ctx, c_loc(Hi), c_loc(Hb), c_loc(SL), c_loc(fraction_gr_b)), g%ctx, &
+                  'ufe_gpu_fraction_gr_b')
+  end subroutine ufe_gpu_fraction_gr_b
+
+  subroutine bc_pointers(m, u, v, bcm, pbm, pbu, pbv)
+    integer, intent(in), optional :: m(:)
+    real(c_double), intent(in), optional, target :: u(:), v(:)
+    integer(c_int), allocatable, target, intent(inout) :: bcm(:)
+    type(c_ptr), intent(out) :: pbm, pbu, pbv
+    pbm = c_null_ptr
+    pbu = c_null_ptr
+    pbv = c_null_ptr
+    if (present(m) .or. present(u) .or. present(v)) then
+      if (.not. (present(m) .and. present(u) .and. present(v))) then
+        write(0, '(a)') 'ufe_velocity_gpu: need to provide prescribed u,v fields and mask!'
+        error stop 1
+      end if
+      bcm = int(m, c_int)
+      pbm = c_loc(bcm)
+      pbu = c_loc(u)
+      pbv = c_loc(v)
+    end if
+  end subroutine bc_pointers
+
+end module ufe_velocity_gpu
