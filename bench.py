@@ -12,8 +12,9 @@ velocity and zero DIVA state, set on the device) until the FP64 block-scaled
 nonlinear residual ||D^-1(b(u) - A(u)u)|| / ||D^-1 b(u)|| <= 1e-6, A and b
 assembled at the final u.  Inputs are uploaded to HBM before the timed region.
 
-  chosen form: Anderson-mixed Picard (depth 2, mixing 0.6), warm-started FP64
-               BiCGStab with the aggregation-AMG preconditioner, rtol 0.1
+  chosen form: Anderson-mixed Picard (depth 2, mixing 1.0) after a 5-decade
+               regularisation continuation, warm-started FP64 BiCGStab with the
+               aggregation-AMG preconditioner, rtol 0.1
   plain form:  the reference iteration (relaxation 0.2 with the adaptive
                relaxation rule, zero Krylov initial guess, rtol 1e-7), FP64
                BiCGStab with the same preconditioner
@@ -42,7 +43,7 @@ sys.path.insert(0, ROOT)
 
 BASE = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-6, visc_it_nit=1000,
             krylov_maxit=20000)
-CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.1, accel_depth=2, accel_mixing=0.6)
+CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.1, accel_depth=2, accel_mixing=1.0, accel_continuation=5)
 PLAIN = dict(precision=0, precond=3, warm_start=0, rtol=1e-7)
 METRIC = "converged velocity solves/s at 1e-6 rel. residual, plain vs chosen form, 1 GPU"
 HBM_PEAK_GBS = 8000.0
@@ -64,10 +65,11 @@ def read_traffic():
     return None
 
 
-def solve_once(s, info_check=True):
+def solve_once(s, stress="DIVA", info_check=True):
     s.set_velocity(None, None)
-    s.reset_diva_state()
-    info = s.run("DIVA")
+    if stress == "DIVA":
+        s.reset_diva_state()
+    info = s.run(stress)
     if info_check and not info.rel_residual <= BASE["nonlinear_rtol"]:
         raise RuntimeError(f"solve did not converge: residual {info.rel_residual}")
     return info
@@ -116,6 +118,48 @@ def cpu_baseline(w, seconds, n_picard_plain):
                       f"{n_picard_plain} Picard iterations of the plain form (estimate, not a full solve)"}
 
 
+WORKLOADS = ("channel", "dome", "continental", "ensemble")
+
+
+def build_workloads(W, args):
+    """The benchmark workloads: the channel (default, the metric's workload),
+    the dome (the CPU-runnable case, SSA), the continental sheet (DIVA, plus
+    its heat columns) and the ensemble of uneven channel members."""
+    if args.workload == "channel":
+        return "channel_diva_200k", [W.channel(target_nV=args.nv, stress_balance="DIVA")]
+    if args.workload == "dome":
+        return "dome_ssa_40km", [W.dome(h=40e3)]
+    if args.workload == "continental":
+        return "continental_diva_400k", [W.continental()]
+    return f"ensemble_{args.members}", [W.ensemble_member(k) for k in range(args.members)]
+
+
+def time_form(V, ws, form, steps, warmup, label):
+    """Warm-up and timed converged solves of one form over the workloads;
+    returns (seconds for `steps` passes, infos, last velocities per workload)."""
+    solvers = []
+    for w in ws:
+        inp = w.inputs()
+        s = V.VelocitySolver(w.descriptor(), 0, **BASE, **form)
+        s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+        solvers.append((w, s))
+    for i in range(warmup):
+        for w, s in solvers:
+            info = solve_once(s, w.stress_balance)
+        log(f"[bench] {label} warmup {i}: picard {info.n_visc_its} krylov {info.n_Axb_its} "
+            f"residual {info.rel_residual:.2e} {info.seconds:.2f}s")
+    infos = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        for w, s in solvers:
+            infos.append(solve_once(s, w.stress_balance))
+        log(f"[bench] {label} step {i}: picard {infos[-1].n_visc_its} krylov {infos[-1].n_Axb_its} "
+            f"residual {infos[-1].rel_residual:.2e} {infos[-1].seconds:.2f}s")
+    t = time.perf_counter() - t0
+    vel = [s.get_velocity() for _, s in solvers]
+    return t, infos, vel, solvers
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +170,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=200)
     ap.add_argument("--nv", type=int, default=200_000, help="target vertices of the channel mesh")
+    ap.add_argument("--workload", choices=WORKLOADS, default="channel")
+    ap.add_argument("--members", type=int, default=64, help="ensemble members (--workload ensemble)")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="one chosen-form solve, then only the block-product and copy-kernel launches "
+                         "(for hardware counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the velocity solve runs on a single MI355X here; "
@@ -138,54 +187,56 @@ def main():
     from ufemism2_amd import velocity as V, workloads as W
 
     t0 = time.perf_counter()
-    w = W.channel(target_nV=args.nv, stress_balance="DIVA")
-    md, inp = w.descriptor(), w.inputs()
-    log(f"[bench] workload {w.describe()} built in {time.perf_counter() - t0:.1f}s")
-    args_up = (inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    name, ws = build_workloads(W, args)
+    log(f"[bench] workload {name}: {len(ws)} mesh(es), {sum(w.mesh.nV for w in ws)} vertices, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    w = ws[0]
 
-    s = V.VelocitySolver(md, 0, **BASE, **CHOSEN)
-    s.upload(*args_up)
-    for i in range(args.warmup):
-        info = solve_once(s)
-        log(f"[bench] warmup {i}: picard {info.n_visc_its} krylov {info.n_Axb_its} "
-            f"residual {info.rel_residual:.2e} {info.seconds:.2f}s")
-    infos = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        infos.append(solve_once(s))
-        log(f"[bench] step {i}: picard {infos[-1].n_visc_its} krylov {infos[-1].n_Axb_its} "
-            f"residual {infos[-1].rel_residual:.2e} {infos[-1].seconds:.2f}s")
-    t_chosen = time.perf_counter() - t0
-    u_c, v_c = s.get_velocity()
-    spmv_ms, spmv_bytes = s.profile_spmv(args.spmv_reps)
-    s.close()
+    if args.profile_kernels:
+        inp = w.inputs()
+        s = V.VelocitySolver(w.descriptor(), 0, **BASE, **CHOSEN)
+        s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+        solve_once(s, w.stress_balance)
+        ms, nbytes = s.profile_spmv(args.spmv_reps)
+        copy_bytes = int(nbytes) // 2 // 16 * 16
+        cms = s.profile_copy(copy_bytes, args.spmv_reps)
+        s.close()
+        print(json.dumps({"spmv_avg_ms": ms, "spmv_algorithmic_bytes": nbytes, "copy_avg_ms": cms,
+                          "copy_read_bytes": copy_bytes, "copy_write_bytes": copy_bytes,
+                          "launches_each": args.spmv_reps + 1}), flush=True)
+        return
 
-    p = V.VelocitySolver(md, 0, **BASE, **PLAIN)
-    p.upload(*args_up)
-    pinfos = []
-    t0 = time.perf_counter()
-    for i in range(args.plain_steps):
-        pinfos.append(solve_once(p))
-        log(f"[bench] plain {i}: picard {pinfos[-1].n_visc_its} krylov {pinfos[-1].n_Axb_its} "
-            f"residual {pinfos[-1].rel_residual:.2e} {pinfos[-1].seconds:.2f}s")
-    t_plain = time.perf_counter() - t0
-    u_p, v_p = p.get_velocity()
-    p.close()
-    scale = max(np.abs(u_p).max(), np.abs(v_p).max())
-    diff = max(np.abs(u_c - u_p).max(), np.abs(v_c - v_p).max()) / scale
+    t_chosen, infos, vel_c, solvers = time_form(V, ws, CHOSEN, args.steps, args.warmup, "chosen")
+    spmv_ms, spmv_bytes = solvers[0][1].profile_spmv(args.spmv_reps)
+    for _, s in solvers:
+        s.close()
+    n_solves = args.steps * len(ws)
 
-    value = args.steps / t_chosen
-    plain_value = args.plain_steps / t_plain if args.plain_steps else None
+    pinfos, t_plain, diff = [], 0.0, None
+    if args.plain_steps > 0:
+        t_plain, pinfos, vel_p, psolvers = time_form(V, ws, PLAIN, args.plain_steps, 0, "plain")
+        for _, s in psolvers:
+            s.close()
+        diff = 0.0
+        for (uc, vc), (up, vp) in zip(vel_c, vel_p):
+            scale = max(np.abs(up).max(), np.abs(vp).max())
+            diff = max(diff, float(max(np.abs(uc - up).max(), np.abs(vc - vp).max()) / scale))
+
+    value = n_solves / t_chosen
+    plain_value = args.plain_steps * len(ws) / t_plain if args.plain_steps else None
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    traffic = read_traffic() if args.workload == "channel" else None
     out = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t_chosen / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "channel_diva_200k", "nV": w.mesh.nV, "nTri": w.mesh.nT, "nz": w.nz,
-                   "unknowns": 2 * w.mesh.nT, "seed": w.seed, "mesh": w.mesh.digest(),
+        "config": {"workload": name, "meshes": len(ws), "nV": sum(x.mesh.nV for x in ws),
+                   "nTri": sum(x.mesh.nT for x in ws), "nz": w.nz,
+                   "unknowns": 2 * sum(x.mesh.nT for x in ws), "seed": w.seed, "mesh": w.mesh.digest(),
+                   "stress_balance": w.stress_balance,
                    "stop": "block-scaled nonlinear residual <= 1e-6",
-                   "chosen": "Anderson(2, 0.6) Picard, FP64 AMG-BiCGStab warm, rtol 0.1",
+                   "chosen": "Anderson(2, 1.0) Picard after 5-decade eps0 continuation, FP64 AMG-BiCGStab warm, "
+                             "rtol 0.1",
                    "plain": "relaxed Picard (reference rule), FP64 AMG-BiCGStab zero guess, rtol 1e-7"},
         "chosen": {"picard": [i.n_visc_its for i in infos], "krylov": [i.n_Axb_its for i in infos],
                    "residual": [i.rel_residual for i in infos]},
@@ -200,10 +251,26 @@ def main():
                      "traffic": traffic, "avg_ms": spmv_ms, "algorithmic_bytes": spmv_bytes},
         "cpu_baseline": None,
     }
+    if args.workload == "continental":
+        out["heat_columns"] = time_heat(V, W, w.mesh.nV, 12, args.steps)
     if not args.no_cpu_baseline:
         n_plain = pinfos[0].n_visc_its if pinfos else 231
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, n_plain)
     print(json.dumps(out), flush=True)
+
+
+def time_heat(V, W, nV, nz, steps):
+    """Heat columns of the continental configuration: nV columns x nz layers,
+    one call = one implicit time step of every column (host arrays in, out)."""
+    c = W.heat_columns_inputs(nV, nz, seed=3)
+    hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], c["fields"], c["dt"])
+    V.heat_columns(hin)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        T, unst, nun = V.heat_columns(hin)
+    t = (time.perf_counter() - t0) / steps
+    return {"columns": nV, "nz": nz, "seconds_per_call": t, "columns_per_s": nV / t, "n_unstable": nun,
+            "note": "includes host<->device transfer of the inputs (the C ABI takes host arrays)"}
 
 
 if __name__ == "__main__":

@@ -214,6 +214,10 @@ int ufe_last_timings(const ufe_ctx *ctx, double *ms5);
  * product y = A x with the system assembled last, and its algorithmic HBM
  * bytes per launch (blocks + column indices + row pointers + x + y). */
 int ufe_profile_spmv(ufe_ctx *ctx, int reps, double *avg_ms, double *bytes);
+/* Measurement hook (no reference counterpart): average duration in ms of a
+ * plain device-to-device copy kernel over `bytes` bytes (read bytes + write
+ * bytes each), the known-traffic reference for hardware byte counters. */
+int ufe_profile_copy(ufe_ctx *ctx, long bytes, int reps, double *avg_ms);
 
 /* ---- linear-algebra boundary ------------------------------------------ */
 
@@ -279,6 +283,32 @@ int ufe_mesh_operators_ba(int nV, int nT, const double *V, const int *TriC, cons
 int ufe_mesh_operators_bb2(int nT, const int *TriC, const double *TriGC, int *nnz, int *ptr, int *ind,
                            double *val_ddx, double *val_ddy, double *val_d2dx2, double *val_d2dxdy,
                            double *val_d2dy2);
+
+/* ---- learned initial guess (engine extension, no reference counterpart) --
+ * A message-passing network on the triangle graph (3 layers, hidden 64,
+ * BF16 matrix-core products with FP32 accumulation) that predicts the
+ * velocity from per-triangle features of the uploaded inputs; it seeds the
+ * Picard iteration, which stays the judge of the result.  Weights: one fp32
+ * vector of ufe_mpnn_num_weights() values (NULL at creation = zeros).
+ * Graph arguments: N nodes, X [N*16] features, nb [N*3] symmetric neighbour
+ * lists (-1 = none), T [N*3] targets (log10 speed in m/yr, flow direction
+ * x, y), Y [N*16] outputs (columns 0..2 used). */
+typedef struct ufe_mpnn ufe_mpnn;
+int ufe_mpnn_num_weights(void);
+int ufe_mpnn_create(ufe_mpnn **net, int device, const float *weights);
+int ufe_mpnn_destroy(ufe_mpnn *net);
+int ufe_mpnn_get_weights(ufe_mpnn *net, float *weights);
+int ufe_mpnn_forward(ufe_mpnn *net, int N, const float *X, const int *nb, float *Y);
+/* loss = sum ||Y[:, 0:3] - T||^2 / N and its gradient (same layout as the weights) */
+int ufe_mpnn_loss_grad(ufe_mpnn *net, int N, const float *X, const int *nb, const float *T, float *loss,
+                       float *grad);
+/* `steps` full-batch Adam steps (lr; betas 0.9, 0.999); loss_history [steps] may be NULL */
+int ufe_mpnn_train(ufe_mpnn *net, int N, const float *X, const int *nb, const float *T, int steps, float lr,
+                   float *loss_history);
+/* features [nT*16] of the context's uploaded inputs, in the caller's triangle order */
+int ufe_mpnn_features(ufe_ctx *ctx, float *X);
+/* sets the context's velocity to the network's prediction (DIVA state zero) */
+int ufe_mpnn_initial_guess(ufe_ctx *ctx, ufe_mpnn *net);
 
 #ifdef __cplusplus
 }

@@ -146,6 +146,17 @@ def lib():
     _sig(L, 'ufe_download_diva', argtypes=[C.c_void_p, dp, dp, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_last_timings', argtypes=[C.c_void_p, dp])
     _sig(L, 'ufe_profile_spmv', argtypes=[C.c_void_p, C.c_int, dp, dp])
+    _sig(L, 'ufe_profile_copy', argtypes=[C.c_void_p, C.c_long, C.c_int, dp])
+    fp = C.POINTER(C.c_float)
+    _sig(L, 'ufe_mpnn_num_weights', restype=C.c_int, argtypes=[])
+    _sig(L, 'ufe_mpnn_create', argtypes=[C.POINTER(C.c_void_p), C.c_int, fp])
+    _sig(L, 'ufe_mpnn_destroy', argtypes=[C.c_void_p])
+    _sig(L, 'ufe_mpnn_get_weights', argtypes=[C.c_void_p, fp])
+    _sig(L, 'ufe_mpnn_forward', argtypes=[C.c_void_p, C.c_int, fp, ip, fp])
+    _sig(L, 'ufe_mpnn_loss_grad', argtypes=[C.c_void_p, C.c_int, fp, ip, fp, fp, fp])
+    _sig(L, 'ufe_mpnn_train', argtypes=[C.c_void_p, C.c_int, fp, ip, fp, C.c_int, C.c_float, fp])
+    _sig(L, 'ufe_mpnn_features', argtypes=[C.c_void_p, fp])
+    _sig(L, 'ufe_mpnn_initial_guess', argtypes=[C.c_void_p, C.c_void_p])
     _sig(L, 'ufe_ssa_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_csr_matvec', argtypes=[C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_csr_solve', argtypes=[C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double,
@@ -163,7 +174,9 @@ EXPORTED = [
     "ufe_params_default", "ufe_last_error", "ufe_version", "ufe_ctx_create", "ufe_ctx_destroy",
     "ufe_set_params", "ufe_solve_ssa", "ufe_solve_diva", "ufe_upload_inputs", "ufe_upload_velocity",
     "ufe_run", "ufe_download_velocity", "ufe_upload_diva_state", "ufe_download_diva",
-    "ufe_last_timings", "ufe_profile_spmv", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns",
+    "ufe_last_timings", "ufe_profile_spmv", "ufe_profile_copy", "ufe_mpnn_num_weights", "ufe_mpnn_create",
+    "ufe_mpnn_destroy", "ufe_mpnn_get_weights", "ufe_mpnn_forward", "ufe_mpnn_loss_grad", "ufe_mpnn_train",
+    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
 ]
 

@@ -19,7 +19,8 @@ constexpr int kAAW = 2 * kAAMaxDepth + 1;
 
 // g := clamp(x) (apply_velocity_limits), f = g - u; if have_prev, ring column
 // p := (f - fprev, g - gprev); fprev := f, gprev := g.  Grid sums:
-// out[j] = dF_j . dF_p, out[m + j] = dF_j . f (j < ncols), out[2m] = f . f.
+// out[j] = dF_j . dF_p, out[m + j] = dF_j . f (j < ncols), out[2m] = f . f
+// (m = kAAMaxDepth), written by the second-phase kernel.
 __global__ void __launch_bounds__(kBlock) k_aa_columns(
     int nT, double *__restrict__ x, const double *__restrict__ u, double *__restrict__ fprev,
     double *__restrict__ gprev, double *__restrict__ dF, double *__restrict__ dG, long stride, int p,
@@ -56,11 +57,7 @@ __global__ void __launch_bounds__(kBlock) k_aa_columns(
   }
   __shared__ double lds[kAAW][kBlock / 64];
   block_sum<kAAW>(acc, lds);
-  double tot[kAAW];
-  if (grid_finish<kAAW>(slot, acc, tot)) {
-#pragma unroll
-    for (int j = 0; j < kAAW; ++j) out[j] = tot[j];
-  }
+  block_partials<kAAW>(slot, acc);
 }
 
 // x := (1 - mix)(u - sum gamma_j (dG_j - dF_j)) + mix (x - sum gamma_j dG_j),
@@ -88,6 +85,8 @@ void launch_aa_columns(int nT, double *x, const double *u, double *fprev, double
   int nb = std::min(blocks_for(nT), kMaxPartials);
   hipLaunchKernelGGL(k_aa_columns, dim3(nb), dim3(kBlock), 0, s, nT, x, u, fprev, gprev, dF, dG, stride, p,
                      have_prev, ncols, vel_max, slot, out);
+  UFE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_partials_out<kAAW>, dim3(1), dim3(kBlock), 0, s, slot, nb, out);
   UFE_LAUNCH_CHECK();
 }
 

@@ -167,14 +167,15 @@ __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__re
   double acc[1] = {nun};
   __shared__ double lds[1][kBlock / 64];
   block_sum<1>(acc, lds);
-  double tot[1];
-  if (grid_finish<1>(slot, acc, tot)) count[0] = tot[0];
+  block_partials<1>(slot, acc);
 }
 
 void launch_heat_columns(const HeatDev &h, double *Tout, int *unstable, const RedSlot &slot, double *count,
                          hipStream_t s) {
   int nb = std::min(blocks_for(h.nV), kMaxPartials);
   hipLaunchKernelGGL(k_heat_columns, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
+  UFE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_partials_out<1>, dim3(1), dim3(kBlock), 0, s, slot, nb, count);
   UFE_LAUNCH_CHECK();
 }
 
@@ -312,9 +313,8 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
     h.dt = in->dt;
     h.Hi_min = in->Hi_min_thermo;
     DBuf Tout(std::max<size_t>(n3, 8)), unst(sizeof(int) * std::max(nV, 1)), cnt(sizeof(double) * 2);
-    DBuf part(sizeof(double) * kMaxPartials), tick(sizeof(unsigned int));
-    UFE_HIP(hipMemsetAsync(tick.p, 0, sizeof(unsigned int), s));
-    RedSlot slot{part.as<double>(), tick.as<unsigned int>(), cnt.as<double>()};
+    DBuf part(sizeof(double) * kMaxPartials);
+    RedSlot slot{part.as<double>(), cnt.as<double>()};
     if (nV > 0) launch_heat_columns(h, Tout.as<double>(), unst.as<int>(), slot, cnt.as<double>(), s);
     double hc = 0.0;
     if (nV > 0) {

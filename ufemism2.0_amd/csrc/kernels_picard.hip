@@ -265,11 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_update(
   }
   __shared__ double lds[2][kBlock / 64];
   block_sum<2>(acc, lds);
-  double tot[2];
-  if (grid_finish<2>(slot, acc, tot)) {
-    out2[0] = tot[0];
-    out2[1] = tot[1];
-  }
+  block_partials<2>(slot, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -329,6 +325,8 @@ void launch_update(int nT, const double *x, double *u, double *uprev, double vel
   if (nb > kMaxPartials) nb = kMaxPartials;
   hipLaunchKernelGGL(k_update, dim3(nb), dim3(kBlock), 0, s, nT, x, u, uprev, vel_max, relax, slot, out2,
                      beta_basal_b, tri_coef, F2b0, no_sliding, ubase, taub);
+  UFE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_partials_out<2>, dim3(1), dim3(kBlock), 0, s, slot, nb, out2);
   UFE_LAUNCH_CHECK();
 }
 

@@ -161,7 +161,7 @@ class Krylov {
   DBuf vec64_[10], vec32_[10];
   DBuf coarse_inv_;      // dense inverse of the coarsest level (column-major)
   DBuf scal_;            // KrylovScalars
-  DBuf red_part_, red_tick_, red_res_;
+  DBuf red_part_, red_res_;
   KrylovScalars *h_scal_ = nullptr;  // pinned host mirror
   int verbose_ = 0;
 };

@@ -46,9 +46,13 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
   using V4 = typename VecT<T>::V4;
   if (sc && sc->done) return;
   const int lane = threadIdx.x % kLpr;
-  const int groups = (gridDim.x * blockDim.x) / kLpr;
+  constexpr int kRowsPerPass = kBlock / kLpr;
+  // contiguous chunk of rows per workgroup, chunks ordered per XCD
+  const int chunk = ((nb + gridDim.x - 1) / gridDim.x + kRowsPerPass - 1) / kRowsPerPass * kRowsPerPass;
+  const int row0 = xcd_block() * chunk;
+  const int row1 = min(nb, row0 + chunk);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr; row < nb; row += groups) {
+  for (int row = row0 + (int)threadIdx.x / kLpr; row < row1; row += kRowsPerPass) {
     T y0 = 0, y1 = 0;
     if (x) {
       const int k1 = ptr[row + 1];
@@ -106,20 +110,28 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
   if (OP == OP_PLAIN) return;
   __shared__ double lds[3][kBlock / 64];
   block_sum<3>(acc, lds);
+  block_partials<3>(slot, acc);
+}
+
+// Second phase of the product's reduction (one workgroup).
+template <int OP>
+__global__ void __launch_bounds__(kBlock) k_spmv_finish(RedSlot slot, int nparts, double *__restrict__ out,
+                                                        KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
   double tot[3];
-  if (grid_finish<3>(slot, acc, tot)) {
-    if (OP == OP_RESID) {
-      out[0] = tot[0];
-      out[1] = tot[1];
-      out[2] = tot[2];
-    } else if (OP == OP_DOT1) {
-      // alpha = rho / (rhat, v)
-      if (tot[0] == 0.0 || !std::isfinite(tot[0])) sc->done = 2;
-      else sc->alpha = sc->rho / tot[0];
-    } else {
-      // omega = (t, s) / (t, t)
-      sc->omega = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
-    }
+  partials_sum<3>(slot, nparts, tot);
+  if (threadIdx.x != 0) return;
+  if (OP == OP_RESID) {
+    out[0] = tot[0];
+    out[1] = tot[1];
+    out[2] = tot[2];
+  } else if (OP == OP_DOT1) {
+    // alpha = rho / (rhat, v)
+    if (tot[0] == 0.0 || !std::isfinite(tot[0])) sc->done = 2;
+    else sc->alpha = sc->rho / tot[0];
+  } else {
+    // omega = (t, s) / (t, t)
+    sc->omega = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
   }
 }
 
@@ -129,12 +141,15 @@ static void spmv_launch(const BsrT<T> &A, const T *x, T *y, const T *b, const T 
                         const typename VecT<T>::V4 *dblk = nullptr, int write_unscaled = 0) {
   using V2 = typename VecT<T>::V2;
   long threads = (long)A.nb * kLpr;
-  int nblk = blocks_for(threads);
-  if (OP != OP_PLAIN && nblk > kMaxPartials) nblk = kMaxPartials;
+  int nblk = xcd_grid(std::min(blocks_for(threads), kMaxPartials));
   hipLaunchKernelGGL((k_bsr_spmv<T, OP>), dim3(nblk), dim3(kBlock), 0, s, A.nb, A.ptr, A.col, A.val,
                      reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y), reinterpret_cast<const V2 *>(b),
                      reinterpret_cast<const V2 *>(w), slot, out, sc, vmax, dblk, write_unscaled);
   UFE_LAUNCH_CHECK();
+  if (OP != OP_PLAIN) {
+    hipLaunchKernelGGL((k_spmv_finish<OP>), dim3(1), dim3(kBlock), 0, s, slot, nblk, out, sc);
+    UFE_LAUNCH_CHECK();
+  }
 }
 
 // ===========================================================================
@@ -235,17 +250,23 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(int nb, typename VecT<T>::V2 
   }
   __shared__ double lds[2][kBlock / 64];
   block_sum<2>(acc, lds);
+  block_partials<2>(slot, acc);
+}
+
+// Second phase of k_bicg_x (one workgroup): rho, ||r||^2 and the stop test.
+__global__ void __launch_bounds__(kBlock) k_bicg_x_finish(RedSlot slot, int nparts, KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
   double tot[2];
-  if (grid_finish<2>(slot, acc, tot)) {
-    sc->its += 1;
-    sc->rho_old = sc->rho;
-    sc->rho = tot[0];
-    sc->rr = tot[1];
-    if (!(tot[1] <= sc->tol2) && !std::isfinite(tot[1])) sc->done = 2;
-    else if (tot[1] <= sc->tol2) sc->done = 1;
-    else if (sc->omega == 0.0 || tot[0] == 0.0) sc->done = 2;
-    else if (sc->its >= sc->maxit) sc->done = 3;
-  }
+  partials_sum<2>(slot, nparts, tot);
+  if (threadIdx.x != 0) return;
+  sc->its += 1;
+  sc->rho_old = sc->rho;
+  sc->rho = tot[0];
+  sc->rr = tot[1];
+  if (!(tot[1] <= sc->tol2) && !std::isfinite(tot[1])) sc->done = 2;
+  else if (tot[1] <= sc->tol2) sc->done = 1;
+  else if (sc->omega == 0.0 || tot[0] == 0.0) sc->done = 2;
+  else if (sc->its >= sc->maxit) sc->done = 3;
 }
 
 // rhat = r, p = v = 0, scalars from ||r0||^2 (rr_in)
@@ -348,7 +369,7 @@ __global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restr
                                                       const KrylovScalars *__restrict__ sc) {
   if (sc && sc->done) return;
   const int lane = threadIdx.x % kLpr;
-  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr;
+  const int row = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
   if (row >= nb) return;
   T y0 = 0, y1 = 0;
   for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLpr) {
@@ -412,7 +433,7 @@ __global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
     const KrylovScalars *__restrict__ sc) {
   if (sc && sc->done) return;
   const int lane = threadIdx.x % kLpr;
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr;
+  const int i = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
   if (i >= nb) return;
   T y0 = 0, y1 = 0;
   for (int k = ptr[i] + lane; k < ptr[i + 1]; k += kLpr) {
@@ -662,9 +683,7 @@ Krylov::Krylov(int nb, hipStream_t s) : nb_(nb), s_(s) {
   for (auto &v : vec32_) v.alloc(sizeof(float) * 2 * (size_t)nb);
   scal_.alloc(sizeof(KrylovScalars));
   red_part_.alloc(sizeof(double) * kMaxPartials * 4 * 4);
-  red_tick_.alloc(sizeof(unsigned int) * 4);
   red_res_.alloc(sizeof(double) * 4 * 4);
-  UFE_HIP(hipMemset(red_tick_.p, 0, sizeof(unsigned int) * 4));
   UFE_HIP(hipMemset(scal_.p, 0, sizeof(KrylovScalars)));
   UFE_HIP(hipHostMalloc((void **)&h_scal_, sizeof(KrylovScalars), hipHostMallocDefault));
 }
@@ -696,7 +715,6 @@ double Krylov::time_spmv64(const BsrT<double> &A, const double *x, double *y, in
 RedSlot Krylov::slot(int i) const {
   RedSlot r;
   r.partials = red_part_.as<double>() + (size_t)i * kMaxPartials * 4;
-  r.ticket = red_tick_.as<unsigned int>() + i;
   r.result = red_res_.as<double>() + 4 * i;
   return r;
 }
@@ -794,7 +812,7 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     bool next_coarsest = (l + 1 == L - 1);
     int nc = lev[l + 1].nb;
-hipLaunchKernelGGL(k_amg_resid<T>, dim3(blocks_for((long)lev[l].nb * kLpr)), dim3(kBlock), 0, s_, lev[l].nb,
+hipLaunchKernelGGL(k_amg_resid<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0, s_, lev[l].nb,
                        lev[l].ptr, lev[l].col, lev[l].val, rin, av.z[l]->template as<V2>(),
                        av.zt[l]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
@@ -813,7 +831,7 @@ hipLaunchKernelGGL(k_amg_resid<T>, dim3(blocks_for((long)lev[l].nb * kLpr)), dim
     const AmgLevelPattern &P = *amg_->levels[l];
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     V2 *zout = l == 0 ? reinterpret_cast<V2 *>(z) : av.zt[l]->template as<V2>();
-    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(blocks_for((long)lev[l].nb * kLpr)), dim3(kBlock), 0, s_, lev[l].nb,
+    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0, s_, lev[l].nb,
                        lev[l].ptr, lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin,
                        av.z[l]->template as<V2>(), av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
     UFE_LAUNCH_CHECK();
@@ -887,6 +905,8 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
                            reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
                            reinterpret_cast<const V2 *>(rhat), sl0, sc,
                            reinterpret_cast<const typename VecT<T>::V4 *>(dscale<T>()));
+        UFE_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_bicg_x_finish, dim3(1), dim3(kBlock), 0, s_, sl0, nbx, sc);
         UFE_LAUNCH_CHECK();
       }
       UFE_HIP(hipMemcpyAsync(h_scal_, sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s_));

@@ -46,12 +46,14 @@ struct ufe_ctx {
   bool diva_state_ready = false;
   // Anderson mixing: ring of m difference columns (dF, dG), previous f and G,
   // reduction slot of width 2 kAAMaxDepth + 1
-  DBuf aa_dF, aa_dG, aa_fprev, aa_gprev, aa_part, aa_tick, aa_out;
+  DBuf aa_dF, aa_dG, aa_fprev, aa_gprev, aa_part, aa_out;
   std::unique_ptr<Krylov> kry;
   std::unique_ptr<AmgHierarchy> amg;
   std::string err;
   double timings[5] = {0, 0, 0, 0, 0};
   int n_fallbacks = 0;
+  double bbox[4] = {0, 1, 0, 1};  // xmin, xmax, ymin, ymax of the vertices
+  DBuf feat;                     // learned-guess features [nT][16] fp32 (device order)
   hipEvent_t ev[8];
 };
 
@@ -286,9 +288,7 @@ static void aa_prepare(ufe_ctx *c) {
     c->aa_fprev.alloc(n * sizeof(double));
     c->aa_gprev.alloc(n * sizeof(double));
     c->aa_part.alloc(sizeof(double) * kMaxPartials * (2 * kAAMaxDepth + 1));
-    c->aa_tick.alloc(sizeof(unsigned int));
     c->aa_out.alloc(sizeof(double) * (2 * kAAMaxDepth + 1));
-    UFE_HIP(hipMemsetAsync(c->aa_tick.p, 0, sizeof(unsigned int), c->s));
   }
 }
 
@@ -342,7 +342,6 @@ static void aa_step(ufe_ctx *c, AndersonState &aa, double resid) {
   }
   RedSlot sl;
   sl.partials = c->aa_part.as<double>();
-  sl.ticket = c->aa_tick.as<unsigned int>();
   sl.result = c->aa_out.as<double>();
   launch_aa_columns(nT, c->x.as<double>(), c->u.as<double>(), c->aa_fprev.as<double>(), c->aa_gprev.as<double>(),
                     c->aa_dF.as<double>(), c->aa_dG.as<double>(), stride, p < 0 ? 0 : p, aa.have_prev ? 1 : 0,
@@ -703,6 +702,14 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
     m.Tri = dev_copy(c.get(), md->Tri, 3 * (size_t)nT);
     m.V = dev_copy(c.get(), md->V, 2 * (size_t)nV);
     m.zeta = dev_copy(c.get(), md->zeta, nz);
+    c->bbox[0] = c->bbox[2] = 1e300;
+    c->bbox[1] = c->bbox[3] = -1e300;
+    for (int vi = 0; vi < nV; ++vi) {
+      c->bbox[0] = std::min(c->bbox[0], md->V[2 * vi]);
+      c->bbox[1] = std::max(c->bbox[1], md->V[2 * vi]);
+      c->bbox[2] = std::min(c->bbox[2], md->V[2 * vi + 1]);
+      c->bbox[3] = std::max(c->bbox[3], md->V[2 * vi + 1]);
+    }
     // system and work arrays
     c->A64.alloc(sizeof(double4) * (size_t)nnz);
     c->A32.alloc(sizeof(float4) * (size_t)nnz);
@@ -967,6 +974,43 @@ extern "C" int ufe_solve_diva(ufe_ctx *c, const double *Hi, const double *Hs, co
   return rc < 0 ? rc : rr;
 }
 
+// Plain streaming copy of n doubles (16-byte vector accesses): the reference
+// kernel of known traffic against which hardware byte counters are scaled.
+__global__ void __launch_bounds__(kBlock) k_stream_copy(long n2, const double2 *__restrict__ in,
+                                                        double2 *__restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+extern "C" int ufe_profile_copy(ufe_ctx *c, long bytes, int reps, double *avg_ms) {
+  if (!c || !avg_ms || reps < 1 || bytes < 16) return fail(c, UFE_ERR_ARGUMENT, "invalid argument");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    long n2 = bytes / 16;
+    DBuf a, b;
+    a.alloc(n2 * 16);
+    b.alloc(n2 * 16);
+    UFE_HIP(hipMemsetAsync(a.p, 0, n2 * 16, c->s));
+    int nb = (int)std::min<long>(blocks_for(n2), 8192);
+    hipEvent_t e0, e1;
+    UFE_HIP(hipEventCreate(&e0));
+    UFE_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(kBlock), 0, c->s, n2, a.as<double2>(), b.as<double2>());
+    UFE_HIP(hipEventRecord(e0, c->s));
+    for (int i = 0; i < reps; ++i)
+      hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(kBlock), 0, c->s, n2, a.as<double2>(), b.as<double2>());
+    UFE_LAUNCH_CHECK();
+    UFE_HIP(hipEventRecord(e1, c->s));
+    UFE_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    UFE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return UFE_OK;
+  });
+}
+
 extern "C" int ufe_profile_spmv(ufe_ctx *c, int reps, double *avg_ms, double *bytes) {
   if (!c || !avg_ms || reps < 1) return fail(c, UFE_ERR_ARGUMENT, "invalid argument");
   return guarded(c, [&]() -> int {
@@ -1123,6 +1167,127 @@ extern "C" int ufe_ssa_residual(ufe_ctx *c, const double *u_b, const double *v_b
     };
     if (r) down(c->r64, r);
     if (b) down(c->rhs, b);
+    return UFE_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Learned initial guess: per-triangle features from the uploaded inputs and
+// the network's output written into the iterate (mpnn.hip runs the network).
+// Features (fp32, 16 per triangle, unused ones zero): H/1e3, tau_d/1e5 (x, y),
+// (log10 beta_sq - 4)/2, log10 A + 16.5, grounded fraction, border flag,
+// centroid scaled to the mesh's bounding box (x, y), Hs/1e3, |tau_d|/1e5,
+// log10 |tau_d| - 4, log10 H - 3, unit driving-stress direction (x, y).
+// The network predicts log10 speed and the flow direction.
+// ---------------------------------------------------------------------------
+constexpr int kFeat = 16;
+
+__global__ void __launch_bounds__(kBlock) k_features(int nT, const int *__restrict__ Tri, const double *__restrict__ V,
+                                                     const double *__restrict__ Hi, const double *__restrict__ Hs,
+                                                     const double *__restrict__ rough, const double *__restrict__ A_vav,
+                                                     const double *__restrict__ fgr, const double *__restrict__ tdx,
+                                                     const double *__restrict__ tdy,
+                                                     const signed char *__restrict__ rowtype, double x0, double sx,
+                                                     double y0, double sy, float *__restrict__ X) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nT) return;
+  double H = 0, S = 0, B = 0, A = 0, cx = 0, cy = 0;
+  for (int n = 0; n < 3; ++n) {
+    int vi = Tri[3 * ti + n];
+    H += Hi[vi]; S += Hs[vi]; B += rough[vi]; A += A_vav[vi];
+    cx += V[2 * vi]; cy += V[2 * vi + 1];
+  }
+  H /= 3; S /= 3; B /= 3; A /= 3; cx /= 3; cy /= 3;
+  float *x = X + (size_t)ti * kFeat;
+  x[0] = (float)(H / 1e3);
+  x[1] = (float)(tdx[ti] / 1e5);
+  x[2] = (float)(tdy[ti] / 1e5);
+  x[3] = (float)((log10(fmax(B, 1e-3)) - 4.0) / 2.0);
+  x[4] = (float)(log10(fmax(A, 1e-30)) + 16.5);
+  x[5] = (float)fgr[ti];
+  x[6] = rowtype[2 * ti] != ROW_FREE ? 1.0f : 0.0f;
+  x[7] = (float)((cx - x0) * sx);
+  x[8] = (float)((cy - y0) * sy);
+  x[9] = (float)(S / 1e3);
+  const double tn = sqrt(tdx[ti] * tdx[ti] + tdy[ti] * tdy[ti]);
+  x[10] = (float)(tn / 1e5);
+  x[11] = (float)(log10(fmax(tn, 1.0)) - 4.0);
+  x[12] = (float)(log10(fmax(H, 1.0)) - 3.0);
+  x[13] = tn > 0.0 ? (float)(tdx[ti] / tn) : 0.0f;
+  x[14] = tn > 0.0 ? (float)(tdy[ti] / tn) : 0.0f;
+  x[15] = 0.0f;
+}
+
+// velocity = 10^y0 * (y1, y2) / |(y1, y2)|  (speed capped at vel_max)
+__global__ void __launch_bounds__(kBlock) k_set_guess(int nT, const float *__restrict__ Y, int ystride,
+                                                      double vel_max, double *__restrict__ u) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nT) return;
+  const float *y = Y + (size_t)ti * ystride;
+  double sp = fmin(pow(10.0, fmin((double)y[0], 6.0)), vel_max);
+  double dx = y[1], dy = y[2], dn = sqrt(dx * dx + dy * dy);
+  if (!(dn > 1e-6)) { dx = 0.0; dy = 0.0; dn = 1.0; }
+  reinterpret_cast<double2 *>(u)[ti] = make_double2(sp * dx / dn, sp * dy / dn);
+}
+
+namespace ufe {
+namespace mp {
+
+// Features of the uploaded inputs into X_dev [nT][16] (device order) and the
+// triangle neighbour lists into nb_dev [nT][3]; with null pointers only N is
+// reported.
+int ctx_features(ufe_ctx *c, float *X_dev, int *nb_dev, hipStream_t *s_out, int *N_out) {
+  *N_out = c->m.nT;
+  *s_out = c->s;
+  if (!X_dev) return UFE_OK;
+  if (!c->inputs_ready) return fail(c, UFE_ERR_ARGUMENT, "inputs not uploaded");
+  UFE_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->s;
+  int nT = c->m.nT;
+  launch_driving_stress(c->m, c->Hi.as<double>(), c->Hs.as<double>(), c->tdx.as<double>(), c->tdy.as<double>(), s);
+  launch_vertical_average(c->m.nV, c->m.nz, c->m.zeta, c->A_vk_vf.as<double>(), c->A_vav.as<double>(), s);
+  double sx = 1.0 / std::max(c->bbox[1] - c->bbox[0], 1e-30), sy = 1.0 / std::max(c->bbox[3] - c->bbox[2], 1e-30);
+  hipLaunchKernelGGL(k_features, dim3(blocks_for(nT)), dim3(kBlock), 0, s, nT, c->m.Tri, c->m.V, c->Hi.as<double>(),
+                     c->Hs.as<double>(), c->rough.as<double>(), c->A_vav.as<double>(), c->fgr.as<double>(),
+                     c->tdx.as<double>(), c->tdy.as<double>(), c->rowtype.as<signed char>(), c->bbox[0], sx,
+                     c->bbox[2], sy, X_dev);
+  UFE_LAUNCH_CHECK();
+  if (nb_dev) UFE_HIP(hipMemcpyAsync(nb_dev, c->m.TriC, sizeof(int) * 3 * (size_t)nT, hipMemcpyDeviceToDevice, s));
+  return UFE_OK;
+}
+
+// u := network output (km/yr -> m/yr); DIVA state cleared.
+int ctx_set_guess(ufe_ctx *c, const float *Y_dev) {
+  UFE_HIP(hipSetDevice(c->device));
+  int nT = c->m.nT;
+  hipLaunchKernelGGL(k_set_guess, dim3(blocks_for(nT)), dim3(kBlock), 0, c->s, nT, Y_dev, 16, c->prm.vel_max,
+                     c->u.as<double>());
+  UFE_LAUNCH_CHECK();
+  UFE_HIP(hipMemsetAsync(c->eta3b.p, 0, c->eta3b.bytes, c->s));
+  UFE_HIP(hipMemsetAsync(c->ubase.p, 0, c->ubase.bytes, c->s));
+  UFE_HIP(hipMemsetAsync(c->taub.p, 0, c->taub.bytes, c->s));
+  c->diva_state_ready = true;
+  UFE_HIP(hipStreamSynchronize(c->s));
+  return UFE_OK;
+}
+
+}  // namespace mp
+}  // namespace ufe
+
+extern "C" int ufe_mpnn_features(ufe_ctx *c, float *X) {
+  if (!c || !X) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  return guarded(c, [&]() -> int {
+    int N;
+    hipStream_t s;
+    int nT = c->m.nT;
+    DBuf xd(sizeof(float) * kFeat * (size_t)nT);
+    int rc = ufe::mp::ctx_features(c, xd.as<float>(), nullptr, &s, &N);
+    if (rc != UFE_OK) return rc;
+    std::vector<float> h((size_t)kFeat * nT);
+    UFE_HIP(hipMemcpyAsync(h.data(), xd.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    UFE_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < nT; ++i)
+      std::memcpy(X + (size_t)kFeat * c->tperm[i], h.data() + (size_t)kFeat * i, sizeof(float) * kFeat);
     return UFE_OK;
   });
 }

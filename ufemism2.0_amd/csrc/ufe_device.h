@@ -27,18 +27,28 @@ struct HipError : std::runtime_error {
 constexpr int kBlock = 256;      // threads per workgroup for 1-D kernels (4 wavefronts)
 constexpr int kMaxPartials = 2048;  // per-reduction partial sums (one per workgroup)
 
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round robin
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Kernels that
+// gather a vector through the mesh connectivity renumber their workgroups so
+// that each XCD owns one contiguous range of rows: with the locality-ordered
+// mesh, that XCD's L2 then holds only its own part of the gathered vector
+// instead of all of it.  The grid must be a multiple of 8 (xcd_grid); the
+// renumbering is a bijection, so correctness does not depend on the dispatch
+// order.
+constexpr int kXcds = 8;
+inline int xcd_grid(int g) { return (g + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ int xcd_block() {
+  return (int)(blockIdx.x % kXcds) * (int)(gridDim.x / kXcds) + (int)(blockIdx.x / kXcds);
+}
+
 inline int blocks_for(long n, int per_block = kBlock) {
   long b = (n + per_block - 1) / per_block;
   return (int)(b < 1 ? 1 : b);
 }
 
-// Deterministic grid reduction slot: each workgroup writes its partial, the
-// last workgroup to arrive (atomic ticket) sums the partials in index order
-// and publishes the result.  The ticket is reset by that last workgroup, so a
-// slot can be reused by the next launch on the same stream.
+// Deterministic grid reduction slot: per-workgroup partials and the result.
 struct RedSlot {
   double *partials;   // [kMaxPartials * width]
-  unsigned int *ticket;
   double *result;     // [width]
 };
 
@@ -61,7 +71,7 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Workgroup-wide sum of up to 4 values; result valid in thread 0.
+// Workgroup-wide sum of W values; result valid in thread 0.
 template <int W>
 __device__ __forceinline__ void block_sum(double (&v)[W], double (*lds)[kBlock / 64]) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -81,41 +91,43 @@ __device__ __forceinline__ void block_sum(double (&v)[W], double (*lds)[kBlock /
   }
 }
 
-// Finish a grid reduction of W values.  Call from all threads after the
-// workgroup sum; returns true in exactly one thread of the last workgroup,
-// with out[] holding the grid totals.
+// Grid reductions are two-phase.  Each workgroup writes its partial sums with
+// plain stores (block_partials); a one-workgroup kernel launched next on the
+// same stream adds the partials in index order (partials_sum), which makes the
+// result deterministic, and runs the epilogue.  The kernel boundary orders the
+// two phases: no device-scope fence or ticket inside the producing kernel
+// (such a fence writes back the L2 of every workgroup's XCD).
 template <int W>
-__device__ __forceinline__ bool grid_finish(const RedSlot &slot, double (&v)[W], double (&out)[W]) {
-  __shared__ bool is_last;
+__device__ __forceinline__ void block_partials(const RedSlot &slot, const double (&v)[W]) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < W; ++i) slot.partials[i * kMaxPartials + blockIdx.x] = v[i];
-    __threadfence();
-    unsigned int t = atomicAdd(slot.ticket, 1u);
-    is_last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!is_last) return false;
-  // last workgroup: sum partials in fixed order (deterministic)
+}
+
+// Sum of the first nparts partials of each of W values; valid in thread 0 of a
+// one-workgroup launch of kBlock threads.
+template <int W>
+__device__ __forceinline__ void partials_sum(const RedSlot &slot, int nparts, double (&out)[W]) {
   __shared__ double lds[W][kBlock / 64];
-  double acc[W];
-  __threadfence();
 #pragma unroll
   for (int i = 0; i < W; ++i) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
-      s += slot.partials[i * kMaxPartials + b];
-    }
-    acc[i] = s;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) s += slot.partials[i * kMaxPartials + b];
+    out[i] = s;
   }
-  block_sum<W>(acc, lds);
+  block_sum<W>(out, lds);
+}
+
+// Second phase writing the W totals to out[0..W).
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_partials_out(RedSlot slot, int nparts, double *__restrict__ out) {
+  double tot[W];
+  partials_sum<W>(slot, nparts, tot);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < W; ++i) out[i] = acc[i];
-    *slot.ticket = 0u;
-    return true;
+    for (int i = 0; i < W; ++i) out[i] = tot[i];
   }
-  return false;
 }
 
 }  // namespace ufe

@@ -121,6 +121,12 @@ class VelocitySolver:
         self._check(self.lib.ufe_profile_spmv(self._ctx, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def profile_copy(self, nbytes, reps=50):
+        """Average ms of a plain copy kernel over nbytes (known-traffic reference)."""
+        ms = C.c_double(0)
+        self._check(self.lib.ufe_profile_copy(self._ctx, int(nbytes), reps, C.byref(ms)))
+        return ms.value
+
     def residual(self, u_b, v_b):
         """FP64 nonlinear SSA residual r = b(u) - A(u) u at (u_b, v_b)."""
         n = 2 * self.md.mesh.nT
