@@ -812,8 +812,8 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     bool next_coarsest = (l + 1 == L - 1);
     int nc = lev[l + 1].nb;
-hipLaunchKernelGGL(k_amg_resid<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0, s_, lev[l].nb,
-                       lev[l].ptr, lev[l].col, lev[l].val, rin, av.z[l]->template as<V2>(),
+    hipLaunchKernelGGL(k_amg_resid<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0, s_,
+                       lev[l].nb, lev[l].ptr, lev[l].col, lev[l].val, rin, av.z[l]->template as<V2>(),
                        av.zt[l]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_amg_restrict<T>, dim3(blocks_for(nc)), dim3(kBlock), 0, s_, nc, P.mem_ptr.as<int>(),
@@ -831,7 +831,8 @@ hipLaunchKernelGGL(k_amg_resid<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kL
     const AmgLevelPattern &P = *amg_->levels[l];
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     V2 *zout = l == 0 ? reinterpret_cast<V2 *>(z) : av.zt[l]->template as<V2>();
-    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0, s_, lev[l].nb,
+    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0,
+                       s_, lev[l].nb,
                        lev[l].ptr, lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin,
                        av.z[l]->template as<V2>(), av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
     UFE_LAUNCH_CHECK();

@@ -1177,7 +1177,8 @@ extern "C" int ufe_ssa_residual(ufe_ctx *c, const double *u_b, const double *v_b
 // Features (fp32, 16 per triangle, unused ones zero): H/1e3, tau_d/1e5 (x, y),
 // (log10 beta_sq - 4)/2, log10 A + 16.5, grounded fraction, border flag,
 // centroid scaled to the mesh's bounding box (x, y), Hs/1e3, |tau_d|/1e5,
-// log10 |tau_d| - 4, log10 H - 3, unit driving-stress direction (x, y).
+// log10 |tau_d| - 4, log10 H - 3, unit driving-stress direction (x, y),
+// log10 of a local shallow-ice speed estimate.
 // The network predicts log10 speed and the flow direction.
 // ---------------------------------------------------------------------------
 constexpr int kFeat = 16;
@@ -1188,7 +1189,8 @@ __global__ void __launch_bounds__(kBlock) k_features(int nT, const int *__restri
                                                      const double *__restrict__ fgr, const double *__restrict__ tdx,
                                                      const double *__restrict__ tdy,
                                                      const signed char *__restrict__ rowtype, double x0, double sx,
-                                                     double y0, double sy, float *__restrict__ X) {
+                                                     double y0, double sy, double glen_n, double weertman_m,
+                                                     float *__restrict__ X) {
   int ti = blockIdx.x * blockDim.x + threadIdx.x;
   if (ti >= nT) return;
   double H = 0, S = 0, B = 0, A = 0, cx = 0, cy = 0;
@@ -1215,7 +1217,10 @@ __global__ void __launch_bounds__(kBlock) k_features(int nT, const int *__restri
   x[12] = (float)(log10(fmax(H, 1.0)) - 3.0);
   x[13] = tn > 0.0 ? (float)(tdx[ti] / tn) : 0.0f;
   x[14] = tn > 0.0 ? (float)(tdy[ti] / tn) : 0.0f;
-  x[15] = 0.0f;
+  // local shallow-ice estimate of the speed: deformation 2A/(n+1) |tau_d|^n H
+  // plus Weertman sliding (|tau_d| / beta_sq)^m, as log10 m/yr
+  const double u_sia = 2.0 * A / (glen_n + 1.0) * pow(tn, glen_n) * H + pow(tn / fmax(B, 1e-3), weertman_m);
+  x[15] = (float)log10(fmax(u_sia, 1e-3));
 }
 
 // velocity = 10^y0 * (y1, y2) / |(y1, y2)|  (speed capped at vel_max)
@@ -1250,7 +1255,7 @@ int ctx_features(ufe_ctx *c, float *X_dev, int *nb_dev, hipStream_t *s_out, int 
   hipLaunchKernelGGL(k_features, dim3(blocks_for(nT)), dim3(kBlock), 0, s, nT, c->m.Tri, c->m.V, c->Hi.as<double>(),
                      c->Hs.as<double>(), c->rough.as<double>(), c->A_vav.as<double>(), c->fgr.as<double>(),
                      c->tdx.as<double>(), c->tdy.as<double>(), c->rowtype.as<signed char>(), c->bbox[0], sx,
-                     c->bbox[2], sy, X_dev);
+                     c->bbox[2], sy, c->prm.glens_n, c->prm.slid_weertman_m, X_dev);
   UFE_LAUNCH_CHECK();
   if (nb_dev) UFE_HIP(hipMemcpyAsync(nb_dev, c->m.TriC, sizeof(int) * 3 * (size_t)nT, hipMemcpyDeviceToDevice, s));
   return UFE_OK;
