@@ -134,7 +134,7 @@ def build_workloads(W, args):
     return f"ensemble_{args.members}", [W.ensemble_member(k) for k in range(args.members)]
 
 
-def time_form(V, ws, form, steps, warmup, label):
+def time_form(V, ws, form, steps, warmup, label, concurrency=1):
     """Warm-up and timed converged solves of one form over the workloads;
     returns (seconds for `steps` passes, infos, last velocities per workload)."""
     solvers = []
@@ -151,8 +151,15 @@ def time_form(V, ws, form, steps, warmup, label):
     infos = []
     t0 = time.perf_counter()
     for i in range(steps):
-        for w, s in solvers:
-            infos.append(solve_once(s, w.stress_balance))
+        if concurrency > 1 and len(solvers) > 1:
+            # members on `concurrency` host threads, one context (and HIP stream) each;
+            # the C ABI releases the interpreter lock during the solve
+            import concurrent.futures as cf
+            with cf.ThreadPoolExecutor(max_workers=concurrency) as ex:
+                infos.extend(ex.map(lambda ws: solve_once(ws[1], ws[0].stress_balance), solvers))
+        else:
+            for w, s in solvers:
+                infos.append(solve_once(s, w.stress_balance))
         log(f"[bench] {label} step {i}: picard {infos[-1].n_visc_its} krylov {infos[-1].n_Axb_its} "
             f"residual {infos[-1].rel_residual:.2e} {infos[-1].seconds:.2f}s")
     t = time.perf_counter() - t0
@@ -172,6 +179,8 @@ def main():
     ap.add_argument("--nv", type=int, default=200_000, help="target vertices of the channel mesh")
     ap.add_argument("--workload", choices=WORKLOADS, default="channel")
     ap.add_argument("--members", type=int, default=64, help="ensemble members (--workload ensemble)")
+    ap.add_argument("--concurrency", type=int, default=4,
+                    help="ensemble members solved at once, one HIP stream each (--workload ensemble)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one chosen-form solve, then only the block-product and copy-kernel launches "
                          "(for hardware counter runs)")
@@ -206,7 +215,8 @@ def main():
                           "launches_each": args.spmv_reps + 1}), flush=True)
         return
 
-    t_chosen, infos, vel_c, solvers = time_form(V, ws, CHOSEN, args.steps, args.warmup, "chosen")
+    conc = args.concurrency if args.workload == "ensemble" else 1
+    t_chosen, infos, vel_c, solvers = time_form(V, ws, CHOSEN, args.steps, args.warmup, "chosen", conc)
     spmv_ms, spmv_bytes = solvers[0][1].profile_spmv(args.spmv_reps)
     for _, s in solvers:
         s.close()
@@ -214,7 +224,7 @@ def main():
 
     pinfos, t_plain, diff = [], 0.0, None
     if args.plain_steps > 0:
-        t_plain, pinfos, vel_p, psolvers = time_form(V, ws, PLAIN, args.plain_steps, 0, "plain")
+        t_plain, pinfos, vel_p, psolvers = time_form(V, ws, PLAIN, args.plain_steps, 0, "plain", conc)
         for _, s in psolvers:
             s.close()
         diff = 0.0
@@ -233,7 +243,7 @@ def main():
         "config": {"workload": name, "meshes": len(ws), "nV": sum(x.mesh.nV for x in ws),
                    "nTri": sum(x.mesh.nT for x in ws), "nz": w.nz,
                    "unknowns": 2 * sum(x.mesh.nT for x in ws), "seed": w.seed, "mesh": w.mesh.digest(),
-                   "stress_balance": w.stress_balance,
+                   "stress_balance": w.stress_balance, "concurrent_streams": conc,
                    "stop": "block-scaled nonlinear residual <= 1e-6",
                    "chosen": "Anderson(2, 1.0) Picard after 5-decade eps0 continuation, FP64 AMG-BiCGStab warm, "
                              "rtol 0.1",

@@ -151,9 +151,9 @@ class Krylov {
   int pc_ = UFE_PC_BLOCK_JACOBI;
   const AmgHierarchy *amg_ = nullptr;
   const int *A_diag_ = nullptr;
-  double amg_omega_ = 0.7;
+  double amg_omega_ = 0.85;
   int amg_coarse_sweeps_ = 20;
-  double amg_coarse_scale_ = 1.0;
+  double amg_coarse_scale_ = 1.5;
   AmgValues<double> amg64_;
   AmgValues<float> amg32_;
   std::vector<BsrT<double>> lev64_;

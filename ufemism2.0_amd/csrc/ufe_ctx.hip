@@ -733,8 +733,8 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
     c->kry = std::make_unique<Krylov>(nT, c->s);
     {
       const char *e;
-      double om = (e = std::getenv("UFE_AMG_OMEGA")) ? std::atof(e) : 0.7;
-      double cs = (e = std::getenv("UFE_AMG_SCALE")) ? std::atof(e) : 1.0;
+      double om = (e = std::getenv("UFE_AMG_OMEGA")) ? std::atof(e) : 0.85;
+      double cs = (e = std::getenv("UFE_AMG_SCALE")) ? std::atof(e) : 1.5;
       c->kry->set_amg_options(om, 0, cs);
     }
     c->kry->set_diag(m.bb_diag);

@@ -207,9 +207,15 @@ def channel(seed: int = 2, target_nV: int = 200_000, nz: int = 10, A: float = 1e
                     {"visc_it_norm_dUV_tol": 1e-3})
 
 
-def continental(seed: int = 3, target_nV: int = 400_000, nz: int = 12) -> Workload:
+def continental(seed: int = 3, target_nV: int = 400_000, nz: int = 12, sea_level: float = -1e4,
+                A: float = 1e-17, beta_sq: float = 1e5) -> Workload:
     """Fractal bed (spectral slope -2) rescaled to [-2000, 2000] m and a
-    perfectly plastic ice profile (yield stress 100 kPa) clipped to the bed."""
+    perfectly plastic ice profile (yield stress 100 kPa) clipped to the bed.
+    Sea level defaults to -10 km, so the sheet and the ice-free land around it
+    are grounded: with an ocean, the ice-free ocean triangles have no friction
+    and a viscosity floor of 0.1 m of ice, and their velocities run to the
+    velocity limit.  A = 1e-17 and beta_sq = 1e5 keep the interior speeds
+    at tens to hundreds of m/yr under the 100 kPa plastic driving stress."""
     rng = np.random.default_rng(seed)
     L = 2000e3
     n = 512
@@ -224,7 +230,7 @@ def continental(seed: int = 3, target_nV: int = 400_000, nz: int = 12) -> Worklo
     g = -L + dx * np.arange(n)
     R = 1500e3
     tau_y = 1e5
-    h_fine = np.sqrt((2 * L) ** 2 / target_nV) * 0.55
+    h_fine = np.sqrt((2 * L) ** 2 / target_nV) * 0.29
     h_coarse = 4.0 * h_fine
 
     def size(x, y):
@@ -239,7 +245,7 @@ def continental(seed: int = 3, target_nV: int = 400_000, nz: int = 12) -> Worklo
     Hi = np.where(d > 0, np.maximum(0.0, Hs_pl - np.maximum(Hb, 0.0)), 0.0)
     nV = mesh.nV
     return Workload("continental_diva", mesh, build_operators(mesh), zeta_regular(nz), Hi, Hb,
-                    np.zeros(nV), np.full(nz * nV, 1e-16), np.full(nV, 1e4), seed, "DIVA",
+                    np.full(nV, sea_level), np.full(nz * nV, A), np.full(nV, beta_sq), seed, "DIVA",
                     {"visc_it_norm_dUV_tol": 1e-3})
 
 
