@@ -8,6 +8,10 @@
 namespace ufe {
 
 constexpr int kMaxNz = 64;
+// Column kernels are instantiated for at most kRegNz layers (the per-column
+// arrays then stay in registers: every index is static after unrolling) and
+// for kMaxNz layers (arrays in per-lane scratch memory).
+constexpr int kRegNz = 16;
 
 // One thread per vertex:
 //   strain rates from u_vav (b->a ddx/ddy)            calc_horizontal_strain_rates
@@ -18,6 +22,7 @@ constexpr int kMaxNz = 64;
 //   (:497-511, integrate_from_zeta_is_one_to_zeta_is_zetap, mesh_zeta.f90:163-194),
 //   basal friction from u_base (:540) and beta_eff = beta / (1 + beta F2(surface))
 //   (:542-557).
+template <int NZ>
 __global__ void __launch_bounds__(kBlock) k_vertex_diva(
     int nV, int nz, const int *__restrict__ ptr, const int *__restrict__ ind, const double *__restrict__ wmap,
     const double *__restrict__ wdx, const double *__restrict__ wdy, const double *__restrict__ zeta,
@@ -29,8 +34,9 @@ __global__ void __launch_bounds__(kBlock) k_vertex_diva(
   int vi = blockIdx.x * blockDim.x + threadIdx.x;
   if (vi >= nV) return;
   double ux = 0, uy = 0, vx = 0, vy = 0, ub = 0, vb = 0;
-  double duz[kMaxNz], dvz[kMaxNz];
-  for (int k = 0; k < nz; ++k) { duz[k] = 0.0; dvz[k] = 0.0; }
+  double duz[NZ], dvz[NZ], et[NZ];
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) { duz[k] = 0.0; dvz[k] = 0.0; et[k] = 0.0; }
   for (int q = ptr[vi]; q < ptr[vi + 1]; ++q) {
     int t = ind[q];
     double2 uv = reinterpret_cast<const double2 *>(x)[t];
@@ -40,36 +46,47 @@ __global__ void __launch_bounds__(kBlock) k_vertex_diva(
     ux += cx * uv.x; uy += cy * uv.x; vx += cx * uv.y; vy += cy * uv.y;
     ub += cm * bs.x; vb += cm * bs.y;
     const double *e = eta3b + (size_t)t * nz;
-    for (int k = 0; k < nz; ++k) {
-      double inv = zeta[k] / fmax(p.visc_eff_min, e[k]);
-      duz[k] += cm * tb.x * inv;
-      dvz[k] += cm * tb.y * inv;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      if (k < nz) {
+        double inv = zeta[k] / fmax(p.visc_eff_min, e[k]);
+        duz[k] += cm * tb.x * inv;
+        dvz[k] += cm * tb.y * inv;
+      }
     }
   }
   double *eo = eta3a + (size_t)vi * nz;
   const double *Av = A_vk + (size_t)vi * nz;
   double eprev = 0.0, evav = 0.0;
-  for (int k = 0; k < nz; ++k) {
-    double e2 = ux * ux + vy * vy + ux * vy + 0.25 * (uy + vx) * (uy + vx) +
-                0.25 * (duz[k] * duz[k] + dvz[k] * dvz[k]) + p.eps0;
-    double eta = 0.5 * pow(Av[k], -1.0 / p.n) * pow(e2, (1.0 - p.n) / (2.0 * p.n));
-    eta = fmin(fmax(eta, p.visc_eff_min), p.eta_max);
-    eo[k] = eta;
-    if (k > 0) evav += 0.5 * (eta + eprev) * (zeta[k] - zeta[k - 1]);
-    eprev = eta;
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    if (k < nz) {
+      double e2 = ux * ux + vy * vy + ux * vy + 0.25 * (uy + vx) * (uy + vx) +
+                  0.25 * (duz[k] * duz[k] + dvz[k] * dvz[k]) + p.eps0;
+      double eta = 0.5 * pow(Av[k], -1.0 / p.n) * pow(e2, (1.0 - p.n) / (2.0 * p.n));
+      eta = fmin(fmax(eta, p.visc_eff_min), p.eta_max);
+      eo[k] = eta;
+      et[k] = eta;
+      if (k > 0) evav += 0.5 * (eta + eprev) * (zeta[k] - zeta[k - 1]);
+      eprev = eta;
+    }
   }
   double h = fmax(0.1, Hi[vi]);
   N_a[vi] = evav * fmax((double)0.1f, Hi[vi]);
   double *f1 = F1a + (size_t)vi * nz, *f2 = F2a + (size_t)vi * nz;
-  double I1 = 0.0, I2 = 0.0;
+  double I1 = 0.0, I2 = 0.0, f2s = 0.0;
   f1[nz - 1] = 0.0;
   f2[nz - 1] = 0.0;
-  for (int k = nz - 2; k >= 0; --k) {
-    double z0 = zeta[k], z1 = zeta[k + 1], e0 = eo[k], e1 = eo[k + 1];
-    I1 = I1 - 0.5 * (z1 / e1 + z0 / e0) * (z1 - z0);
-    I2 = I2 - 0.5 * (z1 * z1 / e1 + z0 * z0 / e0) * (z1 - z0);
-    f1[k] = -h * I1;
-    f2[k] = -h * I2;
+#pragma unroll
+  for (int k = NZ - 2; k >= 0; --k) {
+    if (k <= nz - 2) {
+      double z0 = zeta[k], z1 = zeta[k + 1], e0 = et[k], e1 = et[k + 1];
+      I1 = I1 - 0.5 * (z1 / e1 + z0 / e0) * (z1 - z0);
+      I2 = I2 - 0.5 * (z1 * z1 / e1 + z0 * z0 / e0) * (z1 - z0);
+      f1[k] = -h * I1;
+      f2[k] = -h * I2;
+      f2s = -h * I2;
+    }
   }
   double uabs = sqrt(p.delta_v * p.delta_v + ub * ub + vb * vb);
   double b;
@@ -81,12 +98,15 @@ __global__ void __launch_bounds__(kBlock) k_vertex_diva(
   }
   b = fmin(p.beta_max, b);
   beta_a[vi] = b;
-  beta_eff_a[vi] = p.no_sliding ? 1.0 / f2[0] : b / (1.0 + b * f2[0]);
+  // F2 at the surface (k = 0); 0 for a single-layer column
+  if (nz < 2) f2s = 0.0;
+  beta_eff_a[vi] = p.no_sliding ? 1.0 / f2s : b / (1.0 + b * f2s);
 }
 
 // One thread per triangle: eta_3D_b = map_a_b_3D(eta_3D_a) (DIVA_main.f90:458),
 // F2_3D_b at the surface (needed for the basal velocities) and, at the end of
 // the solve, the full F1_3D_b for calc_3D_velocities (:514).
+template <int NZ>
 __global__ void __launch_bounds__(kBlock) k_triangle_3d(int nT, int nz, const int *__restrict__ ptr,
                                                         const int *__restrict__ ind, const double *__restrict__ wmap,
                                                         const double *__restrict__ eta3a,
@@ -96,26 +116,35 @@ __global__ void __launch_bounds__(kBlock) k_triangle_3d(int nT, int nz, const in
                                                         int full_F1) {
   int ti = blockIdx.x * blockDim.x + threadIdx.x;
   if (ti >= nT) return;
-  double e[kMaxNz], f[kMaxNz];
-  for (int k = 0; k < nz; ++k) { e[k] = 0.0; f[k] = 0.0; }
+  double e[NZ], f[NZ];
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) { e[k] = 0.0; f[k] = 0.0; }
   double f20 = 0.0;
   for (int q = ptr[ti]; q < ptr[ti + 1]; ++q) {
     int v = ind[q];
     double cm = wmap[q];
     const double *ev = eta3a + (size_t)v * nz;
-    for (int k = 0; k < nz; ++k) e[k] += cm * ev[k];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+      if (k < nz) e[k] += cm * ev[k];
     f20 += cm * F2a[(size_t)v * nz];
     if (full_F1) {
       const double *fv = F1a + (size_t)v * nz;
-      for (int k = 0; k < nz; ++k) f[k] += cm * fv[k];
+#pragma unroll
+      for (int k = 0; k < NZ; ++k)
+        if (k < nz) f[k] += cm * fv[k];
     }
   }
   double *eo = eta3b + (size_t)ti * nz;
-  for (int k = 0; k < nz; ++k) eo[k] = e[k];
+#pragma unroll
+  for (int k = 0; k < NZ; ++k)
+    if (k < nz) eo[k] = e[k];
   F2b0[ti] = f20;
   if (full_F1) {
     double *fo = F1b + (size_t)ti * nz;
-    for (int k = 0; k < nz; ++k) fo[k] = f[k];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+      if (k < nz) fo[k] = f[k];
   }
 }
 
@@ -160,16 +189,25 @@ void launch_vertex_diva(const DevMesh &m, const double *x, const double *xbase, 
                         const double *eta3b_tk, const double *A_vk, const double *Hi, const double *rough,
                         const PicardParams &p, double *N_a, double *beta_a, double *beta_eff_a, double *eta3a_vk,
                         double *F1a_vk, double *F2a_vk, hipStream_t s) {
-  hipLaunchKernelGGL(k_vertex_diva, dim3(blocks_for(m.nV)), dim3(kBlock), 0, s, m.nV, m.nz, m.ba_ptr, m.ba_ind,
-                     m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, A_vk, Hi, rough, p, N_a,
-                     beta_a, beta_eff_a, eta3a_vk, F1a_vk, F2a_vk);
+  if (m.nz <= kRegNz)
+    hipLaunchKernelGGL(k_vertex_diva<kRegNz>, dim3(blocks_for(m.nV)), dim3(kBlock), 0, s, m.nV, m.nz, m.ba_ptr,
+                       m.ba_ind, m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, A_vk, Hi, rough, p,
+                       N_a, beta_a, beta_eff_a, eta3a_vk, F1a_vk, F2a_vk);
+  else
+    hipLaunchKernelGGL(k_vertex_diva<kMaxNz>, dim3(blocks_for(m.nV)), dim3(kBlock), 0, s, m.nV, m.nz, m.ba_ptr,
+                       m.ba_ind, m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, A_vk, Hi, rough, p,
+                       N_a, beta_a, beta_eff_a, eta3a_vk, F1a_vk, F2a_vk);
   UFE_LAUNCH_CHECK();
 }
 
 void launch_triangle_3d(const DevMesh &m, const double *eta3a_vk, const double *F1a_vk, const double *F2a_vk,
                         double *eta3b_tk, double *F1b_tk, double *F2b0, int full_F1, hipStream_t s) {
-  hipLaunchKernelGGL(k_triangle_3d, dim3(blocks_for(m.nT)), dim3(kBlock), 0, s, m.nT, m.nz, m.ab_ptr, m.ab_ind,
-                     m.ab_map, eta3a_vk, F1a_vk, F2a_vk, eta3b_tk, F1b_tk, F2b0, full_F1);
+  if (m.nz <= kRegNz)
+    hipLaunchKernelGGL(k_triangle_3d<kRegNz>, dim3(blocks_for(m.nT)), dim3(kBlock), 0, s, m.nT, m.nz, m.ab_ptr,
+                       m.ab_ind, m.ab_map, eta3a_vk, F1a_vk, F2a_vk, eta3b_tk, F1b_tk, F2b0, full_F1);
+  else
+    hipLaunchKernelGGL(k_triangle_3d<kMaxNz>, dim3(blocks_for(m.nT)), dim3(kBlock), 0, s, m.nT, m.nz, m.ab_ptr,
+                       m.ab_ind, m.ab_map, eta3a_vk, F1a_vk, F2a_vk, eta3b_tk, F1b_tk, F2b0, full_F1);
   UFE_LAUNCH_CHECK();
 }
 

@@ -69,9 +69,10 @@ struct AmgHierarchy {
 
 // Build the hierarchy on the host from the block pattern of the system and
 // the triangle adjacency (TriC).  coarse_target: stop when a level has at
-// most this many block rows.
+// most this many block rows.  passes: greedy aggregation passes per level
+// (each pass aggregates the previous pass's aggregates).
 std::unique_ptr<AmgHierarchy> build_amg_hierarchy(int nT, const int *TriC, const int *ptr0, const int *col0,
-                                                  int coarse_target, int max_levels);
+                                                  int coarse_target, int max_levels, int passes = 1);
 
 template <typename T>
 struct AmgValues {

@@ -576,6 +576,26 @@ static std::vector<int> greedy_aggregate(int n, const std::vector<int> &aptr, co
   return agg;
 }
 
+// Graph of aggregates: two aggregates are adjacent if any of their members are.
+static void aggregate_graph(int n, const std::vector<int> &aptr, const std::vector<int> &aind,
+                            const std::vector<int> &agg, int na, std::vector<int> &nptr, std::vector<int> &nind) {
+  std::vector<std::vector<int>> nbrs(na);
+  for (int i = 0; i < n; ++i)
+    for (int k = aptr[i]; k < aptr[i + 1]; ++k) {
+      int a = agg[i], b = agg[aind[k]];
+      if (a != b) nbrs[a].push_back(b);
+    }
+  nptr.assign(na + 1, 0);
+  nind.clear();
+  for (int I = 0; I < na; ++I) {
+    auto &v = nbrs[I];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    nind.insert(nind.end(), v.begin(), v.end());
+    nptr[I + 1] = (int)nind.size();
+  }
+}
+
 template <typename U>
 static void upload(DBuf &d, const std::vector<U> &h) {
   d.alloc(std::max<size_t>(h.size(), 1) * sizeof(U));
@@ -583,7 +603,7 @@ static void upload(DBuf &d, const std::vector<U> &h) {
 }
 
 std::unique_ptr<AmgHierarchy> build_amg_hierarchy(int nT, const int *TriC, const int *ptr0, const int *col0,
-                                                  int coarse_target, int max_levels) {
+                                                  int coarse_target, int max_levels, int passes) {
   auto H = std::make_unique<AmgHierarchy>();
   coarse_target = std::min(coarse_target, kCoarseMax);
   // level-0 aggregation graph: triangle neighbours across edges
@@ -616,6 +636,15 @@ std::unique_ptr<AmgHierarchy> build_amg_hierarchy(int nT, const int *TriC, const
     std::vector<int> agg;
     if (!last) {
       agg = greedy_aggregate(n, aptr, aind, &na);
+      // further passes aggregate the aggregates (coarsening ratio ~4 per pass)
+      for (int pass = 1; pass < passes && na > coarse_target; ++pass) {
+        std::vector<int> gp, gi;
+        aggregate_graph(n, aptr, aind, agg, na, gp, gi);
+        int na2 = 0;
+        std::vector<int> agg2 = greedy_aggregate(na, gp, gi, &na2);
+        for (int i = 0; i < n; ++i) agg[i] = agg2[agg[i]];
+        na = na2;
+      }
       if (na >= n || na < 1) last = true;
     }
     if (last) {
@@ -652,21 +681,9 @@ std::unique_ptr<AmgHierarchy> build_amg_hierarchy(int nT, const int *TriC, const
       cptr[I + 1] = (int)ccol.size();
       std::vector<std::pair<int, int>>().swap(rw);
     }
-    // next-level aggregation graph: aggregates adjacent if any members are
-    std::vector<std::vector<int>> nbrs(na);
-    for (int i = 0; i < n; ++i)
-      for (int k = aptr[i]; k < aptr[i + 1]; ++k) {
-        int a = agg[i], b = agg[aind[k]];
-        if (a != b) nbrs[a].push_back(b);
-      }
-    std::vector<int> nptr(na + 1, 0), nind;
-    for (int I = 0; I < na; ++I) {
-      auto &v = nbrs[I];
-      std::sort(v.begin(), v.end());
-      v.erase(std::unique(v.begin(), v.end()), v.end());
-      nind.insert(nind.end(), v.begin(), v.end());
-      nptr[I + 1] = (int)nind.size();
-    }
+    // next-level aggregation graph
+    std::vector<int> nptr, nind;
+    aggregate_graph(n, aptr, aind, agg, na, nptr, nind);
     H->levels.push_back(std::move(L));
     aptr.swap(nptr);
     aind.swap(nind);

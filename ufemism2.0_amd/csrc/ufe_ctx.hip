@@ -738,7 +738,11 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
       c->kry->set_amg_options(om, 0, cs);
     }
     c->kry->set_diag(m.bb_diag);
-    c->amg = build_amg_hierarchy(nT, md->TriC, md->bb_ptr, ind.data(), 24, 16);
+    {
+      const char *e = std::getenv("UFE_AMG_PASSES");
+      int passes = e ? std::max(1, std::min(3, std::atoi(e))) : 2;
+      c->amg = build_amg_hierarchy(nT, md->TriC, md->bb_ptr, ind.data(), 24, 16, passes);
+    }
     if (c->amg) c->kry->attach_amg(c->amg.get());
     build_rowtypes(c.get(), nullptr);
     UFE_HIP(hipStreamSynchronize(c->s));
