@@ -128,6 +128,12 @@ typedef struct ufe_params {
   double accel_mixing;               /*                                  0.5    */
   double accel_restart;              /*                                  10     */
   int    accel_continuation;         /*                                  0      */
+  /* 1: the multigrid preconditioner reads an FP32 copy of the finest-level
+   * matrix (values rounded once per Picard iteration; vectors and arithmetic
+   * stay FP64, and the Krylov operator itself is the FP64 matrix).  The
+   * preconditioner need not be exact, so this changes iteration counts at most
+   * slightly and halves the bytes of the V-cycle's finest-level passes. */
+  int    precond_fp32;               /*                                  0      */
 } ufe_params;
 
 #define UFE_ACCEL_MAX_DEPTH 8

@@ -4,7 +4,10 @@ residual 1e-8), so the GPU parity tests need not rerun a multi-second CPU
 solve.  The workload is regenerated from its seed; the mesh digest stored
 with the fixture guards against a different mesh.
 
-Run: python tests/golden/make_oracle_fixtures.py
+oracle_channel_seeds.npz holds the same solves for the channel's random-field
+seeds 3..6 (velocities rounded to float32, ample for a 1e-5 comparison).
+
+Run: python tests/golden/make_oracle_fixtures.py [main|seeds]   (both by default)
 """
 
 import os
@@ -23,6 +26,7 @@ from ufemism2_amd import _lib as L, workloads as W  # noqa: E402
 from oracle import binding as OB  # noqa: E402
 
 CHANNEL_NV = 1500
+SEEDS = (3, 4, 5, 6)
 SETTINGS = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, rtol=1e-11, stop_rule=1, nonlinear_rtol=1e-8,
                 visc_it_nit=800)
 
@@ -34,17 +38,39 @@ def params():
     return p
 
 
+def solve(w, sb):
+    o = OB.Oracle(w.descriptor())
+    inp = w.inputs()
+    if sb == "SSA":
+        rc, u, v, info = o.solve_ssa(params(), inp)
+        return rc, u, v, info, None
+    rc, st, info = o.solve_diva(params(), inp)
+    return rc, st["u"], st["v"], info, st
+
+
+def seeds():
+    out = {}
+    for seed in SEEDS:
+        for sb in ("SSA", "DIVA"):
+            w = W.channel(seed=seed, target_nV=CHANNEL_NV, nz=10, stress_balance=sb)
+            rc, u, v, info, _ = solve(w, sb)
+            assert rc == 0, rc
+            key = f"{sb.lower()}_{seed}"
+            out[key + "_u"] = u.astype(np.float32)
+            out[key + "_v"] = v.astype(np.float32)
+            out[key + "_digest"] = w.mesh.digest()
+            print(seed, sb, info.n_visc_its, info.rel_residual, flush=True)
+    path = os.path.join(HERE, "oracle_channel_seeds.npz")
+    np.savez_compressed(path, seeds=np.array(SEEDS), **out)
+    print("->", path, flush=True)
+
+
 def main():
     for sb in ("SSA", "DIVA"):
         w = W.channel_small(target_nV=CHANNEL_NV, stress_balance=sb)
-        o = OB.Oracle(w.descriptor())
-        inp = w.inputs()
-        if sb == "SSA":
-            rc, u, v, info = o.solve_ssa(params(), inp)
-            extra = {}
-        else:
-            rc, st, info = o.solve_diva(params(), inp)
-            u, v = st["u"], st["v"]
+        rc, u, v, info, st = solve(w, sb)
+        extra = {}
+        if sb == "DIVA":
             nz, nT = w.nz, w.mesh.nT
             u3 = st["u3d"].reshape(nz, nT)
             extra = dict(u_base=st["ubase"], v_base=st["vbase"], tau_bx=st["tbx"], tau_by=st["tby"],
@@ -57,4 +83,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    what = sys.argv[1:] or ["main", "seeds"]
+    if "main" in what:
+        main()
+    if "seeds" in what:
+        seeds()

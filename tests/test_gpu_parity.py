@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
 TIGHT = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-8, visc_it_nit=800,
              krylov_maxit=20000)
-CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.1, accel_depth=2, accel_mixing=1.0, accel_continuation=5)
+CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.01, accel_depth=2, accel_mixing=1.0, accel_continuation=5)
 PLAIN = dict(precision=0, precond=3, warm_start=0, rtol=1e-10)
 
 
@@ -168,6 +168,32 @@ def test_channel_converged_solve_matches_oracle(V, channel_fixtures, sb, form):
         nT, nz = w.mesh.nT, w.nz
         u3 = st["u_3D_b"].reshape(nz, nT)
         assert max_rel(u3[0], f["u3d_surface"]) < 1e-5 and max_rel(u3[-1], f["u3d_base"]) < 1e-5
+
+
+SEED_FIXTURES = os.path.join(HERE, "golden", "oracle_channel_seeds.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(SEED_FIXTURES), reason="oracle_channel_seeds.npz not generated")
+@pytest.mark.parametrize("sb", ["SSA", "DIVA"])
+def test_chosen_form_reaches_the_oracle_solution_on_other_seeds(V, W, sb):
+    """The channel's discrete equations admit, besides the reference
+    iteration's solution, localised solutions with a spurious back-flowing
+    patch of grounded ice (nonlinear residual 1e-9, hundreds of m/yr off over
+    ~90 triangles).  An inexact Picard iteration can be drawn into one; the
+    chosen form's inner tolerance is set so that it reaches the oracle's
+    solution.  Four more random-field seeds of the 1.5k-vertex channel."""
+    f = np.load(SEED_FIXTURES)
+    for seed in f["seeds"]:
+        w = W.channel(seed=int(seed), target_nV=1500, nz=10, stress_balance=sb)
+        key = f"{sb.lower()}_{seed}"
+        assert str(f[key + "_digest"]) == w.mesh.digest()
+        st, info = solve(V, w, **TIGHT, **CHOSEN)
+        assert info.rel_residual <= 1e-8, (seed, info.rel_residual)
+        fu, fv = f[key + "_u"].astype(np.float64), f[key + "_v"].astype(np.float64)
+        sc = max(np.abs(fu).max(), np.abs(fv).max())
+        err = max(np.abs(st["u_vav_b"] - fu).max(), np.abs(st["v_vav_b"] - fv).max()) / sc
+        print(f"seed {seed} {sb}: picard {info.n_visc_its} krylov {info.n_Axb_its} err {err:.1e}", flush=True)
+        assert err < 1e-5, (seed, err)
 
 
 def test_chosen_form_is_deterministic(V, channel_fixtures):

@@ -64,6 +64,7 @@ class Params(C.Structure):
         ("accel_mixing", C.c_double),
         ("accel_restart", C.c_double),
         ("accel_continuation", C.c_int),
+        ("precond_fp32", C.c_int),
     ]
 
 

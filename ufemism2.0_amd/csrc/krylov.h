@@ -100,6 +100,8 @@ class Krylov {
   void set_scale(const double4 *d64, const float4 *d32) { dscale64_ = d64; dscale32_ = d32; }
   int precond() const { return pc_; }
   void attach_amg(const AmgHierarchy *h) { amg_ = h; }
+  // FP32 copy of the finest-level matrix for the FP64 V-cycle (null: use the FP64 values)
+  void set_amg_fine32(const float4 *v) { fine32_ = v; }
   // Recompute Galerkin values on all levels from the level-0 matrix.
   void amg_setup64(const BsrT<double> &A0);
   void amg_setup32(const BsrT<float> &A0);
@@ -152,6 +154,7 @@ class Krylov {
   int pc_ = UFE_PC_BLOCK_JACOBI;
   const AmgHierarchy *amg_ = nullptr;
   const int *A_diag_ = nullptr;
+  const float4 *fine32_ = nullptr;
   double amg_omega_ = 0.85;
   int amg_coarse_sweeps_ = 20;
   double amg_coarse_scale_ = 1.5;

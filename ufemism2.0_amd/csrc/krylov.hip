@@ -359,10 +359,11 @@ __global__ void __launch_bounds__(kBlock) k_amg_presmooth(int nb, const typename
   z[i] = o;
 }
 
-// q = r - A z, kLpr lanes per block row (same access pattern as the SpMV).
-template <typename T>
+// q = r - A z, kLpr lanes per block row (same access pattern as the SpMV);
+// MT is the storage type of the matrix values (arithmetic in T).
+template <typename T, typename MT = T>
 __global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restrict__ ptr, const int *__restrict__ col,
-                                                      const typename VecT<T>::V4 *__restrict__ val,
+                                                      const typename VecT<MT>::V4 *__restrict__ val,
                                                       const typename VecT<T>::V2 *__restrict__ r,
                                                       const typename VecT<T>::V2 *__restrict__ z,
                                                       typename VecT<T>::V2 *__restrict__ q,
@@ -373,10 +374,10 @@ __global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restr
   if (row >= nb) return;
   T y0 = 0, y1 = 0;
   for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLpr) {
-    typename VecT<T>::V4 a = val[k];
+    typename VecT<MT>::V4 a = val[k];
     typename VecT<T>::V2 zj = z[col[k]];
-    y0 += a.x * zj.x + a.y * zj.y;
-    y1 += a.z * zj.x + a.w * zj.y;
+    y0 += (T)a.x * zj.x + (T)a.y * zj.y;
+    y1 += (T)a.z * zj.x + (T)a.w * zj.y;
   }
 #pragma unroll
   for (int off = kLpr / 2; off > 0; off >>= 1) {
@@ -424,9 +425,9 @@ __global__ void __launch_bounds__(kBlock) k_amg_restrict(int nc, const int *__re
 
 // zout_i = zn_i + omega D^-1 (r - A zn)_i with zn = z + scale * P ec: the
 // prolongation fused into the post-smoothing sweep, kLpr lanes per block row.
-template <typename T>
+template <typename T, typename MT = T>
 __global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
-    int nb, const int *__restrict__ ptr, const int *__restrict__ col, const typename VecT<T>::V4 *__restrict__ val,
+    int nb, const int *__restrict__ ptr, const int *__restrict__ col, const typename VecT<MT>::V4 *__restrict__ val,
     const typename VecT<T>::V4 *__restrict__ dinv, const int *__restrict__ agg,
     const typename VecT<T>::V2 *__restrict__ r, const typename VecT<T>::V2 *__restrict__ z,
     const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega, typename VecT<T>::V2 *__restrict__ zout,
@@ -438,11 +439,11 @@ __global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
   T y0 = 0, y1 = 0;
   for (int k = ptr[i] + lane; k < ptr[i + 1]; k += kLpr) {
     int j = col[k];
-    typename VecT<T>::V4 a = val[k];
+    typename VecT<MT>::V4 a = val[k];
     typename VecT<T>::V2 zj = z[j], ej = ec[agg[j]];
     T z0 = zj.x + scale * ej.x, z1 = zj.y + scale * ej.y;
-    y0 += a.x * z0 + a.y * z1;
-    y1 += a.z * z0 + a.w * z1;
+    y0 += (T)a.x * z0 + (T)a.y * z1;
+    y1 += (T)a.z * z0 + (T)a.w * z1;
   }
 #pragma unroll
   for (int off = kLpr / 2; off > 0; off >>= 1) {
@@ -829,9 +830,13 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     bool next_coarsest = (l + 1 == L - 1);
     int nc = lev[l + 1].nb;
-    hipLaunchKernelGGL(k_amg_resid<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0, s_,
-                       lev[l].nb, lev[l].ptr, lev[l].col, lev[l].val, rin, av.z[l]->template as<V2>(),
-                       av.zt[l]->template as<V2>(), sc);
+    const dim3 gl(xcd_grid(blocks_for((long)lev[l].nb * kLpr)));
+    if (l == 0 && fine32_ && std::is_same<T, double>::value)
+      hipLaunchKernelGGL((k_amg_resid<T, float>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr, lev[l].col,
+                         fine32_, rin, av.z[l]->template as<V2>(), av.zt[l]->template as<V2>(), sc);
+    else
+      hipLaunchKernelGGL((k_amg_resid<T, T>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr, lev[l].col,
+                         lev[l].val, rin, av.z[l]->template as<V2>(), av.zt[l]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_amg_restrict<T>, dim3(blocks_for(nc)), dim3(kBlock), 0, s_, nc, P.mem_ptr.as<int>(),
                        P.mem_ind.as<int>(), av.zt[l]->template as<V2>(), av.r[l + 1]->template as<V2>(),
@@ -848,10 +853,15 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     const AmgLevelPattern &P = *amg_->levels[l];
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     V2 *zout = l == 0 ? reinterpret_cast<V2 *>(z) : av.zt[l]->template as<V2>();
-    hipLaunchKernelGGL(k_amg_prolong_smooth<T>, dim3(xcd_grid(blocks_for((long)lev[l].nb * kLpr))), dim3(kBlock), 0,
-                       s_, lev[l].nb,
-                       lev[l].ptr, lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin,
-                       av.z[l]->template as<V2>(), av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
+    const dim3 gl(xcd_grid(blocks_for((long)lev[l].nb * kLpr)));
+    if (l == 0 && fine32_ && std::is_same<T, double>::value)
+      hipLaunchKernelGGL((k_amg_prolong_smooth<T, float>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr,
+                         lev[l].col, fine32_, lev[l].dinv, P.agg.as<int>(), rin, av.z[l]->template as<V2>(),
+                         av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
+    else
+      hipLaunchKernelGGL((k_amg_prolong_smooth<T, T>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr,
+                         lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin, av.z[l]->template as<V2>(),
+                         av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
     UFE_LAUNCH_CHECK();
     if (l > 0) std::swap(av.z[l], av.zt[l]);
   }

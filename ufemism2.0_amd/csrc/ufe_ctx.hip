@@ -177,6 +177,7 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
   K.set_scale(nullptr, nullptr);
   if (q.precision == UFE_PREC_FP64) {
     BsrT<double> A = sys64(c);
+    K.set_amg_fine32(q.precond_fp32 && q.precond == UFE_PC_AMG ? c->A32.as<float4>() : nullptr);
     if (q.precond == UFE_PC_AMG) K.amg_setup64(A);
     SolveStats st = K.bicgstab64(A, c->rhs_s.as<double>(), c->x.as<double>(), q.rtol, q.abstol, q.krylov_maxit,
                                  !warm, q.check_every);
@@ -186,6 +187,7 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
   // mixed: FP64 residual, FP32 corrections
   BsrT<double> A = sys64(c);
   BsrT<float> A32 = sys32(c);
+  K.set_amg_fine32(nullptr);
   if (q.precond == UFE_PC_AMG) K.amg_setup32(A32);
   double rr, bb;
   K.residual64(A, c->x.as<double>(), c->rhs_s.as<double>(), c->r64.as<double>(), &rr, &bb);
@@ -258,7 +260,7 @@ static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
                   c->bcu.as<double>(), c->bcv.as<double>(), p, c->tri_coef.as<double4>(),
                   c->beta_basal_b.as<double>(), c->rhs.as<double>(), s);
   UFE_HIP(hipEventRecord(c->ev[2], s));
-  bool mirror = q.precision == UFE_PREC_MIXED;
+  bool mirror = q.precision == UFE_PREC_MIXED || (q.precond_fp32 && q.precond == UFE_PC_AMG);
   launch_assemble(c->m, c->rowtype.as<signed char>(), c->tri_coef.as<double4>(), q.include_crossterms,
                   c->rhs.as<double>(), c->A64.as<double4>(), mirror ? c->A32.as<float4>() : nullptr,
                   c->Dblk.as<double4>(), mirror ? c->Dblk32.as<float4>() : nullptr, c->Dinv64.as<double4>(),
@@ -740,7 +742,7 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
     c->kry->set_diag(m.bb_diag);
     {
       const char *e = std::getenv("UFE_AMG_PASSES");
-      int passes = e ? std::max(1, std::min(3, std::atoi(e))) : 2;
+      int passes = e ? std::max(1, std::min(3, std::atoi(e))) : 1;
       c->amg = build_amg_hierarchy(nT, md->TriC, md->bb_ptr, ind.data(), 24, 16, passes);
     }
     if (c->amg) c->kry->attach_amg(c->amg.get());

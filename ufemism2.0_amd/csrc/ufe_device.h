@@ -110,11 +110,24 @@ __device__ __forceinline__ void block_partials(const RedSlot &slot, const double
 template <int W>
 __device__ __forceinline__ void partials_sum(const RedSlot &slot, int nparts, double (&out)[W]) {
   __shared__ double lds[W][kBlock / 64];
+  constexpr int U = 8;  // independent loads in flight per thread
 #pragma unroll
   for (int i = 0; i < W; ++i) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) s += slot.partials[i * kMaxPartials + b];
-    out[i] = s;
+    const double *p = slot.partials + i * kMaxPartials;
+    double s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] = 0.0;
+    for (int b0 = 0; b0 < nparts; b0 += U * kBlock) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = b0 + u * kBlock + (int)threadIdx.x;
+        if (b < nparts) s[u] += p[b];
+      }
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) t += s[u];
+    out[i] = t;
   }
   block_sum<W>(out, lds);
 }

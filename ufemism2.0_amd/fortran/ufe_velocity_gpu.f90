@@ -52,6 +52,7 @@ module ufe_velocity_gpu
     integer(c_int) :: accel_depth
     real(c_double) :: accel_mixing, accel_restart
     integer(c_int) :: accel_continuation
+    integer(c_int) :: precond_fp32
   end type ufe_gpu_params
 
   ! struct ufe_solve_info
