@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
 TIGHT = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-8, visc_it_nit=800,
              krylov_maxit=20000)
-CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.01, accel_depth=2, accel_mixing=1.0, accel_continuation=5)
+CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.01, accel_depth=3, accel_mixing=1.0, accel_continuation=5,
+              precond_fp32=1)
 PLAIN = dict(precision=0, precond=3, warm_start=0, rtol=1e-10)
 
 
@@ -194,6 +195,21 @@ def test_chosen_form_reaches_the_oracle_solution_on_other_seeds(V, W, sb):
         err = max(np.abs(st["u_vav_b"] - fu).max(), np.abs(st["v_vav_b"] - fv).max()) / sc
         print(f"seed {seed} {sb}: picard {info.n_visc_its} krylov {info.n_Axb_its} err {err:.1e}", flush=True)
         assert err < 1e-5, (seed, err)
+
+
+@pytest.mark.parametrize("sb", ["SSA", "DIVA"])
+def test_amg_coarse_tail_kernel_equals_per_level_kernels(V, channel_fixtures, monkeypatch, sb):
+    """The one-workgroup kernel for the coarse levels of the V-cycle runs the
+    same per-row arithmetic as the per-level kernels: identical iterations and
+    bit-identical velocities (UFE_AMG_TAIL_ROWS sets the largest level it
+    takes, 0 = none, for the contexts created afterwards)."""
+    w, _ = channel_fixtures[sb]
+    monkeypatch.setenv("UFE_AMG_TAIL_ROWS", "4096")
+    a, ia = solve(V, w, **TIGHT, **CHOSEN)
+    monkeypatch.setenv("UFE_AMG_TAIL_ROWS", "0")
+    b, ib = solve(V, w, **TIGHT, **CHOSEN)
+    assert ia.n_visc_its == ib.n_visc_its and ia.n_Axb_its == ib.n_Axb_its
+    assert np.array_equal(a["u_vav_b"], b["u_vav_b"]) and np.array_equal(a["v_vav_b"], b["v_vav_b"])
 
 
 def test_chosen_form_is_deterministic(V, channel_fixtures):

@@ -108,6 +108,9 @@ class Krylov {
   void set_amg_options(double omega, int coarse_sweeps, double coarse_scale) {
     amg_omega_ = omega; amg_coarse_sweeps_ = coarse_sweeps; amg_coarse_scale_ = coarse_scale;
   }
+  // Run the coarse levels of the V-cycle with at most max_rows block rows in
+  // one workgroup (0: one launch per pass on every level).
+  void set_amg_tail(int max_rows) { amg_tail_rows_ = max_rows; }
 
   // BiCGStab: x in (initial guess unless zero_guess) / out.  Stops when
   // ||r|| <= max(rtol*||r0||, abstol).
@@ -155,6 +158,7 @@ class Krylov {
   const AmgHierarchy *amg_ = nullptr;
   const int *A_diag_ = nullptr;
   const float4 *fine32_ = nullptr;
+  int amg_tail_rows_ = 300;
   double amg_omega_ = 0.85;
   int amg_coarse_sweeps_ = 20;
   double amg_coarse_scale_ = 1.5;

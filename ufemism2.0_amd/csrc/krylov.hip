@@ -359,19 +359,18 @@ __global__ void __launch_bounds__(kBlock) k_amg_presmooth(int nb, const typename
   z[i] = o;
 }
 
-// q = r - A z, kLpr lanes per block row (same access pattern as the SpMV);
-// MT is the storage type of the matrix values (arithmetic in T).
-template <typename T, typename MT = T>
-__global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restrict__ ptr, const int *__restrict__ col,
-                                                      const typename VecT<MT>::V4 *__restrict__ val,
-                                                      const typename VecT<T>::V2 *__restrict__ r,
-                                                      const typename VecT<T>::V2 *__restrict__ z,
-                                                      typename VecT<T>::V2 *__restrict__ q,
-                                                      const KrylovScalars *__restrict__ sc) {
-  if (sc && sc->done) return;
-  const int lane = threadIdx.x % kLpr;
-  const int row = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
-  if (row >= nb) return;
+// Per-row bodies of the V-cycle passes, shared by the grid-wide kernels (fine
+// levels) and the one-workgroup kernel of the coarse levels.  kLpr lanes
+// cooperate on one block row or aggregate; all lanes of a group take the same
+// row, so the width-kLpr shuffles stay within converged lanes.
+
+// q_i = r_i - (A z)_i; MT is the storage type of the matrix values (arithmetic in T).
+template <typename T, typename MT>
+__device__ __forceinline__ void row_resid(int row, int lane, const int *__restrict__ ptr,
+                                          const int *__restrict__ col, const typename VecT<MT>::V4 *__restrict__ val,
+                                          const typename VecT<T>::V2 *__restrict__ r,
+                                          const typename VecT<T>::V2 *__restrict__ z,
+                                          typename VecT<T>::V2 *__restrict__ q) {
   T y0 = 0, y1 = 0;
   for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLpr) {
     typename VecT<MT>::V4 a = val[k];
@@ -392,50 +391,52 @@ __global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restr
   }
 }
 
-// rc[I] = sum of q over the members of aggregate I, and zc[I] = omega Dc^-1 rc[I]
-// (the next level's pre-smoothing from zero).
+// rc_I = sum of q over the members of aggregate I, and zc_I = omega Dc^-1 rc_I
+// (the next level's pre-smoothing from zero) when zc is given.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_amg_restrict(int nc, const int *__restrict__ mem_ptr,
-                                                         const int *__restrict__ mem_ind,
-                                                         const typename VecT<T>::V2 *__restrict__ q,
-                                                         typename VecT<T>::V2 *__restrict__ rc,
-                                                         const typename VecT<T>::V4 *__restrict__ cdinv, T omega,
-                                                         typename VecT<T>::V2 *__restrict__ zc,
-                                                         const KrylovScalars *__restrict__ sc) {
-  if (sc && sc->done) return;
-  int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nc) return;
+__device__ __forceinline__ void agg_restrict(int I, int lane, const int *__restrict__ mem_ptr,
+                                             const int *__restrict__ mem_ind,
+                                             const typename VecT<T>::V2 *__restrict__ q,
+                                             typename VecT<T>::V2 *__restrict__ rc,
+                                             const typename VecT<T>::V4 *__restrict__ cdinv, T omega,
+                                             typename VecT<T>::V2 *__restrict__ zc) {
   double s0 = 0, s1 = 0;
-  for (int m = mem_ptr[I]; m < mem_ptr[I + 1]; ++m) {
+  for (int m = mem_ptr[I] + lane; m < mem_ptr[I + 1]; m += kLpr) {
     typename VecT<T>::V2 qi = q[mem_ind[m]];
     s0 += qi.x;
     s1 += qi.y;
   }
-  typename VecT<T>::V2 o;
-  o.x = (T)s0; o.y = (T)s1;
-  rc[I] = o;
-  if (zc) {
-    typename VecT<T>::V4 d = cdinv[I];
-    typename VecT<T>::V2 zz;
-    zz.x = omega * (T)(d.x * s0 + d.y * s1);
-    zz.y = omega * (T)(d.z * s0 + d.w * s1);
-    zc[I] = zz;
+#pragma unroll
+  for (int off = kLpr / 2; off > 0; off >>= 1) {
+    s0 += __shfl_down(s0, off, kLpr);
+    s1 += __shfl_down(s1, off, kLpr);
+  }
+  if (lane == 0) {
+    typename VecT<T>::V2 o;
+    o.x = (T)s0; o.y = (T)s1;
+    rc[I] = o;
+    if (zc) {
+      typename VecT<T>::V4 d = cdinv[I];
+      typename VecT<T>::V2 zz;
+      zz.x = omega * (T)(d.x * s0 + d.y * s1);
+      zz.y = omega * (T)(d.z * s0 + d.w * s1);
+      zc[I] = zz;
+    }
   }
 }
 
 // zout_i = zn_i + omega D^-1 (r - A zn)_i with zn = z + scale * P ec: the
-// prolongation fused into the post-smoothing sweep, kLpr lanes per block row.
-template <typename T, typename MT = T>
-__global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
-    int nb, const int *__restrict__ ptr, const int *__restrict__ col, const typename VecT<MT>::V4 *__restrict__ val,
-    const typename VecT<T>::V4 *__restrict__ dinv, const int *__restrict__ agg,
-    const typename VecT<T>::V2 *__restrict__ r, const typename VecT<T>::V2 *__restrict__ z,
-    const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega, typename VecT<T>::V2 *__restrict__ zout,
-    const KrylovScalars *__restrict__ sc) {
-  if (sc && sc->done) return;
-  const int lane = threadIdx.x % kLpr;
-  const int i = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
-  if (i >= nb) return;
+// prolongation fused into the post-smoothing sweep.
+template <typename T, typename MT>
+__device__ __forceinline__ void row_prolong_smooth(int i, int lane, const int *__restrict__ ptr,
+                                                   const int *__restrict__ col,
+                                                   const typename VecT<MT>::V4 *__restrict__ val,
+                                                   const typename VecT<T>::V4 *__restrict__ dinv,
+                                                   const int *__restrict__ agg,
+                                                   const typename VecT<T>::V2 *__restrict__ r,
+                                                   const typename VecT<T>::V2 *__restrict__ z,
+                                                   const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega,
+                                                   typename VecT<T>::V2 *__restrict__ zout) {
   T y0 = 0, y1 = 0;
   for (int k = ptr[i] + lane; k < ptr[i + 1]; k += kLpr) {
     int j = col[k];
@@ -459,6 +460,46 @@ __global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
     o.y = zi.y + scale * ei.y + omega * (d.z * q0 + d.w * q1);
     zout[i] = o;
   }
+}
+
+template <typename T, typename MT = T>
+__global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restrict__ ptr, const int *__restrict__ col,
+                                                      const typename VecT<MT>::V4 *__restrict__ val,
+                                                      const typename VecT<T>::V2 *__restrict__ r,
+                                                      const typename VecT<T>::V2 *__restrict__ z,
+                                                      typename VecT<T>::V2 *__restrict__ q,
+                                                      const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  const int row = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
+  if (row >= nb) return;
+  row_resid<T, MT>(row, threadIdx.x % kLpr, ptr, col, val, r, z, q);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_amg_restrict(int nc, const int *__restrict__ mem_ptr,
+                                                         const int *__restrict__ mem_ind,
+                                                         const typename VecT<T>::V2 *__restrict__ q,
+                                                         typename VecT<T>::V2 *__restrict__ rc,
+                                                         const typename VecT<T>::V4 *__restrict__ cdinv, T omega,
+                                                         typename VecT<T>::V2 *__restrict__ zc,
+                                                         const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  const int I = (blockIdx.x * blockDim.x + threadIdx.x) / kLpr;
+  if (I >= nc) return;
+  agg_restrict<T>(I, threadIdx.x % kLpr, mem_ptr, mem_ind, q, rc, cdinv, omega, zc);
+}
+
+template <typename T, typename MT = T>
+__global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
+    int nb, const int *__restrict__ ptr, const int *__restrict__ col, const typename VecT<MT>::V4 *__restrict__ val,
+    const typename VecT<T>::V4 *__restrict__ dinv, const int *__restrict__ agg,
+    const typename VecT<T>::V2 *__restrict__ r, const typename VecT<T>::V2 *__restrict__ z,
+    const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega, typename VecT<T>::V2 *__restrict__ zout,
+    const KrylovScalars *__restrict__ sc) {
+  if (sc && sc->done) return;
+  const int i = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
+  if (i >= nb) return;
+  row_prolong_smooth<T, MT>(i, threadIdx.x % kLpr, ptr, col, val, dinv, agg, r, z, ec, scale, omega, zout);
 }
 
 // Coarsest level: an exact solve with the dense inverse of the coarsest
@@ -538,6 +579,69 @@ __global__ void __launch_bounds__(64) k_coarse_apply(int nb, const double *__res
     for (int j = 0; j < n; ++j) s += inv_cm[(size_t)j * n + i] * rs[j];
     T *zz = reinterpret_cast<T *>(z);
     zz[i] = (T)s;
+  }
+}
+
+// The coarse end of the V-cycle in one workgroup: levels first..last of the
+// hierarchy are small (a few thousand block rows at most), where a launch per
+// pass costs more than the pass.  Same arithmetic as the grid-wide kernels,
+// level after level with workgroup barriers between the passes; each level's
+// final correction is left in its z vector (the finer level's prolongation
+// reads it there).
+constexpr int kTailMaxLevels = 8;
+constexpr int kTailThreads = 1024;
+constexpr int kTailMaxRows = 4096;
+struct AmgTailLevel {
+  int nb;
+  const int *ptr, *col, *agg, *mem_ptr, *mem_ind;
+  const void *val, *dinv;
+  void *r, *z, *zt;
+};
+struct AmgTailArgs {
+  int n;  // levels in the tail; lv[n-1] is the coarsest
+  double omega, scale;
+  const double *cinv;  // dense inverse of the coarsest level, column-major
+  AmgTailLevel lv[kTailMaxLevels];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_amg_tail(AmgTailArgs a, const KrylovScalars *__restrict__ sc) {
+  using V2 = typename VecT<T>::V2;
+  using V4 = typename VecT<T>::V4;
+  if (sc && sc->done) return;
+  const int lane = threadIdx.x % kLpr, grp = threadIdx.x / kLpr, ngrp = blockDim.x / kLpr;
+  const T om = (T)a.omega, cs = (T)a.scale;
+  for (int l = 0; l + 1 < a.n; ++l) {
+    const AmgTailLevel &L = a.lv[l];
+    const AmgTailLevel &C = a.lv[l + 1];
+    for (int row = grp; row < L.nb; row += ngrp)
+      row_resid<T, T>(row, lane, L.ptr, L.col, (const V4 *)L.val, (const V2 *)L.r, (const V2 *)L.z, (V2 *)L.zt);
+    __syncthreads();
+    V2 *zc = l + 2 < a.n ? (V2 *)C.z : nullptr;
+    for (int I = grp; I < C.nb; I += ngrp)
+      agg_restrict<T>(I, lane, L.mem_ptr, L.mem_ind, (const V2 *)L.zt, (V2 *)C.r, (const V4 *)C.dinv, om, zc);
+    __syncthreads();
+  }
+  {
+    const AmgTailLevel &C = a.lv[a.n - 1];
+    const int n = 2 * C.nb, i = threadIdx.x;
+    if (i < n) {
+      const T *rr = (const T *)C.r;
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += a.cinv[(size_t)j * n + i] * (double)rr[j];
+      ((T *)C.z)[i] = (T)s;
+    }
+    __syncthreads();
+  }
+  for (int l = a.n - 2; l >= 0; --l) {
+    const AmgTailLevel &L = a.lv[l];
+    const AmgTailLevel &C = a.lv[l + 1];
+    for (int row = grp; row < L.nb; row += ngrp)
+      row_prolong_smooth<T, T>(row, lane, L.ptr, L.col, (const V4 *)L.val, (const V4 *)L.dinv, L.agg,
+                               (const V2 *)L.r, (const V2 *)L.z, (const V2 *)C.z, cs, om, (V2 *)L.zt);
+    __syncthreads();
+    for (int i = threadIdx.x; i < L.nb; i += blockDim.x) ((V2 *)L.z)[i] = ((const V2 *)L.zt)[i];
+    __syncthreads();
   }
 }
 
@@ -825,7 +929,11 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
   hipLaunchKernelGGL(k_amg_presmooth<T>, dim3(blocks_for(lev[0].nb)), dim3(kBlock), 0, s_, lev[0].nb, rl,
                      lev[0].dinv, om, z0, sc);
   UFE_LAUNCH_CHECK();
-  for (int l = 0; l < L - 1; ++l) {
+  // levels t..L-1 run in one workgroup (k_amg_tail); t = L-1: only the coarsest
+  int t = L - 1;
+  for (int l = 1; l < L - 1; ++l)
+    if (lev[l].nb <= std::min(amg_tail_rows_, kTailMaxRows) && L - l <= kTailMaxLevels) { t = l; break; }
+  for (int l = 0; l < t; ++l) {
     const AmgLevelPattern &P = *amg_->levels[l];
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     bool next_coarsest = (l + 1 == L - 1);
@@ -838,18 +946,42 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
       hipLaunchKernelGGL((k_amg_resid<T, T>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr, lev[l].col,
                          lev[l].val, rin, av.z[l]->template as<V2>(), av.zt[l]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_amg_restrict<T>, dim3(blocks_for(nc)), dim3(kBlock), 0, s_, nc, P.mem_ptr.as<int>(),
+    hipLaunchKernelGGL(k_amg_restrict<T>, dim3(blocks_for((long)nc * kLpr)), dim3(kBlock), 0, s_, nc,
+                       P.mem_ptr.as<int>(),
                        P.mem_ind.as<int>(), av.zt[l]->template as<V2>(), av.r[l + 1]->template as<V2>(),
                        lev[l + 1].dinv, om, next_coarsest ? (V2 *)nullptr : av.z[l + 1]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
   }
-  {
+  if (t == L - 1) {
     int c = L - 1;
     hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(64), 0, s_, lev[c].nb, coarse_inv_.as<double>(),
                        av.r[c]->template as<V2>(), av.z[c]->template as<V2>(), sc);
     UFE_LAUNCH_CHECK();
+  } else {
+    AmgTailArgs ta{};
+    ta.n = L - t;
+    ta.omega = amg_omega_;
+    ta.scale = amg_coarse_scale_;
+    ta.cinv = coarse_inv_.as<double>();
+    for (int l = t; l < L; ++l) {
+      const AmgLevelPattern &P = *amg_->levels[l];
+      AmgTailLevel &d = ta.lv[l - t];
+      d.nb = lev[l].nb;
+      d.ptr = lev[l].ptr;
+      d.col = lev[l].col;
+      d.val = lev[l].val;
+      d.dinv = lev[l].dinv;
+      d.agg = l < L - 1 ? P.agg.as<int>() : nullptr;
+      d.mem_ptr = l < L - 1 ? P.mem_ptr.as<int>() : nullptr;
+      d.mem_ind = l < L - 1 ? P.mem_ind.as<int>() : nullptr;
+      d.r = av.r[l]->p;
+      d.z = av.z[l]->p;
+      d.zt = av.zt[l]->p;
+    }
+    hipLaunchKernelGGL(k_amg_tail<T>, dim3(1), dim3(kTailThreads), 0, s_, ta, sc);
+    UFE_LAUNCH_CHECK();
   }
-  for (int l = L - 2; l >= 0; --l) {
+  for (int l = t - 1; l >= 0; --l) {
     const AmgLevelPattern &P = *amg_->levels[l];
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     V2 *zout = l == 0 ? reinterpret_cast<V2 *>(z) : av.zt[l]->template as<V2>();

@@ -738,6 +738,7 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
       double om = (e = std::getenv("UFE_AMG_OMEGA")) ? std::atof(e) : 0.85;
       double cs = (e = std::getenv("UFE_AMG_SCALE")) ? std::atof(e) : 1.5;
       c->kry->set_amg_options(om, 0, cs);
+      if ((e = std::getenv("UFE_AMG_TAIL_ROWS"))) c->kry->set_amg_tail(std::max(0, std::atoi(e)));
     }
     c->kry->set_diag(m.bb_diag);
     {
