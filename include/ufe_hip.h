@@ -224,6 +224,16 @@ int ufe_profile_spmv(ufe_ctx *ctx, int reps, double *avg_ms, double *bytes);
  * plain device-to-device copy kernel over `bytes` bytes (read bytes + write
  * bytes each), the known-traffic reference for hardware byte counters. */
 int ufe_profile_copy(ufe_ctx *ctx, long bytes, int reps, double *avg_ms);
+/* Measurement hook (no reference counterpart): enable != 0 starts timing, with
+ * HIP events on the context's stream, every FP64 block product with a dot
+ * epilogue that the BiCGStab iterations of subsequent solves launch (two per
+ * iteration; launches skipped after convergence are not counted), and resets
+ * the counters; 0 stops it. */
+int ufe_kernel_timing(ufe_ctx *ctx, int enable);
+/* Launches timed so far, their average duration in ms and the algorithmic HBM
+ * bytes of one launch (blocks + column indices + row pointers + x, y and the
+ * dot operand once each). */
+int ufe_kernel_timing_read(const ufe_ctx *ctx, long long *launches, double *avg_ms, double *bytes);
 
 /* ---- linear-algebra boundary ------------------------------------------ */
 

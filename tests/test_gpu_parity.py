@@ -17,8 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = json.load(open(os.path.join(HERE, "golden", "reference_cases.json")))
 TIGHT = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-8, visc_it_nit=800,
              krylov_maxit=20000)
-CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.01, accel_depth=3, accel_mixing=1.0, accel_continuation=5,
-              precond_fp32=1)
+CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.01, accel_depth=2, accel_mixing=1.0, accel_continuation=5)
 PLAIN = dict(precision=0, precond=3, warm_start=0, rtol=1e-10)
 
 
@@ -154,11 +153,14 @@ def channel_fixtures(W):
     return out
 
 
-@pytest.mark.parametrize("form", ["plain", "chosen"])
+FORMS = {"plain": PLAIN, "chosen": CHOSEN, "chosen-fp32-precond": dict(CHOSEN, precond_fp32=1)}
+
+
+@pytest.mark.parametrize("form", list(FORMS))
 @pytest.mark.parametrize("sb", ["SSA", "DIVA"])
 def test_channel_converged_solve_matches_oracle(V, channel_fixtures, sb, form):
     w, f = channel_fixtures[sb]
-    st, info = solve(V, w, **TIGHT, **(PLAIN if form == "plain" else CHOSEN))
+    st, info = solve(V, w, **TIGHT, **FORMS[form])
     assert info.rel_residual <= 1e-8
     sc = max(np.abs(f["u"]).max(), np.abs(f["v"]).max())
     err = max(np.abs(st["u_vav_b"] - f["u"]).max(), np.abs(st["v_vav_b"] - f["v"]).max()) / sc
@@ -210,6 +212,24 @@ def test_amg_coarse_tail_kernel_equals_per_level_kernels(V, channel_fixtures, mo
     b, ib = solve(V, w, **TIGHT, **CHOSEN)
     assert ia.n_visc_its == ib.n_visc_its and ia.n_Axb_its == ib.n_Axb_its
     assert np.array_equal(a["u_vav_b"], b["u_vav_b"]) and np.array_equal(a["v_vav_b"], b["v_vav_b"])
+
+
+def test_live_kernel_timing_counts_every_krylov_product(V, channel_fixtures):
+    """The measurement hook times the two block products of every BiCGStab
+    iteration that ran (none of the launches skipped after convergence)."""
+    w, _ = channel_fixtures["SSA"]
+    s = V.VelocitySolver(w.descriptor(), **TIGHT, **CHOSEN)
+    inp = w.inputs()
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    s.set_velocity(None, None)
+    s.kernel_timing(True)
+    info = s.run("SSA")
+    s.kernel_timing(False)
+    n, ms, by = s.kernel_timing_read()
+    nT, nnzb = w.mesh.nT, w.ops.bb.nnz
+    s.close()
+    assert n == 2 * info.n_Axb_its and 0.0 < ms < 10.0
+    assert by == nnzb * 36 + (nT + 1) * 4 + 3 * nT * 16
 
 
 def test_chosen_form_is_deterministic(V, channel_fixtures):

@@ -79,7 +79,7 @@ def test_learned_guess_gains_over_zero_guess(W, V):
     from ufemism2_amd import learned as LN
     solve_kw = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-6,
                     visc_it_nit=600, krylov_maxit=20000, precision=0, precond=3, warm_start=1, rtol=0.01,
-                    accel_depth=3, accel_mixing=1.0, precond_fp32=1)
+                    accel_depth=2, accel_mixing=1.0)
     train_w = [W.ensemble_member(k, nV_range=(4000, 8000)) for k in range(60)]
     test_w = [W.ensemble_member(k, nV_range=(4000, 8000)) for k in range(100, 104)]
     samples = LN.solver_samples(train_w, dict(solve_kw, accel_continuation=5))

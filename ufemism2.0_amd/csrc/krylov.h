@@ -135,6 +135,13 @@ class Krylov {
   // row pointers, x and y once each).
   double time_spmv64(const BsrT<double> &A, const double *x, double *y, int reps, double *bytes);
 
+  // Live timing of the FP64 products with a dot epilogue inside bicgstab64
+  // (HIP events around each launch; counters reset by set_spmv_timing).
+  void set_spmv_timing(bool on);
+  void spmv_timing(long long *launches, double *total_ms, double *bytes) const {
+    *launches = tn_; *total_ms = tms_; *bytes = tbytes_;
+  }
+
   hipStream_t stream() const { return s_; }
   RedSlot slot(int i) const;
 
@@ -171,6 +178,10 @@ class Krylov {
   DBuf scal_;            // KrylovScalars
   DBuf red_part_, red_res_;
   KrylovScalars *h_scal_ = nullptr;  // pinned host mirror
+  bool time_spmv_ = false;
+  std::vector<hipEvent_t> tev_;
+  long long tn_ = 0;
+  double tms_ = 0.0, tbytes_ = 0.0;
   int verbose_ = 0;
 };
 

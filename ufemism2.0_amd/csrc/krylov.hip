@@ -32,10 +32,13 @@ void DBuf::free() {
 // ===========================================================================
 enum SpmvOp { OP_PLAIN = 0, OP_RESID = 1, OP_DOT1 = 2, OP_DOT2 = 3 };
 
-constexpr int kLpr = 4;
+constexpr int kLpr = 4;          // lanes per block row in the V-cycle kernels
+constexpr int kSpmvLpr = 8;      // lanes per block row in the block product
+constexpr int kSpmvBlock = 512;  // workgroup size of the block product
+constexpr int kSpmvRedGrid = 2048;  // grid cap of the block product with a reduction epilogue
 
 template <typename T, int OP>
-__global__ void __launch_bounds__(kBlock) k_bsr_spmv(
+__global__ void __launch_bounds__(kSpmvBlock) k_bsr_spmv(
     int nb, const int *__restrict__ ptr, const int *__restrict__ col,
     const typename VecT<T>::V4 *__restrict__ val, const typename VecT<T>::V2 *__restrict__ x,
     typename VecT<T>::V2 *__restrict__ y, const typename VecT<T>::V2 *__restrict__ b,
@@ -45,27 +48,26 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
   using V2 = typename VecT<T>::V2;
   using V4 = typename VecT<T>::V4;
   if (sc && sc->done) return;
-  const int lane = threadIdx.x % kLpr;
-  constexpr int kRowsPerPass = kBlock / kLpr;
-  // contiguous chunk of rows per workgroup, chunks ordered per XCD
-  const int chunk = ((nb + gridDim.x - 1) / gridDim.x + kRowsPerPass - 1) / kRowsPerPass * kRowsPerPass;
-  const int row0 = xcd_block() * chunk;
-  const int row1 = min(nb, row0 + chunk);
+  const int lane = threadIdx.x % kSpmvLpr;
+  constexpr int kRowsPerPass = kSpmvBlock / kSpmvLpr;
+  // one row per kSpmvLpr lanes; consecutive rows per workgroup, workgroups ordered
+  // per XCD, grid-stride when the grid is capped
+  const int stride = gridDim.x * kRowsPerPass;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int row = row0 + (int)threadIdx.x / kLpr; row < row1; row += kRowsPerPass) {
+  for (int row = xcd_block() * kRowsPerPass + (int)threadIdx.x / kSpmvLpr; row < nb; row += stride) {
     T y0 = 0, y1 = 0;
     if (x) {
       const int k1 = ptr[row + 1];
-      for (int k = ptr[row] + lane; k < k1; k += kLpr) {
+      for (int k = ptr[row] + lane; k < k1; k += kSpmvLpr) {
         V4 a = val[k];
         V2 xv = x[col[k]];
         y0 += a.x * xv.x + a.y * xv.y;
         y1 += a.z * xv.x + a.w * xv.y;
       }
 #pragma unroll
-      for (int off = kLpr / 2; off > 0; off >>= 1) {
-        y0 += __shfl_down(y0, off, kLpr);
-        y1 += __shfl_down(y1, off, kLpr);
+      for (int off = kSpmvLpr / 2; off > 0; off >>= 1) {
+        y0 += __shfl_down(y0, off, kSpmvLpr);
+        y1 += __shfl_down(y1, off, kSpmvLpr);
       }
     }
     if (lane == 0) {
@@ -108,18 +110,19 @@ __global__ void __launch_bounds__(kBlock) k_bsr_spmv(
     }
   }
   if (OP == OP_PLAIN) return;
-  __shared__ double lds[3][kBlock / 64];
-  block_sum<3>(acc, lds);
+  __shared__ double lds[3][kSpmvBlock / 64];
+  block_sum<3, kSpmvBlock>(acc, lds);
   block_partials<3>(slot, acc);
 }
 
 // Second phase of the product's reduction (one workgroup).
+constexpr int kFinishBlock = 1024;  // the block product's partials: up to 8 * 1024 in one round trip
 template <int OP>
-__global__ void __launch_bounds__(kBlock) k_spmv_finish(RedSlot slot, int nparts, double *__restrict__ out,
-                                                        KrylovScalars *__restrict__ sc) {
+__global__ void __launch_bounds__(kFinishBlock) k_spmv_finish(RedSlot slot, int nparts, double *__restrict__ out,
+                                                              KrylovScalars *__restrict__ sc) {
   if (sc && sc->done) return;
   double tot[3];
-  partials_sum<3>(slot, nparts, tot);
+  partials_sum<3, kFinishBlock>(slot, nparts, tot);
   if (threadIdx.x != 0) return;
   if (OP == OP_RESID) {
     out[0] = tot[0];
@@ -138,16 +141,21 @@ __global__ void __launch_bounds__(kBlock) k_spmv_finish(RedSlot slot, int nparts
 template <typename T, int OP>
 static void spmv_launch(const BsrT<T> &A, const T *x, T *y, const T *b, const T *w, const RedSlot &slot,
                         double *out, KrylovScalars *sc, hipStream_t s, double vmax = 0.0,
-                        const typename VecT<T>::V4 *dblk = nullptr, int write_unscaled = 0) {
+                        const typename VecT<T>::V4 *dblk = nullptr, int write_unscaled = 0,
+                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   using V2 = typename VecT<T>::V2;
-  long threads = (long)A.nb * kLpr;
-  int nblk = xcd_grid(std::min(blocks_for(threads), kMaxPartials));
-  hipLaunchKernelGGL((k_bsr_spmv<T, OP>), dim3(nblk), dim3(kBlock), 0, s, A.nb, A.ptr, A.col, A.val,
+  long threads = (long)A.nb * kSpmvLpr;
+  // with a reduction epilogue, fewer workgroups with several row passes each
+  // amortise the workgroup reduction
+  int nblk = std::min(xcd_grid(blocks_for(threads, kSpmvBlock)), OP == OP_PLAIN ? kMaxPartials : kSpmvRedGrid);
+  if (ev0) UFE_HIP(hipEventRecord(ev0, s));
+  hipLaunchKernelGGL((k_bsr_spmv<T, OP>), dim3(nblk), dim3(kSpmvBlock), 0, s, A.nb, A.ptr, A.col, A.val,
                      reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y), reinterpret_cast<const V2 *>(b),
                      reinterpret_cast<const V2 *>(w), slot, out, sc, vmax, dblk, write_unscaled);
   UFE_LAUNCH_CHECK();
+  if (ev1) UFE_HIP(hipEventRecord(ev1, s));
   if (OP != OP_PLAIN) {
-    hipLaunchKernelGGL((k_spmv_finish<OP>), dim3(1), dim3(kBlock), 0, s, slot, nblk, out, sc);
+    hipLaunchKernelGGL((k_spmv_finish<OP>), dim3(1), dim3(kFinishBlock), 0, s, slot, nblk, out, sc);
     UFE_LAUNCH_CHECK();
   }
 }
@@ -812,12 +820,13 @@ Krylov::Krylov(int nb, hipStream_t s) : nb_(nb), s_(s) {
 
 Krylov::~Krylov() {
   if (h_scal_) (void)hipHostFree(h_scal_);
+  for (hipEvent_t e : tev_) (void)hipEventDestroy(e);
 }
 
 double Krylov::time_spmv64(const BsrT<double> &A, const double *x, double *y, int reps, double *bytes) {
   hipEvent_t e0, e1;
-  UFE_HIP(hipEventCreate(&e0));
-  UFE_HIP(hipEventCreate(&e1));
+  UFE_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+  UFE_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
   RedSlot sl = slot(3);
   spmv_launch<double, OP_PLAIN>(A, x, y, nullptr, nullptr, sl, nullptr, nullptr, s_);  // warm caches/clocks
   UFE_HIP(hipEventRecord(e0, s_));
@@ -832,6 +841,15 @@ double Krylov::time_spmv64(const BsrT<double> &A, const double *x, double *y, in
     *bytes = (double)A.nnzb * (sizeof(double4) + sizeof(int)) + (double)(A.nb + 1) * sizeof(int) +
              2.0 * A.nb * sizeof(double2);
   return reps > 0 ? ms / reps : 0.0;
+}
+
+void Krylov::set_spmv_timing(bool on) {
+  if (on) {
+    tn_ = 0;
+    tms_ = 0.0;
+    tbytes_ = 0.0;
+  }
+  time_spmv_ = on;
 }
 
 RedSlot Krylov::slot(int i) const {
@@ -1046,6 +1064,15 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
                        sl0.result, 0.0, tol_abs, maxit - total_its, sc);
     UFE_LAUNCH_CHECK();
     bool finished = false;
+    const bool timing = time_spmv_ && std::is_same<T, double>::value;
+    if (timing)
+      while ((int)tev_.size() < 4 * check_every) {
+        // timing-only events: no system-scope fence (no L2 writeback) when recorded
+        hipEvent_t e;
+        UFE_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+        tev_.push_back(e);
+      }
+    int its_seen = 0;  // k_bicg_init zeroes the device counter
     while (!finished) {
       for (int k = 0; k < check_every; ++k) {
         hipLaunchKernelGGL(k_bicg_p<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
@@ -1053,13 +1080,15 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
                            reinterpret_cast<V2 *>(amg ? (T *)nullptr : phat), pcl, A.dinv, A.val, A_diag_, sc);
         UFE_LAUNCH_CHECK();
         if (amg) precond_apply<T>(A, p, phat);
-        spmv_launch<T, OP_DOT1>(A, phat, v, nullptr, rhat, sl1, nullptr, sc, s_);
+        spmv_launch<T, OP_DOT1>(A, phat, v, nullptr, rhat, sl1, nullptr, sc, s_, 0.0, nullptr, 0,
+                                timing ? tev_[4 * k] : nullptr, timing ? tev_[4 * k + 1] : nullptr);
         hipLaunchKernelGGL(k_bicg_s<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
                            reinterpret_cast<const V2 *>(r), reinterpret_cast<const V2 *>(v), reinterpret_cast<V2 *>(s),
                            reinterpret_cast<V2 *>(amg ? (T *)nullptr : shat), pcl, A.dinv, A.val, A_diag_, sc);
         UFE_LAUNCH_CHECK();
         if (amg) precond_apply<T>(A, s, shat);
-        spmv_launch<T, OP_DOT2>(A, shat, t, nullptr, s, sl2, nullptr, sc, s_);
+        spmv_launch<T, OP_DOT2>(A, shat, t, nullptr, s, sl2, nullptr, sc, s_, 0.0, nullptr, 0,
+                                timing ? tev_[4 * k + 2] : nullptr, timing ? tev_[4 * k + 3] : nullptr);
         hipLaunchKernelGGL(k_bicg_x<T>, dim3(nbx), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x),
                            reinterpret_cast<const V2 *>(phat), reinterpret_cast<const V2 *>(shat),
                            reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
@@ -1075,6 +1104,20 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
         std::fprintf(stderr, "[ufe:krylov%s] its %d |r| %.3e tol %.3e rho %.3e omega %.3e done %d\n",
                      sizeof(T) == 4 ? "32" : "64", h_scal_->its, std::sqrt(h_scal_->rr), std::sqrt(h_scal_->tol2),
                      h_scal_->rho, h_scal_->omega, h_scal_->done);
+      if (timing) {
+        // launches after convergence return at once: count the iterations done
+        const int real = std::max(0, std::min(check_every, h_scal_->its - its_seen));
+        for (int j = 0; j < real; ++j)
+          for (int q = 0; q < 2; ++q) {
+            float ms = 0.0f;
+            UFE_HIP(hipEventElapsedTime(&ms, tev_[4 * j + 2 * q], tev_[4 * j + 2 * q + 1]));
+            tms_ += ms;
+          }
+        tn_ += 2 * real;
+        tbytes_ = (double)A.nnzb * (sizeof(double4) + sizeof(int)) + (double)(A.nb + 1) * sizeof(int) +
+                  3.0 * A.nb * sizeof(double2);
+      }
+      its_seen = h_scal_->its;
       if (h_scal_->done) finished = true;
     }
     total_its += h_scal_->its;

@@ -1018,6 +1018,22 @@ extern "C" int ufe_profile_copy(ufe_ctx *c, long bytes, int reps, double *avg_ms
   });
 }
 
+extern "C" int ufe_kernel_timing(ufe_ctx *c, int enable) {
+  if (!c) return fail(c, UFE_ERR_ARGUMENT, "invalid argument");
+  return guarded(c, [&]() -> int {
+    c->kry->set_spmv_timing(enable != 0);
+    return UFE_OK;
+  });
+}
+
+extern "C" int ufe_kernel_timing_read(const ufe_ctx *c, long long *launches, double *avg_ms, double *bytes) {
+  if (!c || !launches || !avg_ms || !bytes) return UFE_ERR_ARGUMENT;
+  double tot = 0.0;
+  c->kry->spmv_timing(launches, &tot, bytes);
+  *avg_ms = *launches > 0 ? tot / (double)*launches : 0.0;
+  return UFE_OK;
+}
+
 extern "C" int ufe_profile_spmv(ufe_ctx *c, int reps, double *avg_ms, double *bytes) {
   if (!c || !avg_ms || reps < 1) return fail(c, UFE_ERR_ARGUMENT, "invalid argument");
   return guarded(c, [&]() -> int {

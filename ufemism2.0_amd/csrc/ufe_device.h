@@ -25,7 +25,7 @@ struct HipError : std::runtime_error {
 #define UFE_LAUNCH_CHECK() UFE_HIP(hipGetLastError())
 
 constexpr int kBlock = 256;      // threads per workgroup for 1-D kernels (4 wavefronts)
-constexpr int kMaxPartials = 2048;  // per-reduction partial sums (one per workgroup)
+constexpr int kMaxPartials = 8192;  // per-reduction partial sums (one per workgroup)
 
 // MI355X dispatches the workgroups of a launch to its 8 XCDs round robin
 // (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Kernels that
@@ -71,9 +71,9 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Workgroup-wide sum of W values; result valid in thread 0.
-template <int W>
-__device__ __forceinline__ void block_sum(double (&v)[W], double (*lds)[kBlock / 64]) {
+// Workgroup-wide sum of W values over a BS-thread workgroup; result valid in thread 0.
+template <int W, int BS = kBlock>
+__device__ __forceinline__ void block_sum(double (&v)[W], double (*lds)[BS / 64]) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int i = 0; i < W; ++i) {
@@ -85,7 +85,7 @@ __device__ __forceinline__ void block_sum(double (&v)[W], double (*lds)[kBlock /
 #pragma unroll
     for (int i = 0; i < W; ++i) {
       double s = 0.0;
-      for (int w = 0; w < kBlock / 64; ++w) s += lds[i][w];
+      for (int w = 0; w < BS / 64; ++w) s += lds[i][w];
       v[i] = s;
     }
   }
@@ -106,30 +106,35 @@ __device__ __forceinline__ void block_partials(const RedSlot &slot, const double
 }
 
 // Sum of the first nparts partials of each of W values; valid in thread 0 of a
-// one-workgroup launch of kBlock threads.
-template <int W>
+// one-workgroup launch of BS threads.  The loads of all W values are issued
+// together, U per thread and value in flight (U = 8 for W <= 4), so that up to
+// U * BS partials cost one memory round trip.
+template <int W, int BS = kBlock>
 __device__ __forceinline__ void partials_sum(const RedSlot &slot, int nparts, double (&out)[W]) {
-  __shared__ double lds[W][kBlock / 64];
-  constexpr int U = 8;  // independent loads in flight per thread
+  __shared__ double lds[W][BS / 64];
+  constexpr int U = W > 4 ? 2 : 8;
+  double s[W][U];
 #pragma unroll
-  for (int i = 0; i < W; ++i) {
-    const double *p = slot.partials + i * kMaxPartials;
-    double s[U];
+  for (int i = 0; i < W; ++i)
 #pragma unroll
-    for (int u = 0; u < U; ++u) s[u] = 0.0;
-    for (int b0 = 0; b0 < nparts; b0 += U * kBlock) {
+    for (int u = 0; u < U; ++u) s[i][u] = 0.0;
+  for (int b0 = 0; b0 < nparts; b0 += U * BS) {
+#pragma unroll
+    for (int i = 0; i < W; ++i)
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int b = b0 + u * kBlock + (int)threadIdx.x;
-        if (b < nparts) s[u] += p[b];
+        const int b = b0 + u * BS + (int)threadIdx.x;
+        if (b < nparts) s[i][u] += slot.partials[i * kMaxPartials + b];
       }
-    }
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
     double t = 0.0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) t += s[u];
+    for (int u = 0; u < U; ++u) t += s[i][u];
     out[i] = t;
   }
-  block_sum<W>(out, lds);
+  block_sum<W, BS>(out, lds);
 }
 
 // Second phase writing the W totals to out[0..W).

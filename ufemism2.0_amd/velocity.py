@@ -121,6 +121,16 @@ class VelocitySolver:
         self._check(self.lib.ufe_profile_spmv(self._ctx, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def kernel_timing(self, enable=True):
+        """Start (and reset) or stop live timing of the Krylov block products."""
+        self._check(self.lib.ufe_kernel_timing(self._ctx, int(bool(enable))))
+
+    def kernel_timing_read(self):
+        """(launches, average ms, algorithmic bytes per launch) timed so far."""
+        n, ms, by = C.c_longlong(0), C.c_double(0), C.c_double(0)
+        self._check(self.lib.ufe_kernel_timing_read(self._ctx, C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value
+
     def profile_copy(self, nbytes, reps=50):
         """Average ms of a plain copy kernel over nbytes (known-traffic reference)."""
         ms = C.c_double(0)
