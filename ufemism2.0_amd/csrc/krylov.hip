@@ -147,7 +147,8 @@ static void spmv_launch(const BsrT<T> &A, const T *x, T *y, const T *b, const T 
   long threads = (long)A.nb * kSpmvLpr;
   // with a reduction epilogue, fewer workgroups with several row passes each
   // amortise the workgroup reduction
-  int nblk = std::min(xcd_grid(blocks_for(threads, kSpmvBlock)), OP == OP_PLAIN ? kMaxPartials : kSpmvRedGrid);
+  int nblk = xcd_grid(blocks_for(threads, kSpmvBlock));
+  if (OP != OP_PLAIN) nblk = std::min(nblk, kSpmvRedGrid);
   if (ev0) UFE_HIP(hipEventRecord(ev0, s));
   hipLaunchKernelGGL((k_bsr_spmv<T, OP>), dim3(nblk), dim3(kSpmvBlock), 0, s, A.nb, A.ptr, A.col, A.val,
                      reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y), reinterpret_cast<const V2 *>(b),
