@@ -199,16 +199,20 @@ def test_chosen_form_reaches_the_oracle_solution_on_other_seeds(V, W, sb):
         assert err < 1e-5, (seed, err)
 
 
+@pytest.mark.parametrize("variant", [{"UFE_AMG_TAIL_ROWS": "0"}, {"UFE_AMG_FOLD": "0"}], ids=["no-tail", "no-fold"])
 @pytest.mark.parametrize("sb", ["SSA", "DIVA"])
-def test_amg_coarse_tail_kernel_equals_per_level_kernels(V, channel_fixtures, monkeypatch, sb):
-    """The one-workgroup kernel for the coarse levels of the V-cycle runs the
-    same per-row arithmetic as the per-level kernels: identical iterations and
-    bit-identical velocities (UFE_AMG_TAIL_ROWS sets the largest level it
-    takes, 0 = none, for the contexts created afterwards)."""
+def test_amg_fused_variants_equal_per_level_kernels(V, channel_fixtures, monkeypatch, sb, variant):
+    """The one-workgroup kernel for the coarse V-cycle levels and the finest
+    level's pre-smoothing folded into its passes run the same arithmetic as
+    the separate kernels: identical iterations and bit-identical velocities.
+    UFE_AMG_TAIL_ROWS (largest level given to the one-workgroup kernel, 0 =
+    none) and UFE_AMG_FOLD (0 = separate pre-smoothing kernel) apply to the
+    contexts created afterwards."""
     w, _ = channel_fixtures[sb]
     monkeypatch.setenv("UFE_AMG_TAIL_ROWS", "4096")
     a, ia = solve(V, w, **TIGHT, **CHOSEN)
-    monkeypatch.setenv("UFE_AMG_TAIL_ROWS", "0")
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
     b, ib = solve(V, w, **TIGHT, **CHOSEN)
     assert ia.n_visc_its == ib.n_visc_its and ia.n_Axb_its == ib.n_Axb_its
     assert np.array_equal(a["u_vav_b"], b["u_vav_b"]) and np.array_equal(a["v_vav_b"], b["v_vav_b"])

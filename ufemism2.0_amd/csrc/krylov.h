@@ -108,6 +108,10 @@ class Krylov {
   void set_amg_options(double omega, int coarse_sweeps, double coarse_scale) {
     amg_omega_ = omega; amg_coarse_sweeps_ = coarse_sweeps; amg_coarse_scale_ = coarse_scale;
   }
+  // The finest level of the V-cycle has identity diagonal blocks (the
+  // block-Jacobi-scaled system): its pre-smoothing is folded into the
+  // level's passes.
+  void set_unit_diag0(bool on) { unit_diag0_ = on; }
   // Run the coarse levels of the V-cycle with at most max_rows block rows in
   // one workgroup (0: one launch per pass on every level).
   void set_amg_tail(int max_rows) { amg_tail_rows_ = max_rows; }
@@ -166,6 +170,7 @@ class Krylov {
   const int *A_diag_ = nullptr;
   const float4 *fine32_ = nullptr;
   int amg_tail_rows_ = 300;
+  bool unit_diag0_ = false;
   double amg_omega_ = 0.85;
   int amg_coarse_sweeps_ = 20;
   double amg_coarse_scale_ = 1.5;

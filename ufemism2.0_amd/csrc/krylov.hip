@@ -353,6 +353,13 @@ __global__ void __launch_bounds__(kBlock) k_galerkin(int nnzc, const int *__rest
 }
 
 // z0 = omega D0^-1 r0  (pre-smoothing from a zero guess on the finest level)
+// The V-cycle's per-row arithmetic is written with explicit fused multiply-adds
+// and separately rounded products, so every kernel that inlines it (the
+// per-level kernels, the one-workgroup coarse kernel, with or without the
+// folded pre-smoothing) rounds identically whatever the compiler contracts.
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_amg_presmooth(int nb, const typename VecT<T>::V2 *__restrict__ r,
                                                           const typename VecT<T>::V4 *__restrict__ dinv, T omega,
@@ -363,8 +370,8 @@ __global__ void __launch_bounds__(kBlock) k_amg_presmooth(int nb, const typename
   if (i >= nb) return;
   typename VecT<T>::V2 q = r[i], o;
   typename VecT<T>::V4 d = dinv[i];
-  o.x = omega * (d.x * q.x + d.y * q.y);
-  o.y = omega * (d.z * q.x + d.w * q.y);
+  o.x = mul_rn(omega, fma(d.x, q.x, mul_rn(d.y, q.y)));
+  o.y = mul_rn(omega, fma(d.w, q.y, mul_rn(d.z, q.x)));
   z[i] = o;
 }
 
@@ -373,19 +380,23 @@ __global__ void __launch_bounds__(kBlock) k_amg_presmooth(int nb, const typename
 // cooperate on one block row or aggregate; all lanes of a group take the same
 // row, so the width-kLpr shuffles stay within converged lanes.
 
-// q_i = r_i - (A z)_i; MT is the storage type of the matrix values (arithmetic in T).
+
+// q_i = r_i - (A zs z)_i; MT is the storage type of the matrix values (arithmetic
+// in T).  zs scales z as it is read (the finest level's pre-smoothing from zero,
+// z = omega r, when its diagonal blocks are the identity).
 template <typename T, typename MT>
 __device__ __forceinline__ void row_resid(int row, int lane, const int *__restrict__ ptr,
                                           const int *__restrict__ col, const typename VecT<MT>::V4 *__restrict__ val,
                                           const typename VecT<T>::V2 *__restrict__ r,
                                           const typename VecT<T>::V2 *__restrict__ z,
-                                          typename VecT<T>::V2 *__restrict__ q) {
+                                          typename VecT<T>::V2 *__restrict__ q, T zs = T(1)) {
   T y0 = 0, y1 = 0;
   for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLpr) {
     typename VecT<MT>::V4 a = val[k];
     typename VecT<T>::V2 zj = z[col[k]];
-    y0 += (T)a.x * zj.x + (T)a.y * zj.y;
-    y1 += (T)a.z * zj.x + (T)a.w * zj.y;
+    const T z0 = mul_rn(zs, zj.x), z1 = mul_rn(zs, zj.y);
+    y0 = fma((T)a.x, z0, fma((T)a.y, z1, y0));
+    y1 = fma((T)a.z, z0, fma((T)a.w, z1, y1));
   }
 #pragma unroll
   for (int off = kLpr / 2; off > 0; off >>= 1) {
@@ -427,8 +438,8 @@ __device__ __forceinline__ void agg_restrict(int I, int lane, const int *__restr
     if (zc) {
       typename VecT<T>::V4 d = cdinv[I];
       typename VecT<T>::V2 zz;
-      zz.x = omega * (T)(d.x * s0 + d.y * s1);
-      zz.y = omega * (T)(d.z * s0 + d.w * s1);
+      zz.x = mul_rn(omega, (T)fma((double)d.x, s0, mul_rn((double)d.y, s1)));
+      zz.y = mul_rn(omega, (T)fma((double)d.w, s1, mul_rn((double)d.z, s0)));
       zc[I] = zz;
     }
   }
@@ -445,15 +456,15 @@ __device__ __forceinline__ void row_prolong_smooth(int i, int lane, const int *_
                                                    const typename VecT<T>::V2 *__restrict__ r,
                                                    const typename VecT<T>::V2 *__restrict__ z,
                                                    const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega,
-                                                   typename VecT<T>::V2 *__restrict__ zout) {
+                                                   typename VecT<T>::V2 *__restrict__ zout, T zs = T(1)) {
   T y0 = 0, y1 = 0;
   for (int k = ptr[i] + lane; k < ptr[i + 1]; k += kLpr) {
     int j = col[k];
     typename VecT<MT>::V4 a = val[k];
     typename VecT<T>::V2 zj = z[j], ej = ec[agg[j]];
-    T z0 = zj.x + scale * ej.x, z1 = zj.y + scale * ej.y;
-    y0 += (T)a.x * z0 + (T)a.y * z1;
-    y1 += (T)a.z * z0 + (T)a.w * z1;
+    const T z0 = fma(scale, ej.x, mul_rn(zs, zj.x)), z1 = fma(scale, ej.y, mul_rn(zs, zj.y));
+    y0 = fma((T)a.x, z0, fma((T)a.y, z1, y0));
+    y1 = fma((T)a.z, z0, fma((T)a.w, z1, y1));
   }
 #pragma unroll
   for (int off = kLpr / 2; off > 0; off >>= 1) {
@@ -465,8 +476,8 @@ __device__ __forceinline__ void row_prolong_smooth(int i, int lane, const int *_
     T q0 = ri.x - y0, q1 = ri.y - y1;
     typename VecT<T>::V4 d = dinv[i];
     typename VecT<T>::V2 o;
-    o.x = zi.x + scale * ei.x + omega * (d.x * q0 + d.y * q1);
-    o.y = zi.y + scale * ei.y + omega * (d.z * q0 + d.w * q1);
+    o.x = fma(omega, fma(d.x, q0, mul_rn(d.y, q1)), fma(scale, ei.x, mul_rn(zs, zi.x)));
+    o.y = fma(omega, fma(d.w, q1, mul_rn(d.z, q0)), fma(scale, ei.y, mul_rn(zs, zi.y)));
     zout[i] = o;
   }
 }
@@ -477,11 +488,11 @@ __global__ void __launch_bounds__(kBlock) k_amg_resid(int nb, const int *__restr
                                                       const typename VecT<T>::V2 *__restrict__ r,
                                                       const typename VecT<T>::V2 *__restrict__ z,
                                                       typename VecT<T>::V2 *__restrict__ q,
-                                                      const KrylovScalars *__restrict__ sc) {
+                                                      const KrylovScalars *__restrict__ sc, T zs) {
   if (sc && sc->done) return;
   const int row = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
   if (row >= nb) return;
-  row_resid<T, MT>(row, threadIdx.x % kLpr, ptr, col, val, r, z, q);
+  row_resid<T, MT>(row, threadIdx.x % kLpr, ptr, col, val, r, z, q, zs);
 }
 
 template <typename T>
@@ -504,11 +515,11 @@ __global__ void __launch_bounds__(kBlock) k_amg_prolong_smooth(
     const typename VecT<T>::V4 *__restrict__ dinv, const int *__restrict__ agg,
     const typename VecT<T>::V2 *__restrict__ r, const typename VecT<T>::V2 *__restrict__ z,
     const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega, typename VecT<T>::V2 *__restrict__ zout,
-    const KrylovScalars *__restrict__ sc) {
+    const KrylovScalars *__restrict__ sc, T zs) {
   if (sc && sc->done) return;
   const int i = (xcd_block() * blockDim.x + threadIdx.x) / kLpr;
   if (i >= nb) return;
-  row_prolong_smooth<T, MT>(i, threadIdx.x % kLpr, ptr, col, val, dinv, agg, r, z, ec, scale, omega, zout);
+  row_prolong_smooth<T, MT>(i, threadIdx.x % kLpr, ptr, col, val, dinv, agg, r, z, ec, scale, omega, zout, zs);
 }
 
 // Coarsest level: an exact solve with the dense inverse of the coarsest
@@ -945,9 +956,17 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     UFE_LAUNCH_CHECK();
     return;
   }
-  hipLaunchKernelGGL(k_amg_presmooth<T>, dim3(blocks_for(lev[0].nb)), dim3(kBlock), 0, s_, lev[0].nb, rl,
-                     lev[0].dinv, om, z0, sc);
-  UFE_LAUNCH_CHECK();
+  // finest-level pre-smoothing from zero, z0 = omega D^-1 r: with identity
+  // diagonal blocks (the scaled system) the level-0 passes read omega * r
+  // directly instead (bit-identical: 1 * x + 0 * y == x)
+  const bool fold = unit_diag0_;
+  const V2 *zl0 = fold ? rl : z0;
+  const T zs0 = fold ? om : T(1);
+  if (!fold) {
+    hipLaunchKernelGGL(k_amg_presmooth<T>, dim3(blocks_for(lev[0].nb)), dim3(kBlock), 0, s_, lev[0].nb, rl,
+                       lev[0].dinv, om, z0, sc);
+    UFE_LAUNCH_CHECK();
+  }
   // levels t..L-1 run in one workgroup (k_amg_tail); t = L-1: only the coarsest
   int t = L - 1;
   for (int l = 1; l < L - 1; ++l)
@@ -958,12 +977,14 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     bool next_coarsest = (l + 1 == L - 1);
     int nc = lev[l + 1].nb;
     const dim3 gl(xcd_grid(blocks_for((long)lev[l].nb * kLpr)));
+    const V2 *zl = l == 0 ? zl0 : av.z[l]->template as<V2>();
+    const T zs = l == 0 ? zs0 : T(1);
     if (l == 0 && fine32_ && std::is_same<T, double>::value)
       hipLaunchKernelGGL((k_amg_resid<T, float>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr, lev[l].col,
-                         fine32_, rin, av.z[l]->template as<V2>(), av.zt[l]->template as<V2>(), sc);
+                         fine32_, rin, zl, av.zt[l]->template as<V2>(), sc, zs);
     else
       hipLaunchKernelGGL((k_amg_resid<T, T>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr, lev[l].col,
-                         lev[l].val, rin, av.z[l]->template as<V2>(), av.zt[l]->template as<V2>(), sc);
+                         lev[l].val, rin, zl, av.zt[l]->template as<V2>(), sc, zs);
     UFE_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_amg_restrict<T>, dim3(blocks_for((long)nc * kLpr)), dim3(kBlock), 0, s_, nc,
                        P.mem_ptr.as<int>(),
@@ -1005,14 +1026,16 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     const V2 *rin = l == 0 ? rl : av.r[l]->template as<V2>();
     V2 *zout = l == 0 ? reinterpret_cast<V2 *>(z) : av.zt[l]->template as<V2>();
     const dim3 gl(xcd_grid(blocks_for((long)lev[l].nb * kLpr)));
+    const V2 *zl = l == 0 ? zl0 : av.z[l]->template as<V2>();
+    const T zs = l == 0 ? zs0 : T(1);
     if (l == 0 && fine32_ && std::is_same<T, double>::value)
       hipLaunchKernelGGL((k_amg_prolong_smooth<T, float>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr,
-                         lev[l].col, fine32_, lev[l].dinv, P.agg.as<int>(), rin, av.z[l]->template as<V2>(),
-                         av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
+                         lev[l].col, fine32_, lev[l].dinv, P.agg.as<int>(), rin, zl,
+                         av.z[l + 1]->template as<V2>(), cs, om, zout, sc, zs);
     else
       hipLaunchKernelGGL((k_amg_prolong_smooth<T, T>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr,
-                         lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin, av.z[l]->template as<V2>(),
-                         av.z[l + 1]->template as<V2>(), cs, om, zout, sc);
+                         lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin, zl,
+                         av.z[l + 1]->template as<V2>(), cs, om, zout, sc, zs);
     UFE_LAUNCH_CHECK();
     if (l > 0) std::swap(av.z[l], av.zt[l]);
   }

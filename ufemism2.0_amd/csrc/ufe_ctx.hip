@@ -52,6 +52,7 @@ struct ufe_ctx {
   std::string err;
   double timings[5] = {0, 0, 0, 0, 0};
   int n_fallbacks = 0;
+  bool amg_fold = true;  // UFE_AMG_FOLD=0: keep the separate level-0 pre-smoothing kernel
   double bbox[4] = {0, 1, 0, 1};  // xmin, xmax, ymin, ymax of the vertices
   DBuf feat;                     // learned-guess features [nT][16] fp32 (device order)
   hipEvent_t ev[8];
@@ -175,6 +176,8 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
   // All stopping tests use the block-Jacobi-scaled residual (velocity units),
   // the norm PETSc's left-preconditioned KSP test measures.
   K.set_scale(nullptr, nullptr);
+  // the assembled system is S = D^-1 A: identity diagonal blocks
+  K.set_unit_diag0(c->amg_fold);
   if (q.precision == UFE_PREC_FP64) {
     BsrT<double> A = sys64(c);
     K.set_amg_fine32(q.precond_fp32 && q.precond == UFE_PC_AMG ? c->A32.as<float4>() : nullptr);
@@ -739,6 +742,7 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
       double cs = (e = std::getenv("UFE_AMG_SCALE")) ? std::atof(e) : 1.5;
       c->kry->set_amg_options(om, 0, cs);
       if ((e = std::getenv("UFE_AMG_TAIL_ROWS"))) c->kry->set_amg_tail(std::max(0, std::atoi(e)));
+      if ((e = std::getenv("UFE_AMG_FOLD"))) c->amg_fold = std::atoi(e) != 0;
     }
     c->kry->set_diag(m.bb_diag);
     {
