@@ -21,12 +21,14 @@ assembled at the final u.  Inputs are uploaded to HBM before the timed region.
 
 `value` is the chosen form's solves/s; the plain form is timed in the same run
 (`plain`), and both are checked against each other (max relative velocity
-difference).  `roofline` is the FP64 block product with its dot-product
-epilogue (the two products of every BiCGStab iteration, the kernel family with
-the largest share of the solve), each launch inside the timed solves bracketed
-by HIP events on the solver's stream; `traffic` is its HBM bytes per launch
-from the committed rocprofv3 counter summary.  The plain product timed
-back-to-back after the solves is reported beside it.
+difference).  `roofline` is the largest single kernel of the chosen form, the
+multigrid preconditioner's finest-level prolongation + smoothing sweep (a
+pass over the fine matrix, two per BiCGStab iteration), each launch inside the
+timed solves bracketed by HIP events on the solver's stream; `traffic` is its
+HBM bytes per launch from the committed rocprofv3 counter summary.
+`roofline_krylov_product` is the same for the FP64 block products with their
+dot-product epilogues (two per iteration), with the plain product timed
+back-to-back after the solves beside it.
 `cpu_baseline` is the oracle's FP64 restatement of the reference iteration
 (GMRES(30) + ILU(0)) on the host cores, timed on a bounded sample of Picard
 iterations of the same workload and scaled to solves/s with the plain form's
@@ -56,10 +58,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per launch of the block product from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", "bsr_spmv_dot_channel200k_pmc.txt")
+def read_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -224,14 +226,18 @@ def main():
 
     conc = args.concurrency if args.workload == "ensemble" else 1
     t_chosen, infos, vel_c, solvers = time_form(V, ws, CHOSEN, args.steps, args.warmup, "chosen", conc)
-    # the dominant kernel, timed live: the Krylov block products (with their dot
-    # epilogues) launched inside the timed solves, HIP events on the solver stream
-    live_n, live_ms, live_bytes = 0, 0.0, 0.0
-    for _, s in solvers:
-        n, ms, by = s.kernel_timing_read()
-        live_ms = (live_ms * live_n + ms * n) / max(live_n + n, 1)
-        live_n += n
-        live_bytes = by
+    # kernels timed live inside the timed solves (HIP events on the solver stream):
+    # 1 = the preconditioner's finest-level prolongation + smoothing (the largest
+    # single kernel of the solve), 0 = the Krylov block products
+    live = {}
+    for kern in (0, 1):
+        n_t, ms_t, by_t = 0, 0.0, 0.0
+        for _, s in solvers:
+            n, ms, by = s.kernel_timing_read(kern)
+            ms_t = (ms_t * n_t + ms * n) / max(n_t + n, 1)
+            n_t += n
+            by_t = by
+        live[kern] = (n_t, ms_t, by_t)
     spmv_ms, spmv_bytes = solvers[0][1].profile_spmv(args.spmv_reps)
     for _, s in solvers:
         s.close()
@@ -249,8 +255,11 @@ def main():
 
     value = n_solves / t_chosen
     plain_value = args.plain_steps * len(ws) / t_plain if args.plain_steps else None
-    achieved = live_bytes / (live_ms * 1e-3) / 1e9 if live_n else 0.0
-    traffic = read_traffic() if args.workload == "channel" else None
+    def rate(k):
+        n, ms, by = live[k]
+        return by / (ms * 1e-3) / 1e9 if n else 0.0
+    traffic = read_traffic("amg_prolong_smooth_fine_channel200k_pmc.txt") if args.workload == "channel" else None
+    traffic_spmv = read_traffic("bsr_spmv_dot_channel200k_pmc.txt") if args.workload == "channel" else None
     out = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t_chosen / args.steps, "higher_is_better": True,
@@ -271,12 +280,18 @@ def main():
                   "residual": [i.rel_residual for i in pinfos]},
         "speedup_vs_plain": value / plain_value if plain_value else None,
         "max_rel_vel_diff_chosen_vs_plain": diff,
-        "roofline": {"kernel": "k_bsr_spmv<double, OP_DOT1|OP_DOT2> (Krylov block products, timed live)",
-                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": live_ms,
-                     "launches": live_n, "algorithmic_bytes": live_bytes,
-                     "plain_product_back_to_back": {"avg_ms": spmv_ms, "algorithmic_bytes": spmv_bytes,
-                                                    "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9}},
+        "roofline": {"kernel": "k_amg_prolong_smooth<double, double> on the finest level (timed live)",
+                     "bound": "hbm", "achieved": rate(1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rate(1) / HBM_PEAK_GBS, "traffic": traffic, "avg_ms": live[1][1],
+                     "launches": live[1][0], "algorithmic_bytes": live[1][2]},
+        "roofline_krylov_product": {"kernel": "k_bsr_spmv<double, OP_DOT1|OP_DOT2> (timed live)",
+                                    "bound": "hbm", "achieved": rate(0), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": rate(0) / HBM_PEAK_GBS, "traffic": traffic_spmv,
+                                    "avg_ms": live[0][1], "launches": live[0][0],
+                                    "algorithmic_bytes": live[0][2],
+                                    "plain_product_back_to_back": {
+                                        "avg_ms": spmv_ms, "algorithmic_bytes": spmv_bytes,
+                                        "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9}},
         "cpu_baseline": None,
     }
     if args.workload == "continental":

@@ -225,15 +225,19 @@ int ufe_profile_spmv(ufe_ctx *ctx, int reps, double *avg_ms, double *bytes);
  * bytes each), the known-traffic reference for hardware byte counters. */
 int ufe_profile_copy(ufe_ctx *ctx, long bytes, int reps, double *avg_ms);
 /* Measurement hook (no reference counterpart): enable != 0 starts timing, with
- * HIP events on the context's stream, every FP64 block product with a dot
- * epilogue that the BiCGStab iterations of subsequent solves launch (two per
- * iteration; launches skipped after convergence are not counted), and resets
- * the counters; 0 stops it. */
+ * HIP events on the context's stream, two kernels of the BiCGStab iterations
+ * of subsequent solves (launches skipped after convergence are not counted),
+ * and resets the counters; 0 stops it.  Kernel 0: the FP64 block products
+ * with a dot epilogue (two per iteration); kernel 1: the multigrid
+ * preconditioner's finest-level prolongation + smoothing (two per iteration). */
 int ufe_kernel_timing(ufe_ctx *ctx, int enable);
-/* Launches timed so far, their average duration in ms and the algorithmic HBM
- * bytes of one launch (blocks + column indices + row pointers + x, y and the
- * dot operand once each). */
-int ufe_kernel_timing_read(const ufe_ctx *ctx, long long *launches, double *avg_ms, double *bytes);
+/* Launches of `kernel` timed so far, their average duration in ms and the
+ * algorithmic HBM bytes of one launch (kernel 0: blocks + column indices + row
+ * pointers + x, y and the dot operand once each; kernel 1: blocks + column
+ * indices + row pointers + aggregate map + residual, pre-smoothed iterate
+ * (the same vector when the pre-smoothing is folded), block inverses unless
+ * folded, coarse correction and output once each). */
+int ufe_kernel_timing_read(const ufe_ctx *ctx, int kernel, long long *launches, double *avg_ms, double *bytes);
 
 /* ---- linear-algebra boundary ------------------------------------------ */
 

@@ -219,8 +219,9 @@ def test_amg_fused_variants_equal_per_level_kernels(V, channel_fixtures, monkeyp
 
 
 def test_live_kernel_timing_counts_every_krylov_product(V, channel_fixtures):
-    """The measurement hook times the two block products of every BiCGStab
-    iteration that ran (none of the launches skipped after convergence)."""
+    """The measurement hook times the two block products and the two
+    finest-level prolongations of every BiCGStab iteration that ran (none of
+    the launches skipped after convergence)."""
     w, _ = channel_fixtures["SSA"]
     s = V.VelocitySolver(w.descriptor(), **TIGHT, **CHOSEN)
     inp = w.inputs()
@@ -229,11 +230,15 @@ def test_live_kernel_timing_counts_every_krylov_product(V, channel_fixtures):
     s.kernel_timing(True)
     info = s.run("SSA")
     s.kernel_timing(False)
-    n, ms, by = s.kernel_timing_read()
+    n, ms, by = s.kernel_timing_read(0)
+    n1, ms1, by1 = s.kernel_timing_read(1)
     nT, nnzb = w.mesh.nT, w.ops.bb.nnz
     s.close()
     assert n == 2 * info.n_Axb_its and 0.0 < ms < 10.0
     assert by == nnzb * 36 + (nT + 1) * 4 + 3 * nT * 16
+    # the preconditioner's finest-level prolongation, once per application
+    assert n1 == 2 * info.n_Axb_its and 0.0 < ms1 < 10.0
+    assert nnzb * 36 < by1 < nnzb * 36 + nT * 64
 
 
 def test_chosen_form_is_deterministic(V, channel_fixtures):

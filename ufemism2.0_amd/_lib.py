@@ -149,7 +149,7 @@ def lib():
     _sig(L, 'ufe_profile_spmv', argtypes=[C.c_void_p, C.c_int, dp, dp])
     _sig(L, 'ufe_profile_copy', argtypes=[C.c_void_p, C.c_long, C.c_int, dp])
     _sig(L, 'ufe_kernel_timing', argtypes=[C.c_void_p, C.c_int])
-    _sig(L, 'ufe_kernel_timing_read', argtypes=[C.c_void_p, C.POINTER(C.c_longlong), dp, dp])
+    _sig(L, 'ufe_kernel_timing_read', argtypes=[C.c_void_p, C.c_int, C.POINTER(C.c_longlong), dp, dp])
     fp = C.POINTER(C.c_float)
     _sig(L, 'ufe_mpnn_num_weights', restype=C.c_int, argtypes=[])
     _sig(L, 'ufe_mpnn_create', argtypes=[C.POINTER(C.c_void_p), C.c_int, fp])

@@ -139,11 +139,12 @@ class Krylov {
   // row pointers, x and y once each).
   double time_spmv64(const BsrT<double> &A, const double *x, double *y, int reps, double *bytes);
 
-  // Live timing of the FP64 products with a dot epilogue inside bicgstab64
-  // (HIP events around each launch; counters reset by set_spmv_timing).
+  // Live timing inside bicgstab64 (HIP events around each launch; counters
+  // reset by set_spmv_timing(true)): kernel 0 = the FP64 products with a dot
+  // epilogue, 1 = the V-cycle's finest-level prolongation + smoothing.
   void set_spmv_timing(bool on);
-  void spmv_timing(long long *launches, double *total_ms, double *bytes) const {
-    *launches = tn_; *total_ms = tms_; *bytes = tbytes_;
+  void spmv_timing(int kernel, long long *launches, double *total_ms, double *bytes) const {
+    *launches = tn_[kernel]; *total_ms = tms_[kernel]; *bytes = tbytes_[kernel];
   }
 
   hipStream_t stream() const { return s_; }
@@ -156,7 +157,8 @@ class Krylov {
  private:
   template <typename T> SolveStats bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, double abstol,
                                             int maxit, bool zero_guess, int check_every);
-  template <typename T> void precond_apply(const BsrT<T> &A, const T *r, T *z);
+  template <typename T> void precond_apply(const BsrT<T> &A, const T *r, T *z, hipEvent_t ev0 = nullptr,
+                                           hipEvent_t ev1 = nullptr);
   template <typename T> void amg_setup(const BsrT<T> &A0);
   template <typename T> AmgValues<T> &amg_vals();
   template <typename T> const void *dscale() const;
@@ -185,8 +187,8 @@ class Krylov {
   KrylovScalars *h_scal_ = nullptr;  // pinned host mirror
   bool time_spmv_ = false;
   std::vector<hipEvent_t> tev_;
-  long long tn_ = 0;
-  double tms_ = 0.0, tbytes_ = 0.0;
+  long long tn_[2] = {0, 0};
+  double tms_[2] = {0.0, 0.0}, tbytes_[2] = {0.0, 0.0};
   int verbose_ = 0;
 };
 

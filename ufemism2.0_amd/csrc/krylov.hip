@@ -474,10 +474,15 @@ __device__ __forceinline__ void row_prolong_smooth(int i, int lane, const int *_
   if (lane == 0) {
     typename VecT<T>::V2 ri = r[i], zi = z[i], ei = ec[agg[i]];
     T q0 = ri.x - y0, q1 = ri.y - y1;
-    typename VecT<T>::V4 d = dinv[i];
     typename VecT<T>::V2 o;
-    o.x = fma(omega, fma(d.x, q0, mul_rn(d.y, q1)), fma(scale, ei.x, mul_rn(zs, zi.x)));
-    o.y = fma(omega, fma(d.w, q1, mul_rn(d.z, q0)), fma(scale, ei.y, mul_rn(zs, zi.y)));
+    if (dinv) {  // null: identity diagonal blocks (1 * a + 0 * b == a, so both agree bit for bit)
+      typename VecT<T>::V4 d = dinv[i];
+      const T t0 = fma(d.x, q0, mul_rn(d.y, q1)), t1 = fma(d.w, q1, mul_rn(d.z, q0));
+      q0 = t0;
+      q1 = t1;
+    }
+    o.x = fma(omega, q0, fma(scale, ei.x, mul_rn(zs, zi.x)));
+    o.y = fma(omega, q1, fma(scale, ei.y, mul_rn(zs, zi.y)));
     zout[i] = o;
   }
 }
@@ -856,11 +861,12 @@ double Krylov::time_spmv64(const BsrT<double> &A, const double *x, double *y, in
 }
 
 void Krylov::set_spmv_timing(bool on) {
-  if (on) {
-    tn_ = 0;
-    tms_ = 0.0;
-    tbytes_ = 0.0;
-  }
+  if (on)
+    for (int i = 0; i < 2; ++i) {
+      tn_[i] = 0;
+      tms_[i] = 0.0;
+      tbytes_[i] = 0.0;
+    }
   time_spmv_ = on;
 }
 
@@ -939,7 +945,7 @@ void Krylov::amg_setup64(const BsrT<double> &A0) { amg_setup<double>(A0); }
 void Krylov::amg_setup32(const BsrT<float> &A0) { amg_setup<float>(A0); }
 
 template <typename T>
-void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
+void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z, hipEvent_t ev0, hipEvent_t ev1) {
   using V2 = typename VecT<T>::V2;
   AmgValues<T> &av = amg_vals<T>();
   std::vector<BsrT<T>> &lev = std::is_same<T, double>::value ? *reinterpret_cast<std::vector<BsrT<T>> *>(&lev64_)
@@ -1028,15 +1034,18 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z) {
     const dim3 gl(xcd_grid(blocks_for((long)lev[l].nb * kLpr)));
     const V2 *zl = l == 0 ? zl0 : av.z[l]->template as<V2>();
     const T zs = l == 0 ? zs0 : T(1);
+    const typename VecT<T>::V4 *dl = (l == 0 && fold) ? nullptr : lev[l].dinv;
+    if (l == 0 && ev0) UFE_HIP(hipEventRecord(ev0, s_));
     if (l == 0 && fine32_ && std::is_same<T, double>::value)
       hipLaunchKernelGGL((k_amg_prolong_smooth<T, float>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr,
-                         lev[l].col, fine32_, lev[l].dinv, P.agg.as<int>(), rin, zl,
+                         lev[l].col, fine32_, dl, P.agg.as<int>(), rin, zl,
                          av.z[l + 1]->template as<V2>(), cs, om, zout, sc, zs);
     else
       hipLaunchKernelGGL((k_amg_prolong_smooth<T, T>), gl, dim3(kBlock), 0, s_, lev[l].nb, lev[l].ptr,
-                         lev[l].col, lev[l].val, lev[l].dinv, P.agg.as<int>(), rin, zl,
+                         lev[l].col, lev[l].val, dl, P.agg.as<int>(), rin, zl,
                          av.z[l + 1]->template as<V2>(), cs, om, zout, sc, zs);
     UFE_LAUNCH_CHECK();
+    if (l == 0 && ev1) UFE_HIP(hipEventRecord(ev1, s_));
     if (l > 0) std::swap(av.z[l], av.zt[l]);
   }
 }
@@ -1090,7 +1099,7 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
     bool finished = false;
     const bool timing = time_spmv_ && std::is_same<T, double>::value;
     if (timing)
-      while ((int)tev_.size() < 4 * check_every) {
+      while ((int)tev_.size() < 8 * check_every) {
         // timing-only events: no system-scope fence (no L2 writeback) when recorded
         hipEvent_t e;
         UFE_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
@@ -1103,16 +1112,18 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
                            reinterpret_cast<const V2 *>(r), reinterpret_cast<V2 *>(p), reinterpret_cast<const V2 *>(v),
                            reinterpret_cast<V2 *>(amg ? (T *)nullptr : phat), pcl, A.dinv, A.val, A_diag_, sc);
         UFE_LAUNCH_CHECK();
-        if (amg) precond_apply<T>(A, p, phat);
+        if (amg)
+          precond_apply<T>(A, p, phat, timing ? tev_[8 * k + 4] : nullptr, timing ? tev_[8 * k + 5] : nullptr);
         spmv_launch<T, OP_DOT1>(A, phat, v, nullptr, rhat, sl1, nullptr, sc, s_, 0.0, nullptr, 0,
-                                timing ? tev_[4 * k] : nullptr, timing ? tev_[4 * k + 1] : nullptr);
+                                timing ? tev_[8 * k] : nullptr, timing ? tev_[8 * k + 1] : nullptr);
         hipLaunchKernelGGL(k_bicg_s<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
                            reinterpret_cast<const V2 *>(r), reinterpret_cast<const V2 *>(v), reinterpret_cast<V2 *>(s),
                            reinterpret_cast<V2 *>(amg ? (T *)nullptr : shat), pcl, A.dinv, A.val, A_diag_, sc);
         UFE_LAUNCH_CHECK();
-        if (amg) precond_apply<T>(A, s, shat);
+        if (amg)
+          precond_apply<T>(A, s, shat, timing ? tev_[8 * k + 6] : nullptr, timing ? tev_[8 * k + 7] : nullptr);
         spmv_launch<T, OP_DOT2>(A, shat, t, nullptr, s, sl2, nullptr, sc, s_, 0.0, nullptr, 0,
-                                timing ? tev_[4 * k + 2] : nullptr, timing ? tev_[4 * k + 3] : nullptr);
+                                timing ? tev_[8 * k + 2] : nullptr, timing ? tev_[8 * k + 3] : nullptr);
         hipLaunchKernelGGL(k_bicg_x<T>, dim3(nbx), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x),
                            reinterpret_cast<const V2 *>(phat), reinterpret_cast<const V2 *>(shat),
                            reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
@@ -1132,14 +1143,26 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
         // launches after convergence return at once: count the iterations done
         const int real = std::max(0, std::min(check_every, h_scal_->its - its_seen));
         for (int j = 0; j < real; ++j)
-          for (int q = 0; q < 2; ++q) {
+          for (int q = 0; q < 4; ++q) {
+            if (q >= 2 && !amg) break;
             float ms = 0.0f;
-            UFE_HIP(hipEventElapsedTime(&ms, tev_[4 * j + 2 * q], tev_[4 * j + 2 * q + 1]));
-            tms_ += ms;
+            UFE_HIP(hipEventElapsedTime(&ms, tev_[8 * j + 2 * q], tev_[8 * j + 2 * q + 1]));
+            tms_[q / 2] += ms;
           }
-        tn_ += 2 * real;
-        tbytes_ = (double)A.nnzb * (sizeof(double4) + sizeof(int)) + (double)(A.nb + 1) * sizeof(int) +
-                  3.0 * A.nb * sizeof(double2);
+        tn_[0] += 2 * real;
+        tbytes_[0] = (double)A.nnzb * (sizeof(double4) + sizeof(int)) + (double)(A.nb + 1) * sizeof(int) +
+                     3.0 * A.nb * sizeof(double2);
+        if (amg) {
+          // finest-level prolongation + smoothing: matrix, row pointers, aggregate
+          // map, r (also the pre-smoothed z when folded), D^-1 unless folded, the
+          // coarse correction, the output
+          const double nc = amg_->n_levels() > 1 ? (double)amg_->levels[1]->nb : 0.0;
+          tn_[1] += 2 * real;
+          tbytes_[1] = (double)A.nnzb * ((fine32_ ? sizeof(float4) : sizeof(double4)) + sizeof(int)) +
+                       (double)(A.nb + 1) * sizeof(int) + (double)A.nb * sizeof(int) +
+                       (unit_diag0_ ? 2.0 : 3.0) * A.nb * sizeof(double2) +
+                       (unit_diag0_ ? 0.0 : (double)A.nb * sizeof(double4)) + nc * sizeof(double2);
+        }
       }
       its_seen = h_scal_->its;
       if (h_scal_->done) finished = true;

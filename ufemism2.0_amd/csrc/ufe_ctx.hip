@@ -1030,10 +1030,11 @@ extern "C" int ufe_kernel_timing(ufe_ctx *c, int enable) {
   });
 }
 
-extern "C" int ufe_kernel_timing_read(const ufe_ctx *c, long long *launches, double *avg_ms, double *bytes) {
-  if (!c || !launches || !avg_ms || !bytes) return UFE_ERR_ARGUMENT;
+extern "C" int ufe_kernel_timing_read(const ufe_ctx *c, int kernel, long long *launches, double *avg_ms,
+                                      double *bytes) {
+  if (!c || !launches || !avg_ms || !bytes || kernel < 0 || kernel > 1) return UFE_ERR_ARGUMENT;
   double tot = 0.0;
-  c->kry->spmv_timing(launches, &tot, bytes);
+  c->kry->spmv_timing(kernel, launches, &tot, bytes);
   *avg_ms = *launches > 0 ? tot / (double)*launches : 0.0;
   return UFE_OK;
 }

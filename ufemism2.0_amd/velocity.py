@@ -125,10 +125,12 @@ class VelocitySolver:
         """Start (and reset) or stop live timing of the Krylov block products."""
         self._check(self.lib.ufe_kernel_timing(self._ctx, int(bool(enable))))
 
-    def kernel_timing_read(self):
-        """(launches, average ms, algorithmic bytes per launch) timed so far."""
+    def kernel_timing_read(self, kernel=0):
+        """(launches, average ms, algorithmic bytes per launch) timed so far;
+        kernel 0: Krylov block products, 1: finest-level prolongation + smoothing."""
         n, ms, by = C.c_longlong(0), C.c_double(0), C.c_double(0)
-        self._check(self.lib.ufe_kernel_timing_read(self._ctx, C.byref(n), C.byref(ms), C.byref(by)))
+        self._check(self.lib.ufe_kernel_timing_read(self._ctx, int(kernel), C.byref(n), C.byref(ms),
+                                                    C.byref(by)))
         return n.value, ms.value, by.value
 
     def profile_copy(self, nbytes, reps=50):
