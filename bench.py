@@ -188,7 +188,7 @@ def main():
     ap.add_argument("--nv", type=int, default=200_000, help="target vertices of the channel mesh")
     ap.add_argument("--workload", choices=WORKLOADS, default="channel")
     ap.add_argument("--members", type=int, default=64, help="ensemble members (--workload ensemble)")
-    ap.add_argument("--concurrency", type=int, default=4,
+    ap.add_argument("--concurrency", type=int, default=8,
                     help="ensemble members solved at once, one HIP stream each (--workload ensemble)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one chosen-form solve, then only the block-product and copy-kernel launches "
