@@ -139,14 +139,20 @@ def build_workloads(W, args):
     return f"ensemble_{args.members}", [W.ensemble_member(k) for k in range(args.members)]
 
 
-def time_form(V, ws, form, steps, warmup, label, concurrency=1):
+def time_form(V, ws, form, steps, warmup, label, concurrency=1, autotune=False):
     """Warm-up and timed converged solves of one form over the workloads;
-    returns (seconds for `steps` passes, infos, last velocities per workload)."""
+    returns (seconds for `steps` passes, infos, last velocities per workload).
+    autotune: each solver first runs its correctness-gated coordinate search
+    (outside the timed region) and keeps the winner."""
     solvers = []
     for w in ws:
         inp = w.inputs()
         s = V.VelocitySolver(w.descriptor(), 0, **BASE, **form)
         s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+        if autotune:
+            best, table = s.autotune(w.stress_balance)
+            log(f"[bench] autotune {w.name}: " + "; ".join(
+                f"{cfg} {'%.2fs' % t if t is not None else 'rejected'}" for cfg, t in table) + f" -> {best}")
         solvers.append((w, s))
     for i in range(warmup):
         for w, s in solvers:
@@ -190,6 +196,9 @@ def main():
     ap.add_argument("--members", type=int, default=64, help="ensemble members (--workload ensemble)")
     ap.add_argument("--concurrency", type=int, default=8,
                     help="ensemble members solved at once, one HIP stream each (--workload ensemble)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the chosen form's Anderson depth and multigrid knobs per workload before timing "
+                         "(correctness-gated coordinate search; not part of the default measurement)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one chosen-form solve, then only the block-product and copy-kernel launches "
                          "(for hardware counter runs)")
@@ -225,7 +234,8 @@ def main():
         return
 
     conc = args.concurrency if args.workload == "ensemble" else 1
-    t_chosen, infos, vel_c, solvers = time_form(V, ws, CHOSEN, args.steps, args.warmup, "chosen", conc)
+    t_chosen, infos, vel_c, solvers = time_form(V, ws, CHOSEN, args.steps, args.warmup, "chosen", conc,
+                                                autotune=args.autotune)
     # kernels timed live inside the timed solves (HIP events on the solver stream):
     # 1 = the preconditioner's finest-level prolongation + smoothing (the largest
     # single kernel of the solve), 0 = the Krylov block products
@@ -279,6 +289,8 @@ def main():
                   "picard": [i.n_visc_its for i in pinfos], "krylov": [i.n_Axb_its for i in pinfos],
                   "residual": [i.rel_residual for i in pinfos]},
         "speedup_vs_plain": value / plain_value if plain_value else None,
+        "autotuned": ([{**{k: getattr(s.params, k) for k in ("accel_depth",)}, **s.amg_options}
+                       for _, s in solvers] if args.autotune else None),
         "max_rel_vel_diff_chosen_vs_plain": diff,
         "roofline": {"kernel": "k_amg_prolong_smooth<double, double> on the finest level (timed live)",
                      "bound": "hbm", "achieved": rate(1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

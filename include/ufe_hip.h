@@ -166,6 +166,13 @@ int         ufe_version(void);
 int  ufe_ctx_create(ufe_ctx **ctx, int device, const ufe_mesh_desc *mesh);
 int  ufe_ctx_destroy(ufe_ctx *ctx);
 int  ufe_set_params(ufe_ctx *ctx, const ufe_params *p);
+/* Multigrid preconditioner knobs (no reference counterpart; the reference's
+ * PETSc preconditioner has none of these): damping of the block-Jacobi
+ * smoother (0 < omega <= 1, default 0.85), scale of the coarse-level
+ * correction (0 < coarse_scale <= 3, default 1.5), and the largest level, in
+ * block rows, run in one workgroup (0..4096, default 300).  Takes effect from
+ * the next solve. */
+int  ufe_set_amg_options(ufe_ctx *ctx, double omega, double coarse_scale, int tail_rows);
 
 /* ---- drop-in entry points ---------------------------------------------- */
 

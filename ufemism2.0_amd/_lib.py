@@ -149,6 +149,7 @@ def lib():
     _sig(L, 'ufe_profile_spmv', argtypes=[C.c_void_p, C.c_int, dp, dp])
     _sig(L, 'ufe_profile_copy', argtypes=[C.c_void_p, C.c_long, C.c_int, dp])
     _sig(L, 'ufe_kernel_timing', argtypes=[C.c_void_p, C.c_int])
+    _sig(L, 'ufe_set_amg_options', argtypes=[C.c_void_p, C.c_double, C.c_double, C.c_int])
     _sig(L, 'ufe_kernel_timing_read', argtypes=[C.c_void_p, C.c_int, C.POINTER(C.c_longlong), dp, dp])
     fp = C.POINTER(C.c_float)
     _sig(L, 'ufe_mpnn_num_weights', restype=C.c_int, argtypes=[])
@@ -178,7 +179,7 @@ EXPORTED = [
     "ufe_set_params", "ufe_solve_ssa", "ufe_solve_diva", "ufe_upload_inputs", "ufe_upload_velocity",
     "ufe_run", "ufe_download_velocity", "ufe_upload_diva_state", "ufe_download_diva",
     "ufe_last_timings", "ufe_profile_spmv", "ufe_profile_copy", "ufe_kernel_timing",
-    "ufe_kernel_timing_read", "ufe_mpnn_num_weights", "ufe_mpnn_create",
+    "ufe_kernel_timing_read", "ufe_set_amg_options", "ufe_mpnn_num_weights", "ufe_mpnn_create",
     "ufe_mpnn_destroy", "ufe_mpnn_get_weights", "ufe_mpnn_forward", "ufe_mpnn_loss_grad", "ufe_mpnn_train",
     "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",

@@ -774,6 +774,18 @@ extern "C" int ufe_ctx_destroy(ufe_ctx *c) {
   return UFE_OK;
 }
 
+extern "C" int ufe_set_amg_options(ufe_ctx *c, double omega, double coarse_scale, int tail_rows) {
+  if (!c) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  if (!(omega > 0.0 && omega <= 1.0) || !(coarse_scale > 0.0 && coarse_scale <= 3.0) || tail_rows < 0 ||
+      tail_rows > 4096)
+    return fail(c, UFE_ERR_ARGUMENT, "multigrid options out of range");
+  return guarded(c, [&]() -> int {
+    c->kry->set_amg_options(omega, 0, coarse_scale);
+    c->kry->set_amg_tail(tail_rows);
+    return UFE_OK;
+  });
+}
+
 extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
   if (!c || !p) return fail(c, UFE_ERR_ARGUMENT, "null argument");
   return guarded(c, [&]() -> int {

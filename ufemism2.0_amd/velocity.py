@@ -29,6 +29,10 @@ class VelocitySolverError(RuntimeError):
 
 
 _PARAM_NAMES = [f for f, _ in L.Params._fields_]
+# multigrid preconditioner knobs (ufe_set_amg_options) and their library defaults
+AMG_DEFAULTS = {"omega": 0.85, "coarse_scale": 1.5, "tail_rows": 300}
+# coordinate search of VelocitySolver.autotune: one axis at a time, best kept
+AUTOTUNE_AXES = (("accel_depth", (2, 3)), ("omega", (0.75, 0.85, 0.95)), ("coarse_scale", (1.3, 1.5, 1.7)))
 
 
 class VelocitySolver:
@@ -42,6 +46,7 @@ class VelocitySolver:
         if rc != 0:
             raise VelocitySolverError(f"ufe_ctx_create failed ({rc}): {self.lib.ufe_last_error(None).decode()}")
         self.params = L.default_params()
+        self.amg_options = dict(AMG_DEFAULTS)
         self.set_params(**params)
 
     # -- configuration -------------------------------------------------------
@@ -120,6 +125,70 @@ class VelocitySolver:
         ms, by = C.c_double(0), C.c_double(0)
         self._check(self.lib.ufe_profile_spmv(self._ctx, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def set_amg_options(self, **kw):
+        """Multigrid knobs: omega (smoother damping), coarse_scale (coarse
+        correction scale), tail_rows (largest level run in one workgroup)."""
+        for k in kw:
+            if k not in AMG_DEFAULTS:
+                raise KeyError(f"unknown multigrid option {k}")
+        opts = dict(self.amg_options, **kw)
+        self._check(self.lib.ufe_set_amg_options(self._ctx, float(opts["omega"]), float(opts["coarse_scale"]),
+                                                 int(opts["tail_rows"])))
+        self.amg_options = opts
+
+    def _apply(self, cfg):
+        self.set_params(**{k: v for k, v in cfg.items() if k in _PARAM_NAMES})
+        self.set_amg_options(**{k: v for k, v in cfg.items() if k in AMG_DEFAULTS})
+
+    def autotune(self, stress_balance="DIVA", axes=AUTOTUNE_AXES, tol=1e-5):
+        """Fastest configuration for the uploaded inputs, by coordinate search.
+
+        Every trial solves from zero velocity (and zero DIVA state).  A trial
+        that does not reach the stopping test, or whose velocities differ from
+        those of the configuration the solver had on entry by more than `tol`
+        (max relative difference), is rejected: the discrete equations can have
+        more than one solution, and a tuned configuration must reach the same
+        one.  Iteration counts move with round-off, so the gain is specific to
+        these inputs; re-tune when they change much.  Leaves the solver set to
+        the winner and returns (winner, [(config, seconds or None), ...])."""
+        def trial(cfg):
+            self._apply(cfg)
+            self.set_velocity(None, None)
+            if stress_balance.upper() == "DIVA":
+                self.reset_diva_state()
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore", ViscosityIterationWarning)
+                info = self.run(stress_balance)
+            u, v = self.get_velocity()
+            ok = info.rel_residual <= self.params.nonlinear_rtol
+            return ok, info.seconds, u, v
+
+        cur = {k: getattr(self.params, k) for k, _ in axes if k in _PARAM_NAMES}
+        cur.update({k: self.amg_options[k] for k, _ in axes if k in AMG_DEFAULTS})
+        ok, t_best, u0, v0 = trial(cur)
+        if not ok:
+            raise VelocitySolverError("the configuration on entry does not converge; nothing to tune against")
+        scale = max(np.abs(u0).max(), np.abs(v0).max(), 1e-300)
+        table = [(dict(cur), t_best)]
+        best = dict(cur)
+        for name, values in axes:
+            for val in values:
+                if val == best[name]:
+                    continue
+                cfg = dict(best, **{name: val})
+                try:
+                    ok, t, u, v = trial(cfg)
+                except VelocitySolverError:
+                    ok, t = False, None
+                if ok and max(np.abs(u - u0).max(), np.abs(v - v0).max()) / scale <= tol:
+                    table.append((cfg, t))
+                    if t < t_best:
+                        best, t_best = cfg, t
+                else:
+                    table.append((cfg, None))
+        self._apply(best)
+        return best, table
 
     def kernel_timing(self, enable=True):
         """Start (and reset) or stop live timing of the Krylov block products."""
