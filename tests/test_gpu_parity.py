@@ -80,6 +80,24 @@ def test_ssa_residual_parity(V, OB, L, request, case, cross):
     assert abs(rel_g - rel_o) < 1e-11 * rel_o
 
 
+@pytest.mark.parametrize("law", [0, 2, 3], ids=["no_sliding", "budd", "coulomb"])
+def test_ssa_residual_parity_sliding_laws(V, OB, L, chan, law):
+    """The other sliding laws of sliding_laws.f90 (no sliding, Budd and Coulomb
+    with beta_sq read as till yield stress) through one assembly + product."""
+    inp = chan.inputs()
+    kw = dict(include_crossterms=0, sliding_law=law, slid_delta_v=1e-3)
+    s = V.VelocitySolver(chan.descriptor(), **kw)
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    rng = np.random.default_rng(7)
+    u = rng.normal(0, 80, chan.mesh.nT); v = rng.normal(0, 80, chan.mesh.nT)
+    rel_g, r_g, b_g = s.residual(u, v)
+    s.close()
+    rel_o, r_o, b_o = OB.Oracle(chan.descriptor()).residual(params(L, **kw), inp, u, v)
+    assert max_rel(b_g, b_o) < 1e-12
+    assert max_rel(r_g, r_o) < 1e-11
+    assert abs(rel_g - rel_o) < 1e-11 * rel_o
+
+
 def test_prescribed_velocity_rows(V, OB, L, dome):
     inp = dome.inputs()
     nT = dome.mesh.nT
