@@ -74,95 +74,130 @@ void launch_fraction_gr_b(const DevMesh &m, const double *Hi, const double *Hb, 
 // ---------------------------------------------------------------------------
 constexpr int kHeatMaxNz = 32;
 
+// The kernel is instantiated for NZ >= nz layers; every array index is static
+// once the NZ-loops are unrolled (the layer count enters only through the
+// k < nz and k == nz - 1 tests), so for NZ <= 16 a column stays in registers.
+// The solve updates T in place: the forward sweep reads all of T before the
+// back substitution writes it.
+
+// One implicit step of one column (solve_1D_heat_equation,
+// thermodynamics_3D_heat_equation.f90:309-479): rows 1..nz-2 from the
+// advection/diffusion coefficients, surface row T = min(T_surf, T0), basal row
+// grounded T = min(T_pmp, T(nz-2) - dzeta Q / (dzeta/dz K)) with the old
+// T(nz-2) (explicit), floating T = T_pmp; Thomas (tridiagonal_solver.f90:13-51);
+// clamp to T_pmp.  al, bl, cl are the row's sub-, main- and super-diagonal
+// without the 1/dt term, r0 the explicit right-hand-side terms.
 template <int NZ>
-__device__ __forceinline__ void heat_step(int nz, const double *__restrict__ tri, const double (&Ti)[NZ],
-                                          const double (&c1)[NZ], const double (&c2)[NZ],
-                                          const double (&rhs0)[NZ], double T_surf, const double (&Tpmp)[NZ],
-                                          double Kb, double zzb, double dzb, int grounded, double Q, double Tbf,
-                                          double dt, double (&out)[NZ]) {
-  double bdiag[NZ], cup[NZ], d[NZ];
-  // row 0: surface BC
+__device__ __forceinline__ void heat_step(int nz, double (&T)[NZ], const double (&al)[NZ],
+                                          const double (&bl)[NZ], const double (&cl)[NZ],
+                                          const double (&r0)[NZ], double T_surf, double Hpmp,
+                                          const double *__restrict__ zeta, double Kb, double zzb, double dzb,
+                                          int grounded, double Q, double dt) {
+  double bdiag[NZ], d[NZ];
   bdiag[0] = 1.0;
-  cup[0] = 0.0;
   d[0] = fmin(T_surf, kT0);
-  double aprev = 0.0;
-  // forward elimination interleaved with row construction
-  for (int k = 1; k < nz; ++k) {
-    double a, b, c, dk;
-    if (k < nz - 1) {
-      a = c1[k] * tri[0 * kHeatMaxNz + k - 1] + c2[k] * tri[3 * kHeatMaxNz + k - 1];
-      b = 1.0 / dt + c1[k] * tri[1 * kHeatMaxNz + k] + c2[k] * tri[4 * kHeatMaxNz + k];
-      c = c1[k] * tri[2 * kHeatMaxNz + k] + c2[k] * tri[5 * kHeatMaxNz + k];
-      dk = Ti[k] / dt + rhs0[k];
-    } else {
-      a = 0.0;
-      b = 1.0;
-      c = 0.0;
-      dk = grounded ? fmin(Tpmp[nz - 1], Ti[nz - 2] - dzb * Q / (zzb * Kb)) : fmin(Tbf, Tpmp[nz - 1]);
+  double cprev = 0.0;  // super-diagonal of the previous row (0 for the surface row)
+#pragma unroll
+  for (int k = 1; k < NZ; ++k) {
+    if (k < nz) {
+      double a, b, dk;
+      if (k < nz - 1) {
+        a = al[k];
+        b = 1.0 / dt + bl[k];
+        dk = T[k] / dt + r0[k];
+      } else {
+        // floating: the basal temperature is the pressure-melting point here
+        const double tpmp = kT0 - kClausiusClapeyron * Hpmp * zeta[k];
+        a = 0.0;
+        b = 1.0;
+        dk = grounded ? fmin(tpmp, T[k - 1] - dzb * Q / (zzb * Kb)) : tpmp;
+      }
+      const double w = a / bdiag[k - 1];
+      bdiag[k] = b - w * cprev;
+      d[k] = dk - w * d[k - 1];
+      cprev = k < nz - 1 ? cl[k] : 0.0;
     }
-    double w = a / bdiag[k - 1];
-    bdiag[k] = b - w * cup[k - 1];
-    d[k] = dk - w * d[k - 1];
-    cup[k] = c;
-    (void)aprev;
   }
-  out[nz - 1] = d[nz - 1] / bdiag[nz - 1];
-  for (int k = nz - 2; k >= 0; --k) out[k] = (d[k] - cup[k] * out[k + 1]) / bdiag[k];
-  for (int k = 0; k < nz; ++k) out[k] = fmin(out[k], Tpmp[k]);
+#pragma unroll
+  for (int k = NZ - 1; k >= 0; --k) {
+    if (k == nz - 1) T[k] = d[k] / bdiag[k];
+    else if (k < nz - 1) T[k] = (d[k] - (k == 0 ? 0.0 : cl[k]) * T[k + 1]) / bdiag[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NZ; ++k)
+    if (k < nz) T[k] = fmin(T[k], kT0 - kClausiusClapeyron * Hpmp * zeta[k]);
 }
 
+template <int NZ>
 __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__restrict__ Tout,
                                                          int *__restrict__ unstable, RedSlot slot,
                                                          double *__restrict__ count) {
   __shared__ double tri[6 * kHeatMaxNz];
+  __shared__ double zeta[kHeatMaxNz];
   for (int i = threadIdx.x; i < 6 * kHeatMaxNz; i += blockDim.x) tri[i] = h.tri[i];
+  for (int i = threadIdx.x; i < h.nz; i += blockDim.x) zeta[i] = h.zeta[i];
   __syncthreads();
   const int nV = h.nV, nz = h.nz;
   double nun = 0.0;
+#pragma unroll 1
   for (int vi = blockIdx.x * blockDim.x + threadIdx.x; vi < nV; vi += gridDim.x * blockDim.x) {
     unstable[vi] = 0;
-    if (h.Hi_eff[vi] < h.Hi_min) {
+    const double H = h.Hi_eff[vi];
+    if (H < h.Hi_min) {
       double ts = h.T_surf[vi];
       for (int k = 0; k < nz; ++k) Tout[(size_t)k * nV + vi] = ts;
       continue;
     }
-    double Ti[kHeatMaxNz], c1[kHeatMaxNz], c2[kHeatMaxNz], rhs0[kHeatMaxNz], Tpmp[kHeatMaxNz];
-    double Kb = 0.0;
-    for (int k = 0; k < nz; ++k) {
-      size_t o = (size_t)k * nV + vi;
-      double T = h.Ti[o];
-      Ti[k] = T;
-      Tpmp[k] = kT0 - kClausiusClapeyron * h.Hi_eff[vi] * h.zeta[k];
-      double K = 3.101e8 * exp(-0.0057 * T);
-      double Cp = 2115.3 + 7.79293 * (T - kT0);
-      double zz = h.zz[o];
-      c1[k] = h.zt[o] + h.u[o] * h.zx[o] + h.v[o] * h.zy[o] + h.w[o] * zz;
-      c2[k] = -K / (kIceDensity * Cp) * zz * zz;
-      rhs0[k] = -h.uadv[o] - h.vadv[o] + h.Phi[o] / (kIceDensity * Cp);
-      if (k == nz - 1) Kb = K;
+    // the column's initial temperature, the rows' coefficients (without 1/dt)
+    // and the explicit terms, all from the input temperature
+    double T0[NZ], al[NZ], bl[NZ], cl[NZ], r0[NZ];
+    double Kb = 0.0, zzb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      al[k] = bl[k] = cl[k] = r0[k] = T0[k] = 0.0;
+      if (k < nz) {
+        const size_t o = (size_t)k * nV + vi;
+        const double T = h.Ti[o];
+        T0[k] = T;
+        const double K = 3.101e8 * exp(-0.0057 * T);
+        const double Cp = 2115.3 + 7.79293 * (T - kT0);
+        const double zz = h.zz[o];
+        const double c1 = h.zt[o] + h.u[o] * h.zx[o] + h.v[o] * h.zy[o] + h.w[o] * zz;
+        const double c2 = -K / (kIceDensity * Cp) * zz * zz;
+        if (k >= 1 && k < nz - 1) {
+          al[k] = c1 * tri[0 * kHeatMaxNz + k - 1] + c2 * tri[3 * kHeatMaxNz + k - 1];
+          bl[k] = c1 * tri[1 * kHeatMaxNz + k] + c2 * tri[4 * kHeatMaxNz + k];
+          cl[k] = c1 * tri[2 * kHeatMaxNz + k] + c2 * tri[5 * kHeatMaxNz + k];
+        }
+        r0[k] = -h.uadv[o] - h.vadv[o] + h.Phi[o] / (kIceDensity * Cp);
+        if (k == nz - 1) { Kb = K; zzb = zz; }
+      }
     }
-    double zzb = h.zz[(size_t)(nz - 1) * nV + vi];
-    double dzb = h.zeta[nz - 1] - h.zeta[nz - 2];
-    int grounded = h.mask[vi] == 1;
-    double Q = h.Q[vi], Tbf = Tpmp[nz - 1], ts = h.T_surf[vi];
+    const double dzb = zeta[nz - 1] - zeta[nz - 2];
+    const int grounded = h.mask[vi] == 1;
+    const double Q = h.Q[vi], ts = h.T_surf[vi];
     // dt halving without resetting the column between attempts
     // (thermodynamics_3D_heat_equation.f90:161-239)
-    double Tc[kHeatMaxNz], Tn[kHeatMaxNz];
-    for (int k = 0; k < nz; ++k) Tc[k] = Ti[k];
+    double Tc[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) Tc[k] = T0[k];
     bool stable = false;
+#pragma unroll 1
     for (int it = 1; it <= 10 && !stable; ++it) {
-      double dta = h.dt * pow(0.5, (double)(it - 1));
-      int nsub = 1 << (it - 1);
-      for (int sstep = 0; sstep < nsub; ++sstep) {
-        heat_step<kHeatMaxNz>(nz, tri, Tc, c1, c2, rhs0, ts, Tpmp, Kb, zzb, dzb, grounded, Q, Tbf, dta, Tn);
-        for (int k = 0; k < nz; ++k) Tc[k] = Tn[k];
-      }
+      const double dta = ldexp(h.dt, 1 - it);  // dt / 2^(it-1), exact
+      const int nsub = 1 << (it - 1);
+#pragma unroll 1
+      for (int sstep = 0; sstep < nsub; ++sstep)
+        heat_step<NZ>(nz, Tc, al, bl, cl, r0, ts, H, zeta, Kb, zzb, dzb, grounded, Q, dta);
       stable = true;
-      for (int k = 0; k < nz; ++k)
-        if (Tc[k] != Tc[k] || Tc[k] < 180.0 || Tc[k] > kT0) stable = false;
+#pragma unroll
+      for (int k = 0; k < NZ; ++k)
+        if (k < nz && (Tc[k] != Tc[k] || Tc[k] < 180.0 || Tc[k] > kT0)) stable = false;
     }
     if (!stable) { unstable[vi] = 1; nun += 1.0; }
-    for (int k = 0; k < nz; ++k) Tout[(size_t)k * nV + vi] = Tc[k];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+      if (k < nz) Tout[(size_t)k * nV + vi] = Tc[k];
   }
   double acc[1] = {nun};
   __shared__ double lds[1][kBlock / 64];
@@ -173,7 +208,14 @@ __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__re
 void launch_heat_columns(const HeatDev &h, double *Tout, int *unstable, const RedSlot &slot, double *count,
                          hipStream_t s) {
   int nb = std::min(blocks_for(h.nV), kMaxPartials);
-  hipLaunchKernelGGL(k_heat_columns, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
+  if (h.nz <= 8)
+    hipLaunchKernelGGL(k_heat_columns<8>, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
+  else if (h.nz <= 12)
+    hipLaunchKernelGGL(k_heat_columns<12>, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
+  else if (h.nz <= 16)
+    hipLaunchKernelGGL(k_heat_columns<16>, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
+  else
+    hipLaunchKernelGGL(k_heat_columns<kHeatMaxNz>, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
   UFE_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_partials_out<1>, dim3(1), dim3(kBlock), 0, s, slot, nb, count);
   UFE_LAUNCH_CHECK();
