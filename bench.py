@@ -249,6 +249,7 @@ def main():
             by_t = by
         live[kern] = (n_t, ms_t, by_t)
     spmv_ms, spmv_bytes = solvers[0][1].profile_spmv(args.spmv_reps)
+    heat = time_heat(V, W, w.mesh.nV, 12, args.steps, solvers[0][1]) if args.workload == "continental" else None
     for _, s in solvers:
         s.close()
     n_solves = args.steps * len(ws)
@@ -306,25 +307,31 @@ def main():
                                         "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9}},
         "cpu_baseline": None,
     }
-    if args.workload == "continental":
-        out["heat_columns"] = time_heat(V, W, w.mesh.nV, 12, args.steps)
+    if heat is not None:
+        out["heat_columns"] = heat
     if not args.no_cpu_baseline:
         n_plain = pinfos[0].n_visc_its if pinfos else 231
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, n_plain)
     print(json.dumps(out), flush=True)
 
 
-def time_heat(V, W, nV, nz, steps):
+def time_heat(V, W, nV, nz, steps, solver):
     """Heat columns of the continental configuration: nV columns x nz layers,
-    one call = one implicit time step of every column (host arrays in, out)."""
+    one call = one implicit time step of every column (host arrays in, out),
+    through the solver's context (its stream and device buffers reused from
+    call to call) and without a context (buffers of each call)."""
     c = W.heat_columns_inputs(nV, nz, seed=3)
     hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], c["fields"], c["dt"])
-    V.heat_columns(hin)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        T, unst, nun = V.heat_columns(hin)
-    t = (time.perf_counter() - t0) / steps
+    res = {}
+    for label, sv in (("with_context", solver), ("without_context", None)):
+        V.heat_columns(hin, sv)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            T, unst, nun = V.heat_columns(hin, sv)
+        res[label] = (time.perf_counter() - t0) / steps
+    t = res["with_context"]
     return {"columns": nV, "nz": nz, "seconds_per_call": t, "columns_per_s": nV / t, "n_unstable": nun,
+            "seconds_per_call_without_context": res["without_context"],
             "note": "includes host<->device transfer of the inputs (the C ABI takes host arrays)"}
 
 

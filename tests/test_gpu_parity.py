@@ -345,6 +345,20 @@ def test_heat_columns_parity(V, OB, W, nV, nz):
     assert np.abs(T - To).max() < 1e-9
 
 
+def test_heat_columns_with_context_buffers(V, W, dome):
+    """Through a context (its stream and device buffers kept between calls,
+    grown when a call is larger) the columns equal the context-free call."""
+    s = V.VelocitySolver(dome.descriptor())
+    outs = []
+    for nV, nz in ((5000, 12), (20000, 12), (3000, 10)):
+        c = W.heat_columns_inputs(nV, nz, seed=nV)
+        hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], c["fields"], c["dt"])
+        a = V.heat_columns(hin, s)
+        b = V.heat_columns(hin)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    s.close()
+
+
 def test_heat_steady_profile_fixed_point_gpu(V):
     from test_oracle_fgr_heat import steady_case
     nV, nz, f, Ti = steady_case()

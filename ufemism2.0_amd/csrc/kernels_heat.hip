@@ -273,6 +273,7 @@ struct ufe_ctx;
 namespace ufe {
 int ctx_device(const ufe_ctx *c);
 hipStream_t ctx_stream(const ufe_ctx *c);
+void *ctx_heat_buffer(ufe_ctx *c, int i, size_t bytes);
 const DevMesh *ctx_mesh(const ufe_ctx *c);
 Krylov *ctx_krylov(const ufe_ctx *c);
 int ctx_fail(ufe_ctx *c, int code, const std::string &msg);
@@ -322,14 +323,24 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
   try {
     int dev = c ? ctx_device(c) : 0;
     UFE_HIP(hipSetDevice(dev));
+    // with a context: its stream and device buffers kept between calls (a
+    // time-stepping caller allocates nothing after the first call); without
+    // one: a stream and buffers of this call only
     hipStream_t s;
-    UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (c) s = ctx_stream(c);
+    else UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     size_t n3 = sizeof(double) * (size_t)nV * nz, n1 = sizeof(double) * (size_t)nV;
     std::vector<std::unique_ptr<DBuf>> keep;
-    auto up = [&](const void *h, size_t bytes) {
+    int nbuf = 0;
+    auto buf = [&](size_t bytes) -> void * {
+      if (c) return ctx_heat_buffer(c, nbuf++, bytes);
       keep.push_back(std::make_unique<DBuf>(std::max<size_t>(bytes, 8)));
-      if (bytes) UFE_HIP(hipMemcpyAsync(keep.back()->p, h, bytes, hipMemcpyHostToDevice, s));
       return keep.back()->p;
+    };
+    auto up = [&](const void *h, size_t bytes) {
+      void *d = buf(bytes);
+      if (bytes) UFE_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+      return d;
     };
     double tri[6 * kHeatMaxNz];
     zeta_tridiagonals(nz, in->zeta, tri);
@@ -354,18 +365,20 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
     h.mask = (const int *)up(in->mask, sizeof(int) * nV);
     h.dt = in->dt;
     h.Hi_min = in->Hi_min_thermo;
-    DBuf Tout(std::max<size_t>(n3, 8)), unst(sizeof(int) * std::max(nV, 1)), cnt(sizeof(double) * 2);
-    DBuf part(sizeof(double) * kMaxPartials);
-    RedSlot slot{part.as<double>(), cnt.as<double>()};
-    if (nV > 0) launch_heat_columns(h, Tout.as<double>(), unst.as<int>(), slot, cnt.as<double>(), s);
+    double *Tout = (double *)buf(n3);
+    int *unst = (int *)buf(sizeof(int) * std::max(nV, 1));
+    double *cnt = (double *)buf(sizeof(double) * 2);
+    double *part = (double *)buf(sizeof(double) * kMaxPartials);
+    RedSlot slot{part, cnt};
+    if (nV > 0) launch_heat_columns(h, Tout, unst, slot, cnt, s);
     double hc = 0.0;
     if (nV > 0) {
-      UFE_HIP(hipMemcpyAsync(Tnew, Tout.p, n3, hipMemcpyDeviceToHost, s));
-      UFE_HIP(hipMemcpyAsync(is_unstable, unst.p, sizeof(int) * nV, hipMemcpyDeviceToHost, s));
-      UFE_HIP(hipMemcpyAsync(&hc, cnt.p, sizeof(double), hipMemcpyDeviceToHost, s));
+      UFE_HIP(hipMemcpyAsync(Tnew, Tout, n3, hipMemcpyDeviceToHost, s));
+      UFE_HIP(hipMemcpyAsync(is_unstable, unst, sizeof(int) * nV, hipMemcpyDeviceToHost, s));
+      UFE_HIP(hipMemcpyAsync(&hc, cnt, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     UFE_HIP(hipStreamSynchronize(s));
-    UFE_HIP(hipStreamDestroy(s));
+    if (!c) UFE_HIP(hipStreamDestroy(s));
     *n_unstable = (int)hc;
     return UFE_OK;
   } catch (const std::exception &e) {

@@ -52,6 +52,7 @@ struct ufe_ctx {
   std::string err;
   double timings[5] = {0, 0, 0, 0, 0};
   int n_fallbacks = 0;
+  std::vector<std::unique_ptr<DBuf>> heat_bufs;  // ufe_heat_columns device buffers, reused
   bool amg_fold = true;  // UFE_AMG_FOLD=0: keep the separate level-0 pre-smoothing kernel
   double bbox[4] = {0, 1, 0, 1};  // xmin, xmax, ymin, ymax of the vertices
   DBuf feat;                     // learned-guess features [nT][16] fp32 (device order)
@@ -603,6 +604,14 @@ static void from_dev(const std::vector<int> &n2o, const U *src, U *dst, int laye
 // Accessors for the other translation units (grounded fractions, heat columns).
 namespace ufe {
 int ctx_device(const ufe_ctx *c) { return c->device; }
+// Device buffer i of the heat columns, kept in the context between calls and
+// grown when a call needs more.
+void *ctx_heat_buffer(ufe_ctx *c, int i, size_t bytes) {
+  if ((int)c->heat_bufs.size() <= i) c->heat_bufs.resize(i + 1);
+  auto &b = c->heat_bufs[i];
+  if (!b || b->bytes < bytes) b = std::make_unique<DBuf>(std::max<size_t>(bytes, 8));
+  return b->p;
+}
 hipStream_t ctx_stream(const ufe_ctx *c) { return c->s; }
 const DevMesh *ctx_mesh(const ufe_ctx *c) { return &c->m; }
 Krylov *ctx_krylov(const ufe_ctx *c) { return c->kry.get(); }
