@@ -53,6 +53,24 @@ def test_csr_matvec_large_random(V, OB):
     assert max_rel(V.csr_matvec(A, x), OB.csr_matvec(A.indptr, A.indices, A.data, x)) < 1e-14
 
 
+@pytest.mark.parametrize("pc", ["none", "jacobi"])
+def test_csr_solve_laplace_known_answer(V, L, W, pc):
+    """The reference's Laplace solve test (ct_discretisation_solve_Laplace_eq.f90:142-207)
+    through the linear-algebra boundary: the solution is the exact quadratic."""
+    import scipy.sparse as sp
+    from test_oracle_linear import laplace_case
+    A, b, f_ex = laplace_case(W)
+    # rows scaled by their diagonal (the Laplace rows are ~1e-9, the boundary
+    # rows 1), as the reference's Jacobi solver iterates on D^-1 A; the
+    # stopping test then measures every row on one scale
+    d = 1.0 / A.diagonal()
+    As, bs = sp.csr_matrix(sp.diags(d) @ A), d * b
+    x, its, rel = V.csr_solve(As, bs, rtol=1e-13, maxit=50000,
+                              precond=L.PC_NONE if pc == "none" else L.PC_JACOBI)
+    assert its > 0 and rel <= 1e-13
+    assert np.abs(x - f_ex).max() < 1e-8 * np.abs(f_ex).max()
+
+
 def test_csr_solve_vs_direct(V, OB, L, dome):
     import scipy.sparse.linalg as spl
     rng = np.random.default_rng(3)

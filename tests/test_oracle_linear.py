@@ -75,3 +75,33 @@ def test_gmres_ilu0_vs_direct_ssa_system(OB, L, dome):
     x, its = OB.gmres_ilu0(A, b, rtol=1e-13, maxit=20000)
     xd = spl.spsolve(A.tocsc(), b)
     assert np.abs(x - xd).max() / np.abs(xd).max() < 1e-8
+
+
+def laplace_case(W, h=25e3):
+    """The reference's Laplace known answer (ct_discretisation_solve_Laplace_eq.f90:142-207):
+    d2f/dx2 + d2f/dy2 = c for r < r0 and f = f_ex for r >= r0 on the triangle
+    centres, with f_ex = -c/4 r0^2 + c/4 r^2, c = -1e-9, r0 = 0.8 xmax, the
+    rows assembled from the M2 d2dx2 + d2dy2 operators.  The M2 operators are
+    exact on quadratics in the interior, so f_ex solves the discrete system."""
+    mesh = W.uniform_mesh(-500e3, 500e3, -500e3, 500e3, h, seed=9)
+    ops = W.build_operators(mesh)
+    c, r0 = -1e-9, mesh.xmax * 0.8
+    x, y = mesh.TriGC[:, 0], mesh.TriGC[:, 1]
+    f_ex = -c / 4 * r0 ** 2 + c / 4 * (x ** 2 + y ** 2)
+    lap = ops.bb.matrix("d2dx2", mesh.nT) + ops.bb.matrix("d2dy2", mesh.nT)
+    inside = np.hypot(x, y) < r0
+    A = sp.diags((~inside).astype(float)) + sp.diags(inside.astype(float)) @ lap
+    A = sp.csr_matrix(A)
+    A.eliminate_zeros()
+    A.sort_indices()
+    b = np.where(inside, c, f_ex)
+    return A, b, f_ex
+
+
+def test_laplace_known_answer_oracle(OB, W):
+    A, b, f_ex = laplace_case(W)
+    # f_ex solves the discrete system (interior rows to round-off of c = -1e-9)
+    assert np.abs(A @ f_ex - b).max() < 1e-10 * 1e-9
+    x, its = OB.gmres_ilu0(A, b, rtol=1e-12, maxit=20000)
+    assert its > 0
+    assert np.abs(x - f_ex).max() < 1e-9 * np.abs(f_ex).max()
