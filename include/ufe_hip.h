@@ -297,6 +297,30 @@ typedef struct ufe_heat_inputs {
 int ufe_heat_columns(ufe_ctx *ctx, const ufe_heat_inputs *in, double *Ti_tplusdt, int *is_unstable,
                      int *n_unstable);
 
+/* Unstable columns after the heat solve (thermodynamics_3D_heat_equation.f90:254-279):
+ * when fewer than ceil(nV / 100) columns are flagged, each flagged column of
+ * Ti [nV*nz, vertex fastest] is replaced in place with the Robin solution
+ * (replace_Ti_with_robin_solution, thermodynamics_utilities.f90:256-337):
+ * grounded ice in an accumulation area the steady advection-diffusion
+ * profile of Cuffey & Paterson (2010) eq. 9.13-9.22, grounded ice with
+ * SMB <= 0 a linear profile from T_surf to the basal pressure-melting point,
+ * floating ice a linear profile from T_surf to the basal pressure-melting
+ * point, thin ice T_surf; then clamped to the pressure-melting point.
+ * Otherwise returns UFE_ERR_DIVERGED (the reference stops the model).
+ * Units: K, m, m/yr (SMB), J/m^2/yr (geothermal heat flux). */
+typedef struct ufe_heat_robin_inputs {
+  int nV, nz;
+  const double *zeta;             /* [nz]                                            */
+  const double *Hi_eff;           /* [nV]                                            */
+  const double *T_surf;           /* [nV] annual mean surface temperature            */
+  const double *SMB;              /* [nV] surface mass balance                       */
+  const double *geothermal_flux;  /* [nV]                                            */
+  const int    *mask;             /* [nV] 0 thin / 1 grounded / 2 floating           */
+  double Hi_min_thermo;
+} ufe_heat_robin_inputs;
+int ufe_heat_robin(ufe_ctx *ctx, const ufe_heat_robin_inputs *in, const int *is_unstable, double *Ti,
+                   int *n_replaced);
+
 /* ---- mesh operator construction (host, once per mesh) -----------------
  * calc_matrix_operators_mesh_a_b / _b_a / _b_b_2nd_order
  * (mesh_disc_calc_matrix_operators_2D.f90:198-474,612-764).  Two-call

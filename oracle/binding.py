@@ -51,6 +51,7 @@ def lib():
                                     dp, dp, C.c_int, ip, ip, dp, dp]
     o.oracle_fraction_gr_b.argtypes = [C.POINTER(L.MeshDesc), dp, dp, dp, dp]
     o.oracle_heat_columns.argtypes = [C.POINTER(L.HeatInputs), dp, ip, ip]
+    o.oracle_heat_robin.argtypes = [C.POINTER(L.HeatRobinInputs), ip, dp, ip]
     o.oracle_zeta_tridiagonals.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
     _o = o
     return o
@@ -161,6 +162,16 @@ class Oracle:
         lib().oracle_fraction_gr_b(self.md.byref(), L.ptr(L.f64(Hi)), L.ptr(L.f64(Hb)), L.ptr(L.f64(SL)),
                                    L.ptr(out))
         return out
+
+
+def heat_robin(rin, is_unstable, Ti):
+    """rin: HeatRobinInputs; returns (rc, new Ti, number replaced)."""
+    L = _L()
+    T = L.f64(Ti).copy()
+    flag = L.i32(is_unstable)
+    n = C.c_int(0)
+    rc = lib().oracle_heat_robin(C.byref(rin), L.ptr(flag, L.ip), L.ptr(T), C.byref(n))
+    return rc, T, n.value
 
 
 def heat_columns(hin):

@@ -935,6 +935,49 @@ static void solve_column(int nz, const double *zeta, const double *dl, const dou
   for (int k = 0; k < nz; ++k) out[k] = fmin(out[k], Tpmp[k]);
 }
 
+/* Unstable columns: the 1% rule of thermodynamics_3D_heat_equation.f90:254-279 and
+ * replace_Ti_with_robin_solution (thermodynamics_utilities.f90:256-337) per flagged
+ * column; Ti is [nz][nV] (vertex fastest), updated in place. */
+int oracle_heat_robin(const ufe_heat_robin_inputs *in, const int *is_unstable, double *Ti, int *n_replaced) {
+  const int nV = in->nV, nz = in->nz;
+  int n = 0;
+  for (int vi = 0; vi < nV; ++vi) n += is_unstable[vi] == 1;
+  *n_replaced = 0;
+  if (n >= (nV + 99) / 100) return UFE_ERR_DIVERGED;
+  const double sec_per_year = 31556943.36;                              /* parameters.f90:16 */
+  const double Kr = 9.828 * sec_per_year * exp(-0.0057 * T0);    /* :283-287 */
+  const double kappa = Kr / (ICE_DENSITY * 2127.5);                     /* :285,288 */
+  for (int vi = 0; vi < nV; ++vi) {
+    if (is_unstable[vi] != 1) continue;
+    const double grad_b = -in->geothermal_flux[vi] / Kr;               /* :289 */
+    const double Ts = fmin(T0, in->T_surf[vi]);                  /* :291 */
+    const double H = in->Hi_eff[vi];
+    for (int k = 0; k < nz; ++k) {
+      double *t = &Ti[(size_t)k * nV + vi];
+      const double tpmp = T0 - CLAUSIUS_CLAPEYRON * H * in->zeta[k];
+      if (H > in->Hi_min_thermo) {
+        if (in->mask[vi] == 1) {
+          if (in->SMB[vi] > 0.0) {                                      /* :299-308 */
+            const double L = sqrt(2.0 * kappa * H / in->SMB[vi]);
+            const double d = (1.0 - in->zeta[k]) * H;
+            *t = Ts + sqrt(acos(-1.0)) / 2.0 * L * grad_b * (erf(d / L) - erf(H / L));
+          } else {                                                      /* :310-313 */
+            *t = Ts + ((T0 - CLAUSIUS_CLAPEYRON * H) - Ts) * in->zeta[k];
+          }
+        } else if (in->mask[vi] == 2) {                                 /* :317-321 */
+          const double tpmp_base = T0 - CLAUSIUS_CLAPEYRON * H * in->zeta[nz - 1];
+          *t = Ts + in->zeta[k] * (tpmp_base - Ts);
+        }
+      } else {
+        *t = Ts;                                                        /* :325-329 */
+      }
+      *t = fmin(*t, tpmp);                                              /* :331-334 */
+    }
+  }
+  *n_replaced = n;
+  return 0;
+}
+
 int oracle_heat_columns(const ufe_heat_inputs *in, double *Tnew, int *is_unstable, int *n_unstable) {
   int nV = in->nV, nz = in->nz;
   if (nz > 64 || nz < 3) return UFE_ERR_ARGUMENT;

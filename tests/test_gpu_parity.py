@@ -377,6 +377,23 @@ def test_heat_columns_with_context_buffers(V, W, dome):
     s.close()
 
 
+def test_heat_robin_matches_oracle(V, OB):
+    """Robin replacement of flagged columns on the device equals the oracle's
+    (thermodynamics_utilities.f90:256-337), and the 1% rule stops as the
+    reference does."""
+    from test_oracle_fgr_heat import robin_case
+    nV, nz = 700, 12
+    rin, keep, zeta, H, Ts, SMB, G, mask, T_in = robin_case(nz, nV)
+    flag = np.zeros(nV, np.int32); flag[:6] = 1
+    T_g, n_g = V.heat_robin(rin, flag, T_in)
+    rc, T_o, n_o = OB.heat_robin(rin, flag, T_in)
+    assert rc == 0 and n_g == n_o == 6
+    assert max_rel(T_g, T_o) < 1e-12
+    flag[6] = 1
+    with pytest.raises(V.VelocitySolverError):
+        V.heat_robin(rin, flag, T_in)
+
+
 def test_heat_steady_profile_fixed_point_gpu(V):
     from test_oracle_fgr_heat import steady_case
     nV, nz, f, Ti = steady_case()

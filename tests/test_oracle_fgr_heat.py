@@ -141,3 +141,69 @@ def test_heat_thin_ice_unstable_and_errors(L, OB, W):
     hin, keep = heat_inputs(L, 500, 10, f, 10.0)
     rc, *_ = OB.heat_columns(hin)
     assert rc < 0
+
+
+# -- Robin replacement of unstable columns -------------------------------------
+
+def robin_case(nz=401, nV=200):
+    """Columns of every kind; each flagged column is checked by a property the
+    reference formula fixes (thermodynamics_utilities.f90:256-337)."""
+    from ufemism2_amd import _lib as L
+    zeta = np.linspace(0.0, 1.0, nz)
+    rng = np.random.default_rng(11)
+    H = rng.uniform(500.0, 3000.0, nV)
+    Ts = rng.uniform(230.0, 265.0, nV)
+    SMB = rng.uniform(0.05, 0.5, nV)
+    G = rng.uniform(0.04, 0.07, nV) * 31556943.36  # W/m^2 -> J/m^2/yr
+    mask = np.ones(nV, np.int32)
+    SMB[1] = -0.3          # ablation
+    mask[2] = 2            # floating
+    H[3] = 5.0             # thin ice
+    Ts[4] = 280.0          # surface above T0
+    G[5] *= 40.0           # warm base: clamped to the pressure-melting point
+    keep = [zeta, H, Ts, SMB, G, mask]
+    rin = L.HeatRobinInputs(nV, nz, L.ptr(zeta), L.ptr(H), L.ptr(Ts), L.ptr(SMB), L.ptr(G), L.ptr(mask, L.ip), 10.0)
+    T_in = np.full(nz * nV, 250.0)
+    return rin, keep, zeta, H, Ts, SMB, G, mask, T_in
+
+
+def test_robin_profiles_oracle(OB):
+    nV, nz = 700, 401
+    rin, keep, zeta, H, Ts, SMB, G, mask, T_in = robin_case(nz, nV)
+    flag = np.zeros(nV, np.int32); flag[:6] = 1
+    rc, T, n = OB.heat_robin(rin, flag, T_in)
+    assert rc == 0 and n == 6
+    T = T.reshape(nz, nV)
+    T0, cc = 273.16, 8.7e-4
+    # unflagged columns untouched
+    assert np.array_equal(T[:, 6:], T_in.reshape(nz, nV)[:, 6:])
+    # accumulation: T(surface) = Ts and dT/dz at the bed = -G / K(T0) (central difference on 401 layers)
+    Kr = 9.828 * 31556943.36 * np.exp(-0.0057 * T0)
+    col = T[:, 0]
+    assert abs(col[0] - min(T0, Ts[0])) < 1e-9
+    dz = (zeta[1] - zeta[0]) * H[0]
+    grad_bed = (col[-2] - col[-1]) / dz   # d/d(height above bed) at the bed, one-sided
+    grad_bed2 = (-3 * col[-1] + 4 * col[-2] - col[-3]) / (2 * dz)
+    assert abs(grad_bed2 - (-G[0] / Kr)) < 1e-3 * abs(G[0] / Kr)
+    assert np.all(np.diff(col) > 0) == (G[0] > 0)  # temperature rises towards the bed
+    # ablation and floating: linear from Ts to the basal pressure-melting point
+    for vi in (1, 2):
+        lin = min(T0, Ts[vi]) + zeta * ((T0 - cc * H[vi]) - min(T0, Ts[vi]))
+        assert np.allclose(T[:, vi], np.minimum(lin, T0 - cc * H[vi] * zeta), rtol=0, atol=1e-9)
+    # thin ice: the surface temperature; surface above T0 capped at T0
+    assert np.allclose(T[:, 3], min(T0, Ts[3]), atol=1e-12)
+    assert T[0, 4] <= T0 + 1e-12
+    # warm base: clamped to the pressure-melting point at depth
+    assert np.all(T[:, 5] <= T0 - cc * H[5] * zeta + 1e-9)
+    assert abs(T[-1, 5] - (T0 - cc * H[5])) < 1e-9
+
+
+def test_robin_one_percent_rule_oracle(OB):
+    nV, nz = 700, 11
+    rin, keep, *_ , T_in = robin_case(nz, nV)
+    flag = np.zeros(nV, np.int32); flag[:7] = 1   # ceil(700/100) = 7 flagged: the reference stops
+    rc, T, n = OB.heat_robin(rin, flag, T_in)
+    assert rc < 0 and n == 0
+    flag[6] = 0
+    rc, T, n = OB.heat_robin(rin, flag, T_in)
+    assert rc == 0 and n == 6

@@ -90,6 +90,13 @@ class HeatInputs(C.Structure):
     ]
 
 
+class HeatRobinInputs(C.Structure):
+    _fields_ = [
+        ("nV", C.c_int), ("nz", C.c_int), ("zeta", dp), ("Hi_eff", dp), ("T_surf", dp), ("SMB", dp),
+        ("geothermal_flux", dp), ("mask", ip), ("Hi_min_thermo", C.c_double),
+    ]
+
+
 def ptr(a, kind=dp):
     """Pointer to a contiguous numpy array (None -> NULL)."""
     if a is None:
@@ -150,6 +157,7 @@ def lib():
     _sig(L, 'ufe_profile_copy', argtypes=[C.c_void_p, C.c_long, C.c_int, dp])
     _sig(L, 'ufe_kernel_timing', argtypes=[C.c_void_p, C.c_int])
     _sig(L, 'ufe_set_amg_options', argtypes=[C.c_void_p, C.c_double, C.c_double, C.c_int])
+    _sig(L, 'ufe_heat_robin', argtypes=[C.c_void_p, C.POINTER(HeatRobinInputs), ip, dp, ip])
     _sig(L, 'ufe_kernel_timing_read', argtypes=[C.c_void_p, C.c_int, C.POINTER(C.c_longlong), dp, dp])
     fp = C.POINTER(C.c_float)
     _sig(L, 'ufe_mpnn_num_weights', restype=C.c_int, argtypes=[])
@@ -181,7 +189,7 @@ EXPORTED = [
     "ufe_last_timings", "ufe_profile_spmv", "ufe_profile_copy", "ufe_kernel_timing",
     "ufe_kernel_timing_read", "ufe_set_amg_options", "ufe_mpnn_num_weights", "ufe_mpnn_create",
     "ufe_mpnn_destroy", "ufe_mpnn_get_weights", "ufe_mpnn_forward", "ufe_mpnn_loss_grad", "ufe_mpnn_train",
-    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns",
+    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_robin",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
 ]
 

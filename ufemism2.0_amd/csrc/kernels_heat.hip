@@ -221,6 +221,46 @@ void launch_heat_columns(const HeatDev &h, double *Tout, int *unstable, const Re
   UFE_LAUNCH_CHECK();
 }
 
+// Robin solution of one flagged column (replace_Ti_with_robin_solution,
+// thermodynamics_utilities.f90:256-337).  One thread per vertex; unflagged
+// columns are left alone.
+constexpr double kSecPerYear = 31556943.36;  // parameters.f90:16
+__global__ void __launch_bounds__(kBlock) k_heat_robin(int nV, int nz, const double *__restrict__ zeta,
+                                                       const double *__restrict__ Hi_eff,
+                                                       const double *__restrict__ T_surf,
+                                                       const double *__restrict__ SMB,
+                                                       const double *__restrict__ G, const int *__restrict__ mask,
+                                                       const int *__restrict__ flag, double Hi_min,
+                                                       double *__restrict__ Ti) {
+  const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (vi >= nV || flag[vi] != 1) return;
+  // conductivity at T0 (Ritz 1987), diffusivity with c_0 = 2127.5 J/kg/K
+  const double Kr = 9.828 * kSecPerYear * exp(-0.0057 * kT0);
+  const double kappa = Kr / (kIceDensity * 2127.5);
+  const double grad_b = -G[vi] / Kr;
+  const double Ts = fmin(kT0, T_surf[vi]);
+  const double H = Hi_eff[vi];
+  for (int k = 0; k < nz; ++k) {
+    double T = Ti[(size_t)k * nV + vi];
+    if (H > Hi_min) {
+      if (mask[vi] == 1) {
+        if (SMB[vi] > 0.0) {
+          const double Lt = sqrt(2.0 * kappa * H / SMB[vi]);
+          const double d = (1.0 - zeta[k]) * H;
+          T = Ts + sqrt(M_PI) / 2.0 * Lt * grad_b * (erf(d / Lt) - erf(H / Lt));
+        } else {
+          T = Ts + ((kT0 - kClausiusClapeyron * H) - Ts) * zeta[k];
+        }
+      } else if (mask[vi] == 2) {
+        T = Ts + zeta[k] * ((kT0 - kClausiusClapeyron * H * zeta[nz - 1]) - Ts);
+      }
+    } else {
+      T = Ts;
+    }
+    Ti[(size_t)k * nV + vi] = fmin(T, kT0 - kClausiusClapeyron * H * zeta[k]);
+  }
+}
+
 // zeta tridiagonals: calc_vertical_operators_reg_1D (mesh_zeta.f90:287-365)
 // with 1-D 2nd-order shape functions (shape_functions.f90:17-79), then the
 // tridiagonal extraction of calc_zeta_operators_tridiagonal (:487-605).
@@ -302,6 +342,52 @@ extern "C" int ufe_fraction_gr_b(ufe_ctx *c, const double *Hi, const double *Hb,
     return UFE_OK;
   } catch (const std::exception &e) {
     return ctx_fail(c, UFE_ERR_DEVICE, e.what());
+  }
+}
+
+extern "C" int ufe_heat_robin(ufe_ctx *c, const ufe_heat_robin_inputs *in, const int *is_unstable, double *Ti,
+                              int *n_replaced) {
+  if (!in || !is_unstable || !Ti || !n_replaced) return heat_fail(UFE_ERR_ARGUMENT, "null argument");
+  if (in->nz < 2 || in->nV < 0 || !in->zeta || !in->Hi_eff || !in->T_surf || !in->SMB || !in->geothermal_flux ||
+      !in->mask)
+    return heat_fail(UFE_ERR_ARGUMENT, "invalid Robin inputs");
+  const int nV = in->nV, nz = in->nz;
+  int n = 0;
+  for (int vi = 0; vi < nV; ++vi) n += is_unstable[vi] == 1;
+  *n_replaced = 0;
+  // thermodynamics_3D_heat_equation.f90:256: more than 1% unstable stops the model
+  if (n >= (nV + 99) / 100)
+    return heat_fail(UFE_ERR_DIVERGED, "heat equation unstable in " + std::to_string(n) + " of " +
+                                           std::to_string(nV) + " columns (at most 1% may be replaced)");
+  if (n == 0) return UFE_OK;
+  try {
+    int dev = c ? ctx_device(c) : 0;
+    UFE_HIP(hipSetDevice(dev));
+    hipStream_t s;
+    if (c) s = ctx_stream(c);
+    else UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const size_t n3 = sizeof(double) * (size_t)nV * nz, n1 = sizeof(double) * (size_t)nV;
+    DBuf dz(sizeof(double) * nz), dH(n1), dTs(n1), dS(n1), dG(n1), dm(sizeof(int) * nV), df(sizeof(int) * nV),
+        dT(n3);
+    UFE_HIP(hipMemcpyAsync(dz.p, in->zeta, dz.bytes, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dH.p, in->Hi_eff, n1, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dTs.p, in->T_surf, n1, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dS.p, in->SMB, n1, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dG.p, in->geothermal_flux, n1, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dm.p, in->mask, sizeof(int) * nV, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(df.p, is_unstable, sizeof(int) * nV, hipMemcpyHostToDevice, s));
+    UFE_HIP(hipMemcpyAsync(dT.p, Ti, n3, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_heat_robin, dim3(blocks_for(nV)), dim3(kBlock), 0, s, nV, nz, dz.as<double>(),
+                       dH.as<double>(), dTs.as<double>(), dS.as<double>(), dG.as<double>(), dm.as<int>(),
+                       df.as<int>(), in->Hi_min_thermo, dT.as<double>());
+    UFE_LAUNCH_CHECK();
+    UFE_HIP(hipMemcpyAsync(Ti, dT.p, n3, hipMemcpyDeviceToHost, s));
+    UFE_HIP(hipStreamSynchronize(s));
+    if (!c) UFE_HIP(hipStreamDestroy(s));
+    *n_replaced = n;
+    return UFE_OK;
+  } catch (const std::exception &e) {
+    return heat_fail(UFE_ERR_DEVICE, e.what());
   }
 }
 

@@ -305,6 +305,29 @@ def heat_columns(hin, solver: VelocitySolver | None = None):
     return T, unst, nun.value
 
 
+def make_robin_inputs(nV, nz, zeta, Hi_eff, T_surf, SMB, geothermal_flux, mask, Hi_min_thermo=10.0):
+    """ufe_heat_robin_inputs from host arrays; returns (struct, arrays kept alive)."""
+    keep = [L.f64(zeta), L.f64(Hi_eff), L.f64(T_surf), L.f64(SMB), L.f64(geothermal_flux), L.i32(mask)]
+    r = L.HeatRobinInputs(nV, nz, L.ptr(keep[0]), L.ptr(keep[1]), L.ptr(keep[2]), L.ptr(keep[3]), L.ptr(keep[4]),
+                          L.ptr(keep[5], L.ip), float(Hi_min_thermo))
+    return r, keep
+
+
+def heat_robin(rin, is_unstable, Ti, solver: VelocitySolver | None = None):
+    """Robin replacement of the flagged columns (the reference's handling of
+    unstable heat columns); returns (new Ti, number replaced).  Raises when
+    ceil(nV / 100) or more columns are flagged, as the reference stops."""
+    lib = L.lib()
+    T = L.f64(Ti).copy()
+    flag = L.i32(is_unstable)
+    n = C.c_int(0)
+    ctx = solver._ctx if solver is not None else None
+    rc = lib.ufe_heat_robin(ctx, C.byref(rin), L.ptr(flag, L.ip), L.ptr(T), C.byref(n))
+    if rc < 0:
+        raise VelocitySolverError(f"ufe_heat_robin failed ({rc}): {lib.ufe_last_error(None).decode()}")
+    return T, n.value
+
+
 def csr_matvec(A, x):
     import scipy.sparse as sp
     A = sp.csr_matrix(A)
