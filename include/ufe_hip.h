@@ -339,7 +339,7 @@ int ufe_mesh_operators_bb2(int nT, const int *TriC, const double *TriGC, int *nn
  * A message-passing network on the triangle graph (3 layers, hidden 64,
  * BF16 matrix-core products with FP32 accumulation) that predicts the
  * velocity from per-triangle features of the uploaded inputs; it seeds the
- * Picard iteration, which stays the judge of the result.  Weights: one fp32
+ * Picard iteration, which stays the arbiter of the result.  Weights: one fp32
  * vector of ufe_mpnn_num_weights() values (NULL at creation = zeros).
  * Graph arguments: N nodes, X [N*16] features, nb [N*3] symmetric neighbour
  * lists (-1 = none), T [N*3] targets (log10 speed in m/yr, flow direction

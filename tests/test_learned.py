@@ -72,7 +72,7 @@ def test_training_reduces_loss(graph):
 def test_learned_guess_gains_over_zero_guess(W, V):
     """Train on solver-generated channel-ensemble solutions (60 members), then
     solve held-out members from the learned guess and from zero.  The Picard
-    iteration stays the judge: where both converge they give the same
+    iteration stays the arbiter: where both converge they give the same
     velocities (<= 1e-5).  The guess's accuracy and the iteration counts are
     reported; the gain is not asserted, because on this solver it is not
     consistently positive."""
