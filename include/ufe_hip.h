@@ -275,7 +275,12 @@ int ufe_fraction_gr_b(ufe_ctx *ctx, const double *Hi, const double *Hb, const do
  * melting point recomputed per layer (thermodynamics_utilities.f90:130-229).
  * Per-layer inputs are [nV*nz] vertex-fastest; per-vertex inputs [nV].
  *   mask: 0 = column below Hi_min_thermo (set to T_surf), 1 = grounded
- *   (basal flux BC), 2 = floating (T_base_float BC)
+ *   (basal flux BC), 2 = floating (T_base_float BC), 3 = grounded side of
+ *   the grounding line with choice_GL_temperature_BC = 'subgrid': each
+ *   sub-step solves the column with both basal conditions and keeps
+ *   fraction_gr * grounded + (1 - fraction_gr) * floating
+ *   (thermodynamics_3D_heat_equation.f90:171-203; 'grounded' and 'pmp'
+ *   are masks 1 and 2 for those vertices)
  * Columns still unstable after 10 halvings are reported in is_unstable and
  * counted in *n_unstable; the caller applies its fallback profile. */
 typedef struct ufe_heat_inputs {
@@ -289,9 +294,11 @@ typedef struct ufe_heat_inputs {
   const double *Hi_eff;                                 /* [nV]                                */
   const double *T_surf;                                 /* [nV] annual mean surface T          */
   const double *Q_base_grnd;                            /* [nV] frictional + geothermal flux   */
-  const int    *mask;                                   /* [nV] 0 thin / 1 grounded / 2 floating */
+  const int    *mask;                                   /* [nV] 0 thin / 1 grounded / 2 floating / 3 GL blend */
   double dt;
   double Hi_min_thermo;
+  const double *fraction_gr;                            /* [nV] vertex grounded fraction, read where
+                                                           mask = 3; may be NULL when no mask is 3 */
 } ufe_heat_inputs;
 
 int ufe_heat_columns(ufe_ctx *ctx, const ufe_heat_inputs *in, double *Ti_tplusdt, int *is_unstable,
@@ -315,7 +322,7 @@ typedef struct ufe_heat_robin_inputs {
   const double *T_surf;           /* [nV] annual mean surface temperature            */
   const double *SMB;              /* [nV] surface mass balance                       */
   const double *geothermal_flux;  /* [nV]                                            */
-  const int    *mask;             /* [nV] 0 thin / 1 grounded / 2 floating           */
+  const int    *mask;             /* [nV] 0 thin / 1 or 3 grounded / 2 floating      */
   double Hi_min_thermo;
 } ufe_heat_robin_inputs;
 int ufe_heat_robin(ufe_ctx *ctx, const ufe_heat_robin_inputs *in, const int *is_unstable, double *Ti,

@@ -956,7 +956,7 @@ int oracle_heat_robin(const ufe_heat_robin_inputs *in, const int *is_unstable, d
       double *t = &Ti[(size_t)k * nV + vi];
       const double tpmp = T0 - CLAUSIUS_CLAPEYRON * H * in->zeta[k];
       if (H > in->Hi_min_thermo) {
-        if (in->mask[vi] == 1) {
+        if (in->mask[vi] == 1 || in->mask[vi] == 3) {
           if (in->SMB[vi] > 0.0) {                                      /* :299-308 */
             const double L = sqrt(2.0 * kappa * H / in->SMB[vi]);
             const double d = (1.0 - in->zeta[k]) * H;
@@ -984,7 +984,8 @@ int oracle_heat_columns(const ufe_heat_inputs *in, double *Tnew, int *is_unstabl
   double dl[64], dd[64], du[64], d2l[64], d2d[64], d2u[64];
   oracle_zeta_tridiagonals(nz, in->zeta, dl, dd, du, d2l, d2d, d2u);
   for (int vi = 0; vi < nV; ++vi)
-    if (!(in->Hi_eff[vi] < in->Hi_min_thermo) && in->mask[vi] != 1 && in->mask[vi] != 2)
+    if (!(in->Hi_eff[vi] < in->Hi_min_thermo) && in->mask[vi] != 1 && in->mask[vi] != 2 &&
+        !(in->mask[vi] == 3 && in->fraction_gr))
       return UFE_ERR_ARGUMENT; /* reference crash: thick ice neither grounded nor floating */
   int nun = 0;
 #pragma omp parallel for reduction(+ : nun) schedule(static)
@@ -1010,7 +1011,8 @@ int oracle_heat_columns(const ufe_heat_inputs *in, double *Tnew, int *is_unstabl
       Ki[k] = 3.101e8 * exp(-0.0057 * T);
       Cpi[k] = 2115.3 + 7.79293 * (T - T0);
     }
-    int grounded = in->mask[vi] == 1;
+    int grounded = in->mask[vi] != 2;
+    int gl = in->mask[vi] == 3;
     double Tbf = Tpmp[nz - 1];
     /* The reference does not reset the column between halving attempts: each
      * attempt starts from the profile the previous attempt left
@@ -1025,6 +1027,18 @@ int oracle_heat_columns(const ufe_heat_inputs *in, double *Tnew, int *is_unstabl
         solve_column(nz, in->zeta, dl, dd, du, d2l, d2d, d2u, Tcur, col[0], col[1], col[2], col[3], col[4],
                      in->T_surf[vi], Tpmp, Ki, Cpi, col[5], col[6], col[7], col[8], col[9], dta,
                      grounded, in->Q_base_grnd[vi], Tbf, Tn);
+        if (gl) {
+          /* grounding line, choice_GL_temperature_BC = 'subgrid': the fully
+           * floating solution from the same profile, then
+           * fraction_gr * grounded + (1 - fraction_gr) * floating
+           * (thermodynamics_3D_heat_equation.f90:171-203) */
+          double Tf[64];
+          solve_column(nz, in->zeta, dl, dd, du, d2l, d2d, d2u, Tcur, col[0], col[1], col[2], col[3], col[4],
+                       in->T_surf[vi], Tpmp, Ki, Cpi, col[5], col[6], col[7], col[8], col[9], dta,
+                       0, in->Q_base_grnd[vi], Tbf, Tf);
+          const double f = in->fraction_gr[vi];
+          for (int k = 0; k < nz; ++k) Tn[k] = f * Tn[k] + (1.0 - f) * Tf[k];
+        }
         for (int k = 0; k < nz; ++k) Tcur[k] = Tn[k];
       }
       stable = 1;

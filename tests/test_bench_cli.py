@@ -1,6 +1,5 @@
 """bench.py's command-line contract that does not need a GPU."""
 
-import json
 import os
 import subprocess
 import sys

@@ -1,6 +1,6 @@
-"""The Fortran host interface (ufe_velocity_gpu, ISO_C_BINDING): the driver
-built from it solves the same case as the Python host through the same C ABI
-and must return the same velocities."""
+"""The Fortran host interface (ufe_velocity_gpu, ISO_C_BINDING): the drivers
+built from it solve the same cases as the Python host through the same C ABI
+and must return the same velocities and temperatures."""
 
 import os
 
@@ -13,6 +13,33 @@ from conftest import max_rel
 def test_fortran_driver_built(pkg):
     from ufemism2_amd import fortran_case as FC
     assert os.path.exists(FC.DRIVER), "build() compiles the Fortran driver with amdflang"
+    assert os.path.exists(FC.HEAT_DRIVER), "build() compiles the Fortran heat driver with amdflang"
+
+
+def heat_case(W, nV=2000, nz=12, n_hot=3):
+    """Heat columns, 20 % of the grounded ones on the grounding line, of which
+    n_hot get a heat sink (negative Phi) that no dt halving keeps above
+    180 K, so they end unstable and take the Robin solution (heating cannot
+    do it: every step clamps to the melting point)."""
+    c = W.heat_columns_inputs(nV, nz, seed=11, gl=0.2)
+    f = dict(c["fields"])
+    phi = f["Phi"].reshape(nz, nV).copy()
+    hot = np.flatnonzero(c["fields"]["Hi_eff"] > 1000.0)[:n_hot]
+    phi[1:, hot] = -1e14
+    f["Phi"] = phi.reshape(-1)
+    rng = np.random.default_rng(12)
+    SMB = rng.uniform(-0.5, 1.0, nV)
+    G = rng.uniform(1.2e6, 2.2e6, nV)
+    return c, f, SMB, G, hot
+
+
+def test_heat_case_file_layout(W, tmp_path):
+    from ufemism2_amd import fortran_case as FC
+    nV, nz = 300, 12
+    c, f, SMB, G, _ = heat_case(W, nV, nz)
+    path = str(tmp_path / "heat.bin")
+    FC.write_heat_case(path, nV, nz, c["zeta"], f, SMB, G, c["dt"])
+    assert os.path.getsize(path) == 4 * (3 + nV) + 8 * (2 + nz + 11 * nz * nV + 6 * nV)
 
 
 def test_case_file_layout(W, tmp_path):
@@ -54,3 +81,40 @@ def test_fortran_solve_matches_c_abi(V, W, tmp_path, stress):
     s.close()
     assert res <= 1e-8 and nvi == nv2
     assert max_rel(uf, u) < 1e-12 and max_rel(vf, v) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gl_bc", ["subgrid", "pmp"])
+def test_fortran_heat_step_matches_c_abi(V, W, OB, tmp_path, gl_bc):
+    """solve_3D_heat_equation_gpu (column solve with the grounding-line
+    choice, 1% rule, Robin replacement) equals the same two C-ABI calls made
+    from Python, bit for bit, and the oracle's column solve and Robin
+    replacement to 1e-9 K."""
+    from ufemism2_amd import fortran_case as FC
+    nV, nz = 2000, 12
+    c, f, SMB, G, hot = heat_case(W, nV, nz)
+    assert (f["mask"] == 3).sum() > 100
+    case, out = str(tmp_path / "heat.bin"), str(tmp_path / "heat_out.bin")
+    FC.write_heat_case(case, nV, nz, c["zeta"], f, SMB, G, c["dt"], gl_bc=gl_bc)
+    if gl_bc == "pmp":  # grounding-line columns take the floating solve; Robin still sees them grounded
+        f = dict(f)
+        f["mask"] = np.where(f["mask"] == 3, 2, f["mask"]).astype(np.int32)
+        robin_mask = np.where(c["fields"]["mask"] == 3, 1, f["mask"])
+    else:
+        robin_mask = f["mask"]
+    print(FC.run_driver(case, out, driver=FC.HEAT_DRIVER), flush=True)
+    Tf, nf = FC.read_heat_result(out, nV, nz)
+    hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], f, c["dt"])
+    T, unst, nun = V.heat_columns(hin)
+    assert nun == len(hot) and set(np.flatnonzero(unst)) == set(hot)
+    rin, keep_r = V.make_robin_inputs(nV, nz, c["zeta"], f["Hi_eff"], f["T_surf"], SMB, G, robin_mask)
+    T, nrep = V.heat_robin(rin, unst, T)
+    assert nf == nun and nrep == nun
+    assert np.array_equal(Tf, T)
+    Tc = T.reshape(nz, nV)
+    assert np.all(Tc[:, hot] < 273.16) and np.all(Tc[:, hot] > 180.0)
+    rc, To, uo, no = OB.heat_columns(hin)
+    assert rc == 0 and np.array_equal(uo, unst)
+    rc, To, no = OB.heat_robin(rin, uo, To)
+    assert rc == 0 and no == nrep
+    assert np.abs(T - To).max() < 1e-9

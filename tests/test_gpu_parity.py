@@ -363,6 +363,25 @@ def test_heat_columns_parity(V, OB, W, nV, nz):
     assert np.abs(T - To).max() < 1e-9
 
 
+@pytest.mark.parametrize("nz", [12, 30])
+def test_heat_grounding_line_blend_matches_oracle(V, W, OB, nz):
+    """Grounding-line columns blended with fraction_gr ('subgrid') equal the
+    oracle's two-solve blend; without fraction_gr the call is rejected."""
+    nV = 20000
+    c = W.heat_columns_inputs(nV, nz, seed=nz, gl=0.3)
+    f = dict(c["fields"])
+    assert (f["mask"] == 3).sum() > 1000
+    hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], f, c["dt"])
+    T, unst, nun = V.heat_columns(hin)
+    rc, To, uo, no = OB.heat_columns(hin)
+    assert rc == 0 and np.array_equal(unst, uo) and nun == no
+    assert np.abs(T - To).max() < 1e-9
+    f.pop("fraction_gr")
+    hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], f, c["dt"])
+    with pytest.raises(V.VelocitySolverError):
+        V.heat_columns(hin)
+
+
 def test_heat_columns_with_context_buffers(V, W, dome):
     """Through a context (its stream and device buffers kept between calls,
     grown when a call is larger) the columns equal the context-free call."""

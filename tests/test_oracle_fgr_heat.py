@@ -143,6 +143,41 @@ def test_heat_thin_ice_unstable_and_errors(L, OB, W):
     assert rc < 0
 
 
+def test_heat_grounding_line_blend_oracle(L, OB, W):
+    """Grounding-line columns with choice_GL_temperature_BC = 'subgrid'
+    (thermodynamics_3D_heat_equation.f90:171-203): fraction 1 is the grounded
+    solve and fraction 0 the floating solve, bit for bit; for one step the
+    blend equals fraction_gr * grounded + (1 - fraction_gr) * floating of two
+    separate solves from the same profile (a sequential grounded-then-floating
+    solve or swapped weights fails this); no fraction_gr is an error."""
+    nV, nz = 400, 12
+    c = W.heat_columns_inputs(nV, nz, seed=21, dt=1.0, gl=0.5)
+    f = dict(c["fields"]); f["zeta"] = c["zeta"]
+    gl = np.flatnonzero(f["mask"] == 3)
+    assert gl.size > 50
+
+    def run(mask, frac):
+        g = dict(f); g["mask"] = mask.astype(np.int32); g["fraction_gr"] = frac
+        hin, keep = heat_inputs(L, nV, nz, g, c["dt"])
+        rc, T, unst, nun = OB.heat_columns(hin)
+        assert rc == 0 and nun == 0
+        return T.reshape(nz, nV)
+
+    m1 = np.where(f["mask"] == 3, 1, f["mask"]); m2 = np.where(f["mask"] == 3, 2, f["mask"])
+    Tg, Tf = run(m1, f["fraction_gr"]), run(m2, f["fraction_gr"])
+    assert np.array_equal(run(f["mask"], np.ones(nV)), Tg)
+    assert np.array_equal(run(f["mask"], np.zeros(nV)), Tf)
+    Tb = run(f["mask"], f["fraction_gr"])
+    fr = f["fraction_gr"][gl]
+    assert np.abs(Tb[:, gl] - (fr * Tg[:, gl] + (1 - fr) * Tf[:, gl])).max() < 1e-10
+    assert np.abs(Tg[:, gl] - Tf[:, gl]).max() > 0.1  # the two basal conditions differ
+    other = np.setdiff1d(np.arange(nV), gl)
+    assert np.array_equal(Tb[:, other], Tg[:, other])
+    g = dict(f); g.pop("fraction_gr")
+    hin, keep = heat_inputs(L, nV, nz, g, c["dt"])
+    assert OB.heat_columns(hin)[0] < 0
+
+
 # -- Robin replacement of unstable columns -------------------------------------
 
 def robin_case(nz=401, nV=200):

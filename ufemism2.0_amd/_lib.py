@@ -86,7 +86,7 @@ class HeatInputs(C.Structure):
         ("u_3D", dp), ("v_3D", dp), ("w_3D", dp), ("u_dTdxp_upwind", dp), ("v_dTdyp_upwind", dp),
         ("dzeta_dx", dp), ("dzeta_dy", dp), ("dzeta_dz", dp), ("dzeta_dt", dp), ("Phi", dp),
         ("Hi_eff", dp), ("T_surf", dp), ("Q_base_grnd", dp), ("mask", ip),
-        ("dt", C.c_double), ("Hi_min_thermo", C.c_double),
+        ("dt", C.c_double), ("Hi_min_thermo", C.c_double), ("fraction_gr", dp),
     ]
 
 

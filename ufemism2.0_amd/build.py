@@ -78,8 +78,16 @@ def build_fortran(verbose=True):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    heat = os.path.join(FORTRAN, "ufe_fortran_heat_driver")
+    cmd = [flang, "-O2", "-module-dir", out, "-o", heat,
+           os.path.join(FORTRAN, "ufe_velocity_gpu.f90"),
+           os.path.join(FORTRAN, "ufe_fortran_heat_driver.f90"),
+           f"-L{PKG}", "-lufe_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {exe}")
+        print(f"built {exe}, {heat}")
     return exe
 
 

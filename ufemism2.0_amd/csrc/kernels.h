@@ -109,6 +109,7 @@ struct HeatDev {
   int nV, nz;
   const double *zeta, *Ti, *u, *v, *w, *uadv, *vadv, *zx, *zy, *zz, *zt, *Phi, *Hi_eff, *T_surf, *Q;
   const int *mask;
+  const double *fgr;  // [nV] vertex grounded fraction, read where mask = 3 (may be null otherwise)
   double dt, Hi_min;
   const double *tri;  // [6][nz] d/dzeta and d2/dzeta2 tridiagonals (ldiag, diag, udiag)
 };

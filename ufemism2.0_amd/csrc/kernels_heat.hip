@@ -87,13 +87,18 @@ constexpr int kHeatMaxNz = 32;
 // T(nz-2) (explicit), floating T = T_pmp; Thomas (tridiagonal_solver.f90:13-51);
 // clamp to T_pmp.  al, bl, cl are the row's sub-, main- and super-diagonal
 // without the 1/dt term, r0 the explicit right-hand-side terms.
+// mode 1 grounded, 2 floating, 3 both, kept as fgr * grounded + (1 - fgr) *
+// floating (grounding line, :171-203).  The two systems differ only in the
+// basal row, which has no sub-diagonal, so they share the forward sweep; the
+// back substitution carries both recursions and blends the clamped profiles.
 template <int NZ>
 __device__ __forceinline__ void heat_step(int nz, double (&T)[NZ], const double (&al)[NZ],
                                           const double (&bl)[NZ], const double (&cl)[NZ],
                                           const double (&r0)[NZ], double T_surf, double Hpmp,
                                           const double *__restrict__ zeta, double Kb, double zzb, double dzb,
-                                          int grounded, double Q, double dt) {
+                                          int mode, double fgr, double Q, double dt) {
   double bdiag[NZ], d[NZ];
+  double dfl = 0.0;  // basal entry of the floating system (mode 3)
   bdiag[0] = 1.0;
   d[0] = fmin(T_surf, kT0);
   double cprev = 0.0;  // super-diagonal of the previous row (0 for the surface row)
@@ -110,22 +115,38 @@ __device__ __forceinline__ void heat_step(int nz, double (&T)[NZ], const double 
         const double tpmp = kT0 - kClausiusClapeyron * Hpmp * zeta[k];
         a = 0.0;
         b = 1.0;
-        dk = grounded ? fmin(tpmp, T[k - 1] - dzb * Q / (zzb * Kb)) : tpmp;
+        dk = mode != 2 ? fmin(tpmp, T[k - 1] - dzb * Q / (zzb * Kb)) : tpmp;
+        dfl = tpmp;
       }
       const double w = a / bdiag[k - 1];
       bdiag[k] = b - w * cprev;
       d[k] = dk - w * d[k - 1];
+      if (k == nz - 1) dfl = dfl - w * d[k - 1];
       cprev = k < nz - 1 ? cl[k] : 0.0;
     }
   }
+  if (mode != 3) {
 #pragma unroll
-  for (int k = NZ - 1; k >= 0; --k) {
-    if (k == nz - 1) T[k] = d[k] / bdiag[k];
-    else if (k < nz - 1) T[k] = (d[k] - (k == 0 ? 0.0 : cl[k]) * T[k + 1]) / bdiag[k];
+    for (int k = NZ - 1; k >= 0; --k) {
+      if (k == nz - 1) T[k] = d[k] / bdiag[k];
+      else if (k < nz - 1) T[k] = (d[k] - (k == 0 ? 0.0 : cl[k]) * T[k + 1]) / bdiag[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+      if (k < nz) T[k] = fmin(T[k], kT0 - kClausiusClapeyron * Hpmp * zeta[k]);
+  } else {
+    double tg = 0.0, tf = 0.0;  // unclamped solutions at layer k + 1
+#pragma unroll
+    for (int k = NZ - 1; k >= 0; --k) {
+      if (k < nz) {
+        const double c = (k == 0 || k == nz - 1) ? 0.0 : cl[k];
+        tg = (d[k] - c * tg) / bdiag[k];
+        tf = ((k == nz - 1 ? dfl : d[k]) - c * tf) / bdiag[k];
+        const double tpmp = kT0 - kClausiusClapeyron * Hpmp * zeta[k];
+        T[k] = fgr * fmin(tg, tpmp) + (1.0 - fgr) * fmin(tf, tpmp);
+      }
+    }
   }
-#pragma unroll
-  for (int k = 0; k < NZ; ++k)
-    if (k < nz) T[k] = fmin(T[k], kT0 - kClausiusClapeyron * Hpmp * zeta[k]);
 }
 
 template <int NZ>
@@ -174,7 +195,8 @@ __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__re
       }
     }
     const double dzb = zeta[nz - 1] - zeta[nz - 2];
-    const int grounded = h.mask[vi] == 1;
+    const int mode = h.mask[vi];
+    const double fgr = mode == 3 ? h.fgr[vi] : 1.0;
     const double Q = h.Q[vi], ts = h.T_surf[vi];
     // dt halving without resetting the column between attempts
     // (thermodynamics_3D_heat_equation.f90:161-239)
@@ -188,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__re
       const int nsub = 1 << (it - 1);
 #pragma unroll 1
       for (int sstep = 0; sstep < nsub; ++sstep)
-        heat_step<NZ>(nz, Tc, al, bl, cl, r0, ts, H, zeta, Kb, zzb, dzb, grounded, Q, dta);
+        heat_step<NZ>(nz, Tc, al, bl, cl, r0, ts, H, zeta, Kb, zzb, dzb, mode, fgr, Q, dta);
       stable = true;
 #pragma unroll
       for (int k = 0; k < NZ; ++k)
@@ -243,7 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_heat_robin(int nV, int nz, const dou
   for (int k = 0; k < nz; ++k) {
     double T = Ti[(size_t)k * nV + vi];
     if (H > Hi_min) {
-      if (mask[vi] == 1) {
+      if (mask[vi] == 1 || mask[vi] == 3) {
         if (SMB[vi] > 0.0) {
           const double Lt = sqrt(2.0 * kappa * H / SMB[vi]);
           const double d = (1.0 - zeta[k]) * H;
@@ -402,10 +424,17 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
     if (!f) return heat_fail(UFE_ERR_ARGUMENT, "null input field");
   if (!in->mask) return heat_fail(UFE_ERR_ARGUMENT, "null mask");
   int nV = in->nV, nz = in->nz;
-  for (int vi = 0; vi < nV; ++vi)
-    if (!(in->Hi_eff[vi] < in->Hi_min_thermo) && in->mask[vi] != 1 && in->mask[vi] != 2)
+  bool any_gl = false;
+  for (int vi = 0; vi < nV; ++vi) {
+    if (in->Hi_eff[vi] < in->Hi_min_thermo) continue;
+    const int m = in->mask[vi];
+    if (m != 1 && m != 2 && m != 3)
       return heat_fail(UFE_ERR_ARGUMENT,
                        "Hi_eff > Hi_min_thermo, but mask_grounded_ice and mask_floating_ice are both false");
+    any_gl |= m == 3;
+  }
+  if (any_gl && !in->fraction_gr)
+    return heat_fail(UFE_ERR_ARGUMENT, "grounding-line columns (mask 3) need fraction_gr");
   try {
     int dev = c ? ctx_device(c) : 0;
     UFE_HIP(hipSetDevice(dev));
@@ -449,6 +478,7 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
     h.T_surf = (const double *)up(in->T_surf, n1);
     h.Q = (const double *)up(in->Q_base_grnd, n1);
     h.mask = (const int *)up(in->mask, sizeof(int) * nV);
+    h.fgr = any_gl ? (const double *)up(in->fraction_gr, n1) : nullptr;
     h.dt = in->dt;
     h.Hi_min = in->Hi_min_thermo;
     double *Tout = (double *)buf(n3);

@@ -4,6 +4,8 @@
 ! that keep the argument meaning of the reference's ice-velocity module:
 !   solve_SSA_gpu  <-> solve_SSA  (SSA_main.f90:87-242)
 !   solve_DIVA_gpu <-> solve_DIVA (DIVA_main.f90:88-262)
+!   solve_3D_heat_equation_gpu <-> the column solve, the 1% rule and the Robin
+!     replacement of solve_3D_heat_equation (thermodynamics_3D_heat_equation.f90:33-307)
 ! Arrays are the caller's 1-based Fortran arrays; (n, nz) fields are passed
 ! as they lie in memory (first index fastest).  Mesh connectivity and operator
 ! indices are converted to 0-based once, in ufe_gpu_init.  A failing call
@@ -18,7 +20,7 @@ module ufe_velocity_gpu
   private
   public :: ufe_gpu_context, ufe_gpu_params, ufe_gpu_info, ufe_gpu_init, ufe_gpu_finalise, &
             ufe_gpu_set_params, ufe_gpu_default_params, solve_SSA_gpu, solve_DIVA_gpu, &
-            ufe_gpu_fraction_gr_b
+            ufe_gpu_fraction_gr_b, solve_3D_heat_equation_gpu
 
   integer(c_int), parameter, public :: UFE_OK = 0, UFE_WARN_NOT_CONVERGED = 1
 
@@ -61,6 +63,22 @@ module ufe_velocity_gpu
     real(c_double) :: L2_uv, rel_residual, visc_it_relax_applied, epsilon_sq_0_applied, seconds
     integer(c_int) :: n_clamped, n_fallbacks
   end type ufe_gpu_info
+
+  ! struct ufe_heat_inputs
+  type, bind(C) :: ufe_heat_inputs
+    integer(c_int) :: nV, nz
+    type(c_ptr)    :: zeta, Ti, u_3D, v_3D, w_3D, u_dTdxp_upwind, v_dTdyp_upwind, dzeta_dx, dzeta_dy, &
+                      dzeta_dz, dzeta_dt, Phi, Hi_eff, T_surf, Q_base_grnd, mask
+    real(c_double) :: dt, Hi_min_thermo
+    type(c_ptr)    :: fraction_gr
+  end type ufe_heat_inputs
+
+  ! struct ufe_heat_robin_inputs
+  type, bind(C) :: ufe_heat_robin_inputs
+    integer(c_int) :: nV, nz
+    type(c_ptr)    :: zeta, Hi_eff, T_surf, SMB, geothermal_flux, mask
+    real(c_double) :: Hi_min_thermo
+  end type ufe_heat_robin_inputs
 
   ! A device context plus the 0-based copies of the mesh arrays it was built from.
   type :: ufe_gpu_context
@@ -109,6 +127,20 @@ module ufe_velocity_gpu
       type(c_ptr), value :: ctx, Hi, Hs, A_flow, beta_sq, fgr, mask, bcm, bcu, bcv, u, v, ubase, vbase, &
                             u3, v3, eta3, tbx, tby
       type(ufe_gpu_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+    function c_ufe_heat_columns(ctx, hin, Tnew, unstable, n_unstable) bind(C, name='ufe_heat_columns') result(rc)
+      import :: c_ptr, c_int, ufe_heat_inputs
+      type(c_ptr), value :: ctx, Tnew, unstable
+      type(ufe_heat_inputs), intent(in) :: hin
+      integer(c_int), intent(out) :: n_unstable
+      integer(c_int) :: rc
+    end function
+    function c_ufe_heat_robin(ctx, rin, unstable, Ti, n_replaced) bind(C, name='ufe_heat_robin') result(rc)
+      import :: c_ptr, c_int, ufe_heat_robin_inputs
+      type(c_ptr), value :: ctx, unstable, Ti
+      type(ufe_heat_robin_inputs), intent(in) :: rin
+      integer(c_int), intent(out) :: n_replaced
       integer(c_int) :: rc
     end function
     function c_ufe_fraction_gr_b(ctx, Hi, Hb, SL, fgr) bind(C, name='ufe_fraction_gr_b') result(rc)
@@ -329,6 +361,81 @@ contains
     call crash_on(c_ufe_fraction_gr_b(g%ctx, c_loc(Hi), c_loc(Hb), c_loc(SL), c_loc(fraction_gr_b)), g%ctx, &
                   'ufe_gpu_fraction_gr_b')
   end subroutine ufe_gpu_fraction_gr_b
+
+  ! solve_3D_heat_equation (thermodynamics_3D_heat_equation.f90:33-307) from the
+  ! column inputs onward: every column (nV, nz), vertex index fastest, is
+  ! solved implicitly with dt halving; when fewer than 1% of the columns stay
+  ! unstable they are replaced with the Robin solution
+  ! (replace_Ti_with_robin_solution, thermodynamics_utilities.f90:256-337),
+  ! otherwise the program stops as the reference's does.  Thin ice
+  ! (Hi_eff < Hi_min_thermo) takes the surface temperature.  Grounding-line
+  ! vertices (mask_gl_gr) follow choice_GL_temperature_BC ('grounded' by
+  ! default, 'pmp', or 'subgrid', which needs the vertex fraction_gr).
+  subroutine solve_3D_heat_equation_gpu(g, zeta, Ti, u_3D, v_3D, w_3D, u_dTdxp_upwind, v_dTdyp_upwind, &
+      dzeta_dx, dzeta_dy, dzeta_dz, dzeta_dt, Phi, Hi_eff, T_surf, Q_base_grnd, SMB, geothermal_heat_flux, &
+      mask_grounded_ice, mask_floating_ice, dt, Hi_min_thermo, Ti_tplusdt, n_unstable, &
+      mask_gl_gr, fraction_gr, choice_GL_temperature_BC)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: zeta(:), Ti(:, :), u_3D(:, :), v_3D(:, :), w_3D(:, :)
+    real(c_double), intent(in), target :: u_dTdxp_upwind(:, :), v_dTdyp_upwind(:, :)
+    real(c_double), intent(in), target :: dzeta_dx(:, :), dzeta_dy(:, :), dzeta_dz(:, :), dzeta_dt(:, :), Phi(:, :)
+    real(c_double), intent(in), target :: Hi_eff(:), T_surf(:), Q_base_grnd(:), SMB(:), geothermal_heat_flux(:)
+    logical, intent(in) :: mask_grounded_ice(:), mask_floating_ice(:)
+    real(c_double), intent(in) :: dt, Hi_min_thermo
+    real(c_double), intent(out), target :: Ti_tplusdt(:, :)
+    integer, intent(out) :: n_unstable
+    logical, intent(in), optional :: mask_gl_gr(:)
+    real(c_double), intent(in), optional, target :: fraction_gr(:)
+    character(len=*), intent(in), optional :: choice_GL_temperature_BC
+    type(ufe_heat_inputs) :: hin
+    type(ufe_heat_robin_inputs) :: rin
+    integer(c_int), allocatable, target :: mask(:), mask_col(:), unstable(:)
+    type(c_ptr) :: pfgr
+    integer(c_int) :: nun, nrep
+    integer :: nV, nz
+    character(len=16) :: gl_bc
+    nV = size(Ti, 1)
+    nz = size(Ti, 2)
+    allocate(mask(nV), mask_col(nV), unstable(nV))
+    ! grounded / floating as replace_Ti_with_robin_solution reads them
+    mask = 0
+    where (mask_grounded_ice) mask = 1
+    where (mask_floating_ice .and. .not. mask_grounded_ice) mask = 2
+    ! the column solve's basal condition (thermodynamics_3D_heat_equation.f90:171-224)
+    mask_col = mask
+    pfgr = c_null_ptr
+    gl_bc = 'grounded'
+    if (present(choice_GL_temperature_BC)) gl_bc = choice_GL_temperature_BC
+    if (present(mask_gl_gr)) then
+      select case (trim(gl_bc))
+      case ('grounded')
+        where (mask_gl_gr) mask_col = 1
+      case ('pmp')
+        where (mask_gl_gr) mask_col = 2
+      case ('subgrid')
+        if (.not. present(fraction_gr)) then
+          write(0, '(a)') 'ufe_velocity_gpu: choice_GL_temperature_BC = "subgrid" needs fraction_gr'
+          error stop 1
+        end if
+        where (mask_gl_gr) mask_col = 3
+        pfgr = c_loc(fraction_gr)
+      case default
+        write(0, '(3a)') 'ufe_velocity_gpu: unknown choice_GL_temperature_BC "', trim(gl_bc), '"!'
+        error stop 1
+      end select
+    end if
+    hin = ufe_heat_inputs(int(nV, c_int), int(nz, c_int), c_loc(zeta), c_loc(Ti), c_loc(u_3D), c_loc(v_3D), &
+                          c_loc(w_3D), c_loc(u_dTdxp_upwind), c_loc(v_dTdyp_upwind), c_loc(dzeta_dx), &
+                          c_loc(dzeta_dy), c_loc(dzeta_dz), c_loc(dzeta_dt), c_loc(Phi), c_loc(Hi_eff), &
+                          c_loc(T_surf), c_loc(Q_base_grnd), c_loc(mask_col), dt, Hi_min_thermo, pfgr)
+    call crash_on(c_ufe_heat_columns(g%ctx, hin, c_loc(Ti_tplusdt), c_loc(unstable), nun), c_null_ptr, &
+                  'solve_3D_heat_equation_gpu')
+    rin = ufe_heat_robin_inputs(int(nV, c_int), int(nz, c_int), c_loc(zeta), c_loc(Hi_eff), c_loc(T_surf), &
+                                c_loc(SMB), c_loc(geothermal_heat_flux), c_loc(mask), Hi_min_thermo)
+    call crash_on(c_ufe_heat_robin(g%ctx, rin, c_loc(unstable), c_loc(Ti_tplusdt), nrep), c_null_ptr, &
+                  'solve_3D_heat_equation_gpu (unstable columns)')
+    n_unstable = int(nun)
+  end subroutine solve_3D_heat_equation_gpu
 
   subroutine bc_pointers(m, u, v, bcm, pbm, pbu, pbv)
     integer, intent(in), optional :: m(:)

@@ -1,7 +1,9 @@
-"""Case files for the Fortran driver (ufe_fortran_driver): the mesh in the
-reference's 1-based layout (V(nV,2), Tri/TriC(nT,3) with TriC = 0 for none,
-TriGC(nT,2)), the operators as 1-based CSR, a few solver controls and the
-inputs, written as one raw little-endian stream in Fortran array order."""
+"""Case files for the Fortran drivers, each one raw little-endian stream in
+Fortran array order.  ufe_fortran_driver: the mesh in the reference's 1-based
+layout (V(nV,2), Tri/TriC(nT,3) with TriC = 0 for none, TriGC(nT,2)), the
+operators as 1-based CSR, a few solver controls and the inputs.
+ufe_fortran_heat_driver: the column fields (nV,nz), vertex index fastest,
+and the per-vertex fields of one heat-equation step."""
 
 from __future__ import annotations
 
@@ -11,6 +13,9 @@ import subprocess
 import numpy as np
 
 DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "ufe_fortran_driver")
+HEAT_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "ufe_fortran_heat_driver")
+HEAT_COLUMN_FIELDS = ("Ti", "u_3D", "v_3D", "w_3D", "u_dTdxp_upwind", "v_dTdyp_upwind", "dzeta_dx", "dzeta_dy",
+                      "dzeta_dz", "dzeta_dt", "Phi")
 
 
 def write_case(path, w, stress="SSA", **ctl):
@@ -35,10 +40,39 @@ def write_case(path, w, stress="SSA", **ctl):
             f.write(np.ascontiguousarray(p).tobytes())
 
 
-def run_driver(case, out, timeout=300):
-    if not os.path.exists(DRIVER):
-        raise RuntimeError(f"{DRIVER} is missing: build it first")
-    r = subprocess.run([DRIVER, case, out], capture_output=True, text=True, timeout=timeout)
+GL_BC = {"grounded": 0, "subgrid": 1, "pmp": 2}
+
+
+def write_heat_case(path, nV, nz, zeta, fields, SMB, geothermal_flux, dt, Hi_min_thermo=10.0,
+                    gl_bc="grounded"):
+    """fields: the column arrays of HEAT_COLUMN_FIELDS (nz * nV, vertex
+    fastest), Hi_eff, T_surf, Q_base_grnd, mask (1 grounded, 2 floating,
+    3 grounded at the grounding line) and optionally fraction_gr; gl_bc is
+    choice_GL_temperature_BC for the mask-3 vertices."""
+    f64 = lambda a: np.asarray(a, dtype="<f8").reshape(-1)  # noqa: E731
+    parts = [np.asarray([nV, nz, GL_BC[gl_bc]], "<i4"), f64([dt, Hi_min_thermo]), f64(zeta)]
+    parts += [f64(fields[k]) for k in HEAT_COLUMN_FIELDS]
+    parts += [f64(fields["Hi_eff"]), f64(fields["T_surf"]), f64(fields["Q_base_grnd"]), f64(SMB),
+              f64(geothermal_flux), np.asarray(fields["mask"], "<i4"),
+              f64(fields.get("fraction_gr", np.ones(nV)))]
+    for p in parts:
+        assert p.size in (2, 3, nz, nV, nz * nV), p.size
+    with open(path, "wb") as f:
+        for p in parts:
+            f.write(np.ascontiguousarray(p).tobytes())
+
+
+def read_heat_result(path, nV, nz):
+    raw = open(path, "rb").read()
+    T = np.frombuffer(raw, "<f8", nV * nz, 0)
+    n_unstable = int(np.frombuffer(raw, "<i4", 1, 8 * nV * nz)[0])
+    return T, n_unstable
+
+
+def run_driver(case, out, timeout=300, driver=DRIVER):
+    if not os.path.exists(driver):
+        raise RuntimeError(f"{driver} is missing: build it first")
+    r = subprocess.run([driver, case, out], capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran driver failed ({r.returncode}): {r.stdout}\n{r.stderr}")
     return r.stdout

@@ -277,7 +277,9 @@ class VelocitySolver:
 
 
 def make_heat_inputs(nV, nz, zeta, fields: dict, dt: float, Hi_min_thermo: float = 10.0):
-    """HeatInputs with its arrays kept alive in the returned tuple."""
+    """HeatInputs with its arrays kept alive in the returned tuple.  fields
+    may hold fraction_gr (vertex grounded fraction) for mask-3 columns, the
+    grounding-line blend of choice_GL_temperature_BC = 'subgrid'."""
     keep = {k: (L.i32(v) if k == "mask" else L.f64(v)) for k, v in fields.items()}
     keep["zeta"] = L.f64(zeta)
     h = L.HeatInputs()
@@ -286,6 +288,8 @@ def make_heat_inputs(nV, nz, zeta, fields: dict, dt: float, Hi_min_thermo: float
                  "dzeta_dz", "dzeta_dt", "Phi", "Hi_eff", "T_surf", "Q_base_grnd"):
         setattr(h, name, L.ptr(keep[name]))
     h.mask = L.ptr(keep["mask"], L.ip)
+    if "fraction_gr" in keep:
+        h.fraction_gr = L.ptr(keep["fraction_gr"])
     h.dt = dt
     h.Hi_min_thermo = Hi_min_thermo
     return h, keep

@@ -291,12 +291,14 @@ def schoof_icestream(h: float = 16e3, seed: int = 5, nz: int = 10, A: float = 1e
                     params)
 
 
-def heat_columns_inputs(nV: int, nz: int = 12, seed: int = 6, dt: float = 10.0) -> dict:
+def heat_columns_inputs(nV: int, nz: int = 12, seed: int = 6, dt: float = 10.0, gl: float = 0.0) -> dict:
     """Synthetic inputs of the vertical heat columns for nV vertices, in the
     reference's [k][vi] layer-major layout (vertex fastest): a cold surface,
     a near-steady geothermal profile, horizontal velocities decaying with
     depth, upwind advection terms, strain heating, 10 % floating columns and
     5 % thin-ice columns (thermodynamics_3D_heat_equation.f90:47-160 inputs).
+    With gl > 0 that share of the grounded columns is put on the grounding
+    line (mask 3) with a uniform random fraction_gr.
     Units: K, m/yr, J/m^2/yr (Q), J/m^3/yr (Phi)."""
     rng = np.random.default_rng(seed)
     zeta = zeta_regular(nz)
@@ -331,4 +333,9 @@ def heat_columns_inputs(nV: int, nz: int = 12, seed: int = 6, dt: float = 10.0) 
     out = {k: np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1)) for k, v in f.items()
            if k not in ("mask", "zeta")}
     out["mask"] = f["mask"]
+    if gl > 0.0:
+        rg = np.random.default_rng(seed + 7919)
+        on_gl = (out["mask"] == 1) & (rg.random(nV) < gl)
+        out["mask"] = np.where(on_gl, 3, out["mask"]).astype(np.int32)
+        out["fraction_gr"] = rg.uniform(0.0, 1.0, nV)
     return {"nV": nV, "nz": nz, "zeta": zeta, "fields": out, "dt": dt}
