@@ -317,9 +317,12 @@ def main():
 
 def time_heat(V, W, nV, nz, steps, solver):
     """Heat columns of the continental configuration: nV columns x nz layers,
-    one call = one implicit time step of every column (host arrays in, out),
-    through the solver's context (its stream and device buffers reused from
-    call to call) and without a context (buffers of each call)."""
+    one call = one implicit time step of every column.  Inputs resident in
+    device memory (ufe_heat_columns_device through the solver's context), then
+    host arrays in and out (ufe_heat_columns, through the context, whose
+    stream and device buffers are reused, and without one)."""
+    import numpy as np
+    from ufemism2_amd.devmem import DeviceArray
     c = W.heat_columns_inputs(nV, nz, seed=3)
     hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], c["fields"], c["dt"])
     res = {}
@@ -329,10 +332,27 @@ def time_heat(V, W, nV, nz, steps, solver):
         for _ in range(steps):
             T, unst, nun = V.heat_columns(hin, sv)
         res[label] = (time.perf_counter() - t0) / steps
+    dev = {k: DeviceArray.from_host(v) for k, v in c["fields"].items()}
+    dev["zeta"] = DeviceArray.from_host(c["zeta"])
+    hd = V.make_heat_inputs_device(nV, nz, {k: d.ptr for k, d in dev.items()}, c["dt"])
+    Tout = DeviceArray(nV * nz, np.float64)
+    un = DeviceArray(nV, np.int32)
+    V.heat_columns_device(hd, Tout.ptr, un.ptr, solver)
+    reps = max(steps, 50)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nd = V.heat_columns_device(hd, Tout.ptr, un.ptr, solver)
+    td = (time.perf_counter() - t0) / reps
+    same = bool(np.array_equal(Tout.to_host(), T) and nd == nun)
+    for d in list(dev.values()) + [Tout, un]:
+        d.free()
     t = res["with_context"]
-    return {"columns": nV, "nz": nz, "seconds_per_call": t, "columns_per_s": nV / t, "n_unstable": nun,
-            "seconds_per_call_without_context": res["without_context"],
-            "note": "includes host<->device transfer of the inputs (the C ABI takes host arrays)"}
+    return {"columns": nV, "nz": nz, "columns_per_s": nV / td, "seconds_per_call": td, "n_unstable": nd,
+            "device_result_equals_host_call": same,
+            "host_arrays": {"columns_per_s": nV / t, "seconds_per_call": t,
+                            "seconds_per_call_without_context": res["without_context"],
+                            "note": "includes the host<->device copies of the inputs and outputs"},
+            "note": "inputs resident in device memory; one call = launch, reduction and count read-back"}
 
 
 if __name__ == "__main__":

@@ -304,6 +304,18 @@ typedef struct ufe_heat_inputs {
 int ufe_heat_columns(ufe_ctx *ctx, const ufe_heat_inputs *in, double *Ti_tplusdt, int *is_unstable,
                      int *n_unstable);
 
+/* The same step with every array of *in, Ti_tplusdt and is_unstable in
+ * device memory (the same replaced interface, solve_3D_heat_equation,
+ * thermodynamics_3D_heat_equation.f90:33-307, for a caller whose column
+ * fields already live on the GPU, e.g. the 3-D velocities of a DIVA solve):
+ * nothing crosses PCIe but nz + 2 doubles.  The call waits for the device's
+ * queued work before it starts and returns when the step is done.  Masks
+ * are checked on the device: a thick column that is neither grounded nor
+ * floating (or mask 3 without fraction_gr) gives UFE_ERR_ARGUMENT after the
+ * launch, with Ti_tplusdt undefined for that column. */
+int ufe_heat_columns_device(ufe_ctx *ctx, const ufe_heat_inputs *in, double *Ti_tplusdt, int *is_unstable,
+                            int *n_unstable);
+
 /* Unstable columns after the heat solve (thermodynamics_3D_heat_equation.f90:254-279):
  * when fewer than ceil(nV / 100) columns are flagged, each flagged column of
  * Ti [nV*nz, vertex fastest] is replaced in place with the Robin solution

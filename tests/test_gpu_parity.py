@@ -396,6 +396,36 @@ def test_heat_columns_with_context_buffers(V, W, dome):
     s.close()
 
 
+def test_heat_columns_device_equals_host(V, W, dome):
+    """ufe_heat_columns_device on device-resident inputs (grounding-line
+    columns included) gives the host-array call's result bit for bit, with
+    and without a context; invalid masks found on the device are an error."""
+    from ufemism2_amd.devmem import DeviceArray
+    nV, nz = 50000, 12
+    c = W.heat_columns_inputs(nV, nz, seed=5, gl=0.2)
+    f = dict(c["fields"])
+    hin, keep = V.make_heat_inputs(nV, nz, c["zeta"], f, c["dt"])
+    T_h, u_h, n_h = V.heat_columns(hin)
+    dev = {k: DeviceArray.from_host(v) for k, v in f.items()}
+    dev["zeta"] = DeviceArray.from_host(c["zeta"])
+    assert dev["mask"].dtype == np.int32 and dev["Ti"].dtype == np.float64
+    ptrs = {k: d.ptr for k, d in dev.items()}
+    Tout = DeviceArray(nV * nz, np.float64)
+    un = DeviceArray(nV, np.int32)
+    s = V.VelocitySolver(dome.descriptor())
+    for sv in (None, s, s):
+        Tout.upload(np.full(nV * nz, -1.0))
+        n = V.heat_columns_device(V.make_heat_inputs_device(nV, nz, ptrs, c["dt"]), Tout.ptr, un.ptr, sv)
+        assert n == n_h
+        assert np.array_equal(Tout.to_host(), T_h) and np.array_equal(un.to_host(), u_h)
+    bad = f["mask"].copy()
+    bad[np.flatnonzero(f["Hi_eff"] >= 10.0)[:3]] = 0
+    dev["mask"].upload(bad)
+    with pytest.raises(V.VelocitySolverError, match="3 columns"):
+        V.heat_columns_device(V.make_heat_inputs_device(nV, nz, ptrs, c["dt"]), Tout.ptr, un.ptr, s)
+    s.close()
+
+
 def test_heat_robin_matches_oracle(V, OB):
     """Robin replacement of flagged columns on the device equals the oracle's
     (thermodynamics_utilities.f90:256-337), and the 1% rule stops as the

@@ -1,5 +1,6 @@
 // Grounded fractions on the triangles and the vertical heat columns, with
 // their C ABI entry points.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -77,59 +78,91 @@ constexpr int kHeatMaxNz = 32;
 // The kernel is instantiated for NZ >= nz layers; every array index is static
 // once the NZ-loops are unrolled (the layer count enters only through the
 // k < nz and k == nz - 1 tests), so for NZ <= 16 a column stays in registers.
-// The solve updates T in place: the forward sweep reads all of T before the
-// back substitution writes it.
+// Only the column state, the Thomas pivots and the super-diagonal are held:
+// each implicit step streams the layer's inputs from memory and forms that
+// row's coefficients inside the forward sweep, which keeps the register
+// footprint low enough for several wavefronts per SIMD (the inputs of a
+// block's columns stay in L2 for the rare extra steps of dt halving).
 
-// One implicit step of one column (solve_1D_heat_equation,
+// One implicit step of column vi (solve_1D_heat_equation,
 // thermodynamics_3D_heat_equation.f90:309-479): rows 1..nz-2 from the
 // advection/diffusion coefficients, surface row T = min(T_surf, T0), basal row
 // grounded T = min(T_pmp, T(nz-2) - dzeta Q / (dzeta/dz K)) with the old
 // T(nz-2) (explicit), floating T = T_pmp; Thomas (tridiagonal_solver.f90:13-51);
-// clamp to T_pmp.  al, bl, cl are the row's sub-, main- and super-diagonal
-// without the 1/dt term, r0 the explicit right-hand-side terms.
+// clamp to T_pmp.  Conductivity and heat capacity come from the input
+// temperature Ti of the call (the reference forms them once before the
+// column loop, :90-112); T is the column at the start of the step (on the
+// first step, first = true, it is read from Ti as the sweep passes).  The
+// forward sweep stores its right-hand side in T; the back substitution
+// overwrites it with the new column.
 // mode 1 grounded, 2 floating, 3 both, kept as fgr * grounded + (1 - fgr) *
 // floating (grounding line, :171-203).  The two systems differ only in the
 // basal row, which has no sub-diagonal, so they share the forward sweep; the
 // back substitution carries both recursions and blends the clamped profiles.
 template <int NZ>
-__device__ __forceinline__ void heat_step(int nz, double (&T)[NZ], const double (&al)[NZ],
-                                          const double (&bl)[NZ], const double (&cl)[NZ],
-                                          const double (&r0)[NZ], double T_surf, double Hpmp,
-                                          const double *__restrict__ zeta, double Kb, double zzb, double dzb,
-                                          int mode, double fgr, double Q, double dt) {
-  double bdiag[NZ], d[NZ];
-  double dfl = 0.0;  // basal entry of the floating system (mode 3)
+__device__ __forceinline__ void heat_step(const HeatDev &h, const double *__restrict__ tri,
+                                          const double *__restrict__ zeta, int vi, bool first, double (&T)[NZ],
+                                          double T_surf, double Hpmp, double dzb, int mode, double fgr, double Q,
+                                          double dt) {
+  const int nV = h.nV, nz = h.nz;
+  // The layer loads are the same on every sub-step; an empty asm on the
+  // column index makes them opaque to loop-invariant code motion, which
+  // would otherwise lift all of them (nz * 11 values) out of the dt-halving
+  // loops and spill.  It emits no instruction.
+  size_t col = (size_t)vi;
+  asm volatile("" : "+v"(col));
+  double bdiag[NZ], cl[NZ];
   bdiag[0] = 1.0;
-  d[0] = fmin(T_surf, kT0);
-  double cprev = 0.0;  // super-diagonal of the previous row (0 for the surface row)
+  cl[0] = 0.0;
+  if (first) T[0] = h.Ti[col];
+  double told = T[0];        // the step's starting value of the previous layer
+  T[0] = fmin(T_surf, kT0);  // right-hand side of the surface row
+  double cprev = 0.0;        // super-diagonal of the previous row (0 for the surface row)
+  double dfl = 0.0;          // basal entry of the floating system (mode 3)
 #pragma unroll
   for (int k = 1; k < NZ; ++k) {
+    cl[k] = 0.0;
+    bdiag[k] = 1.0;
     if (k < nz) {
-      double a, b, dk;
+      const size_t o = (size_t)k * nV + col;
+      const double Tin = h.Ti[o];
+      const double K = 3.101e8 * exp(-0.0057 * Tin);
+      const double zz = h.zz[o];
+      const double tk = first ? Tin : T[k];
+      double a, b, dk, c = 0.0;
       if (k < nz - 1) {
-        a = al[k];
-        b = 1.0 / dt + bl[k];
-        dk = T[k] / dt + r0[k];
+        const double Cp = 2115.3 + 7.79293 * (Tin - kT0);
+        const double c1 = h.zt[o] + h.u[o] * h.zx[o] + h.v[o] * h.zy[o] + h.w[o] * zz;
+        const double c2 = -K / (kIceDensity * Cp) * zz * zz;
+        a = c1 * tri[0 * kHeatMaxNz + k - 1] + c2 * tri[3 * kHeatMaxNz + k - 1];
+        const double bl = c1 * tri[1 * kHeatMaxNz + k] + c2 * tri[4 * kHeatMaxNz + k];
+        c = c1 * tri[2 * kHeatMaxNz + k] + c2 * tri[5 * kHeatMaxNz + k];
+        const double r0 = -h.uadv[o] - h.vadv[o] + h.Phi[o] / (kIceDensity * Cp);
+        b = 1.0 / dt + bl;
+        dk = tk / dt + r0;
       } else {
         // floating: the basal temperature is the pressure-melting point here
         const double tpmp = kT0 - kClausiusClapeyron * Hpmp * zeta[k];
         a = 0.0;
         b = 1.0;
-        dk = mode != 2 ? fmin(tpmp, T[k - 1] - dzb * Q / (zzb * Kb)) : tpmp;
+        dk = mode != 2 ? fmin(tpmp, told - dzb * Q / (zz * K)) : tpmp;
         dfl = tpmp;
       }
       const double w = a / bdiag[k - 1];
       bdiag[k] = b - w * cprev;
-      d[k] = dk - w * d[k - 1];
-      if (k == nz - 1) dfl = dfl - w * d[k - 1];
-      cprev = k < nz - 1 ? cl[k] : 0.0;
+      const double dprev = T[k - 1];
+      told = tk;
+      T[k] = dk - w * dprev;
+      if (k == nz - 1) dfl = dfl - w * dprev;
+      cl[k] = c;
+      cprev = c;
     }
   }
   if (mode != 3) {
 #pragma unroll
     for (int k = NZ - 1; k >= 0; --k) {
-      if (k == nz - 1) T[k] = d[k] / bdiag[k];
-      else if (k < nz - 1) T[k] = (d[k] - (k == 0 ? 0.0 : cl[k]) * T[k + 1]) / bdiag[k];
+      if (k == nz - 1) T[k] = T[k] / bdiag[k];
+      else if (k < nz - 1) T[k] = (T[k] - cl[k] * T[k + 1]) / bdiag[k];
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
@@ -139,9 +172,9 @@ __device__ __forceinline__ void heat_step(int nz, double (&T)[NZ], const double 
 #pragma unroll
     for (int k = NZ - 1; k >= 0; --k) {
       if (k < nz) {
-        const double c = (k == 0 || k == nz - 1) ? 0.0 : cl[k];
-        tg = (d[k] - c * tg) / bdiag[k];
-        tf = ((k == nz - 1 ? dfl : d[k]) - c * tf) / bdiag[k];
+        const double dk = T[k];
+        tg = (dk - cl[k] * tg) / bdiag[k];
+        tf = ((k == nz - 1 ? dfl : dk) - cl[k] * tf) / bdiag[k];
         const double tpmp = kT0 - kClausiusClapeyron * Hpmp * zeta[k];
         T[k] = fgr * fmin(tg, tpmp) + (1.0 - fgr) * fmin(tf, tpmp);
       }
@@ -149,8 +182,16 @@ __device__ __forceinline__ void heat_step(int nz, double (&T)[NZ], const double 
   }
 }
 
+// Occupancy target per column height: the register budget it leaves lets
+// the scheduler prefetch a few layers ahead without spilling.  Measured on
+// MI355X (399k columns): nz = 12 and 30 run fastest at 2 wavefronts per SIMD
+// (115 and 122 us), nz = 8 at 4 (75 us).
 template <int NZ>
-__global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__restrict__ Tout,
+constexpr int heat_waves() { return NZ <= 8 ? 4 : 2; }
+
+template <int NZ>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(heat_waves<NZ>())))
+k_heat_columns(HeatDev h, double *__restrict__ Tout,
                                                          int *__restrict__ unstable, RedSlot slot,
                                                          double *__restrict__ count) {
   __shared__ double tri[6 * kHeatMaxNz];
@@ -159,50 +200,30 @@ __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__re
   for (int i = threadIdx.x; i < h.nz; i += blockDim.x) zeta[i] = h.zeta[i];
   __syncthreads();
   const int nV = h.nV, nz = h.nz;
-  double nun = 0.0;
+  const double dzb = zeta[nz - 1] - zeta[nz - 2];
+  double nun = 0.0, nbad = 0.0;
 #pragma unroll 1
   for (int vi = blockIdx.x * blockDim.x + threadIdx.x; vi < nV; vi += gridDim.x * blockDim.x) {
     unstable[vi] = 0;
     const double H = h.Hi_eff[vi];
+    const double ts = h.T_surf[vi];
     if (H < h.Hi_min) {
-      double ts = h.T_surf[vi];
       for (int k = 0; k < nz; ++k) Tout[(size_t)k * nV + vi] = ts;
       continue;
     }
-    // the column's initial temperature, the rows' coefficients (without 1/dt)
-    // and the explicit terms, all from the input temperature
-    double T0[NZ], al[NZ], bl[NZ], cl[NZ], r0[NZ];
-    double Kb = 0.0, zzb = 0.0;
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-      al[k] = bl[k] = cl[k] = r0[k] = T0[k] = 0.0;
-      if (k < nz) {
-        const size_t o = (size_t)k * nV + vi;
-        const double T = h.Ti[o];
-        T0[k] = T;
-        const double K = 3.101e8 * exp(-0.0057 * T);
-        const double Cp = 2115.3 + 7.79293 * (T - kT0);
-        const double zz = h.zz[o];
-        const double c1 = h.zt[o] + h.u[o] * h.zx[o] + h.v[o] * h.zy[o] + h.w[o] * zz;
-        const double c2 = -K / (kIceDensity * Cp) * zz * zz;
-        if (k >= 1 && k < nz - 1) {
-          al[k] = c1 * tri[0 * kHeatMaxNz + k - 1] + c2 * tri[3 * kHeatMaxNz + k - 1];
-          bl[k] = c1 * tri[1 * kHeatMaxNz + k] + c2 * tri[4 * kHeatMaxNz + k];
-          cl[k] = c1 * tri[2 * kHeatMaxNz + k] + c2 * tri[5 * kHeatMaxNz + k];
-        }
-        r0[k] = -h.uadv[o] - h.vadv[o] + h.Phi[o] / (kIceDensity * Cp);
-        if (k == nz - 1) { Kb = K; zzb = zz; }
-      }
-    }
-    const double dzb = zeta[nz - 1] - zeta[nz - 2];
     const int mode = h.mask[vi];
+    if (mode < 1 || mode > 3 || (mode == 3 && !h.fgr)) {
+      // thick ice neither grounded nor floating: the call reports an error
+      nbad += 1.0;
+      continue;
+    }
     const double fgr = mode == 3 ? h.fgr[vi] : 1.0;
-    const double Q = h.Q[vi], ts = h.T_surf[vi];
+    const double Q = h.Q[vi];
     // dt halving without resetting the column between attempts
     // (thermodynamics_3D_heat_equation.f90:161-239)
     double Tc[NZ];
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) Tc[k] = T0[k];
+    for (int k = 0; k < NZ; ++k) Tc[k] = 0.0;
     bool stable = false;
 #pragma unroll 1
     for (int it = 1; it <= 10 && !stable; ++it) {
@@ -210,7 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__re
       const int nsub = 1 << (it - 1);
 #pragma unroll 1
       for (int sstep = 0; sstep < nsub; ++sstep)
-        heat_step<NZ>(nz, Tc, al, bl, cl, r0, ts, H, zeta, Kb, zzb, dzb, mode, fgr, Q, dta);
+        heat_step<NZ>(h, tri, zeta, vi, it == 1, Tc, ts, H, dzb, mode, fgr, Q, dta);
       stable = true;
 #pragma unroll
       for (int k = 0; k < NZ; ++k)
@@ -221,10 +242,10 @@ __global__ void __launch_bounds__(kBlock) k_heat_columns(HeatDev h, double *__re
     for (int k = 0; k < NZ; ++k)
       if (k < nz) Tout[(size_t)k * nV + vi] = Tc[k];
   }
-  double acc[1] = {nun};
-  __shared__ double lds[1][kBlock / 64];
-  block_sum<1>(acc, lds);
-  block_partials<1>(slot, acc);
+  double acc[2] = {nun, nbad};
+  __shared__ double lds[2][kBlock / 64];
+  block_sum<2>(acc, lds);
+  block_partials<2>(slot, acc);
 }
 
 void launch_heat_columns(const HeatDev &h, double *Tout, int *unstable, const RedSlot &slot, double *count,
@@ -239,7 +260,7 @@ void launch_heat_columns(const HeatDev &h, double *Tout, int *unstable, const Re
   else
     hipLaunchKernelGGL(k_heat_columns<kHeatMaxNz>, dim3(nb), dim3(kBlock), 0, s, h, Tout, unstable, slot, count);
   UFE_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_partials_out<1>, dim3(1), dim3(kBlock), 0, s, slot, nb, count);
+  hipLaunchKernelGGL(k_partials_out<2>, dim3(1), dim3(kBlock), 0, s, slot, nb, count);
   UFE_LAUNCH_CHECK();
 }
 
@@ -413,8 +434,10 @@ extern "C" int ufe_heat_robin(ufe_ctx *c, const ufe_heat_robin_inputs *in, const
   }
 }
 
-extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *is_unstable,
-                                int *n_unstable) {
+namespace {
+// ufe_heat_columns (host arrays) and ufe_heat_columns_device (device arrays).
+int heat_columns_call(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *is_unstable, int *n_unstable,
+                      bool device) {
   if (!in || !Tnew || !is_unstable || !n_unstable) return heat_fail(UFE_ERR_ARGUMENT, "null argument");
   if (in->nz < 3 || in->nz > kHeatMaxNz || in->nV < 0) return heat_fail(UFE_ERR_ARGUMENT, "unsupported nz");
   const double *fields[] = {in->zeta, in->Ti, in->u_3D, in->v_3D, in->w_3D, in->u_dTdxp_upwind, in->v_dTdyp_upwind,
@@ -424,20 +447,25 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
     if (!f) return heat_fail(UFE_ERR_ARGUMENT, "null input field");
   if (!in->mask) return heat_fail(UFE_ERR_ARGUMENT, "null mask");
   int nV = in->nV, nz = in->nz;
-  bool any_gl = false;
-  for (int vi = 0; vi < nV; ++vi) {
-    if (in->Hi_eff[vi] < in->Hi_min_thermo) continue;
-    const int m = in->mask[vi];
-    if (m != 1 && m != 2 && m != 3)
-      return heat_fail(UFE_ERR_ARGUMENT,
-                       "Hi_eff > Hi_min_thermo, but mask_grounded_ice and mask_floating_ice are both false");
-    any_gl |= m == 3;
+  bool any_gl = device && in->fraction_gr;
+  if (!device) {
+    // host arrays: the reference's checks before any device work
+    for (int vi = 0; vi < nV; ++vi) {
+      if (in->Hi_eff[vi] < in->Hi_min_thermo) continue;
+      const int m = in->mask[vi];
+      if (m != 1 && m != 2 && m != 3)
+        return heat_fail(UFE_ERR_ARGUMENT,
+                         "Hi_eff > Hi_min_thermo, but mask_grounded_ice and mask_floating_ice are both false");
+      any_gl |= m == 3;
+    }
+    if (any_gl && !in->fraction_gr)
+      return heat_fail(UFE_ERR_ARGUMENT, "grounding-line columns (mask 3) need fraction_gr");
   }
-  if (any_gl && !in->fraction_gr)
-    return heat_fail(UFE_ERR_ARGUMENT, "grounding-line columns (mask 3) need fraction_gr");
   try {
     int dev = c ? ctx_device(c) : 0;
     UFE_HIP(hipSetDevice(dev));
+    // device arrays may come from work on any stream of the caller's
+    if (device) UFE_HIP(hipDeviceSynchronize());
     // with a context: its stream and device buffers kept between calls (a
     // time-stepping caller allocates nothing after the first call); without
     // one: a stream and buffers of this call only
@@ -457,47 +485,68 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
       if (bytes) UFE_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
       return d;
     };
+    auto in_dev = [&](const void *p, size_t bytes) { return device ? p : (const void *)up(p, bytes); };
+    double zeta_h[kHeatMaxNz];
+    if (device) UFE_HIP(hipMemcpy(zeta_h, in->zeta, sizeof(double) * nz, hipMemcpyDeviceToHost));
+    else std::copy(in->zeta, in->zeta + nz, zeta_h);
     double tri[6 * kHeatMaxNz];
-    zeta_tridiagonals(nz, in->zeta, tri);
+    zeta_tridiagonals(nz, zeta_h, tri);
     HeatDev h;
     h.nV = nV; h.nz = nz;
-    h.zeta = (const double *)up(in->zeta, sizeof(double) * nz);
+    h.zeta = (const double *)in_dev(in->zeta, sizeof(double) * nz);
     h.tri = (const double *)up(tri, sizeof(tri));
-    h.Ti = (const double *)up(in->Ti, n3);
-    h.u = (const double *)up(in->u_3D, n3);
-    h.v = (const double *)up(in->v_3D, n3);
-    h.w = (const double *)up(in->w_3D, n3);
-    h.uadv = (const double *)up(in->u_dTdxp_upwind, n3);
-    h.vadv = (const double *)up(in->v_dTdyp_upwind, n3);
-    h.zx = (const double *)up(in->dzeta_dx, n3);
-    h.zy = (const double *)up(in->dzeta_dy, n3);
-    h.zz = (const double *)up(in->dzeta_dz, n3);
-    h.zt = (const double *)up(in->dzeta_dt, n3);
-    h.Phi = (const double *)up(in->Phi, n3);
-    h.Hi_eff = (const double *)up(in->Hi_eff, n1);
-    h.T_surf = (const double *)up(in->T_surf, n1);
-    h.Q = (const double *)up(in->Q_base_grnd, n1);
-    h.mask = (const int *)up(in->mask, sizeof(int) * nV);
-    h.fgr = any_gl ? (const double *)up(in->fraction_gr, n1) : nullptr;
+    h.Ti = (const double *)in_dev(in->Ti, n3);
+    h.u = (const double *)in_dev(in->u_3D, n3);
+    h.v = (const double *)in_dev(in->v_3D, n3);
+    h.w = (const double *)in_dev(in->w_3D, n3);
+    h.uadv = (const double *)in_dev(in->u_dTdxp_upwind, n3);
+    h.vadv = (const double *)in_dev(in->v_dTdyp_upwind, n3);
+    h.zx = (const double *)in_dev(in->dzeta_dx, n3);
+    h.zy = (const double *)in_dev(in->dzeta_dy, n3);
+    h.zz = (const double *)in_dev(in->dzeta_dz, n3);
+    h.zt = (const double *)in_dev(in->dzeta_dt, n3);
+    h.Phi = (const double *)in_dev(in->Phi, n3);
+    h.Hi_eff = (const double *)in_dev(in->Hi_eff, n1);
+    h.T_surf = (const double *)in_dev(in->T_surf, n1);
+    h.Q = (const double *)in_dev(in->Q_base_grnd, n1);
+    h.mask = (const int *)in_dev(in->mask, sizeof(int) * nV);
+    h.fgr = any_gl ? (const double *)in_dev(in->fraction_gr, n1) : nullptr;
     h.dt = in->dt;
     h.Hi_min = in->Hi_min_thermo;
-    double *Tout = (double *)buf(n3);
-    int *unst = (int *)buf(sizeof(int) * std::max(nV, 1));
+    double *Tout = device ? Tnew : (double *)buf(n3);
+    int *unst = device ? is_unstable : (int *)buf(sizeof(int) * std::max(nV, 1));
     double *cnt = (double *)buf(sizeof(double) * 2);
-    double *part = (double *)buf(sizeof(double) * kMaxPartials);
+    double *part = (double *)buf(sizeof(double) * 2 * kMaxPartials);
     RedSlot slot{part, cnt};
     if (nV > 0) launch_heat_columns(h, Tout, unst, slot, cnt, s);
-    double hc = 0.0;
+    double hc[2] = {0.0, 0.0};
     if (nV > 0) {
-      UFE_HIP(hipMemcpyAsync(Tnew, Tout, n3, hipMemcpyDeviceToHost, s));
-      UFE_HIP(hipMemcpyAsync(is_unstable, unst, sizeof(int) * nV, hipMemcpyDeviceToHost, s));
-      UFE_HIP(hipMemcpyAsync(&hc, cnt, sizeof(double), hipMemcpyDeviceToHost, s));
+      if (!device) {
+        UFE_HIP(hipMemcpyAsync(Tnew, Tout, n3, hipMemcpyDeviceToHost, s));
+        UFE_HIP(hipMemcpyAsync(is_unstable, unst, sizeof(int) * nV, hipMemcpyDeviceToHost, s));
+      }
+      UFE_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
     }
     UFE_HIP(hipStreamSynchronize(s));
     if (!c) UFE_HIP(hipStreamDestroy(s));
-    *n_unstable = (int)hc;
+    if (hc[1] > 0.0)
+      return heat_fail(UFE_ERR_ARGUMENT, std::to_string((long)hc[1]) +
+                                             " columns with Hi_eff > Hi_min_thermo are neither grounded nor floating "
+                                             "(or mask 3 without fraction_gr)");
+    *n_unstable = (int)hc[0];
     return UFE_OK;
   } catch (const std::exception &e) {
     return heat_fail(UFE_ERR_DEVICE, e.what());
   }
+}
+}  // namespace
+
+extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *is_unstable,
+                                int *n_unstable) {
+  return heat_columns_call(c, in, Tnew, is_unstable, n_unstable, false);
+}
+
+extern "C" int ufe_heat_columns_device(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *is_unstable,
+                                       int *n_unstable) {
+  return heat_columns_call(c, in, Tnew, is_unstable, n_unstable, true);
 }

@@ -309,6 +309,38 @@ def heat_columns(hin, solver: VelocitySolver | None = None):
     return T, unst, nun.value
 
 
+HEAT_FIELDS = ("zeta", "Ti", "u_3D", "v_3D", "w_3D", "u_dTdxp_upwind", "v_dTdyp_upwind", "dzeta_dx", "dzeta_dy",
+               "dzeta_dz", "dzeta_dt", "Phi", "Hi_eff", "T_surf", "Q_base_grnd")
+
+
+def make_heat_inputs_device(nV, nz, ptrs: dict, dt: float, Hi_min_thermo: float = 10.0):
+    """HeatInputs whose arrays are device addresses (ints): ptrs holds one per
+    name of HEAT_FIELDS, "mask" (int32) and optionally "fraction_gr"."""
+    h = L.HeatInputs()
+    h.nV, h.nz = nV, nz
+    for name in HEAT_FIELDS:
+        setattr(h, name, C.cast(C.c_void_p(ptrs[name]), L.dp))
+    h.mask = C.cast(C.c_void_p(ptrs["mask"]), L.ip)
+    if ptrs.get("fraction_gr"):
+        h.fraction_gr = C.cast(C.c_void_p(ptrs["fraction_gr"]), L.dp)
+    h.dt = dt
+    h.Hi_min_thermo = Hi_min_thermo
+    return h
+
+
+def heat_columns_device(hin, Tout_ptr: int, unstable_ptr: int, solver: VelocitySolver | None = None) -> int:
+    """ufe_heat_columns_device: inputs and outputs in device memory (addresses
+    as ints); returns the number of unstable columns."""
+    lib = L.lib()
+    nun = C.c_int(0)
+    ctx = solver._ctx if solver is not None else None
+    rc = lib.ufe_heat_columns_device(ctx, C.byref(hin), C.c_void_p(Tout_ptr), C.c_void_p(unstable_ptr),
+                                     C.byref(nun))
+    if rc < 0:
+        raise VelocitySolverError(f"ufe_heat_columns_device failed ({rc}): {lib.ufe_last_error(None).decode()}")
+    return nun.value
+
+
 def make_robin_inputs(nV, nz, zeta, Hi_eff, T_surf, SMB, geothermal_flux, mask, Hi_min_thermo=10.0):
     """ufe_heat_robin_inputs from host arrays; returns (struct, arrays kept alive)."""
     keep = [L.f64(zeta), L.f64(Hi_eff), L.f64(T_surf), L.f64(SMB), L.f64(geothermal_flux), L.i32(mask)]
