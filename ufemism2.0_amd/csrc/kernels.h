@@ -81,6 +81,7 @@ void launch_triangle_3d(const DevMesh &m, const double *eta3a_vk, const double *
 void launch_velocities_3d(int nT, int nz, const double *ubase, const double *beta_basal_b, const double *taub,
                           const double *F1b_tk, int no_sliding, double *u3_out, double *v3_out, hipStream_t s);
 void launch_transpose(int rows, int cols, const double *in, double *out, hipStream_t s);
+void launch_flow_factor_pow(long n, const double *A, double glen_n, double *out, hipStream_t s);
 
 // linear algebra (kernels_linalg.hip)
 struct Bsr64 {

@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_diva(
     int nV, int nz, const int *__restrict__ ptr, const int *__restrict__ ind, const double *__restrict__ wmap,
     const double *__restrict__ wdx, const double *__restrict__ wdy, const double *__restrict__ zeta,
     const double *__restrict__ x, const double *__restrict__ xbase, const double *__restrict__ taub,
-    const double *__restrict__ eta3b, const double *__restrict__ A_vk, const double *__restrict__ Hi,
+    const double *__restrict__ eta3b, const double *__restrict__ Apow_vk, const double *__restrict__ Hi,
     const double *__restrict__ rough, PicardParams p, double *__restrict__ N_a, double *__restrict__ beta_a,
     double *__restrict__ beta_eff_a, double *__restrict__ eta3a, double *__restrict__ F1a,
     double *__restrict__ F2a) {
@@ -56,14 +56,14 @@ __global__ void __launch_bounds__(kBlock) k_vertex_diva(
     }
   }
   double *eo = eta3a + (size_t)vi * nz;
-  const double *Av = A_vk + (size_t)vi * nz;
+  const double *Ap = Apow_vk + (size_t)vi * nz;
   double eprev = 0.0, evav = 0.0;
 #pragma unroll
   for (int k = 0; k < NZ; ++k) {
     if (k < nz) {
       double e2 = ux * ux + vy * vy + ux * vy + 0.25 * (uy + vx) * (uy + vx) +
                   0.25 * (duz[k] * duz[k] + dvz[k] * dvz[k]) + p.eps0;
-      double eta = 0.5 * pow(Av[k], -1.0 / p.n) * pow(e2, (1.0 - p.n) / (2.0 * p.n));
+      double eta = 0.5 * Ap[k] * pow(e2, (1.0 - p.n) / (2.0 * p.n));
       eta = fmin(fmax(eta, p.visc_eff_min), p.eta_max);
       eo[k] = eta;
       et[k] = eta;
@@ -186,16 +186,16 @@ __global__ void __launch_bounds__(256) k_transpose(int rows, int cols, const dou
 }
 
 void launch_vertex_diva(const DevMesh &m, const double *x, const double *xbase, const double *taub,
-                        const double *eta3b_tk, const double *A_vk, const double *Hi, const double *rough,
+                        const double *eta3b_tk, const double *Apow_vk, const double *Hi, const double *rough,
                         const PicardParams &p, double *N_a, double *beta_a, double *beta_eff_a, double *eta3a_vk,
                         double *F1a_vk, double *F2a_vk, hipStream_t s) {
   if (m.nz <= kRegNz)
     hipLaunchKernelGGL(k_vertex_diva<kRegNz>, dim3(blocks_for(m.nV)), dim3(kBlock), 0, s, m.nV, m.nz, m.ba_ptr,
-                       m.ba_ind, m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, A_vk, Hi, rough, p,
+                       m.ba_ind, m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, Apow_vk, Hi, rough, p,
                        N_a, beta_a, beta_eff_a, eta3a_vk, F1a_vk, F2a_vk);
   else
     hipLaunchKernelGGL(k_vertex_diva<kMaxNz>, dim3(blocks_for(m.nV)), dim3(kBlock), 0, s, m.nV, m.nz, m.ba_ptr,
-                       m.ba_ind, m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, A_vk, Hi, rough, p,
+                       m.ba_ind, m.ba_map, m.ba_ddx, m.ba_ddy, m.zeta, x, xbase, taub, eta3b_tk, Apow_vk, Hi, rough, p,
                        N_a, beta_a, beta_eff_a, eta3a_vk, F1a_vk, F2a_vk);
   UFE_LAUNCH_CHECK();
 }
@@ -215,6 +215,22 @@ void launch_velocities_3d(int nT, int nz, const double *ubase, const double *bet
                           const double *F1b_tk, int no_sliding, double *u3_out, double *v3_out, hipStream_t s) {
   hipLaunchKernelGGL(k_velocities_3d, dim3(blocks_for(nT)), dim3(kBlock), 0, s, nT, nz, ubase, beta_basal_b, taub,
                      F1b_tk, no_sliding, u3_out, v3_out);
+  UFE_LAUNCH_CHECK();
+}
+
+// A^(-1/n) of Glen's law per vertex and layer (constitutive_equation.f90:25-51),
+// once per solve: the flow factor does not change between Picard iterations,
+// so the vertex kernel reads this instead of evaluating the power each time.
+// Same expression as the kernel used inline, so the products are unchanged.
+__global__ void __launch_bounds__(kBlock) k_flow_factor_pow(long n, const double *__restrict__ A, double glen_n,
+                                                            double *__restrict__ out) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = pow(A[i], -1.0 / glen_n);
+}
+
+void launch_flow_factor_pow(long n, const double *A, double glen_n, double *out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_flow_factor_pow, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, A, glen_n, out);
   UFE_LAUNCH_CHECK();
 }
 

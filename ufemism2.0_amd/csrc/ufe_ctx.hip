@@ -36,6 +36,7 @@ struct ufe_ctx {
   DBuf A64, A32, Dinv64, Dinv32, Dblk, Dblk32, rhs_s;  // left-scaled system S = D^-1 A, b_s = D^-1 b
   // inputs
   DBuf Hi, Hs, A_vk_vf, A_vk_lf, A_vav, rough, fgr, bcu, bcv, rowtype;
+  DBuf Apow_lf;  // A^(-1/n), layer-fastest, formed at the start of each DIVA solve
   std::vector<signed char> h_rowtype;
   bool grounded_exists = true;
   bool inputs_ready = false;
@@ -252,7 +253,7 @@ static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
                       c->N_a.as<double>(), c->beta_a.as<double>(), s);
   } else {
     launch_vertex_diva(c->m, c->u.as<double>(), c->ubase.as<double>(), c->taub.as<double>(), c->eta3b.as<double>(),
-                       c->A_vk_lf.as<double>(), c->Hi.as<double>(), c->rough.as<double>(), p, c->N_a.as<double>(),
+                       c->Apow_lf.as<double>(), c->Hi.as<double>(), c->rough.as<double>(), p, c->N_a.as<double>(),
                        c->beta_a.as<double>(), c->beta_eff_a.as<double>(), c->eta3a.as<double>(),
                        c->F1a.as<double>(), c->F2a.as<double>(), s);
     launch_triangle_3d(c->m, c->eta3a.as<double>(), c->F1a.as<double>(), c->F2a.as<double>(), c->eta3b.as<double>(),
@@ -395,6 +396,8 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
   launch_driving_stress(c->m, c->Hi.as<double>(), c->Hs.as<double>(), c->tdx.as<double>(), c->tdy.as<double>(), s);
   if (!diva)
     launch_vertical_average(c->m.nV, c->m.nz, c->m.zeta, c->A_vk_vf.as<double>(), c->A_vav.as<double>(), s);
+  else
+    launch_flow_factor_pow((long)c->m.nV * c->m.nz, c->A_vk_lf.as<double>(), q.glens_n, c->Apow_lf.as<double>(), s);
   if (diva && !c->diva_state_ready) {
     UFE_HIP(hipMemsetAsync(c->eta3b.p, 0, c->eta3b.bytes, s));
     UFE_HIP(hipMemsetAsync(c->ubase.p, 0, c->ubase.bytes, s));
@@ -739,7 +742,7 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
     c->tri_coef.alloc(sizeof(double4) * (size_t)nT);
     for (DBuf *b : {&c->N_a, &c->beta_a, &c->beta_eff_a, &c->A_vav, &c->Hi, &c->Hs, &c->rough})
       b->alloc(sizeof(double) * (size_t)nV);
-    for (DBuf *b : {&c->eta3a, &c->F1a, &c->F2a, &c->A_vk_vf, &c->A_vk_lf})
+    for (DBuf *b : {&c->eta3a, &c->F1a, &c->F2a, &c->A_vk_vf, &c->A_vk_lf, &c->Apow_lf})
       b->alloc(sizeof(double) * (size_t)nV * nz);
     for (DBuf *b : {&c->eta3b, &c->F1b, &c->tmp3}) b->alloc(sizeof(double) * (size_t)nT * nz);
     c->out2.alloc(sizeof(double) * 4);
