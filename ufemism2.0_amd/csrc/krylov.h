@@ -190,6 +190,9 @@ class Krylov {
   long long tn_[2] = {0, 0};
   double tms_[2] = {0.0, 0.0}, tbytes_[2] = {0.0, 0.0};
   int verbose_ = 0;
+  // iterations of the previous solve per precision ([0] FP64, [1] FP32): the
+  // first host poll of the next solve is placed there
+  int last_its_[2] = {0, 0};
 };
 
 }  // namespace ufe

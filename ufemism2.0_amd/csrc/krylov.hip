@@ -1106,8 +1106,18 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
         tev_.push_back(e);
       }
     int its_seen = 0;  // k_bicg_init zeroes the device counter
+    // Iterations are launched in chunks and the host polls the device's
+    // convergence flag after each; launches after convergence return at once
+    // but still cost a dispatch each (~50 per iteration with the multigrid
+    // preconditioner).  Chunks end at the previous solve's iteration count
+    // (consecutive Picard systems need similar counts), then grow 1, 2, 4, ...
+    // up to check_every past it.  Which iterations run is decided on the
+    // device, so the chunking does not change the result.
+    const int pred = restart == 0 ? last_its_[sizeof(T) == 4] : 0;
     while (!finished) {
-      for (int k = 0; k < check_every; ++k) {
+      const int chunk = pred > its_seen ? std::min(check_every, pred - its_seen)
+                                        : std::min(check_every, std::max(1, its_seen - pred + 1));
+      for (int k = 0; k < chunk; ++k) {
         hipLaunchKernelGGL(k_bicg_p<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
                            reinterpret_cast<const V2 *>(r), reinterpret_cast<V2 *>(p), reinterpret_cast<const V2 *>(v),
                            reinterpret_cast<V2 *>(amg ? (T *)nullptr : phat), pcl, A.dinv, A.val, A_diag_, sc);
@@ -1141,7 +1151,7 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
                      h_scal_->rho, h_scal_->omega, h_scal_->done);
       if (timing) {
         // launches after convergence return at once: count the iterations done
-        const int real = std::max(0, std::min(check_every, h_scal_->its - its_seen));
+        const int real = std::max(0, std::min(chunk, h_scal_->its - its_seen));
         for (int j = 0; j < real; ++j)
           for (int q = 0; q < 4; ++q) {
             if (q >= 2 && !amg) break;
@@ -1168,6 +1178,7 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
       if (h_scal_->done) finished = true;
     }
     total_its += h_scal_->its;
+    if (restart == 0) last_its_[sizeof(T) == 4] = h_scal_->its;
     st.status = h_scal_->done;
     // verify the true residual (recurrence drift, breakdown) and restart if needed
     RedSlot sl3 = slot(3);
