@@ -287,6 +287,20 @@ def test_chosen_form_is_deterministic(V, channel_fixtures):
     assert np.array_equal(a["u_vav_b"], b["u_vav_b"]) and np.array_equal(a["v_vav_b"], b["v_vav_b"])
 
 
+@pytest.mark.parametrize("stress", ["SSA", "DIVA"])
+def test_krylov_poll_chunks_do_not_change_the_solve(V, channel_fixtures, stress):
+    """BiCGStab iterations are launched in chunks between host polls; which
+    iterations run is decided on the device, so the chunk size (check_every,
+    and the chunks ended at the previous solve's count) leaves velocities and
+    iteration counts bit-identical."""
+    w, _ = channel_fixtures[stress]
+    runs = [solve(V, w, **TIGHT, **CHOSEN, check_every=k) for k in (1, 3, 8, 64)]
+    (a, ia) = runs[0]
+    for b, ib in runs[1:]:
+        assert (ib.n_visc_its, ib.n_Axb_its) == (ia.n_visc_its, ia.n_Axb_its)
+        assert np.array_equal(a["u_vav_b"], b["u_vav_b"]) and np.array_equal(a["v_vav_b"], b["v_vav_b"])
+
+
 def test_mixed_precision_matches_fp64(V, dome):
     kw = dict(stop_rule=1, nonlinear_rtol=1e-8, visc_it_nit=300, abstol=0.0, rtol=1e-10)
     a, _ = solve(V, dome, precision=0, **kw)
