@@ -112,7 +112,8 @@ typedef struct ufe_params {
   double inner_rtol;                 /* MIXED: smallest residual reduction asked of one FP32 inner solve 1e-3 */
   int    stop_rule;                  /* UFE_STOP_*                       L2UV   */
   double nonlinear_rtol;             /* for UFE_STOP_RESIDUAL            1e-6   */
-  int    check_every;                /* host polls Krylov convergence every k iterations */
+  int    check_every;                /* host polls Krylov convergence at most every k iterations
+                                        (polls also land on the previous solve's count) */
   /* Nonlinear acceleration.  accel_depth = 0 keeps the reference's relaxed
    * Picard iteration with its adaptive relaxation rule.  accel_depth = m > 0
    * replaces the relaxation step by Anderson mixing over the last m Picard
