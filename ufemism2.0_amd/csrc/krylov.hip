@@ -621,7 +621,12 @@ struct AmgTailLevel {
   const int *ptr, *col, *agg, *mem_ptr, *mem_ind;
   const void *val, *dinv;
   void *r, *z, *zt;
+  // byte offsets of r, z, zt in the kernel's dynamic LDS, or -1 for the
+  // device-memory vectors above (the top level's r and z are the kernel's
+  // input and output and always live in device memory)
+  int off_r, off_z, off_zt;
 };
+constexpr int kTailLdsMax = 48 * 1024;  // dynamic LDS of the tail kernel, at most
 struct AmgTailArgs {
   int n;  // levels in the tail; lv[n-1] is the coarsest
   double omega, scale;
@@ -629,43 +634,59 @@ struct AmgTailArgs {
   AmgTailLevel lv[kTailMaxLevels];
 };
 
+// A tail level's vector: in the kernel's LDS when it has an offset there,
+// else in device memory.  The work-group barriers between the phases order
+// both kinds of access the same way.
+__device__ __forceinline__ void *tail_vec(unsigned char *lds, int off, void *dev) {
+  return off >= 0 ? (void *)(lds + off) : dev;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kTailThreads) k_amg_tail(AmgTailArgs a, const KrylovScalars *__restrict__ sc) {
   using V2 = typename VecT<T>::V2;
   using V4 = typename VecT<T>::V4;
+  extern __shared__ __align__(16) unsigned char tail_lds[];
   if (sc && sc->done) return;
   const int lane = threadIdx.x % kLpr, grp = threadIdx.x / kLpr, ngrp = blockDim.x / kLpr;
   const T om = (T)a.omega, cs = (T)a.scale;
   for (int l = 0; l + 1 < a.n; ++l) {
     const AmgTailLevel &L = a.lv[l];
     const AmgTailLevel &C = a.lv[l + 1];
+    const V2 *Lr = (const V2 *)tail_vec(tail_lds, L.off_r, L.r);
+    const V2 *Lz = (const V2 *)tail_vec(tail_lds, L.off_z, L.z);
+    V2 *Lzt = (V2 *)tail_vec(tail_lds, L.off_zt, L.zt);
     for (int row = grp; row < L.nb; row += ngrp)
-      row_resid<T, T>(row, lane, L.ptr, L.col, (const V4 *)L.val, (const V2 *)L.r, (const V2 *)L.z, (V2 *)L.zt);
+      row_resid<T, T>(row, lane, L.ptr, L.col, (const V4 *)L.val, Lr, Lz, Lzt);
     __syncthreads();
-    V2 *zc = l + 2 < a.n ? (V2 *)C.z : nullptr;
+    V2 *zc = l + 2 < a.n ? (V2 *)tail_vec(tail_lds, C.off_z, C.z) : nullptr;
+    V2 *Cr = (V2 *)tail_vec(tail_lds, C.off_r, C.r);
     for (int I = grp; I < C.nb; I += ngrp)
-      agg_restrict<T>(I, lane, L.mem_ptr, L.mem_ind, (const V2 *)L.zt, (V2 *)C.r, (const V4 *)C.dinv, om, zc);
+      agg_restrict<T>(I, lane, L.mem_ptr, L.mem_ind, Lzt, Cr, (const V4 *)C.dinv, om, zc);
     __syncthreads();
   }
   {
     const AmgTailLevel &C = a.lv[a.n - 1];
     const int n = 2 * C.nb, i = threadIdx.x;
     if (i < n) {
-      const T *rr = (const T *)C.r;
+      const T *rr = (const T *)tail_vec(tail_lds, C.off_r, C.r);
       double s = 0.0;
       for (int j = 0; j < n; ++j) s += a.cinv[(size_t)j * n + i] * (double)rr[j];
-      ((T *)C.z)[i] = (T)s;
+      ((T *)tail_vec(tail_lds, C.off_z, C.z))[i] = (T)s;
     }
     __syncthreads();
   }
   for (int l = a.n - 2; l >= 0; --l) {
     const AmgTailLevel &L = a.lv[l];
     const AmgTailLevel &C = a.lv[l + 1];
+    const V2 *Lr = (const V2 *)tail_vec(tail_lds, L.off_r, L.r);
+    V2 *Lz = (V2 *)tail_vec(tail_lds, L.off_z, L.z);
+    V2 *Lzt = (V2 *)tail_vec(tail_lds, L.off_zt, L.zt);
+    const V2 *Cz = (const V2 *)tail_vec(tail_lds, C.off_z, C.z);
     for (int row = grp; row < L.nb; row += ngrp)
-      row_prolong_smooth<T, T>(row, lane, L.ptr, L.col, (const V4 *)L.val, (const V4 *)L.dinv, L.agg,
-                               (const V2 *)L.r, (const V2 *)L.z, (const V2 *)C.z, cs, om, (V2 *)L.zt);
+      row_prolong_smooth<T, T>(row, lane, L.ptr, L.col, (const V4 *)L.val, (const V4 *)L.dinv, L.agg, Lr, Lz, Cz,
+                               cs, om, Lzt);
     __syncthreads();
-    for (int i = threadIdx.x; i < L.nb; i += blockDim.x) ((V2 *)L.z)[i] = ((const V2 *)L.zt)[i];
+    for (int i = threadIdx.x; i < L.nb; i += blockDim.x) Lz[i] = Lzt[i];
     __syncthreads();
   }
 }
@@ -1023,8 +1044,26 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z, hipEvent_t ev0, h
       d.r = av.r[l]->p;
       d.z = av.z[l]->p;
       d.zt = av.zt[l]->p;
+      d.off_r = d.off_z = d.off_zt = -1;
     }
-    hipLaunchKernelGGL(k_amg_tail<T>, dim3(1), dim3(kTailThreads), 0, s_, ta, sc);
+    // the vectors that stay inside the kernel go to LDS: every level's zt and
+    // the r and z of the levels below the top one
+    int lds = 0;
+    auto take = [&](int nb) {
+      const int o = lds;
+      lds += ((int)(nb * sizeof(V2)) + 15) & ~15;
+      return o;
+    };
+    for (int l = 0; l < ta.n; ++l) {
+      AmgTailLevel &d = ta.lv[l];
+      if (l + 1 < ta.n) d.off_zt = take(d.nb);
+      if (l > 0) { d.off_r = take(d.nb); d.off_z = take(d.nb); }
+    }
+    if (lds > kTailLdsMax) {
+      for (int l = 0; l < ta.n; ++l) ta.lv[l].off_r = ta.lv[l].off_z = ta.lv[l].off_zt = -1;
+      lds = 0;
+    }
+    hipLaunchKernelGGL(k_amg_tail<T>, dim3(1), dim3(kTailThreads), lds, s_, ta, sc);
     UFE_LAUNCH_CHECK();
   }
   for (int l = t - 1; l >= 0; --l) {
