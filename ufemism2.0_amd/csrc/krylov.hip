@@ -58,9 +58,20 @@ __global__ void __launch_bounds__(kSpmvBlock) k_bsr_spmv(
     T y0 = 0, y1 = 0;
     if (x) {
       const int k1 = ptr[row + 1];
-      for (int k = ptr[row] + lane; k < k1; k += kSpmvLpr) {
-        V4 a = val[k];
-        V2 xv = x[col[k]];
+      // two blocks' gathers in flight per lane; the sums keep the block order
+      int k = ptr[row] + lane;
+      for (; k + kSpmvLpr < k1; k += 2 * kSpmvLpr) {
+        const V4 a0 = val[k], a1 = val[k + kSpmvLpr];
+        const int c0 = col[k], c1 = col[k + kSpmvLpr];
+        const V2 xv0 = x[c0], xv1 = x[c1];
+        y0 += a0.x * xv0.x + a0.y * xv0.y;
+        y1 += a0.z * xv0.x + a0.w * xv0.y;
+        y0 += a1.x * xv1.x + a1.y * xv1.y;
+        y1 += a1.z * xv1.x + a1.w * xv1.y;
+      }
+      if (k < k1) {
+        const V4 a = val[k];
+        const V2 xv = x[col[k]];
         y0 += a.x * xv.x + a.y * xv.y;
         y1 += a.z * xv.x + a.w * xv.y;
       }
@@ -403,13 +414,22 @@ __device__ __forceinline__ void row_resid(int row, int lane, const int *__restri
                                           const typename VecT<T>::V2 *__restrict__ z,
                                           typename VecT<T>::V2 *__restrict__ q, T zs = T(1)) {
   T y0 = 0, y1 = 0;
-  for (int k = ptr[row] + lane; k < ptr[row + 1]; k += kLpr) {
-    typename VecT<MT>::V4 a = val[k];
-    typename VecT<T>::V2 zj = z[col[k]];
+  auto add = [&](const typename VecT<MT>::V4 &a, const typename VecT<T>::V2 &zj) {
     const T z0 = mul_rn(zs, zj.x), z1 = mul_rn(zs, zj.y);
     y0 = fma((T)a.x, z0, fma((T)a.y, z1, y0));
     y1 = fma((T)a.z, z0, fma((T)a.w, z1, y1));
+  };
+  // two blocks' gathers in flight per lane; the FMAs keep the block order
+  int k = ptr[row] + lane;
+  const int k1 = ptr[row + 1];
+  for (; k + kLpr < k1; k += 2 * kLpr) {
+    const typename VecT<MT>::V4 a0 = val[k], a1 = val[k + kLpr];
+    const int c0 = col[k], c1 = col[k + kLpr];
+    const typename VecT<T>::V2 zj0 = z[c0], zj1 = z[c1];
+    add(a0, zj0);
+    add(a1, zj1);
   }
+  if (k < k1) add(val[k], z[col[k]]);
 #pragma unroll
   for (int off = kLpr / 2; off > 0; off >>= 1) {
     y0 += __shfl_down(y0, off, kLpr);
@@ -470,13 +490,26 @@ __device__ __forceinline__ void row_prolong_smooth(int i, int lane, const int *_
                                                    const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega,
                                                    typename VecT<T>::V2 *__restrict__ zout, T zs = T(1)) {
   T y0 = 0, y1 = 0;
-  for (int k = ptr[i] + lane; k < ptr[i + 1]; k += kLpr) {
-    int j = col[k];
-    typename VecT<MT>::V4 a = val[k];
-    typename VecT<T>::V2 zj = z[j], ej = ec[agg[j]];
+  auto add = [&](const typename VecT<MT>::V4 &a, const typename VecT<T>::V2 &zj, const typename VecT<T>::V2 &ej) {
     const T z0 = fma(scale, ej.x, mul_rn(zs, zj.x)), z1 = fma(scale, ej.y, mul_rn(zs, zj.y));
     y0 = fma((T)a.x, z0, fma((T)a.y, z1, y0));
     y1 = fma((T)a.z, z0, fma((T)a.w, z1, y1));
+  };
+  // two blocks' gathers (column, aggregate, coarse correction) in flight per
+  // lane; the FMAs keep the block order
+  int k = ptr[i] + lane;
+  const int k1 = ptr[i + 1];
+  for (; k + kLpr < k1; k += 2 * kLpr) {
+    const int j0 = col[k], j1 = col[k + kLpr];
+    const typename VecT<MT>::V4 a0 = val[k], a1 = val[k + kLpr];
+    const int g0 = agg[j0], g1 = agg[j1];
+    const typename VecT<T>::V2 zj0 = z[j0], zj1 = z[j1], ej0 = ec[g0], ej1 = ec[g1];
+    add(a0, zj0, ej0);
+    add(a1, zj1, ej1);
+  }
+  if (k < k1) {
+    const int j = col[k];
+    add(val[k], z[j], ec[agg[j]]);
   }
 #pragma unroll
   for (int off = kLpr / 2; off > 0; off >>= 1) {
