@@ -111,8 +111,10 @@ __device__ __forceinline__ void heat_step(const HeatDev &h, const double *__rest
   // loops and spill.  It emits no instruction.
   size_t col = (size_t)vi;
   asm volatile("" : "+v"(col));
-  double bdiag[NZ], cl[NZ];
-  bdiag[0] = 1.0;
+  // rb: reciprocals of the Thomas pivots, formed once and used by both sweeps
+  double rb[NZ], cl[NZ];
+  rb[0] = 1.0;
+  const double idt = 1.0 / dt;
   cl[0] = 0.0;
   if (first) T[0] = h.Ti[col];
   double told = T[0];        // the step's starting value of the previous layer
@@ -122,7 +124,7 @@ __device__ __forceinline__ void heat_step(const HeatDev &h, const double *__rest
 #pragma unroll
   for (int k = 1; k < NZ; ++k) {
     cl[k] = 0.0;
-    bdiag[k] = 1.0;
+    rb[k] = 1.0;
     if (k < nz) {
       const size_t o = (size_t)k * nV + col;
       const double Tin = h.Ti[o];
@@ -133,13 +135,14 @@ __device__ __forceinline__ void heat_step(const HeatDev &h, const double *__rest
       if (k < nz - 1) {
         const double Cp = 2115.3 + 7.79293 * (Tin - kT0);
         const double c1 = h.zt[o] + h.u[o] * h.zx[o] + h.v[o] * h.zy[o] + h.w[o] * zz;
-        const double c2 = -K / (kIceDensity * Cp) * zz * zz;
+        const double rcp = 1.0 / (kIceDensity * Cp);
+        const double c2 = -K * rcp * zz * zz;
         a = c1 * tri[0 * kHeatMaxNz + k - 1] + c2 * tri[3 * kHeatMaxNz + k - 1];
         const double bl = c1 * tri[1 * kHeatMaxNz + k] + c2 * tri[4 * kHeatMaxNz + k];
         c = c1 * tri[2 * kHeatMaxNz + k] + c2 * tri[5 * kHeatMaxNz + k];
-        const double r0 = -h.uadv[o] - h.vadv[o] + h.Phi[o] / (kIceDensity * Cp);
-        b = 1.0 / dt + bl;
-        dk = tk / dt + r0;
+        const double r0 = -h.uadv[o] - h.vadv[o] + h.Phi[o] * rcp;
+        b = idt + bl;
+        dk = tk * idt + r0;
       } else {
         // floating: the basal temperature is the pressure-melting point here
         const double tpmp = kT0 - kClausiusClapeyron * Hpmp * zeta[k];
@@ -148,8 +151,8 @@ __device__ __forceinline__ void heat_step(const HeatDev &h, const double *__rest
         dk = mode != 2 ? fmin(tpmp, told - dzb * Q / (zz * K)) : tpmp;
         dfl = tpmp;
       }
-      const double w = a / bdiag[k - 1];
-      bdiag[k] = b - w * cprev;
+      const double w = a * rb[k - 1];
+      rb[k] = 1.0 / (b - w * cprev);
       const double dprev = T[k - 1];
       told = tk;
       T[k] = dk - w * dprev;
@@ -161,8 +164,8 @@ __device__ __forceinline__ void heat_step(const HeatDev &h, const double *__rest
   if (mode != 3) {
 #pragma unroll
     for (int k = NZ - 1; k >= 0; --k) {
-      if (k == nz - 1) T[k] = T[k] / bdiag[k];
-      else if (k < nz - 1) T[k] = (T[k] - cl[k] * T[k + 1]) / bdiag[k];
+      if (k == nz - 1) T[k] = T[k] * rb[k];
+      else if (k < nz - 1) T[k] = (T[k] - cl[k] * T[k + 1]) * rb[k];
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
@@ -173,8 +176,8 @@ __device__ __forceinline__ void heat_step(const HeatDev &h, const double *__rest
     for (int k = NZ - 1; k >= 0; --k) {
       if (k < nz) {
         const double dk = T[k];
-        tg = (dk - cl[k] * tg) / bdiag[k];
-        tf = ((k == nz - 1 ? dfl : dk) - cl[k] * tf) / bdiag[k];
+        tg = (dk - cl[k] * tg) * rb[k];
+        tf = ((k == nz - 1 ? dfl : dk) - cl[k] * tf) * rb[k];
         const double tpmp = kT0 - kClausiusClapeyron * Hpmp * zeta[k];
         T[k] = fgr * fmin(tg, tpmp) + (1.0 - fgr) * fmin(tf, tpmp);
       }
