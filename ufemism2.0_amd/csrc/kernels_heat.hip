@@ -185,10 +185,12 @@ __device__ __forceinline__ void heat_step(const HeatDev &h, const double *__rest
   }
 }
 
-// Occupancy target per column height: the register budget it leaves lets
-// the scheduler prefetch a few layers ahead without spilling.  Measured on
-// MI355X (399k columns): nz = 12 and 30 run fastest at 2 wavefronts per SIMD
-// (115 and 122 us), nz = 8 at 4 (75 us).
+// Occupancy target per column height (a minimum: the compiler may fit more):
+// the register budget it leaves lets the scheduler prefetch a few layers
+// ahead without spilling.  Measured on MI355X (399k columns, 100k for
+// nz = 30): a target of 2 wavefronts per SIMD was fastest for nz = 12 and 30,
+// 4 for nz = 8; with the reciprocal form the kernels take 101, 104 and 63 us
+// (nz = 12 then fits 3 wavefronts).
 template <int NZ>
 constexpr int heat_waves() { return NZ <= 8 ? 4 : 2; }
 
