@@ -452,6 +452,11 @@ __device__ __forceinline__ void agg_restrict(int I, int lane, const int *__restr
                                              typename VecT<T>::V2 *__restrict__ rc,
                                              const typename VecT<T>::V4 *__restrict__ cdinv, T omega,
                                              typename VecT<T>::V2 *__restrict__ zc) {
+  // the coarse diagonal-block inverse is loaded before the member sums so its
+  // latency overlaps them (the epilogue's rounding is explicit, so moving the
+  // load changes no result)
+  typename VecT<T>::V4 d{};
+  if (zc && lane == 0) d = cdinv[I];
   double s0 = 0, s1 = 0;
   for (int m = mem_ptr[I] + lane; m < mem_ptr[I + 1]; m += kLpr) {
     typename VecT<T>::V2 qi = q[mem_ind[m]];
@@ -468,7 +473,6 @@ __device__ __forceinline__ void agg_restrict(int I, int lane, const int *__restr
     o.x = (T)s0; o.y = (T)s1;
     rc[I] = o;
     if (zc) {
-      typename VecT<T>::V4 d = cdinv[I];
       typename VecT<T>::V2 zz;
       zz.x = mul_rn(omega, (T)fma((double)d.x, s0, mul_rn((double)d.y, s1)));
       zz.y = mul_rn(omega, (T)fma((double)d.w, s1, mul_rn((double)d.z, s0)));
