@@ -413,6 +413,10 @@ __device__ __forceinline__ void row_resid(int row, int lane, const int *__restri
                                           const typename VecT<T>::V2 *__restrict__ r,
                                           const typename VecT<T>::V2 *__restrict__ z,
                                           typename VecT<T>::V2 *__restrict__ q, T zs = T(1)) {
+  // lane 0's epilogue operand, loaded before the row's products (its
+  // arithmetic is a single subtraction, so the order changes no result)
+  typename VecT<T>::V2 ri{};
+  if (lane == 0) ri = r[row];
   T y0 = 0, y1 = 0;
   auto add = [&](const typename VecT<MT>::V4 &a, const typename VecT<T>::V2 &zj) {
     const T z0 = mul_rn(zs, zj.x), z1 = mul_rn(zs, zj.y);
@@ -436,7 +440,7 @@ __device__ __forceinline__ void row_resid(int row, int lane, const int *__restri
     y1 += __shfl_down(y1, off, kLpr);
   }
   if (lane == 0) {
-    typename VecT<T>::V2 ri = r[row], o;
+    typename VecT<T>::V2 o;
     o.x = ri.x - y0;
     o.y = ri.y - y1;
     q[row] = o;
@@ -493,6 +497,16 @@ __device__ __forceinline__ void row_prolong_smooth(int i, int lane, const int *_
                                                    const typename VecT<T>::V2 *__restrict__ z,
                                                    const typename VecT<T>::V2 *__restrict__ ec, T scale, T omega,
                                                    typename VecT<T>::V2 *__restrict__ zout, T zs = T(1)) {
+  // lane 0's epilogue operands, loaded before the row's products (the
+  // epilogue's rounding is explicit, so the order changes no result)
+  typename VecT<T>::V2 ri{}, zi{}, ei{};
+  typename VecT<T>::V4 di{};
+  if (lane == 0) {
+    ri = r[i];
+    zi = z[i];
+    ei = ec[agg[i]];
+    if (dinv) di = dinv[i];
+  }
   T y0 = 0, y1 = 0;
   auto add = [&](const typename VecT<MT>::V4 &a, const typename VecT<T>::V2 &zj, const typename VecT<T>::V2 &ej) {
     const T z0 = fma(scale, ej.x, mul_rn(zs, zj.x)), z1 = fma(scale, ej.y, mul_rn(zs, zj.y));
@@ -521,11 +535,10 @@ __device__ __forceinline__ void row_prolong_smooth(int i, int lane, const int *_
     y1 += __shfl_down(y1, off, kLpr);
   }
   if (lane == 0) {
-    typename VecT<T>::V2 ri = r[i], zi = z[i], ei = ec[agg[i]];
     T q0 = ri.x - y0, q1 = ri.y - y1;
     typename VecT<T>::V2 o;
     if (dinv) {  // null: identity diagonal blocks (1 * a + 0 * b == a, so both agree bit for bit)
-      typename VecT<T>::V4 d = dinv[i];
+      const typename VecT<T>::V4 d = di;
       const T t0 = fma(d.x, q0, mul_rn(d.y, q1)), t1 = fma(d.w, q1, mul_rn(d.z, q0));
       q0 = t0;
       q1 = t1;
