@@ -220,6 +220,24 @@ int ufe_download_diva(ufe_ctx *ctx, double *u_base_b, double *v_base_b, double *
  * (solve_linearised_SSA_DIVA.f90:23-178) assembled at u, applied to u. */
 int ufe_ssa_residual(ufe_ctx *ctx, const double *u_b, const double *v_b, double *r, double *b,
                      double *rel_residual);
+/* Nonlinear DIVA residual: one evaluation of the state of solve_DIVA's
+ * viscosity iteration (DIVA_main.f90:174-186: vertical shear strain rates
+ * from tau_b and the previous eta_3D_b, 3-D Glen viscosity, F-integrals,
+ * basal friction from u_base and beta_eff; :375-574) at u_vav and the given
+ * DIVA state, with the linearised system assembled there and applied to u_vav
+ * (solve_linearised_SSA_DIVA.f90:23-178), with the uploaded inputs.
+ *   in : u_vav_b, v_vav_b [nT]; eta_3D_b [nT*nz] (NULL = zero), tau_bx_b,
+ *        tau_by_b, u_base_b, v_base_b [nT] (NULL = zero)
+ *   out (each may be NULL): r, b [2*nT] rows 2*ti+uv; eta_3D_b_new, F1_3D_b
+ *        [nT*nz] of the evaluated state; F2_b_surface [nT] (F2_3D_b at k = 1);
+ *        beta_b [nT] (map_a_b of the basal friction coefficient) and
+ *        beta_eff_b [nT] (after the sub-grid scaling)
+ * The context's velocity and DIVA state become u_vav and the evaluated state
+ * (a following ufe_run starts from them). */
+int ufe_diva_residual(ufe_ctx *ctx, const double *u_vav_b, const double *v_vav_b, const double *eta_3D_b,
+                      const double *tau_bx_b, const double *tau_by_b, const double *u_base_b,
+                      const double *v_base_b, double *r, double *b, double *rel_residual, double *eta_3D_b_new,
+                      double *F1_3D_b, double *F2_b_surface, double *beta_b, double *beta_eff_b);
 /* Time of the last ufe_run spent per stage, from HIP events, in ms:
  * [0] vertex/viscosity, [1] triangle/a->b, [2] assembly, [3] SpMV, [4] Krylov vector ops */
 int ufe_last_timings(const ufe_ctx *ctx, double *ms5);

@@ -47,6 +47,9 @@ def lib():
     o.oracle_ssa_residual.restype = C.c_double
     o.oracle_ssa_residual.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), dp, dp, dp, dp, dp,
                                       ip, dp, dp, dp, dp, dp, dp]
+    o.oracle_diva_residual.restype = C.c_double
+    o.oracle_diva_residual.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), dp, dp, dp, dp, dp, dp, dp,
+                                       dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     o.oracle_ssa_system.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), dp, dp, dp, dp, dp,
                                     dp, dp, C.c_int, ip, ip, dp, dp]
     o.oracle_fraction_gr_b.argtypes = [C.POINTER(L.MeshDesc), dp, dp, dp, dp]
@@ -111,7 +114,9 @@ class Oracle:
                                     L.ptr(bcv), L.ptr(u), L.ptr(v), C.byref(info))
         return rc, u, v, info
 
-    def solve_diva(self, params, inp, state=None):
+    def solve_diva(self, params, inp, state=None, with_info=True):
+        """with_info=False passes no info structure (the oracle then skips its
+        diagnostic residual); info is returned as None."""
         L = _L()
         nT, nz = self.md.mesh.nT, len(self.md.zeta)
         st = {k: np.zeros(nT) for k in ("u", "v", "ubase", "vbase", "tbx", "tby")}
@@ -127,9 +132,9 @@ class Oracle:
                                      L.ptr(inp.get("bc_u")), L.ptr(inp.get("bc_v")),
                                      L.ptr(st["u"]), L.ptr(st["v"]), L.ptr(st["ubase"]), L.ptr(st["vbase"]),
                                      L.ptr(u3), L.ptr(v3), L.ptr(st["eta3b"]), L.ptr(st["tbx"]),
-                                     L.ptr(st["tby"]), C.byref(info))
+                                     L.ptr(st["tby"]), C.byref(info) if with_info else None)
         st["u3d"], st["v3d"] = u3, v3
-        return rc, st, info
+        return rc, st, (info if with_info else None)
 
     def residual(self, params, inp, u, v):
         L = _L()
@@ -141,6 +146,25 @@ class Oracle:
                                         L.ptr(inp.get("bc_v")), L.ptr(L.f64(u)), L.ptr(L.f64(v)),
                                         L.ptr(r), L.ptr(b))
         return rel, r, b
+
+    def diva_residual(self, params, inp, u, v, eta_3D_b=None, tau_bx_b=None, tau_by_b=None, u_base_b=None,
+                      v_base_b=None):
+        """One DIVA state evaluation at u_vav = (u, v) from the given state
+        (3-D arrays [k][ti]); the same dict as VelocitySolver.diva_residual."""
+        L = _L()
+        nT, nz = self.md.mesh.nT, len(self.md.zeta)
+        z = np.zeros(nT)
+        eta = L.f64(np.zeros(nT * nz) if eta_3D_b is None else eta_3D_b).copy()
+        st = [L.f64(z if a is None else a) for a in (tau_bx_b, tau_by_b, u_base_b, v_base_b)]
+        out = {"r": np.zeros(2 * nT), "b": np.zeros(2 * nT), "F1_3D_b": np.zeros(nT * nz),
+               "F2_b_surface": np.zeros(nT), "beta_b": np.zeros(nT), "beta_eff_b": np.zeros(nT)}
+        out["rel_residual"] = lib().oracle_diva_residual(
+            self.md.byref(), C.byref(params), L.ptr(inp["Hi"]), L.ptr(inp["Hs"]), L.ptr(inp["A_flow"]),
+            L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]), L.ptr(L.f64(u)), L.ptr(L.f64(v)), L.ptr(eta),
+            *(L.ptr(a) for a in st), L.ptr(out["r"]), L.ptr(out["b"]), L.ptr(out["F1_3D_b"]),
+            L.ptr(out["F2_b_surface"]), L.ptr(out["beta_b"]), L.ptr(out["beta_eff_b"]))
+        out["eta_3D_b"] = eta
+        return out
 
     def system(self, params, inp, u, v):
         import scipy.sparse as sp

@@ -677,9 +677,29 @@ static int picard(picard_t *s, double *u, double *v, double *eta3b, double *tbx,
   }
   if (info) {
     if (p->stop_rule != UFE_STOP_RESIDUAL || !converged) {
+      /* Diagnostic residual at the returned iterate.  The evaluation
+       * overwrites the DIVA state; the returned eta_3D_b and the F1_3D_b and
+       * beta_b that calc_3D_velocities reads (DIVA_main.f90:254) stay those
+       * of the last state the loop evaluated, as without info. */
+      double *keep_e = NULL, *keep_f = NULL, *keep_b = NULL;
+      size_t n3 = (size_t)nT * nz;
+      if (s->is_diva) {
+        keep_e = malloc(sizeof(double) * n3);
+        keep_f = malloc(sizeof(double) * n3);
+        keep_b = malloc(sizeof(double) * nT);
+        memcpy(keep_e, eta3b, sizeof(double) * n3);
+        memcpy(keep_f, s->F1b, sizeof(double) * n3);
+        memcpy(keep_b, s->beta_b, sizeof(double) * nT);
+      }
       picard_state(s, u, v, eps0, eta3b, tbx, tby, ubase, vbase);
       picard_assemble(s);
       resid = rel_residual(s, u, v);
+      if (s->is_diva) {
+        memcpy(eta3b, keep_e, sizeof(double) * n3);
+        memcpy(s->F1b, keep_f, sizeof(double) * n3);
+        memcpy(s->beta_b, keep_b, sizeof(double) * nT);
+        free(keep_e); free(keep_f); free(keep_b);
+      }
     }
     info->n_visc_its = it;
     info->n_Axb_its = n_axb;
@@ -790,6 +810,46 @@ double oracle_ssa_residual(const ufe_mesh_desc *m, const ufe_params *p, const do
   free(r);
   double rel = scaled_rel_residual(m->nT, s.Aptr, s.Aind, s.Aval, s.bb, s.xx, p->vel_max);
   (void)rr; (void)b2;
+  picard_free(&s);
+  return rel;
+}
+
+/* Nonlinear DIVA residual at (u, v) and a given DIVA state (eta3b, tau_b,
+ * u_base), one state evaluation of solve_DIVA (DIVA_main.f90:174-186, with
+ * calc_vertical_shear_strain_rates :375-410, calc_effective_viscosity
+ * :412-479, calc_F_integrals :481-520 and
+ * calc_effective_basal_friction_coefficient :522-574) followed by the
+ * assembly of solve_SSA_DIVA_linearised.  3-D arrays are [k][ti]; eta3b is
+ * read as the previous state and receives the evaluated one.  Outputs may be
+ * NULL.  Returns the block-scaled relative residual. */
+double oracle_diva_residual(const ufe_mesh_desc *m, const ufe_params *p, const double *Hi,
+                            const double *Hs, const double *A_flow, const double *beta_sq,
+                            const double *fgr_b, const double *u, const double *v, double *eta3b,
+                            const double *tbx, const double *tby, const double *ubase,
+                            const double *vbase, double *r_out, double *b_out, double *F1b_out,
+                            double *F2s_out, double *beta_b_out, double *beta_eff_b_out) {
+  picard_t s;
+  memset(&s, 0, sizeof(s));
+  s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
+  s.is_diva = 1;
+  picard_alloc(&s);
+  int nT = m->nT, nz = m->nz, n = 2 * nT;
+  driving_stress(m, Hi, Hs, s.tdx, s.tdy);
+  picard_state(&s, u, v, p->epsilon_sq_0, eta3b, tbx, tby, ubase, vbase);
+  picard_assemble(&s);
+  for (int ti = 0; ti < nT; ++ti) { s.xx[2 * ti] = u[ti]; s.xx[2 * ti + 1] = v[ti]; }
+  double *r = malloc(sizeof(double) * n);
+  oracle_csr_matvec(n, s.Aptr, s.Aind, s.Aval, s.xx, r);
+  for (int i = 0; i < n; ++i) {
+    if (r_out) r_out[i] = s.bb[i] - r[i];
+    if (b_out) b_out[i] = s.bb[i];
+  }
+  free(r);
+  if (F1b_out) memcpy(F1b_out, s.F1b, sizeof(double) * (size_t)nT * nz);
+  if (F2s_out) memcpy(F2s_out, s.F2b, sizeof(double) * nT); /* layer k = 1 (surface) */
+  if (beta_b_out) memcpy(beta_b_out, s.beta_b, sizeof(double) * nT);
+  if (beta_eff_b_out) memcpy(beta_eff_b_out, s.beta_eff_b, sizeof(double) * nT);
+  double rel = scaled_rel_residual(nT, s.Aptr, s.Aind, s.Aval, s.bb, s.xx, p->vel_max);
   picard_free(&s);
   return rel;
 }

@@ -69,13 +69,14 @@ def test_training_reduces_loss(graph):
     assert hist[-1] < 0.1 * hist[0]
 
 
-def test_learned_guess_gains_over_zero_guess(W, V):
+def test_learned_guess_reaches_the_zero_guess_solution(W, V):
     """Train on solver-generated channel-ensemble solutions (60 members), then
-    solve held-out members from the learned guess and from zero.  The Picard
-    iteration stays the arbiter: where both converge they give the same
-    velocities (<= 1e-5).  The guess's accuracy and the iteration counts are
-    reported; the gain is not asserted, because on this solver it is not
-    consistently positive."""
+    solve held-out members from the learned guess and from zero with the same
+    solver settings.  The Picard iteration stays the arbiter: where both
+    converge they give the same velocities (<= 1e-5).  The guess's accuracy and
+    the iteration counts of both arms are reported; no gain is asserted: on
+    this solver the learned guess does not lower the Picard count
+    consistently."""
     from ufemism2_amd import learned as LN
     solve_kw = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-6,
                     visc_it_nit=600, krylov_maxit=20000, precision=0, precond=3, warm_start=1, rtol=0.01,
@@ -94,7 +95,7 @@ def test_learned_guess_gains_over_zero_guess(W, V):
     for w in test_w:
         inp = w.inputs()
         args = (inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
-        a = V.VelocitySolver(w.descriptor(), 0, **solve_kw, accel_continuation=5)
+        a = V.VelocitySolver(w.descriptor(), 0, **solve_kw)
         a.upload(*args)
         a.set_velocity(None, None)
         a.reset_diva_state()

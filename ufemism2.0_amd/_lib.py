@@ -170,6 +170,8 @@ def lib():
     _sig(L, 'ufe_mpnn_features', argtypes=[C.c_void_p, fp])
     _sig(L, 'ufe_mpnn_initial_guess', argtypes=[C.c_void_p, C.c_void_p])
     _sig(L, 'ufe_ssa_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp])
+    _sig(L, 'ufe_diva_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                           dp])
     _sig(L, 'ufe_csr_matvec', argtypes=[C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_csr_solve', argtypes=[C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double,
                                 C.c_int, C.c_int, ip, dp])
@@ -191,7 +193,7 @@ EXPORTED = [
     "ufe_last_timings", "ufe_profile_spmv", "ufe_profile_copy", "ufe_kernel_timing",
     "ufe_kernel_timing_read", "ufe_set_amg_options", "ufe_mpnn_num_weights", "ufe_mpnn_create",
     "ufe_mpnn_destroy", "ufe_mpnn_get_weights", "ufe_mpnn_forward", "ufe_mpnn_loss_grad", "ufe_mpnn_train",
-    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_robin",
+    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_diva_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_robin",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
 ]
 

@@ -95,9 +95,6 @@ class Krylov {
   // Preconditioner choice for subsequent solves.
   void set_precond(int pc) { pc_ = pc; }
   void set_diag(const int *diag) { A_diag_ = diag; }
-  // Left scaling of the systems solved next (their diagonal blocks D): the
-  // stopping tests then use ||D r_s|| instead of ||r_s||.  null = none.
-  void set_scale(const double4 *d64, const float4 *d32) { dscale64_ = d64; dscale32_ = d32; }
   int precond() const { return pc_; }
   void attach_amg(const AmgHierarchy *h) { amg_ = h; }
   // FP32 copy of the finest-level matrix for the FP64 V-cycle (null: use the FP64 values)
@@ -161,9 +158,6 @@ class Krylov {
                                            hipEvent_t ev1 = nullptr);
   template <typename T> void amg_setup(const BsrT<T> &A0);
   template <typename T> AmgValues<T> &amg_vals();
-  template <typename T> const void *dscale() const;
-  const double4 *dscale64_ = nullptr;
-  const float4 *dscale32_ = nullptr;
 
   int nb_;
   hipStream_t s_;

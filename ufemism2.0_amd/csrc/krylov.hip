@@ -245,8 +245,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(int nb, typename VecT<T>::V2 
                                                    const typename VecT<T>::V2 *__restrict__ t,
                                                    typename VecT<T>::V2 *__restrict__ r,
                                                    const typename VecT<T>::V2 *__restrict__ rhat, RedSlot slot,
-                                                   KrylovScalars *__restrict__ sc,
-                                                   const typename VecT<T>::V4 *__restrict__ dblk) {
+                                                   KrylovScalars *__restrict__ sc) {
   if (sc->done) return;
   T al = (T)sc->alpha, om = (T)sc->omega;
   double acc[2] = {0.0, 0.0};
@@ -259,14 +258,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(int nb, typename VecT<T>::V2 
     rr.y = ss.y - om * tt.y;
     r[i] = rr;
     acc[0] += (double)rh.x * rr.x + (double)rh.y * rr.y;
-    if (dblk) {
-      // stopping norm of a left-scaled system measured unscaled: ||D r_s||
-      typename VecT<T>::V4 d = dblk[i];
-      double u0 = (double)d.x * rr.x + (double)d.y * rr.y, u1 = (double)d.z * rr.x + (double)d.w * rr.y;
-      acc[1] += u0 * u0 + u1 * u1;
-    } else {
-      acc[1] += (double)rr.x * rr.x + (double)rr.y * rr.y;
-    }
+    acc[1] += (double)rr.x * rr.x + (double)rr.y * rr.y;
   }
   __shared__ double lds[2][kBlock / 64];
   block_sum<2>(acc, lds);
@@ -976,8 +968,6 @@ void Krylov::residual64(const BsrT<double> &A, const double *x, const double *b,
 }
 
 template <> AmgValues<double> &Krylov::amg_vals<double>() { return amg64_; }
-template <> const void *Krylov::dscale<double>() const { return dscale64_; }
-template <> const void *Krylov::dscale<float>() const { return dscale32_; }
 template <> AmgValues<float> &Krylov::amg_vals<float>() { return amg32_; }
 
 template <typename T>
@@ -1177,9 +1167,9 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
       hipLaunchKernelGGL(k_zero2<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x));
       UFE_LAUNCH_CHECK();
     }
-    // r (scaled) for the iteration; its unscaled norm for the stopping test
-    spmv_launch<T, OP_RESID>(A, x, r, b, nullptr, sl0, sl0.result, nullptr, s_, 0.0,
-                             reinterpret_cast<const typename VecT<T>::V4 *>(dscale<T>()));
+    // r and its norm (the norm of the system given: for the block-scaled
+    // system, the block-Jacobi-scaled residual) for the stopping test
+    spmv_launch<T, OP_RESID>(A, x, r, b, nullptr, sl0, sl0.result, nullptr, s_, 0.0);
     double h0[3];
     UFE_HIP(hipMemcpyAsync(h0, sl0.result, sizeof(h0), hipMemcpyDeviceToHost, s_));
     UFE_HIP(hipStreamSynchronize(s_));
@@ -1238,8 +1228,7 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
         hipLaunchKernelGGL(k_bicg_x<T>, dim3(nbx), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x),
                            reinterpret_cast<const V2 *>(phat), reinterpret_cast<const V2 *>(shat),
                            reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
-                           reinterpret_cast<const V2 *>(rhat), sl0, sc,
-                           reinterpret_cast<const typename VecT<T>::V4 *>(dscale<T>()));
+                           reinterpret_cast<const V2 *>(rhat), sl0, sc);
         UFE_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_bicg_x_finish, dim3(1), dim3(kBlock), 0, s_, sl0, nbx, sc);
         UFE_LAUNCH_CHECK();
@@ -1283,8 +1272,7 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
     st.status = h_scal_->done;
     // verify the true residual (recurrence drift, breakdown) and restart if needed
     RedSlot sl3 = slot(3);
-    spmv_launch<T, OP_RESID>(A, x, t, b, nullptr, sl3, sl3.result, nullptr, s_, 0.0,
-                             reinterpret_cast<const typename VecT<T>::V4 *>(dscale<T>()));
+    spmv_launch<T, OP_RESID>(A, x, t, b, nullptr, sl3, sl3.result, nullptr, s_, 0.0);
     double hr[3];
     UFE_HIP(hipMemcpyAsync(hr, sl3.result, sizeof(hr), hipMemcpyDeviceToHost, s_));
     UFE_HIP(hipStreamSynchronize(s_));

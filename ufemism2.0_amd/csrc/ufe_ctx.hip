@@ -44,6 +44,7 @@ struct ufe_ctx {
   DBuf u, uprev, x, rhs, r64, tdx, tdy, N_a, beta_a, beta_eff_a, tri_coef, beta_basal_b, out2;
   // DIVA
   DBuf eta3b, eta3a, F1a, F2a, F1b, F2b0, ubase, taub, tmp3;
+  DBuf beta_save;  // beta_b kept aside while the diagnostic residual is evaluated
   bool diva_state_ready = false;
   // Anderson mixing: ring of m difference columns (dF, dG), previous f and G,
   // reduction slot of width 2 kAAMaxDepth + 1
@@ -175,9 +176,9 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
   UFE_LAUNCH_CHECK();
   *its = 0;
   *nref = 0;
-  // All stopping tests use the block-Jacobi-scaled residual (velocity units),
-  // the norm PETSc's left-preconditioned KSP test measures.
-  K.set_scale(nullptr, nullptr);
+  // All stopping tests use the block-Jacobi-scaled residual (velocity units)
+  // of the assembled system S = D^-1 A, the norm PETSc's left-preconditioned
+  // KSP test measures with a block-Jacobi preconditioner.
   // the assembled system is S = D^-1 A: identity diagonal blocks
   K.set_unit_diag0(c->amg_fold);
   if (q.precision == UFE_PREC_FP64) {
@@ -462,15 +463,29 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
     if (q.stop_rule == UFE_STOP_L2UV && L2 < q.visc_it_norm_dUV_tol) converged = true;
     if (it > q.visc_it_nit) { rc = UFE_WARN_NOT_CONVERGED; break; }
   }
-  if (rc >= 0 && (q.stop_rule != UFE_STOP_RESIDUAL || !converged)) {
-    // state at the returned iterate (for the residual and, for DIVA, eta_3D_b/F)
-    picard_state(c, diva, eps0, false);
-    resid = rel_residual(c, &n_clamped);
-  }
   if (diva && rc >= 0) {
-    // F1 on the b-grid at the returned state for calc_3D_velocities
+    // F1_3D_b of the last state the loop evaluated: calc_3D_velocities
+    // (DIVA_main.f90:254, :636-676) combines it with that state's beta_b and
+    // the u_base / tau_b formed after the last update
     launch_triangle_3d(c->m, c->eta3a.as<double>(), c->F1a.as<double>(), c->F2a.as<double>(),
                        c->tmp3.as<double>(), c->F1b.as<double>(), c->F2b0.as<double>(), 1, s);
+  }
+  if (rc >= 0 && (q.stop_rule != UFE_STOP_RESIDUAL || !converged)) {
+    // Diagnostic residual at the returned iterate.  The evaluation overwrites
+    // the DIVA state, so eta_3D_b (returned, and the next call's warm start)
+    // and beta_b (read by the 3-D velocities) are kept aside and restored.
+    const size_t e_bytes = sizeof(double) * (size_t)nT * c->m.nz, b_bytes = sizeof(double) * (size_t)nT;
+    if (diva) {
+      if (c->beta_save.bytes < b_bytes) c->beta_save.alloc(b_bytes);
+      UFE_HIP(hipMemcpyAsync(c->tmp3.p, c->eta3b.p, e_bytes, hipMemcpyDeviceToDevice, s));
+      UFE_HIP(hipMemcpyAsync(c->beta_save.p, c->beta_basal_b.p, b_bytes, hipMemcpyDeviceToDevice, s));
+    }
+    picard_state(c, diva, eps0, false);
+    resid = rel_residual(c, &n_clamped);
+    if (diva) {
+      UFE_HIP(hipMemcpyAsync(c->eta3b.p, c->tmp3.p, e_bytes, hipMemcpyDeviceToDevice, s));
+      UFE_HIP(hipMemcpyAsync(c->beta_basal_b.p, c->beta_save.p, b_bytes, hipMemcpyDeviceToDevice, s));
+    }
   }
   UFE_HIP(hipStreamSynchronize(s));
   inf.n_visc_its = it;
@@ -661,6 +676,8 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
       if (md->TriC[k] < -1 || md->TriC[k] >= nT) throw std::runtime_error("TriC out of range");
       if (md->Tri[k] < 0 || md->Tri[k] >= nV) throw std::runtime_error("Tri out of range");
     }
+    for (int ti = 0; ti < nT; ++ti)
+      if (md->TriBI[ti] < 0 || md->TriBI[ti] > 8) throw std::runtime_error("TriBI out of range (0..8)");
     PermutedMesh pm;
     permute_mesh(*md, pm);
     c->vperm = pm.v_n2o;
@@ -1219,6 +1236,64 @@ extern "C" int ufe_ssa_residual(ufe_ctx *c, const double *u_b, const double *v_b
     };
     if (r) down(c->r64, r);
     if (b) down(c->rhs, b);
+    return UFE_OK;
+  });
+}
+
+extern "C" int ufe_diva_residual(ufe_ctx *c, const double *u_b, const double *v_b, const double *eta_3D_b,
+                                 const double *tau_bx_b, const double *tau_by_b, const double *u_base_b,
+                                 const double *v_base_b, double *r, double *b, double *rel_out, double *eta_out,
+                                 double *F1_out, double *F2s_out, double *beta_out, double *beta_eff_out) {
+  if (!c || !u_b || !v_b) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  if ((!tau_bx_b) != (!tau_by_b) || (!u_base_b) != (!v_base_b))
+    return fail(c, UFE_ERR_ARGUMENT, "tau_b and u_base need both components or neither");
+  return guarded(c, [&]() -> int {
+    UFE_HIP(hipSetDevice(c->device));
+    if (!c->inputs_ready) return fail(c, UFE_ERR_ARGUMENT, "inputs not uploaded");
+    const int nT = c->m.nT, nz = c->m.nz;
+    int rc = ufe_upload_diva_state(c, eta_3D_b, tau_bx_b, tau_by_b, u_base_b, v_base_b);
+    if (rc < 0) return rc;
+    interleave_up(c, c->u, u_b, v_b);
+    hipStream_t s = c->s;
+    launch_driving_stress(c->m, c->Hi.as<double>(), c->Hs.as<double>(), c->tdx.as<double>(), c->tdy.as<double>(), s);
+    launch_flow_factor_pow((long)c->m.nV * nz, c->A_vk_lf.as<double>(), c->prm.glens_n, c->Apow_lf.as<double>(), s);
+    picard_state(c, true, c->prm.epsilon_sq_0, true);
+    double rel = rel_residual(c);
+    if (rel_out) *rel_out = rel;
+    std::vector<double> h(2 * (size_t)nT);
+    auto down2 = [&](const DBuf &src, double *dst) {
+      UFE_HIP(hipMemcpy(h.data(), src.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < nT; ++i) {
+        dst[2 * c->tperm[i]] = h[2 * i];
+        dst[2 * c->tperm[i] + 1] = h[2 * i + 1];
+      }
+    };
+    if (r) down2(c->r64, r);
+    if (b) down2(c->rhs, b);
+    auto down3 = [&](const DBuf &src_tk, double *dst) {  // [triangle][k] -> caller's [k][triangle]
+      launch_transpose(nT, nz, src_tk.as<double>(), c->tmp3.as<double>(), s);
+      UFE_HIP(hipStreamSynchronize(s));
+      std::vector<double> t((size_t)nT * nz);
+      UFE_HIP(hipMemcpy(t.data(), c->tmp3.p, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+      from_dev(c->tperm, t.data(), dst, nz);
+    };
+    if (eta_out) down3(c->eta3b, eta_out);
+    if (F1_out) down3(c->F1b, F1_out);
+    std::vector<double> t1(nT);
+    if (F2s_out) {
+      UFE_HIP(hipMemcpy(t1.data(), c->F2b0.p, sizeof(double) * nT, hipMemcpyDeviceToHost));
+      from_dev(c->tperm, t1.data(), F2s_out);
+    }
+    if (beta_out) {
+      UFE_HIP(hipMemcpy(t1.data(), c->beta_basal_b.p, sizeof(double) * nT, hipMemcpyDeviceToHost));
+      from_dev(c->tperm, t1.data(), beta_out);
+    }
+    if (beta_eff_out) {
+      std::vector<double4> tc(nT);
+      UFE_HIP(hipMemcpy(tc.data(), c->tri_coef.p, sizeof(double4) * nT, hipMemcpyDeviceToHost));
+      for (int i = 0; i < nT; ++i) t1[i] = tc[i].w;
+      from_dev(c->tperm, t1.data(), beta_eff_out);
+    }
     return UFE_OK;
   });
 }

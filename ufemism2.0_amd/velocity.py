@@ -218,6 +218,24 @@ class VelocitySolver:
                                               L.ptr(b), C.byref(rel)))
         return rel.value, r, b
 
+    def diva_residual(self, u_b, v_b, eta_3D_b=None, tau_bx_b=None, tau_by_b=None, u_base_b=None, v_base_b=None):
+        """One DIVA state evaluation and the nonlinear residual at u_vav =
+        (u_b, v_b) from the given DIVA state (eta_3D_b in the (nT, nz)
+        triangle-fastest layout).  Returns a dict with rel_residual, r, b,
+        eta_3D_b, F1_3D_b, F2_b_surface, beta_b, beta_eff_b."""
+        nT, nz = self.md.mesh.nT, len(self.md.zeta)
+        out = {"r": np.zeros(2 * nT), "b": np.zeros(2 * nT), "eta_3D_b": np.zeros(nT * nz),
+               "F1_3D_b": np.zeros(nT * nz), "F2_b_surface": np.zeros(nT), "beta_b": np.zeros(nT),
+               "beta_eff_b": np.zeros(nT)}
+        rel = C.c_double()
+        opt = [None if a is None else L.f64(a) for a in (eta_3D_b, tau_bx_b, tau_by_b, u_base_b, v_base_b)]
+        self._check(self.lib.ufe_diva_residual(
+            self._ctx, L.ptr(L.f64(u_b)), L.ptr(L.f64(v_b)), *(L.ptr(a) for a in opt), L.ptr(out["r"]),
+            L.ptr(out["b"]), C.byref(rel), L.ptr(out["eta_3D_b"]), L.ptr(out["F1_3D_b"]),
+            L.ptr(out["F2_b_surface"]), L.ptr(out["beta_b"]), L.ptr(out["beta_eff_b"])))
+        out["rel_residual"] = rel.value
+        return out
+
     # -- drop-in entry points ---------------------------------------------------
     def solve_SSA(self, Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice=None, u_b=None, v_b=None,
                   BC_prescr_mask_b=None, BC_prescr_u_b=None, BC_prescr_v_b=None):
