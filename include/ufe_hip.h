@@ -335,6 +335,18 @@ int ufe_heat_columns(ufe_ctx *ctx, const ufe_heat_inputs *in, double *Ti_tplusdt
 int ufe_heat_columns_device(ufe_ctx *ctx, const ufe_heat_inputs *in, double *Ti_tplusdt, int *is_unstable,
                             int *n_unstable);
 
+/* Allocation-free form of ufe_heat_columns_device (no counterpart in the
+ * reference: for callers that cannot allocate device memory, e.g. code under
+ * HIP graph capture or a serving loop that forbids hipMalloc).  Same inputs,
+ * outputs, checks and results; no context.  It runs on the caller's current
+ * device and the null stream (waiting for the device's queued work first) and
+ * calls no hipMalloc/hipFree: its scratch memory (the zeta tridiagonals and
+ * the reduction partials, 132,624 bytes) is one static device array per GPU,
+ * held until the process ends.  Limit: nz <= 32 (UFE_ERR_ARGUMENT beyond);
+ * nV is not limited.  Concurrent calls on one GPU share that array and race. */
+int ufe_heat_columns_device_noalloc(const ufe_heat_inputs *in, double *Ti_tplusdt, int *is_unstable,
+                                    int *n_unstable);
+
 /* Unstable columns after the heat solve (thermodynamics_3D_heat_equation.f90:254-279):
  * when fewer than ceil(nV / 100) columns are flagged, each flagged column of
  * Ti [nV*nz, vertex fastest] is replaced in place with the Robin solution

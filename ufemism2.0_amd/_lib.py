@@ -179,6 +179,8 @@ def lib():
     _sig(L, 'ufe_heat_columns_device', argtypes=[C.c_void_p, C.POINTER(HeatInputs), C.c_void_p, C.c_void_p,
                                                  C.POINTER(C.c_int)])
     _sig(L, 'ufe_heat_columns', argtypes=[C.c_void_p, C.POINTER(HeatInputs), dp, ip, ip])
+    _sig(L, 'ufe_heat_columns_device_noalloc', argtypes=[C.POINTER(HeatInputs), C.c_void_p, C.c_void_p,
+                                                         C.POINTER(C.c_int)])
     _sig(L, 'ufe_mesh_operators_ab', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_ba', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_bb2', argtypes=[C.c_int, ip, dp, ip, ip, ip, dp, dp, dp, dp, dp])
@@ -193,7 +195,7 @@ EXPORTED = [
     "ufe_last_timings", "ufe_profile_spmv", "ufe_profile_copy", "ufe_kernel_timing",
     "ufe_kernel_timing_read", "ufe_set_amg_options", "ufe_mpnn_num_weights", "ufe_mpnn_create",
     "ufe_mpnn_destroy", "ufe_mpnn_get_weights", "ufe_mpnn_forward", "ufe_mpnn_loss_grad", "ufe_mpnn_train",
-    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_diva_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_robin",
+    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_diva_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_columns_device_noalloc", "ufe_heat_robin",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
 ]
 

@@ -2,6 +2,7 @@
 // their C ABI entry points.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "krylov.h"
@@ -439,12 +440,54 @@ extern "C" int ufe_heat_robin(ufe_ctx *c, const ufe_heat_robin_inputs *in, const
   }
 }
 
+// ---------------------------------------------------------------------------
+// Scratch memory of the allocation-free heat step
+// (ufe_heat_columns_device_noalloc): the zeta tridiagonals, the per-workgroup
+// partial counts and the two totals live in one file-scope device array.
+// Every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` only names the symbol: its device address on the
+// current GPU is looked up once per device (hipGetSymbolAddress) and only that
+// address reaches kernels and copies.  Layout, in doubles:
+//   [0, kScratchTri)                       zeta tridiagonals (6 x kHeatMaxNz)
+//   [kScratchPart, kScratchPart + 2 * kMaxPartials)   partials of the two counts
+//   [kScratchCnt, kScratchCnt + 2)         the totals (unstable, invalid mask)
+// Nothing in it has to start at zero: each workgroup writes its partials
+// before the one-workgroup pass reads them, and that pass writes both totals.
+// The array keeps the last call's contents; calls that overlap on one GPU
+// race on it.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchTri = 6 * (size_t)kHeatMaxNz;
+constexpr size_t kScratchPart = kScratchTri;
+constexpr size_t kScratchCnt = kScratchPart + 2 * (size_t)kMaxPartials;
+constexpr size_t kScratchDoubles = kScratchCnt + 2;  // 16,578 doubles = 132,624 bytes
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kScratchMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kScratchMaxDevices] = {};
+
+// Device address of `scratch` on device dev (which must be current).
+static double *scratch_on_device(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    UFE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the heat-step scratch array");
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
 namespace {
-// ufe_heat_columns (host arrays) and ufe_heat_columns_device (device arrays).
+// ufe_heat_columns (host arrays), ufe_heat_columns_device (device arrays) and,
+// with use_scratch, ufe_heat_columns_device_noalloc (device arrays, scratch
+// in the file-scope array, the null stream, no allocation).
 int heat_columns_call(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *is_unstable, int *n_unstable,
-                      bool device) {
+                      bool device, bool use_scratch = false) {
   if (!in || !Tnew || !is_unstable || !n_unstable) return heat_fail(UFE_ERR_ARGUMENT, "null argument");
-  if (in->nz < 3 || in->nz > kHeatMaxNz || in->nV < 0) return heat_fail(UFE_ERR_ARGUMENT, "unsupported nz");
+  if (in->nz < 3 || in->nz > kHeatMaxNz || in->nV < 0)
+    return heat_fail(UFE_ERR_ARGUMENT, "unsupported column size: nz must be 3.." + std::to_string(kHeatMaxNz) +
+                                           " and nV >= 0");
   const double *fields[] = {in->zeta, in->Ti, in->u_3D, in->v_3D, in->w_3D, in->u_dTdxp_upwind, in->v_dTdyp_upwind,
                             in->dzeta_dx, in->dzeta_dy, in->dzeta_dz, in->dzeta_dt, in->Phi, in->Hi_eff,
                             in->T_surf, in->Q_base_grnd};
@@ -468,15 +511,22 @@ int heat_columns_call(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *
   }
   try {
     int dev = c ? ctx_device(c) : 0;
+    if (use_scratch) {
+      // the caller's current device; its copy of the scratch array
+      UFE_HIP(hipGetDevice(&dev));
+      if (dev < 0 || dev >= kScratchMaxDevices)
+        return heat_fail(UFE_ERR_DEVICE, "device index beyond the scratch table (64 devices)");
+    }
     UFE_HIP(hipSetDevice(dev));
+    double *scr = use_scratch ? scratch_on_device(dev) : nullptr;
     // device arrays may come from work on any stream of the caller's
     if (device) UFE_HIP(hipDeviceSynchronize());
     // with a context: its stream and device buffers kept between calls (a
     // time-stepping caller allocates nothing after the first call); without
     // one: a stream and buffers of this call only
-    hipStream_t s;
+    hipStream_t s = nullptr;  // the null stream for the allocation-free form
     if (c) s = ctx_stream(c);
-    else UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    else if (!use_scratch) UFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     size_t n3 = sizeof(double) * (size_t)nV * nz, n1 = sizeof(double) * (size_t)nV;
     std::vector<std::unique_ptr<DBuf>> keep;
     int nbuf = 0;
@@ -499,7 +549,12 @@ int heat_columns_call(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *
     HeatDev h;
     h.nV = nV; h.nz = nz;
     h.zeta = (const double *)in_dev(in->zeta, sizeof(double) * nz);
-    h.tri = (const double *)up(tri, sizeof(tri));
+    if (use_scratch) {
+      UFE_HIP(hipMemcpyAsync(scr, tri, sizeof(double) * kScratchTri, hipMemcpyHostToDevice, s));
+      h.tri = scr;
+    } else {
+      h.tri = (const double *)up(tri, sizeof(tri));
+    }
     h.Ti = (const double *)in_dev(in->Ti, n3);
     h.u = (const double *)in_dev(in->u_3D, n3);
     h.v = (const double *)in_dev(in->v_3D, n3);
@@ -520,8 +575,8 @@ int heat_columns_call(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *
     h.Hi_min = in->Hi_min_thermo;
     double *Tout = device ? Tnew : (double *)buf(n3);
     int *unst = device ? is_unstable : (int *)buf(sizeof(int) * std::max(nV, 1));
-    double *cnt = (double *)buf(sizeof(double) * 2);
-    double *part = (double *)buf(sizeof(double) * 2 * kMaxPartials);
+    double *cnt = use_scratch ? scr + kScratchCnt : (double *)buf(sizeof(double) * 2);
+    double *part = use_scratch ? scr + kScratchPart : (double *)buf(sizeof(double) * 2 * kMaxPartials);
     RedSlot slot{part, cnt};
     if (nV > 0) launch_heat_columns(h, Tout, unst, slot, cnt, s);
     double hc[2] = {0.0, 0.0};
@@ -533,7 +588,7 @@ int heat_columns_call(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *
       UFE_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
     }
     UFE_HIP(hipStreamSynchronize(s));
-    if (!c) UFE_HIP(hipStreamDestroy(s));
+    if (!c && !use_scratch) UFE_HIP(hipStreamDestroy(s));
     if (hc[1] > 0.0)
       return heat_fail(UFE_ERR_ARGUMENT, std::to_string((long)hc[1]) +
                                              " columns with Hi_eff > Hi_min_thermo are neither grounded nor floating "
@@ -554,4 +609,9 @@ extern "C" int ufe_heat_columns(ufe_ctx *c, const ufe_heat_inputs *in, double *T
 extern "C" int ufe_heat_columns_device(ufe_ctx *c, const ufe_heat_inputs *in, double *Tnew, int *is_unstable,
                                        int *n_unstable) {
   return heat_columns_call(c, in, Tnew, is_unstable, n_unstable, true);
+}
+
+extern "C" int ufe_heat_columns_device_noalloc(const ufe_heat_inputs *in, double *Tnew, int *is_unstable,
+                                               int *n_unstable) {
+  return heat_columns_call(nullptr, in, Tnew, is_unstable, n_unstable, true, true);
 }

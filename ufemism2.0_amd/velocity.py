@@ -359,6 +359,20 @@ def heat_columns_device(hin, Tout_ptr: int, unstable_ptr: int, solver: VelocityS
     return nun.value
 
 
+def heat_columns_device_noalloc(hin, Tout_ptr: int, unstable_ptr: int) -> int:
+    """ufe_heat_columns_device_noalloc: the device-array heat step with no
+    device allocation (scratch in the library's static array) on the current
+    device and the null stream; returns the number of unstable columns."""
+    lib = L.lib()
+    nun = C.c_int(0)
+    rc = lib.ufe_heat_columns_device_noalloc(C.byref(hin), C.c_void_p(Tout_ptr), C.c_void_p(unstable_ptr),
+                                             C.byref(nun))
+    if rc < 0:
+        raise VelocitySolverError(
+            f"ufe_heat_columns_device_noalloc failed ({rc}): {lib.ufe_last_error(None).decode()}")
+    return nun.value
+
+
 def make_robin_inputs(nV, nz, zeta, Hi_eff, T_surf, SMB, geothermal_flux, mask, Hi_min_thermo=10.0):
     """ufe_heat_robin_inputs from host arrays; returns (struct, arrays kept alive)."""
     keep = [L.f64(zeta), L.f64(Hi_eff), L.f64(T_surf), L.f64(SMB), L.f64(geothermal_flux), L.i32(mask)]
