@@ -32,7 +32,14 @@ back-to-back after the solves beside it.
 `cpu_baseline` is the oracle's FP64 restatement of the reference iteration
 (GMRES(30) + ILU(0)) on the host cores, timed on a bounded sample of Picard
 iterations of the same workload and scaled to solves/s with the plain form's
-Picard count.
+Picard count; beside it, full measured CPU solves of the dome (the
+CPU-runnable configuration, 2,205 vertices) with the same stopping test, and
+the GPU's plain and chosen forms on the same dome.
+Other keys: `roofline_out_of_cache` (the Krylov block product on the
+continental mesh, whose 326 MB per product exceed the 256 MB Infinity Cache),
+`oracle_parity_at_scale` (the chosen form against the committed oracle
+solution of a 20k-vertex DIVA channel), and the unscaled residual
+||b - A u|| / ||b|| of every solve next to the block-scaled one.
 """
 
 from __future__ import annotations
@@ -80,6 +87,124 @@ def solve_once(s, stress="DIVA", info_check=True):
     return info
 
 
+def cpu_info():
+    """CPU model and the cores this process may use (the oracle's OpenMP threads)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", avail))
+    return {"model": model, "nproc": avail, "threads": threads}
+
+
+def dome_full_solves(V, W, reps=3):
+    """The CPU-runnable configuration measured end to end: full converged SSA
+    solves of the dome (block-scaled residual <= 1e-6, relaxed Picard with the
+    reference rule) by the oracle on the host cores (median of `reps`), and by
+    the GPU's plain and chosen forms on the same inputs."""
+    import numpy as np
+    from ufemism2_amd import _lib as L
+    from oracle import binding as OB
+    w = W.dome(h=40e3)
+    inp = w.inputs()
+    p = L.default_params()
+    for k, v in dict(BASE, rtol=1e-7).items():
+        setattr(p, k, v)
+    o = OB.Oracle(w.descriptor())
+    times, info = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rc, uo, vo, info = o.solve_ssa(p, inp)
+        times.append(time.perf_counter() - t0)
+        if rc != 0 or not info.rel_residual <= BASE["nonlinear_rtol"]:
+            raise RuntimeError(f"oracle dome solve failed: rc {rc} residual {info.rel_residual}")
+    t_cpu = float(np.median(times))
+    out = {"workload": f"dome_ssa_40km ({w.mesh.nV} vertices, {w.mesh.nT} triangles)",
+           "cpu_oracle": {"solves_per_s": 1.0 / t_cpu, "seconds": t_cpu, "picard": info.n_visc_its,
+                          "krylov": info.n_Axb_its, "residual": info.rel_residual,
+                          "residual_unscaled": info.rel_residual_unscaled,
+                          "method": "oracle: relaxed Picard (reference rule), GMRES(30) + ILU(0), rtol 1e-7"}}
+    for label, form, reps_g in (("gpu_plain", PLAIN, 3), ("gpu_chosen", CHOSEN, 10)):
+        s = V.VelocitySolver(w.descriptor(), 0, **BASE, **form)
+        s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+        solve_once(s, "SSA")
+        t0 = time.perf_counter()
+        for _ in range(reps_g):
+            gi = solve_once(s, "SSA")
+        t = (time.perf_counter() - t0) / reps_g
+        u, v = s.get_velocity()
+        s.close()
+        sc = max(np.abs(uo).max(), np.abs(vo).max())
+        out[label] = {"solves_per_s": 1.0 / t, "seconds": t, "picard": gi.n_visc_its, "krylov": gi.n_Axb_its,
+                      "residual": gi.rel_residual, "residual_unscaled": gi.rel_residual_unscaled,
+                      "max_rel_vel_diff_vs_cpu_oracle": float(max(np.abs(u - uo).max(), np.abs(v - vo).max()) / sc)}
+    out["gpu_chosen_vs_cpu"] = out["gpu_chosen"]["solves_per_s"] / out["cpu_oracle"]["solves_per_s"]
+    return out
+
+
+def oracle_parity_at_scale(V, W):
+    """The chosen form on the 20k-vertex DIVA channel (seed 2) against the
+    committed oracle solution of the reference iteration (nonlinear residual
+    1e-8): max relative velocity error, normalised by max |u_oracle|."""
+    import numpy as np
+    path = os.path.join(ROOT, "tests", "golden", "oracle_channel_diva_20k.npz")
+    if not os.path.exists(path):
+        return None
+    f = np.load(path)
+    w = W.channel(seed=2, target_nV=20000, nz=10, stress_balance="DIVA")
+    if str(f["digest"]) != w.mesh.digest():
+        return {"error": "mesh digest differs from the fixture's"}
+    inp = w.inputs()
+    s = V.VelocitySolver(w.descriptor(), 0, **dict(BASE, nonlinear_rtol=1e-8), **CHOSEN)
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    s.set_velocity(None, None)
+    s.reset_diva_state()
+    info = s.run("DIVA")
+    u, v = s.get_velocity()
+    s.close()
+    fu, fv = f["u"].astype(np.float64), f["v"].astype(np.float64)
+    sc = max(np.abs(fu).max(), np.abs(fv).max())
+    return {"workload": f"channel_diva seed 2 ({w.mesh.nV} vertices, {w.mesh.nT} triangles, nz 10)",
+            "max_rel_vel_err_vs_oracle": float(max(np.abs(u - fu).max(), np.abs(v - fv).max()) / sc),
+            "chosen_residual": info.rel_residual, "chosen_residual_unscaled": info.rel_residual_unscaled,
+            "oracle_residual": float(f["rel_residual"]), "stop": "block-scaled nonlinear residual <= 1e-8"}
+
+
+def out_of_cache_roofline(V, W, steps=1):
+    """The Krylov block products with their dot epilogues, timed live (HIP
+    events on the solver's stream) inside chosen-form solves of the
+    continental mesh, whose matrix stream (326 MB per product) exceeds the
+    256 MB Infinity Cache; and the same product launched back to back."""
+    w = W.continental()
+    inp = w.inputs()
+    s = V.VelocitySolver(w.descriptor(), 0, **BASE, **CHOSEN)
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    solve_once(s, "DIVA")
+    s.kernel_timing(True)
+    for _ in range(steps):
+        info = solve_once(s, "DIVA")
+    s.kernel_timing(False)
+    n, ms, by = s.kernel_timing_read(0)
+    spmv_ms, spmv_bytes = s.profile_spmv(100)
+    s.close()
+    rate = by / (ms * 1e-3) / 1e9 if n else 0.0
+    return {"kernel": "k_bsr_spmv<double, OP_DOT1|OP_DOT2> (timed live)", "workload":
+            f"continental_diva_400k ({w.mesh.nV} vertices, {w.mesh.nT} triangles)",
+            "bound": "hbm", "achieved": rate, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rate / HBM_PEAK_GBS,
+            "traffic": read_traffic("bsr_spmv_dot_continental400k_pmc.txt"), "avg_ms": ms, "launches": n,
+            "algorithmic_bytes": by, "solve_seconds": info.seconds,
+            "plain_product_back_to_back": {"avg_ms": spmv_ms, "algorithmic_bytes": spmv_bytes,
+                                           "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9}}
+
+
 def cpu_baseline(w, seconds, n_picard_plain):
     """Oracle (CPU restatement of the reference's linear solve, GMRES(30) with
     left ILU(0), one rank) on a bounded sample of the same workload.  A full
@@ -115,7 +240,7 @@ def cpu_baseline(w, seconds, n_picard_plain):
     r0 = r1 / rate ** i1
     its_needed = math.log(1e-7) / math.log(rate) if rate < 1.0 else float("inf")
     t_solve = per_it * its_needed * n_picard_plain
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = cpu_info()["threads"]
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": cores, "kind": "port",
             "sample": f"oracle GMRES(30)+ILU(0) on the first Picard system ({2 * nT} unknowns): {i1}+{i2} "
                       f"iterations in {t1 + t2:.1f} s, {per_it * 1e3:.1f} ms/iteration, residual rate "
@@ -190,6 +315,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1, help="timed plain-form solves")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the legs outside the metric (out-of-cache roofline, oracle parity at scale, "
+                         "dome CPU/GPU solves)")
     ap.add_argument("--spmv-reps", type=int, default=200)
     ap.add_argument("--nv", type=int, default=200_000, help="target vertices of the channel mesh")
     ap.add_argument("--workload", choices=WORKLOADS, default="channel")
@@ -284,11 +412,13 @@ def main():
                              "rtol 0.01",
                    "plain": "relaxed Picard (reference rule), FP64 AMG-BiCGStab zero guess, rtol 1e-7"},
         "chosen": {"picard": [i.n_visc_its for i in infos], "krylov": [i.n_Axb_its for i in infos],
-                   "residual": [i.rel_residual for i in infos]},
+                   "residual": [i.rel_residual for i in infos],
+                   "residual_unscaled": [i.rel_residual_unscaled for i in infos]},
         "plain": {"value": plain_value, "unit": "solves/s",
                   "ms_per_step": 1e3 * t_plain / max(args.plain_steps, 1),
                   "picard": [i.n_visc_its for i in pinfos], "krylov": [i.n_Axb_its for i in pinfos],
-                  "residual": [i.rel_residual for i in pinfos]},
+                  "residual": [i.rel_residual for i in pinfos],
+                  "residual_unscaled": [i.rel_residual_unscaled for i in pinfos]},
         "speedup_vs_plain": value / plain_value if plain_value else None,
         "autotuned": ([{**{k: getattr(s.params, k) for k in ("accel_depth",)}, **s.amg_options}
                        for _, s in solvers] if args.autotune else None),
@@ -309,9 +439,18 @@ def main():
     }
     if heat is not None:
         out["heat_columns"] = heat
+    if not args.no_extra and args.workload == "channel":
+        log("[bench] out-of-cache roofline (continental mesh)")
+        out["roofline_out_of_cache"] = out_of_cache_roofline(V, W)
+        log("[bench] oracle parity at scale (20k-vertex channel)")
+        out["oracle_parity_at_scale"] = oracle_parity_at_scale(V, W)
     if not args.no_cpu_baseline:
         n_plain = pinfos[0].n_visc_its if pinfos else 231
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, n_plain)
+        out["cpu_baseline"]["cpu"] = cpu_info()
+        if not args.no_extra:
+            log("[bench] full dome solves, CPU oracle and GPU")
+            out["cpu_baseline"]["measured_full_solves"] = dome_full_solves(V, W)
     print(json.dumps(out), flush=True)
 
 

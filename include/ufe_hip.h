@@ -43,7 +43,8 @@ extern "C" {
 #define UFE_PREC_MIXED             1  /* FP32 Krylov inner solves, FP64 refinement residual */
 
 #define UFE_KRYLOV_BICGSTAB        0
-#define UFE_KRYLOV_GMRES           1
+#define UFE_KRYLOV_GMRES           1  /* restarted GMRES(gmres_restart), right-preconditioned (PETSc's default KSP
+                                         is GMRES(30); its default ILU(0) preconditioner is not provided) */
 
 #define UFE_PC_NONE                0
 #define UFE_PC_JACOBI              1  /* point Jacobi */
@@ -106,7 +107,7 @@ typedef struct ufe_params {
   int    precision;                  /* UFE_PREC_*                       FP64   */
   int    krylov;                     /* UFE_KRYLOV_*                     BICGSTAB */
   int    precond;                    /* UFE_PC_*                         BLOCK_JACOBI */
-  int    gmres_restart;              /*                                  30     */
+  int    gmres_restart;              /* GMRES restart length, 1..64      30     */
   int    krylov_maxit;               /* per linear solve                 10000  */
   int    warm_start;                 /* 0: zero initial guess (PETSc default), 1: previous iterate */
   double inner_rtol;                 /* MIXED: smallest residual reduction asked of one FP32 inner solve 1e-3 */
@@ -153,6 +154,8 @@ typedef struct ufe_solve_info {
   double seconds;                    /* wall time of the Picard loop                        */
   int    n_clamped;                  /* triangles at the velocity limit (left out of rel_residual) */
   int    n_fallbacks;                /* MIXED: linear solves redone in FP64 after FP32 refinement stalled */
+  double rel_residual_unscaled;      /* FP64 ||b(u)-A(u)u|| / ||b(u)|| of the assembled (unscaled) system at the
+                                        returned u, over the same rows as rel_residual                 */
 } ufe_solve_info;
 
 typedef struct ufe_ctx ufe_ctx;
@@ -174,6 +177,12 @@ int  ufe_set_params(ufe_ctx *ctx, const ufe_params *p);
  * block rows, run in one workgroup (0..4096, default 300).  Takes effect from
  * the next solve. */
 int  ufe_set_amg_options(ufe_ctx *ctx, double omega, double coarse_scale, int tail_rows);
+/* Launch mode of the Krylov iterations (no reference counterpart): enable != 0
+ * replays chunks of BiCGStab iterations from HIP graphs captured once per
+ * context (one hipGraphLaunch instead of ~50 kernel launches per iteration;
+ * the same kernels in the same order, so results are identical).  Default 0
+ * (the environment variable UFE_GRAPHS=1 turns it on for new contexts). */
+int  ufe_set_graph_launch(ufe_ctx *ctx, int enable);
 
 /* ---- drop-in entry points ---------------------------------------------- */
 
@@ -278,6 +287,13 @@ int ufe_csr_matvec(int m, int n, int nnz, const int *ptr, const int *ind, const 
 int ufe_csr_solve(int n, int nnz, const int *ptr, const int *ind, const double *val,
                   const double *b, double *x, double rtol, double abstol, int maxit,
                   int precond, int *n_its, double *rel_residual);
+/* The same with the Krylov method chosen: krylov = UFE_KRYLOV_BICGSTAB or
+ * UFE_KRYLOV_GMRES with restart length `restart` (1..64; PETSc's default
+ * KSPGMRES uses 30).  Replaces solve_matrix_equation_CSR_PETSc
+ * (petsc_basic.f90:32-64) with the method named. */
+int ufe_csr_solve_ex(int n, int nnz, const int *ptr, const int *ind, const double *val,
+                     const double *b, double *x, double rtol, double abstol, int maxit,
+                     int precond, int krylov, int restart, int *n_its, double *rel_residual);
 
 /* ---- grounded fractions ------------------------------------------------
  * calc_grounded_fractions_bilin_interp_TAF_b
@@ -370,6 +386,73 @@ typedef struct ufe_heat_robin_inputs {
 } ufe_heat_robin_inputs;
 int ufe_heat_robin(ufe_ctx *ctx, const ufe_heat_robin_inputs *in, const int *is_unstable, double *Ti,
                    int *n_replaced);
+
+/* ---- ice thickness: semi-implicit time step ---------------------------
+ * calc_dHi_dt_semiimplicit (conservation_of_mass_semiimplicit.f90:23-159), the
+ * reference's second PETSc call site:
+ *   M    upwind flux-divergence matrix on the Voronoi cells
+ *        (calc_ice_flux_divergence_matrix_upwind,
+ *        conservation_of_mass_utilities.f90:17-110), edge velocities from the
+ *        triangles either side (map_velocities_from_b_to_c_2D,
+ *        map_velocities_to_c_grid.f90:17-69);
+ *   the explicit step (calc_dHi_dt_explicit, conservation_of_mass_explicit.f90)
+ *        for the flux-limited time step and the border values, with the
+ *        'zero' / 'infinite' thickness boundary conditions
+ *        (apply_ice_thickness_BC_explicit);
+ *   (I + dt fs M) H(t+dt) = H - dt (1 - fs) M H + max(-H, dt (fm (SMB + BMB - target) + LMB))
+ *        with border, prescribed-thickness and no-ice rows replaced by
+ *        identity rows, solved on the device with the engine's Krylov solver.
+ * The vertex geometry (the reference's mesh%C, mesh%A, mesh%Cw and the edge
+ * triangles mesh%ETri(mesh%VE)) is attached to a context once. */
+typedef struct ufe_vertex_geometry {
+  int nC;                   /* connections, sum of mesh%nC                              */
+  const int    *C_ptr;      /* [nV+1] neighbours of vertex vi: C_ind[C_ptr[vi] .. C_ptr[vi+1])  */
+  const int    *C_ind;      /* [nC]   mesh%C (counter-clockwise)                         */
+  const int    *VBI;        /* [nV]   vertex border index 0..8                           */
+  const double *A;          /* [nV]   Voronoi cell area                  mesh%A          */
+  const double *Cw;         /* [nC]   shared Voronoi boundary length      mesh%Cw         */
+  const int    *ETri;       /* [nC*2] triangles left and right of the connection's edge, -1 = none */
+} ufe_vertex_geometry;
+
+typedef struct ufe_thickness_params {
+  double fs;                /* dHi_semiimplicit_fs                1.5     */
+  int    bc_H[4];           /* BC_H_{north,east,south,west}       ZERO    */
+  double dt_ice_max;        /* dt_ice_max                         10 yr   */
+  double dt_ice_min;        /* dt_ice_min                         0.1 yr  */
+  double Hi_min;            /* Hi_min                             0 m     */
+  double rtol;              /* dHi_PETSc_rtol                     1e-8    */
+  double abstol;            /* dHi_PETSc_abstol                   1e-6    */
+  int    krylov;            /* UFE_KRYLOV_*                       GMRES   */
+  int    gmres_restart;     /*                                    30      */
+  int    maxit;             /*                                    10000   */
+} ufe_thickness_params;
+
+/* Per-vertex fields [nV] unless stated; velocities [nT] on the triangles.
+ * u_vav_b = v_vav_b = NULL takes the context's current velocity (the last
+ * velocity solve's result, still on the device).  mask_noice and
+ * BC_prescr_mask are 0/1; BC_prescr_mask and BC_prescr_Hi both or neither.
+ * dHi_dt_guess: the initial guess of the linear solve is Hi + dt dHi_dt_guess
+ * (the reference passes the previous time step's dH/dt); NULL = Hi. */
+typedef struct ufe_thickness_inputs {
+  const double *Hi, *Hb, *SL;
+  const double *u_vav_b, *v_vav_b;
+  const double *SMB, *BMB, *LMB, *fraction_margin, *dHi_dt_target;
+  const int    *mask_noice;
+  const int    *BC_prescr_mask;
+  const double *BC_prescr_Hi;
+  const double *dHi_dt_guess;
+  double dt;
+} ufe_thickness_inputs;
+
+void ufe_thickness_params_default(ufe_thickness_params *p);
+/* Attach the vertex geometry of the context's mesh (checked and kept on the
+ * device; a second call replaces it). */
+int ufe_thickness_setup(ufe_ctx *ctx, const ufe_vertex_geometry *g);
+/* One semi-implicit step.  Outputs [nV]: Hi_tplusdt, dHi_dt = (Hi_tplusdt - Hi)/dt,
+ * divQ = M Hi (m/yr); *dt_max: the explicit scheme's flux-limited time step
+ * (informational: the semi-implicit step uses dt); *n_its Krylov iterations. */
+int ufe_thickness_semiimplicit(ufe_ctx *ctx, const ufe_thickness_params *p, const ufe_thickness_inputs *in,
+                               double *Hi_tplusdt, double *dHi_dt, double *divQ, double *dt_max, int *n_its);
 
 /* ---- mesh operator construction (host, once per mesh) -----------------
  * calc_matrix_operators_mesh_a_b / _b_a / _b_b_2nd_order

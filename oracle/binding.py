@@ -55,6 +55,9 @@ def lib():
     o.oracle_fraction_gr_b.argtypes = [C.POINTER(L.MeshDesc), dp, dp, dp, dp]
     o.oracle_heat_columns.argtypes = [C.POINTER(L.HeatInputs), dp, ip, ip]
     o.oracle_heat_robin.argtypes = [C.POINTER(L.HeatRobinInputs), ip, dp, ip]
+    o.oracle_thickness_semiimplicit.argtypes = [C.c_int, dp, C.POINTER(L.VertexGeometry),
+                                                C.POINTER(L.ThicknessParams), C.POINTER(L.ThicknessInputs),
+                                                dp, dp, dp, dp, ip]
     o.oracle_zeta_tridiagonals.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
     _o = o
     return o
@@ -205,3 +208,27 @@ def heat_columns(hin):
     T = np.zeros(n); unst = np.zeros(hin.nV, np.int32); nun = C.c_int(0)
     rc = lib().oracle_heat_columns(C.byref(hin), L.ptr(T), L.ptr(unst, L.ip), C.byref(nun))
     return rc, T, unst, nun.value
+
+
+def thickness_semiimplicit(mesh, geom, Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, LMB, fraction_margin, mask_noice,
+                           dt, dHi_dt_target=None, BC_prescr_mask=None, BC_prescr_Hi=None, dHi_dt_guess=None,
+                           **params):
+    """One semi-implicit thickness step of the oracle, same arguments and
+    returns as VelocitySolver.calc_dHi_dt_semiimplicit (velocities required):
+    (rc, dHi_dt, Hi_tplusdt, divQ, dt_max, n_its)."""
+    L = _L()
+    from ufemism2_amd import velocity as V
+    nV = mesh.nV
+    p = V.thickness_params(**params)
+    inp, _keep = V.make_thickness_inputs(nV, Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, LMB, fraction_margin,
+                                         mask_noice, dt, dHi_dt_target, BC_prescr_mask, BC_prescr_Hi,
+                                         dHi_dt_guess)
+    keep = (L.i32(mesh.C_ptr), L.i32(mesh.C_ind), L.i32(mesh.VBI), L.f64(geom.A), L.f64(geom.Cw),
+            L.i32(geom.ETri), L.f64(mesh.V))
+    g = L.VertexGeometry(int(keep[0][-1]), L.ptr(keep[0], L.ip), L.ptr(keep[1], L.ip), L.ptr(keep[2], L.ip),
+                         L.ptr(keep[3]), L.ptr(keep[4]), L.ptr(keep[5], L.ip))
+    Hn, rate, divQ = np.zeros(nV), np.zeros(nV), np.zeros(nV)
+    dt_max, its = C.c_double(), C.c_int()
+    rc = lib().oracle_thickness_semiimplicit(nV, L.ptr(keep[6]), C.byref(g), C.byref(p), C.byref(inp), L.ptr(Hn),
+                                             L.ptr(rate), L.ptr(divQ), C.byref(dt_max), C.byref(its))
+    return rc, rate, Hn, divQ, dt_max.value, its.value

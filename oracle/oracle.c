@@ -541,11 +541,11 @@ static void picard_assemble(picard_t *s) {
  * is not active.  Scaling each block row by its diagonal block makes every row a
  * velocity equation, so momentum rows and boundary rows are on one scale. */
 static double scaled_rel_residual(int nT, const int *Aptr, const int *Aind, const double *Aval,
-                                  const double *bvec, const double *x, double vel_max) {
+                                  const double *bvec, const double *x, double vel_max, double *unscaled) {
   int n = 2 * nT;
   double *r = malloc(sizeof(double) * n);
   oracle_csr_matvec(n, Aptr, Aind, Aval, x, r);
-  double rr = 0, b2 = 0;
+  double rr = 0, b2 = 0, ru = 0, bu = 0;
   for (int ti = 0; ti < nT; ++ti) {
     double u = x[2 * ti], v = x[2 * ti + 1];
     if (vel_max > 0 && sqrt(u * u + v * v) >= vel_max * (1.0 - 1e-9)) continue;
@@ -563,14 +563,18 @@ static double scaled_rel_residual(int nT, const int *Aptr, const int *Aind, cons
     double c0 = i0 * bvec[2 * ti] + i1 * bvec[2 * ti + 1], c1 = i2 * bvec[2 * ti] + i3 * bvec[2 * ti + 1];
     rr += s0 * s0 + s1 * s1;
     b2 += c0 * c0 + c1 * c1;
+    ru += r0 * r0 + r1 * r1;
+    bu += bvec[2 * ti] * bvec[2 * ti] + bvec[2 * ti + 1] * bvec[2 * ti + 1];
   }
   free(r);
+  /* the unscaled ||b - A u|| / ||b|| over the same rows */
+  if (unscaled) *unscaled = bu > 0 ? sqrt(ru / bu) : sqrt(ru);
   return b2 > 0 ? sqrt(rr / b2) : sqrt(rr);
 }
 
-static double rel_residual(picard_t *s, const double *u, const double *v) {
+static double rel_residual(picard_t *s, const double *u, const double *v, double *unscaled) {
   for (int ti = 0; ti < s->m->nT; ++ti) { s->xx[2 * ti] = u[ti]; s->xx[2 * ti + 1] = v[ti]; }
-  return scaled_rel_residual(s->m->nT, s->Aptr, s->Aind, s->Aval, s->bb, s->xx, s->p->vel_max);
+  return scaled_rel_residual(s->m->nT, s->Aptr, s->Aind, s->Aval, s->bb, s->xx, s->p->vel_max, unscaled);
 }
 
 static int picard_alloc(picard_t *s) {
@@ -614,13 +618,13 @@ static int picard(picard_t *s, double *u, double *v, double *eta3b, double *tbx,
   double relax = p->visc_it_relax, eps0 = p->epsilon_sq_0;
   int n_axb = 0, it = 0, converged = 0, rc = 0;
   double *up = malloc(sizeof(double) * nT), *vp = malloc(sizeof(double) * nT);
-  double resid = -1.0;
+  double resid = -1.0, resid_u = -1.0;
   while (!converged) {
     ++it;
     picard_state(s, u, v, eps0, eta3b, tbx, tby, ubase, vbase);
     picard_assemble(s);
     if (p->stop_rule == UFE_STOP_RESIDUAL) {
-      resid = rel_residual(s, u, v);
+      resid = rel_residual(s, u, v, &resid_u);
       if (resid <= p->nonlinear_rtol) { --it; converged = 1; break; }
     }
     /* solve_SSA_DIVA_linearised: keep previous solution, solve, split u/v */
@@ -693,7 +697,7 @@ static int picard(picard_t *s, double *u, double *v, double *eta3b, double *tbx,
       }
       picard_state(s, u, v, eps0, eta3b, tbx, tby, ubase, vbase);
       picard_assemble(s);
-      resid = rel_residual(s, u, v);
+      resid = rel_residual(s, u, v, &resid_u);
       if (s->is_diva) {
         memcpy(eta3b, keep_e, sizeof(double) * n3);
         memcpy(s->F1b, keep_f, sizeof(double) * n3);
@@ -706,6 +710,7 @@ static int picard(picard_t *s, double *u, double *v, double *eta3b, double *tbx,
     info->n_refinements = 0;
     info->L2_uv = L2;
     info->rel_residual = resid;
+    info->rel_residual_unscaled = resid_u;
     info->visc_it_relax_applied = relax;
     info->epsilon_sq_0_applied = eps0;
     info->seconds = now_s() - t0;
@@ -808,7 +813,7 @@ double oracle_ssa_residual(const ufe_mesh_desc *m, const ufe_params *p, const do
     if (b_out) b_out[i] = s.bb[i];
   }
   free(r);
-  double rel = scaled_rel_residual(m->nT, s.Aptr, s.Aind, s.Aval, s.bb, s.xx, p->vel_max);
+  double rel = scaled_rel_residual(m->nT, s.Aptr, s.Aind, s.Aval, s.bb, s.xx, p->vel_max, NULL);
   (void)rr; (void)b2;
   picard_free(&s);
   return rel;
@@ -849,7 +854,7 @@ double oracle_diva_residual(const ufe_mesh_desc *m, const ufe_params *p, const d
   if (F2s_out) memcpy(F2s_out, s.F2b, sizeof(double) * nT); /* layer k = 1 (surface) */
   if (beta_b_out) memcpy(beta_b_out, s.beta_b, sizeof(double) * nT);
   if (beta_eff_b_out) memcpy(beta_eff_b_out, s.beta_eff_b, sizeof(double) * nT);
-  double rel = scaled_rel_residual(nT, s.Aptr, s.Aind, s.Aval, s.bb, s.xx, p->vel_max);
+  double rel = scaled_rel_residual(nT, s.Aptr, s.Aind, s.Aval, s.bb, s.xx, p->vel_max, NULL);
   picard_free(&s);
   return rel;
 }
@@ -1110,4 +1115,157 @@ int oracle_heat_columns(const ufe_heat_inputs *in, double *Tnew, int *is_unstabl
   }
   *n_unstable = nun;
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Ice thickness, semi-implicit step (conservation_of_mass_semiimplicit.f90) */
+/* ---------------------------------------------------------------------- */
+
+/* ice_surface_elevation, Hi_from_Hb_Hs_and_SL (ice_geometry_basics.f90:28-82) */
+static double surface_elevation(double Hi, double Hb, double SL) {
+  return Hi + fmax(SL - ICE_DENSITY / SEAWATER_DENSITY * Hi, Hb);
+}
+static double thickness_from_surface(double Hb, double Hs, double SL) {
+  double Hi_float = fmax(0.0, (SL - Hb) * (SEAWATER_DENSITY / ICE_DENSITY));
+  double Hs_float = Hb + Hi_float;
+  if (Hs > Hs_float) return Hs - Hb;
+  return fmin(Hi_float, (Hs - SL) / (1.0 - (ICE_DENSITY / SEAWATER_DENSITY)));
+}
+
+static int border_side(int vbi) {  /* 1,2 north; 3,4 east; 5,6 south; 7,8 west */
+  return vbi > 0 ? (vbi - 1) / 2 : -1;
+}
+
+/* apply_ice_thickness_BC_explicit (conservation_of_mass_explicit.f90): 'zero'
+ * borders get no ice; 'infinite' borders take the mean surface elevation of
+ * their interior neighbours (vertices not on the border and not in the
+ * no-ice mask), and border vertices without such neighbours, in a second
+ * pass, the mean surface elevation of all their neighbours. */
+static void thickness_bc_explicit(int nV, const double *Vx, const ufe_vertex_geometry *g, const int *bcH,
+                                  const int *mask_noice, const double *Hb, const double *SL, double *Hi) {
+  (void)Vx;
+  double *Hs = malloc(sizeof(double) * nV);
+  int *nint = malloc(sizeof(int) * nV);
+  for (int vi = 0; vi < nV; ++vi) {
+    Hs[vi] = surface_elevation(Hi[vi], Hb[vi], SL[vi]);
+    nint[vi] = 0;
+    for (int c = g->C_ptr[vi]; c < g->C_ptr[vi + 1]; ++c) {
+      int vj = g->C_ind[c];
+      if (g->VBI[vj] == 0 && !mask_noice[vj]) ++nint[vi];
+    }
+  }
+  /* each pass reads the surface elevations as they were when it started
+   * (the reference gathers them to all processes before each pass) */
+  double *Hs0 = malloc(sizeof(double) * nV);
+  for (int pass = 0; pass < 2; ++pass) {
+    memcpy(Hs0, Hs, sizeof(double) * nV);
+    for (int vi = 0; vi < nV; ++vi) {
+      int side = border_side(g->VBI[vi]);
+      if (side < 0) continue;
+      if (bcH[side] == UFE_BC_ZERO) { Hi[vi] = 0.0; continue; }
+      if ((pass == 0) != (nint[vi] > 0)) continue;
+      double sum = 0.0;
+      int n = 0;
+      for (int c = g->C_ptr[vi]; c < g->C_ptr[vi + 1]; ++c) {
+        int vj = g->C_ind[c];
+        if (pass == 1 || (g->VBI[vj] == 0 && !mask_noice[vj])) { sum += Hs0[vj]; ++n; }
+      }
+      double Hs_av = sum / (double)(pass == 0 ? nint[vi] : n);
+      Hs[vi] = fmax(Hb[vi], Hs_av);
+      Hi[vi] = thickness_from_surface(Hb[vi], Hs[vi], SL[vi]);
+    }
+  }
+  free(Hs0);
+  free(Hs);
+  free(nint);
+}
+
+int oracle_thickness_semiimplicit(int nV, const double *V, const ufe_vertex_geometry *g,
+                                  const ufe_thickness_params *p, const ufe_thickness_inputs *in,
+                                  double *Hi_next, double *dHi_dt, double *divQ, double *dt_max_out,
+                                  int *n_its) {
+  const int nC = g->nC;
+  const double dt = in->dt;
+  /* calc_ice_flux_divergence_matrix_upwind (conservation_of_mass_utilities.f90):
+   * row vi = [diag, neighbours in C order] */
+  int *ptr = malloc(sizeof(int) * (nV + 1));
+  int *ind = malloc(sizeof(int) * (nV + nC));
+  double *M = malloc(sizeof(double) * (nV + nC));
+  ptr[0] = 0;
+  for (int vi = 0; vi < nV; ++vi) {
+    int k0 = ptr[vi];
+    double diag = 0.0;
+    ind[k0] = vi;
+    for (int c = g->C_ptr[vi]; c < g->C_ptr[vi + 1]; ++c) {
+      int vj = g->C_ind[c], k = k0 + 1 + (c - g->C_ptr[vi]);
+      /* map_velocities_from_b_to_c_2D: mean of the triangles either side */
+      int tl = g->ETri[2 * c], tr = g->ETri[2 * c + 1];
+      double uc, vc;
+      if (tl < 0) { uc = in->u_vav_b[tr]; vc = in->v_vav_b[tr]; }
+      else if (tr < 0) { uc = in->u_vav_b[tl]; vc = in->v_vav_b[tl]; }
+      else { uc = (in->u_vav_b[tl] + in->u_vav_b[tr]) / 2.0; vc = (in->v_vav_b[tl] + in->v_vav_b[tr]) / 2.0; }
+      double Dx = V[2 * vj] - V[2 * vi], Dy = V[2 * vj + 1] - V[2 * vi + 1];
+      double D = sqrt(Dx * Dx + Dy * Dy);
+      double u_perp = uc * Dx / D + vc * Dy / D;
+      double A_i = g->A[vi], L_c = g->Cw[c];
+      if (in->fraction_margin[vi] >= 1.0) diag = diag + L_c * fmax(0.0, u_perp) / A_i;
+      ind[k] = vj;
+      M[k] = in->fraction_margin[vj] >= 1.0 ? L_c * fmin(0.0, u_perp) / A_i : 0.0;
+    }
+    M[k0] = diag;
+    ptr[vi + 1] = k0 + 1 + (g->C_ptr[vi + 1] - g->C_ptr[vi]);
+  }
+  oracle_csr_matvec(nV, ptr, ind, M, in->Hi, divQ);
+  /* calc_dHi_dt_explicit: rates, flux-limited time step
+   * (calc_flux_limited_timestep restated as written: a thinning vertex's
+   * limit is Hi / max(dHi_dt, 1e-9), with dHi_dt < 0 there), explicit
+   * thickness and its boundary conditions */
+  double *Hex = malloc(sizeof(double) * nV), *rate = malloc(sizeof(double) * nV);
+  double dt_max = p->dt_ice_max;
+  for (int vi = 0; vi < nV; ++vi) {
+    double m = in->fraction_margin[vi] * (in->SMB[vi] + in->BMB[vi] - in->dHi_dt_target[vi]) + in->LMB[vi];
+    rate[vi] = -divQ[vi] + m;
+    double lim = p->dt_ice_max;
+    if (in->Hi[vi] > p->Hi_min && rate[vi] < 0.0) lim = in->Hi[vi] / fmax(rate[vi], 1e-9);
+    dt_max = fmin(dt_max, lim);
+  }
+  dt_max = fmax(p->dt_ice_min, dt_max);
+  double dt_ex = fmin(dt, dt_max);
+  for (int vi = 0; vi < nV; ++vi) Hex[vi] = fmax(0.0, in->Hi[vi] + rate[vi] * dt_ex);
+  thickness_bc_explicit(nV, V, g, p->bc_H, in->mask_noice, in->Hb, in->SL, Hex);
+  for (int vi = 0; vi < nV; ++vi) {
+    if (in->BC_prescr_mask && in->BC_prescr_mask[vi] == 1) Hex[vi] = fmax(0.0, in->BC_prescr_Hi[vi]);
+    if (in->mask_noice[vi]) Hex[vi] = 0.0;
+  }
+  /* the matrix boundary conditions apply the explicit ones again
+   * (apply_ice_thickness_BC_matrix_domain_border) */
+  thickness_bc_explicit(nV, V, g, p->bc_H, in->mask_noice, in->Hb, in->SL, Hex);
+  /* stiffness matrix and load vector, calc_dHi_dt_semiimplicit :112-140 */
+  double *A = malloc(sizeof(double) * (nV + nC)), *bb = malloc(sizeof(double) * nV);
+  for (int vi = 0; vi < nV; ++vi) {
+    for (int k = ptr[vi]; k < ptr[vi + 1]; ++k) A[k] = M[k] * dt * p->fs;
+    A[ptr[vi]] += 1.0;
+    double m = in->fraction_margin[vi] * (in->SMB[vi] + in->BMB[vi] - in->dHi_dt_target[vi]) + in->LMB[vi];
+    bb[vi] = in->Hi[vi] - (dt * (1.0 - p->fs) * divQ[vi]) + fmax(-1.0 * in->Hi[vi], dt * m);
+    Hi_next[vi] = in->Hi[vi] + dt * (in->dHi_dt_guess ? in->dHi_dt_guess[vi] : 0.0);
+  }
+  /* border, prescribed and no-ice rows (:165-315), in that order */
+  for (int vi = 0; vi < nV; ++vi) {
+    int fixed = 0;
+    double val = 0.0;
+    if (g->VBI[vi] > 0) { fixed = 1; val = Hex[vi]; }
+    if (in->BC_prescr_mask && in->BC_prescr_mask[vi] == 1) { fixed = 1; val = in->BC_prescr_Hi[vi]; }
+    if (in->mask_noice[vi]) { fixed = 1; val = 0.0; }
+    if (!fixed) continue;
+    for (int k = ptr[vi]; k < ptr[vi + 1]; ++k) A[k] = 0.0;
+    A[ptr[vi]] = 1.0;
+    bb[vi] = val;
+    Hi_next[vi] = val;
+  }
+  int its = oracle_gmres_ilu0(nV, ptr, ind, A, bb, Hi_next, p->rtol, p->abstol, p->maxit, p->gmres_restart, 0, NULL);
+  for (int vi = 0; vi < nV; ++vi) dHi_dt[vi] = (Hi_next[vi] - in->Hi[vi]) / dt;
+  if (dt_max_out) *dt_max_out = dt_max;
+  if (n_its) *n_its = its;
+  free(ptr); free(ind); free(M); free(Hex); free(rate); free(A); free(bb);
+  return its < 0 ? UFE_ERR_ARGUMENT : 0;
 }

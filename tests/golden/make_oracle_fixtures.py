@@ -7,7 +7,16 @@ with the fixture guards against a different mesh.
 oracle_channel_seeds.npz holds the same solves for the channel's random-field
 seeds 3..6 (velocities rounded to float32, ample for a 1e-5 comparison).
 
-Run: python tests/golden/make_oracle_fixtures.py [main|seeds]   (both by default)
+oracle_channel_diva_20k.npz, oracle_ensemble_members.npz and
+oracle_continental_small.npz ("scale") hold converged oracle solutions (the
+reference iteration, nonlinear residual 1e-8) of larger meshes: the DIVA
+channel at ~20k vertices, three members of the ensemble generator at
+~12k vertices and a continental-generator mesh at ~10k vertices, with the
+oracle's wall time and thread count (the measured CPU solves against which the
+benchmark's CPU-time model is checked).  These take tens of minutes each.
+
+Run: python tests/golden/make_oracle_fixtures.py [main|seeds|scale:<name>]
+     (main and seeds by default; scale:channel20k, scale:members, scale:continental)
 """
 
 import os
@@ -82,9 +91,47 @@ def main():
         print(sb, w.mesh.nV, w.mesh.nT, info.n_visc_its, info.rel_residual, "->", path, flush=True)
 
 
+def timed_solve(w):
+    import time
+    t0 = time.perf_counter()
+    rc, u, v, info, st = solve(w, "DIVA")
+    t = time.perf_counter() - t0
+    assert rc == 0, (w.name, rc)
+    meta = dict(digest=w.mesh.digest(), nV=w.mesh.nV, nT=w.mesh.nT, n_visc_its=info.n_visc_its,
+                n_Axb_its=info.n_Axb_its, rel_residual=info.rel_residual,
+                rel_residual_unscaled=info.rel_residual_unscaled, seconds=t,
+                threads=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    print(w.name, meta, flush=True)
+    return u.astype(np.float32), v.astype(np.float32), meta
+
+
+def scale(name):
+    if name == "channel20k":
+        w = W.channel(seed=2, target_nV=20000, nz=10, stress_balance="DIVA")
+        u, v, meta = timed_solve(w)
+        np.savez_compressed(os.path.join(HERE, "oracle_channel_diva_20k.npz"), u=u, v=v, **meta)
+    elif name == "members":
+        out = {}
+        for k in range(3):
+            w = W.ensemble_member(k, nV_range=(10000, 14000))
+            u, v, meta = timed_solve(w)
+            out.update({f"m{k}_u": u, f"m{k}_v": v, **{f"m{k}_{a}": b for a, b in meta.items()}})
+        np.savez_compressed(os.path.join(HERE, "oracle_ensemble_members.npz"), members=np.arange(3),
+                            nV_range=np.array([10000, 14000]), **out)
+    elif name == "continental":
+        w = W.continental(target_nV=10000)
+        u, v, meta = timed_solve(w)
+        np.savez_compressed(os.path.join(HERE, "oracle_continental_small.npz"), u=u, v=v, target_nV=10000, **meta)
+    else:
+        raise SystemExit(f"unknown scale case {name}")
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["main", "seeds"]
     if "main" in what:
         main()
     if "seeds" in what:
         seeds()
+    for a in what:
+        if a.startswith("scale:"):
+            scale(a.split(":", 1)[1])

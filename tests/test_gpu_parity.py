@@ -176,6 +176,25 @@ def test_dome_ssa_reference_rule_matches_oracle(V, OB, L, dome):
     rc, uo, vo, io = OB.Oracle(dome.descriptor()).solve_ssa(params(L, **kw), dome.inputs())
     assert rc == 0 and info.n_visc_its == io.n_visc_its
     assert max_rel(st["u_vav_b"], uo) < 1e-6 and max_rel(st["v_vav_b"], vo) < 1e-6
+    # both residual norms at the returned iterate: block-scaled and unscaled ||b - A u|| / ||b||
+    assert abs(info.rel_residual / io.rel_residual - 1.0) < 1e-3
+    assert abs(info.rel_residual_unscaled / io.rel_residual_unscaled - 1.0) < 1e-3
+
+
+def test_dome_configuration_1_matches_oracle(V, OB, L, W):
+    """Configuration 1 as stated: the 40 km dome (2,205 vertices), Glen n = 3,
+    A = 1e-16, Weertman m = 3 with beta_sq = 1e4, Picard stop L2_uv < 1e-3,
+    Krylov rtol 1e-8, the reference relaxation, FP64: the device runs the
+    oracle's iteration (same Picard count) and the velocities agree to the
+    linear-solve tolerance."""
+    w = W.dome(h=40e3)
+    assert 2000 < w.mesh.nV < 3000
+    kw = dict(w.params, abstol=0.0, visc_it_nit=300)
+    assert kw["visc_it_norm_dUV_tol"] == 1e-3 and kw["rtol"] == 1e-8
+    st, info = solve(V, w, **kw)
+    rc, uo, vo, io = OB.Oracle(w.descriptor()).solve_ssa(params(L, **kw), w.inputs())
+    assert rc == 0 and info.n_visc_its == io.n_visc_its
+    assert max_rel(st["u_vav_b"], uo) < 1e-5 and max_rel(st["v_vav_b"], vo) < 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -254,6 +273,66 @@ def test_amg_fused_variants_equal_per_level_kernels(V, channel_fixtures, monkeyp
     b, ib = solve(V, w, **TIGHT, **CHOSEN)
     assert ia.n_visc_its == ib.n_visc_its and ia.n_Axb_its == ib.n_Axb_its
     assert np.array_equal(a["u_vav_b"], b["u_vav_b"]) and np.array_equal(a["v_vav_b"], b["v_vav_b"])
+
+
+@pytest.mark.parametrize("nz", [10, 12])
+def test_diva_column_kernels_equal_per_thread_kernels(V, W, monkeypatch, nz):
+    """The DIVA vertex and triangle kernels with 16 lanes per column (the
+    row's gathers issued together, the sums in row order) and the
+    one-lane-per-column kernels (UFE_DIVA_COLUMNS=0, read when a context is
+    created) evaluate the same expressions; the compiler fuses their products
+    and sums into different FMAs, so they agree to round-off per evaluation
+    (each is held to 1e-12 against the oracle at full size in
+    test_gpu_diva.py) and the converged solves reach the same solution."""
+    w = W.channel(seed=2, target_nV=1500, nz=nz, stress_balance="DIVA")
+    a, ia = solve(V, w, **TIGHT, **CHOSEN)
+    monkeypatch.setenv("UFE_DIVA_COLUMNS", "0")
+    b, ib = solve(V, w, **TIGHT, **CHOSEN)
+    assert ia.rel_residual <= 1e-8 and ib.rel_residual <= 1e-8
+    for k in ("u_vav_b", "v_vav_b", "u_3D_b"):
+        assert max_rel(a[k], b[k]) < 1e-6, k
+    # the viscosity and basal stress follow the strain rates, one derivative more sensitive
+    for k in ("eta_3D_b", "tau_bx_b"):
+        assert max_rel(a[k], b[k]) < 1e-5, k
+
+
+def _solve_on(s, w):
+    inp = w.inputs()
+    s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    s.set_velocity(None, None)
+    if w.stress_balance == "DIVA":
+        s.reset_diva_state()
+    info = s.run(w.stress_balance)
+    return s.get_velocity(), info
+
+
+GRAPH_FORMS = {"chosen": dict(TIGHT, **CHOSEN), "plain": dict(TIGHT, **PLAIN),
+               "block-jacobi": dict(TIGHT, precision=0, precond=2, warm_start=0, rtol=1e-9, visc_it_nit=2000)}
+
+
+# block-Jacobi BiCGStab without warm starts does not bring the DIVA channel's
+# Picard iteration to 1e-8, so that preconditioner is covered on SSA only
+@pytest.mark.parametrize("sb,form", [("SSA", "chosen"), ("SSA", "plain"), ("SSA", "block-jacobi"),
+                                     ("DIVA", "chosen"), ("DIVA", "plain")])
+def test_graph_launch_equals_eager(V, channel_fixtures, sb, form):
+    """Chunks of BiCGStab iterations replayed from captured HIP graphs run the
+    same kernels in the same order as eager launches: three solves in a row on
+    one context (later ones replay the graphs captured in the first, with
+    eager work, host polls and the next Picard system's assembly between the
+    replays) give the eager solve's iteration counts and velocities bit for
+    bit."""
+    w, _ = channel_fixtures[sb]
+    kw = GRAPH_FORMS[form]
+    e = V.VelocitySolver(w.descriptor(), **kw)
+    (ue, ve), ie = _solve_on(e, w)
+    e.close()
+    g = V.VelocitySolver(w.descriptor(), **kw)
+    g.set_graph_launch(True)
+    for rep in range(3):
+        (ug, vg), ig = _solve_on(g, w)
+        assert (ig.n_visc_its, ig.n_Axb_its) == (ie.n_visc_its, ie.n_Axb_its), rep
+        assert np.array_equal(ug, ue) and np.array_equal(vg, ve), rep
+    g.close()
 
 
 def test_live_kernel_timing_counts_every_krylov_product(V, channel_fixtures):
@@ -463,3 +542,64 @@ def test_heat_steady_profile_fixed_point_gpu(V):
     hin, keep = V.make_heat_inputs(nV, nz, f["zeta"], {k: v for k, v in f.items() if k != "zeta"}, 100.0)
     T, unst, nun = V.heat_columns(hin)
     assert nun == 0 and np.abs(T.reshape(nz, nV) - Ti).max() < 1e-9 * 273
+
+
+# -- configurations beyond the small channel: oracle solutions at larger sizes ---------
+
+SCALE = os.path.join(HERE, "golden")
+
+
+def _chosen_vs_fixture(V, w, fu, fv, rtol=1e-8):
+    st, info = solve(V, w, **dict(TIGHT, nonlinear_rtol=rtol), **CHOSEN)
+    assert info.rel_residual <= rtol
+    fu, fv = fu.astype(np.float64), fv.astype(np.float64)
+    sc = max(np.abs(fu).max(), np.abs(fv).max())
+    return max(np.abs(st["u_vav_b"] - fu).max(), np.abs(st["v_vav_b"] - fv).max()) / sc, info
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(SCALE, "oracle_channel_diva_20k.npz")),
+                    reason="oracle_channel_diva_20k.npz not generated")
+def test_channel_20k_chosen_form_matches_oracle(V, W):
+    """The chosen form (the benchmark's method) on a 20k-vertex DIVA channel
+    against the oracle's converged reference iteration: <= 1e-5."""
+    f = np.load(os.path.join(SCALE, "oracle_channel_diva_20k.npz"))
+    w = W.channel(seed=2, target_nV=20000, nz=10, stress_balance="DIVA")
+    assert str(f["digest"]) == w.mesh.digest()
+    err, info = _chosen_vs_fixture(V, w, f["u"], f["v"])
+    print(f"channel 20k: picard {info.n_visc_its} krylov {info.n_Axb_its} err {err:.1e}", flush=True)
+    assert err < 1e-5, err
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(SCALE, "oracle_continental_small.npz")),
+                    reason="oracle_continental_small.npz not generated")
+def test_continental_generator_chosen_form_matches_oracle(V, W):
+    """Configuration 3's generator (fractal bed, plastic profile) at ~23k
+    vertices: the chosen form reaches the oracle's converged reference
+    iteration (<= 1e-5)."""
+    f = np.load(os.path.join(SCALE, "oracle_continental_small.npz"))
+    w = W.continental(target_nV=int(f["target_nV"]))
+    assert str(f["digest"]) == w.mesh.digest()
+    err, info = _chosen_vs_fixture(V, w, f["u"], f["v"])
+    print(f"continental {w.mesh.nV}: picard {info.n_visc_its} krylov {info.n_Axb_its} err {err:.1e}", flush=True)
+    assert err < 1e-5, err
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(SCALE, "oracle_ensemble_members.npz")),
+                    reason="oracle_ensemble_members.npz not generated")
+def test_ensemble_members_match_oracle(V, W):
+    """Configuration 4's generator (channel members with their own A, beta_sq
+    and resolution): three members solved on one device, each by its own
+    context on its own stream from concurrent host threads (the benchmark's
+    ensemble mode), against the oracle's converged solutions (<= 1e-5)."""
+    import concurrent.futures as cf
+    f = np.load(os.path.join(SCALE, "oracle_ensemble_members.npz"))
+    lo, hi = (int(x) for x in f["nV_range"])
+    ws = [W.ensemble_member(int(k), nV_range=(lo, hi)) for k in f["members"]]
+    for k, w in zip(f["members"], ws):
+        assert str(f[f"m{k}_digest"]) == w.mesh.digest()
+    with cf.ThreadPoolExecutor(max_workers=len(ws)) as ex:
+        res = list(ex.map(lambda kw: _chosen_vs_fixture(V, kw[1], f[f"m{kw[0]}_u"], f[f"m{kw[0]}_v"]),
+                          zip(f["members"], ws)))
+    for k, (err, info) in zip(f["members"], res):
+        print(f"member {k}: picard {info.n_visc_its} krylov {info.n_Axb_its} err {err:.1e}", flush=True)
+        assert err < 1e-5, (k, err)

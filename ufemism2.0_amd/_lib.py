@@ -74,6 +74,7 @@ class SolveInfo(C.Structure):
         ("L2_uv", C.c_double), ("rel_residual", C.c_double),
         ("visc_it_relax_applied", C.c_double), ("epsilon_sq_0_applied", C.c_double),
         ("seconds", C.c_double), ("n_clamped", C.c_int), ("n_fallbacks", C.c_int),
+        ("rel_residual_unscaled", C.c_double),
     ]
 
     def as_dict(self):
@@ -94,6 +95,27 @@ class HeatRobinInputs(C.Structure):
     _fields_ = [
         ("nV", C.c_int), ("nz", C.c_int), ("zeta", dp), ("Hi_eff", dp), ("T_surf", dp), ("SMB", dp),
         ("geothermal_flux", dp), ("mask", ip), ("Hi_min_thermo", C.c_double),
+    ]
+
+
+class VertexGeometry(C.Structure):
+    _fields_ = [("nC", C.c_int), ("C_ptr", ip), ("C_ind", ip), ("VBI", ip), ("A", dp), ("Cw", dp),
+                ("ETri", ip)]
+
+
+class ThicknessParams(C.Structure):
+    _fields_ = [
+        ("fs", C.c_double), ("bc_H", C.c_int * 4), ("dt_ice_max", C.c_double), ("dt_ice_min", C.c_double),
+        ("Hi_min", C.c_double), ("rtol", C.c_double), ("abstol", C.c_double), ("krylov", C.c_int),
+        ("gmres_restart", C.c_int), ("maxit", C.c_int),
+    ]
+
+
+class ThicknessInputs(C.Structure):
+    _fields_ = [
+        ("Hi", dp), ("Hb", dp), ("SL", dp), ("u_vav_b", dp), ("v_vav_b", dp), ("SMB", dp), ("BMB", dp),
+        ("LMB", dp), ("fraction_margin", dp), ("dHi_dt_target", dp), ("mask_noice", ip),
+        ("BC_prescr_mask", ip), ("BC_prescr_Hi", dp), ("dHi_dt_guess", dp), ("dt", C.c_double),
     ]
 
 
@@ -169,12 +191,15 @@ def lib():
     _sig(L, 'ufe_mpnn_train', argtypes=[C.c_void_p, C.c_int, fp, ip, fp, C.c_int, C.c_float, fp])
     _sig(L, 'ufe_mpnn_features', argtypes=[C.c_void_p, fp])
     _sig(L, 'ufe_mpnn_initial_guess', argtypes=[C.c_void_p, C.c_void_p])
+    _sig(L, 'ufe_set_graph_launch', argtypes=[C.c_void_p, C.c_int])
     _sig(L, 'ufe_ssa_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_diva_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
                                            dp])
     _sig(L, 'ufe_csr_matvec', argtypes=[C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_csr_solve', argtypes=[C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double,
                                 C.c_int, C.c_int, ip, dp])
+    _sig(L, 'ufe_csr_solve_ex', argtypes=[C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_double,
+                                          C.c_int, C.c_int, C.c_int, C.c_int, ip, dp])
     _sig(L, 'ufe_fraction_gr_b', argtypes=[C.c_void_p, dp, dp, dp, dp])
     _sig(L, 'ufe_heat_columns_device', argtypes=[C.c_void_p, C.POINTER(HeatInputs), C.c_void_p, C.c_void_p,
                                                  C.POINTER(C.c_int)])
@@ -184,6 +209,10 @@ def lib():
     _sig(L, 'ufe_mesh_operators_ab', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_ba', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_bb2', argtypes=[C.c_int, ip, dp, ip, ip, ip, dp, dp, dp, dp, dp])
+    _sig(L, 'ufe_thickness_params_default', argtypes=[C.POINTER(ThicknessParams)])
+    _sig(L, 'ufe_thickness_setup', argtypes=[C.c_void_p, C.POINTER(VertexGeometry)])
+    _sig(L, 'ufe_thickness_semiimplicit', argtypes=[C.c_void_p, C.POINTER(ThicknessParams),
+                                                    C.POINTER(ThicknessInputs), dp, dp, dp, dp, ip])
     _lib = L
     return L
 
@@ -195,8 +224,9 @@ EXPORTED = [
     "ufe_last_timings", "ufe_profile_spmv", "ufe_profile_copy", "ufe_kernel_timing",
     "ufe_kernel_timing_read", "ufe_set_amg_options", "ufe_mpnn_num_weights", "ufe_mpnn_create",
     "ufe_mpnn_destroy", "ufe_mpnn_get_weights", "ufe_mpnn_forward", "ufe_mpnn_loss_grad", "ufe_mpnn_train",
-    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_ssa_residual", "ufe_diva_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_columns_device_noalloc", "ufe_heat_robin",
+    "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_set_graph_launch", "ufe_ssa_residual", "ufe_diva_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_csr_solve_ex", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_columns_device_noalloc", "ufe_heat_robin",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
+    "ufe_thickness_params_default", "ufe_thickness_setup", "ufe_thickness_semiimplicit",
 ]
 
 
@@ -205,6 +235,12 @@ def check(rc, ctx=None, allow_warn=True):
         return rc
     msg = lib().ufe_last_error(ctx)
     raise RuntimeError(f"libufe_hip error {rc}: {msg.decode() if msg else ''}")
+
+
+def default_thickness_params() -> ThicknessParams:
+    p = ThicknessParams()
+    lib().ufe_thickness_params_default(C.byref(p))
+    return p
 
 
 def default_params() -> Params:

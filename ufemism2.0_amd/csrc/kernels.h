@@ -31,6 +31,9 @@ struct DevMesh {
   int *TriC = nullptr, *Tri = nullptr;
   double *V = nullptr;
   double *zeta = nullptr;
+  // DIVA column kernels with 16 lanes per column (nz <= 16); false: one lane
+  // per column (same results, kept for the equivalence test)
+  bool diva_columns = true;
 };
 
 // Scalars of one Picard iteration (copied by value into kernels).

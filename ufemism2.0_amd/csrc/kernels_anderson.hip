@@ -82,7 +82,9 @@ __global__ void __launch_bounds__(kBlock) k_aa_mix(int n, double *__restrict__ x
 void launch_aa_columns(int nT, double *x, const double *u, double *fprev, double *gprev, double *dF, double *dG,
                        long stride, int p, int have_prev, int ncols, double vel_max, const RedSlot &slot,
                        double *out, hipStream_t s) {
-  int nb = std::min(blocks_for(nT), kMaxPartials);
+  // grid-stride over at most 1,024 workgroups: the one-workgroup second phase
+  // then sums 17 x 1,024 partials instead of 17 x nT / 256
+  int nb = std::min(blocks_for(nT), 1024);
   hipLaunchKernelGGL(k_aa_columns, dim3(nb), dim3(kBlock), 0, s, nT, x, u, fprev, gprev, dF, dG, stride, p,
                      have_prev, ncols, vel_max, slot, out);
   UFE_LAUNCH_CHECK();

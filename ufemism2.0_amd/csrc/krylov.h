@@ -19,6 +19,8 @@
 //       iteration).  It is a fixed linear operator, so plain BiCGStab applies.
 #pragma once
 
+#include <cstdint>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -112,6 +114,9 @@ class Krylov {
   // Run the coarse levels of the V-cycle with at most max_rows block rows in
   // one workgroup (0: one launch per pass on every level).
   void set_amg_tail(int max_rows) { amg_tail_rows_ = max_rows; }
+  // Launch BiCGStab iteration chunks as replays of captured HIP graphs
+  // (same kernels in the same order: identical results).
+  void set_graphs(bool on) { use_graphs_ = on; }
 
   // BiCGStab: x in (initial guess unless zero_guess) / out.  Stops when
   // ||r|| <= max(rtol*||r0||, abstol).
@@ -119,6 +124,15 @@ class Krylov {
                         bool zero_guess, int check_every);
   SolveStats bicgstab32(const BsrT<float> &A, const float *b, float *x, double rtol, double abstol, int maxit,
                         bool zero_guess, int check_every);
+
+  // GMRES(restart), right-preconditioned with the current preconditioner:
+  // x in (initial guess unless zero_guess) / out.  Stops when ||b - A x||
+  // (the Arnoldi estimate inside a cycle, the true residual at each restart)
+  // <= max(rtol*||r0||, abstol); iterations count Arnoldi steps.  restart <= 64.
+  SolveStats gmres64(const BsrT<double> &A, const double *b, double *x, double rtol, double abstol, int maxit,
+                     bool zero_guess, int restart, int check_every);
+  SolveStats gmres32(const BsrT<float> &A, const float *b, float *x, double rtol, double abstol, int maxit,
+                     bool zero_guess, int restart, int check_every);
 
   // FP64 residual r = b - A x with ||r||^2 and ||b||^2 (blocking read).
   // vmax > 0: rows whose block of x has magnitude >= vmax are left out of both
@@ -154,6 +168,11 @@ class Krylov {
  private:
   template <typename T> SolveStats bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, double abstol,
                                             int maxit, bool zero_guess, int check_every);
+  template <typename T> std::vector<uintptr_t> graph_key(const BsrT<T> &A, const T *b, const T *x, int nbx);
+  template <typename T> hipGraphExec_t iteration_graph(int n, const std::function<void(int)> &launch_iter,
+                                                       const BsrT<T> &A, const T *b, T *x, int nbx);
+  template <typename T> SolveStats gmres(const BsrT<T> &A, const T *b, T *x, double rtol, double abstol, int maxit,
+                                         bool zero_guess, int restart, int check_every);
   template <typename T> void precond_apply(const BsrT<T> &A, const T *r, T *z, hipEvent_t ev0 = nullptr,
                                            hipEvent_t ev1 = nullptr);
   template <typename T> void amg_setup(const BsrT<T> &A0);
@@ -187,6 +206,18 @@ class Krylov {
   // iterations of the previous solve per precision ([0] FP64, [1] FP32): the
   // first host poll of the next solve is placed there
   int last_its_[2] = {0, 0};
+  int last_gmres_its_[2] = {0, 0};
+  // captured iteration graphs of 1, 2, 4 and 8 BiCGStab iterations ([0] FP64, [1] FP32)
+  struct GraphCache {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    void clear();
+  };
+  GraphCache graphs_[2];
+  bool use_graphs_ = false;
+  // GMRES: basis (restart + 1 vectors), preconditioned vector, combination,
+  // device scalars (Hessenberg matrix, rotations) and reduction partials
+  DBuf gm_V_, gm_z_, gm_t_, gm_sc_, gm_part_;
 };
 
 }  // namespace ufe

@@ -7,6 +7,9 @@
 #include <cstring>
 #include <numeric>
 
+#include <cstring>
+#include <functional>
+
 #include "krylov.h"
 
 namespace ufe {
@@ -911,6 +914,7 @@ Krylov::Krylov(int nb, hipStream_t s) : nb_(nb), s_(s) {
 }
 
 Krylov::~Krylov() {
+  for (GraphCache &g : graphs_) g.clear();
   if (h_scal_) (void)hipHostFree(h_scal_);
   for (hipEvent_t e : tev_) (void)hipEventDestroy(e);
 }
@@ -1141,6 +1145,75 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z, hipEvent_t ev0, h
   }
 }
 
+// ---------------------------------------------------------------------------
+// Captured iteration graphs.  A chunk of BiCGStab iterations is a fixed
+// sequence of about 50 kernel launches per iteration whose arguments depend
+// only on the solver's buffers, the matrix and the preconditioner settings, so
+// it is captured once (hipStreamBeginCapture on the solver's stream) and
+// replayed with one hipGraphLaunch.  The capture is redone when anything its
+// launches read from the host side changes (the key below).  The V-cycle's
+// level buffers alternate between the two applications of one iteration and
+// are back in place after it, so replays and eager launches interleave freely.
+// ---------------------------------------------------------------------------
+template <typename T>
+std::vector<uintptr_t> Krylov::graph_key(const BsrT<T> &A, const T *b, const T *x, int nbx) {
+  auto bits = [](double d) { uintptr_t u = 0; std::memcpy(&u, &d, sizeof(d)); return u; };
+  std::vector<uintptr_t> k = {(uintptr_t)A.nb, (uintptr_t)A.nnzb, (uintptr_t)A.ptr, (uintptr_t)A.col,
+                              (uintptr_t)A.val, (uintptr_t)A.dinv, (uintptr_t)b, (uintptr_t)x, (uintptr_t)nbx,
+                              (uintptr_t)pc_, (uintptr_t)amg_, (uintptr_t)A_diag_, (uintptr_t)fine32_,
+                              (uintptr_t)amg_tail_rows_, (uintptr_t)unit_diag0_, bits(amg_omega_),
+                              bits(amg_coarse_scale_), (uintptr_t)coarse_inv_.p, (uintptr_t)red_part_.p,
+                              (uintptr_t)vec64_[0].p, (uintptr_t)vec32_[0].p, (uintptr_t)scal_.p};
+  if (pc_ == UFE_PC_AMG && amg_) {
+    AmgValues<T> &av = amg_vals<T>();
+    const std::vector<BsrT<T>> &lev = std::is_same<T, double>::value
+                                          ? *reinterpret_cast<const std::vector<BsrT<T>> *>(&lev64_)
+                                          : *reinterpret_cast<const std::vector<BsrT<T>> *>(&lev32_);
+    for (const BsrT<T> &l : lev)
+      for (uintptr_t v : {(uintptr_t)l.nb, (uintptr_t)l.ptr, (uintptr_t)l.col, (uintptr_t)l.val, (uintptr_t)l.dinv})
+        k.push_back(v);
+    for (size_t l = 0; l < av.z.size(); ++l)
+      for (uintptr_t v : {(uintptr_t)av.r[l]->p, (uintptr_t)av.z[l]->p, (uintptr_t)av.zt[l]->p}) k.push_back(v);
+  }
+  return k;
+}
+
+template <typename T>
+hipGraphExec_t Krylov::iteration_graph(int n, const std::function<void(int)> &launch_iter, const BsrT<T> &A,
+                                       const T *b, T *x, int nbx) {
+  GraphCache &gc = graphs_[sizeof(T) == 4];
+  std::vector<uintptr_t> key = graph_key<T>(A, b, x, nbx);
+  if (gc.key != key) {
+    gc.clear();
+    gc.key = std::move(key);
+  }
+  const int idx = n >= 8 ? 3 : n >= 4 ? 2 : n >= 2 ? 1 : 0;
+  if (!gc.exec[idx]) {
+    hipGraph_t g = nullptr;
+    UFE_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int k = 0; k < (1 << idx); ++k) launch_iter(k);
+    } catch (...) {
+      (void)hipStreamEndCapture(s_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    UFE_HIP(hipStreamEndCapture(s_, &g));
+    const hipError_t e = hipGraphInstantiate(&gc.exec[idx], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    UFE_HIP(e);
+  }
+  return gc.exec[idx];
+}
+
+void Krylov::GraphCache::clear() {
+  for (hipGraphExec_t &e : exec) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
+  key.clear();
+}
+
 template <typename T>
 SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, double abstol, int maxit,
                             bool zero_guess, int check_every) {
@@ -1204,34 +1277,46 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
     // (consecutive Picard systems need similar counts), then grow 1, 2, 4, ...
     // up to check_every past it.  Which iterations run is decided on the
     // device, so the chunking does not change the result.
+    // One BiCGStab iteration: every kernel returns at once after convergence
+    // (device flag), so a chunk may run past it.  k picks the timing events.
+    auto launch_iter = [&](int k) {
+      hipLaunchKernelGGL(k_bicg_p<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
+                         reinterpret_cast<const V2 *>(r), reinterpret_cast<V2 *>(p), reinterpret_cast<const V2 *>(v),
+                         reinterpret_cast<V2 *>(amg ? (T *)nullptr : phat), pcl, A.dinv, A.val, A_diag_, sc);
+      UFE_LAUNCH_CHECK();
+      if (amg)
+        precond_apply<T>(A, p, phat, timing ? tev_[8 * k + 4] : nullptr, timing ? tev_[8 * k + 5] : nullptr);
+      spmv_launch<T, OP_DOT1>(A, phat, v, nullptr, rhat, sl1, nullptr, sc, s_, 0.0, nullptr, 0,
+                              timing ? tev_[8 * k] : nullptr, timing ? tev_[8 * k + 1] : nullptr);
+      hipLaunchKernelGGL(k_bicg_s<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
+                         reinterpret_cast<const V2 *>(r), reinterpret_cast<const V2 *>(v), reinterpret_cast<V2 *>(s),
+                         reinterpret_cast<V2 *>(amg ? (T *)nullptr : shat), pcl, A.dinv, A.val, A_diag_, sc);
+      UFE_LAUNCH_CHECK();
+      if (amg)
+        precond_apply<T>(A, s, shat, timing ? tev_[8 * k + 6] : nullptr, timing ? tev_[8 * k + 7] : nullptr);
+      spmv_launch<T, OP_DOT2>(A, shat, t, nullptr, s, sl2, nullptr, sc, s_, 0.0, nullptr, 0,
+                              timing ? tev_[8 * k + 2] : nullptr, timing ? tev_[8 * k + 3] : nullptr);
+      hipLaunchKernelGGL(k_bicg_x<T>, dim3(nbx), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x),
+                         reinterpret_cast<const V2 *>(phat), reinterpret_cast<const V2 *>(shat),
+                         reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
+                         reinterpret_cast<const V2 *>(rhat), sl0, sc);
+      UFE_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_bicg_x_finish, dim3(1), dim3(kBlock), 0, s_, sl0, nbx, sc);
+      UFE_LAUNCH_CHECK();
+    };
     const int pred = restart == 0 ? last_its_[sizeof(T) == 4] : 0;
     while (!finished) {
       const int chunk = pred > its_seen ? std::min(check_every, pred - its_seen)
                                         : std::min(check_every, std::max(1, its_seen - pred + 1));
-      for (int k = 0; k < chunk; ++k) {
-        hipLaunchKernelGGL(k_bicg_p<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
-                           reinterpret_cast<const V2 *>(r), reinterpret_cast<V2 *>(p), reinterpret_cast<const V2 *>(v),
-                           reinterpret_cast<V2 *>(amg ? (T *)nullptr : phat), pcl, A.dinv, A.val, A_diag_, sc);
-        UFE_LAUNCH_CHECK();
-        if (amg)
-          precond_apply<T>(A, p, phat, timing ? tev_[8 * k + 4] : nullptr, timing ? tev_[8 * k + 5] : nullptr);
-        spmv_launch<T, OP_DOT1>(A, phat, v, nullptr, rhat, sl1, nullptr, sc, s_, 0.0, nullptr, 0,
-                                timing ? tev_[8 * k] : nullptr, timing ? tev_[8 * k + 1] : nullptr);
-        hipLaunchKernelGGL(k_bicg_s<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
-                           reinterpret_cast<const V2 *>(r), reinterpret_cast<const V2 *>(v), reinterpret_cast<V2 *>(s),
-                           reinterpret_cast<V2 *>(amg ? (T *)nullptr : shat), pcl, A.dinv, A.val, A_diag_, sc);
-        UFE_LAUNCH_CHECK();
-        if (amg)
-          precond_apply<T>(A, s, shat, timing ? tev_[8 * k + 6] : nullptr, timing ? tev_[8 * k + 7] : nullptr);
-        spmv_launch<T, OP_DOT2>(A, shat, t, nullptr, s, sl2, nullptr, sc, s_, 0.0, nullptr, 0,
-                                timing ? tev_[8 * k + 2] : nullptr, timing ? tev_[8 * k + 3] : nullptr);
-        hipLaunchKernelGGL(k_bicg_x<T>, dim3(nbx), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x),
-                           reinterpret_cast<const V2 *>(phat), reinterpret_cast<const V2 *>(shat),
-                           reinterpret_cast<const V2 *>(s), reinterpret_cast<const V2 *>(t), reinterpret_cast<V2 *>(r),
-                           reinterpret_cast<const V2 *>(rhat), sl0, sc);
-        UFE_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bicg_x_finish, dim3(1), dim3(kBlock), 0, s_, sl0, nbx, sc);
-        UFE_LAUNCH_CHECK();
+      if (use_graphs_ && !timing) {
+        // the chunk as replays of captured graphs of 8, 4, 2 and 1 iterations
+        for (int left = chunk; left > 0;) {
+          const int n = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+          UFE_HIP(hipGraphLaunch(iteration_graph<T>(n, launch_iter, A, b, x, nbx), s_));
+          left -= n;
+        }
+      } else {
+        for (int k = 0; k < chunk; ++k) launch_iter(k);
       }
       UFE_HIP(hipMemcpyAsync(h_scal_, sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s_));
       UFE_HIP(hipStreamSynchronize(s_));
@@ -1294,6 +1379,385 @@ SolveStats Krylov::bicgstab64(const BsrT<double> &A, const double *b, double *x,
 SolveStats Krylov::bicgstab32(const BsrT<float> &A, const float *b, float *x, double rtol, double abstol,
                               int maxit, bool zero_guess, int check_every) {
   return bicgstab<float>(A, b, x, rtol, abstol, maxit, zero_guess, check_every);
+}
+
+// ===========================================================================
+// GMRES(m), right-preconditioned, on the device.
+//
+// PETSc's default KSP (petsc_basic.f90:106-119 keeps it through
+// KSPSetFromOptions) is restarted GMRES(30) with classical Gram-Schmidt.  Here
+// the preconditioner is applied on the right, so the Arnoldi residual
+// estimate is the norm of b - A x itself: with the block-Jacobi-scaled system
+// that is the norm PETSc's left block-Jacobi test measures, the same norm the
+// BiCGStab stopping test uses.  Per Arnoldi step j (all on the stream, no host
+// synchronisation): z = M^-1 v_j; w = A z; h_ij = (w, v_i), i <= j (one pass
+// over the basis with w held in registers, deterministic two-phase sums);
+// w -= sum h_ij v_i with ||w||^2 (second pass); then one thread applies the
+// previous Givens rotations to column j, forms the new one, updates the
+// residual estimate |g_j+1| and sets the done flag; v_j+1 = w / h_j+1,j.  At
+// the end of a cycle one thread solves the triangular system H y = g and
+// x += M^-1 (V y).  The host polls the done flag between chunks of steps, as
+// for BiCGStab, and recomputes the true residual at each restart.
+// ===========================================================================
+constexpr int kGmresMaxRestart = 64;
+constexpr int kGmresLd = kGmresMaxRestart + 1;  // leading dimension of H
+
+struct GmresScalars {
+  double H[kGmresLd * kGmresMaxRestart];  // column j at H[j * kGmresLd + i]
+  double cs[kGmresMaxRestart], sn[kGmresMaxRestart];
+  double g[kGmresLd], y[kGmresMaxRestart];
+  double inv_h;  // 1 / h_j+1,j of the last column (normalisation of v_j+1)
+  int cols;      // Arnoldi columns completed in this cycle
+};
+
+// v0 = r / ||r||, g = (||r||, 0, ...), cycle counters reset
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gmres_init(long n, T *__restrict__ v0, const double *__restrict__ rr,
+                                                       double tol_abs, int maxit, KrylovScalars *__restrict__ sc,
+                                                       GmresScalars *__restrict__ gs) {
+  const double beta = sqrt(rr[0]);
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) v0[i] = (T)((double)v0[i] / beta);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < kGmresLd; ++k) gs->g[k] = 0.0;
+    gs->g[0] = beta;
+    gs->cols = 0;
+    gs->inv_h = 0.0;
+    sc->done = 0;
+    sc->its = 0;
+    sc->maxit = maxit;
+    sc->tol2 = tol_abs * tol_abs;
+    sc->rr = beta * beta;
+  }
+}
+
+// z = M_local v (none, point or block Jacobi)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pc_apply(int nb, const typename VecT<T>::V2 *__restrict__ v,
+                                                     typename VecT<T>::V2 *__restrict__ z, int pc,
+                                                     const typename VecT<T>::V4 *__restrict__ dinv,
+                                                     const typename VecT<T>::V4 *__restrict__ aval,
+                                                     const int *__restrict__ diag, const KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) z[i] = pc_local<T>(pc, dinv, aval, diag, i, v[i]);
+}
+
+// Partial sums of h_i = (w, v_i), i < ncol: R entries of w per lane stay in
+// registers while the basis vectors stream past.
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) k_gmres_dots(long n, int ncol, const T *__restrict__ V, long ldv,
+                                                       const T *__restrict__ w, double *__restrict__ part,
+                                                       const KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  const long base = (long)blockIdx.x * kBlock * R + threadIdx.x;
+  double wr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const long idx = base + (long)r * kBlock;
+    wr[r] = idx < n ? (double)w[idx] : 0.0;
+  }
+  __shared__ double lds[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int i = 0; i < ncol; ++i) {
+    const T *v = V + (long)i * ldv;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const long idx = base + (long)r * kBlock;
+      if (idx < n) acc += wr[r] * (double)v[idx];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) lds[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int q = 0; q < kBlock / 64; ++q) t += lds[q];
+      part[(long)i * kMaxPartials + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Sum of `ncol` partial sets (nparts each, index order) in one workgroup.
+__device__ __forceinline__ double gmres_partial_sum(const double *__restrict__ part, int nparts, double (*lds)) {
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) t += part[b];
+  t = wave_sum(t);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wid] = t;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < (int)(blockDim.x / 64); ++q) s += lds[q];
+  return s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_gmres_dots_finish(int ncol, int nparts, const double *__restrict__ part,
+                                                              GmresScalars *__restrict__ gs, int j,
+                                                              const KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double lds[kBlock / 64];
+  for (int i = 0; i < ncol; ++i) {
+    const double h = gmres_partial_sum(part + (long)i * kMaxPartials, nparts, lds);
+    if (threadIdx.x == 0) gs->H[j * kGmresLd + i] = h;
+  }
+}
+
+// w -= sum_i h_ij v_i, partial sums of ||w||^2
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) k_gmres_update(long n, int ncol, const T *__restrict__ V, long ldv,
+                                                         T *__restrict__ w, const GmresScalars *__restrict__ gs, int j,
+                                                         double *__restrict__ part,
+                                                         const KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double h[kGmresLd];
+  for (int i = threadIdx.x; i < ncol; i += blockDim.x) h[i] = gs->H[j * kGmresLd + i];
+  __syncthreads();
+  const long base = (long)blockIdx.x * kBlock * R + threadIdx.x;
+  double wr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const long idx = base + (long)r * kBlock;
+    wr[r] = idx < n ? (double)w[idx] : 0.0;
+  }
+  for (int i = 0; i < ncol; ++i) {
+    const T *v = V + (long)i * ldv;
+    const double hi = h[i];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const long idx = base + (long)r * kBlock;
+      if (idx < n) wr[r] -= hi * (double)v[idx];
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const long idx = base + (long)r * kBlock;
+    if (idx < n) {
+      const T wt = (T)wr[r];
+      w[idx] = wt;
+      acc += (double)wt * (double)wt;
+    }
+  }
+  __shared__ double lds[kBlock / 64];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kBlock / 64; ++q) t += lds[q];
+    part[blockIdx.x] = t;
+  }
+}
+
+// h_j+1,j = ||w||; rotations; residual estimate; stop flags (one workgroup)
+__global__ void __launch_bounds__(kBlock) k_gmres_norm_finish(int nparts, const double *__restrict__ part,
+                                                              GmresScalars *__restrict__ gs, int j,
+                                                              KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double lds[kBlock / 64];
+  const double hn = sqrt(gmres_partial_sum(part, nparts, lds));
+  if (threadIdx.x != 0) return;
+  double *H = gs->H + j * kGmresLd;
+  for (int i = 0; i < j; ++i) {
+    const double a = H[i], b = H[i + 1];
+    H[i] = gs->cs[i] * a + gs->sn[i] * b;
+    H[i + 1] = -gs->sn[i] * a + gs->cs[i] * b;
+  }
+  const double a = H[j];
+  const double r = sqrt(a * a + hn * hn);
+  const double c = r > 0.0 ? a / r : 1.0, s = r > 0.0 ? hn / r : 0.0;
+  gs->cs[j] = c;
+  gs->sn[j] = s;
+  H[j] = r;
+  H[j + 1] = 0.0;
+  const double gj = gs->g[j];
+  gs->g[j] = c * gj;
+  gs->g[j + 1] = -s * gj;
+  gs->cols = j + 1;
+  gs->inv_h = hn > 0.0 ? 1.0 / hn : 0.0;
+  sc->its += 1;
+  const double res = fabs(gs->g[j + 1]);
+  sc->rr = res * res;
+  if (!std::isfinite(res) || !std::isfinite(r)) sc->done = 2;
+  else if (res * res <= sc->tol2 || hn == 0.0) sc->done = 1;  // converged, or the Krylov space is invariant
+  else if (r == 0.0) sc->done = 2;
+  else if (sc->its >= sc->maxit) sc->done = 3;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gmres_scale(long n, T *__restrict__ w, const GmresScalars *__restrict__ gs,
+                                                        const KrylovScalars *__restrict__ sc) {
+  if (sc->done) return;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) w[i] = (T)((double)w[i] * gs->inv_h);
+}
+
+// y = H^-1 g over the columns of the cycle (upper triangular after the rotations)
+__global__ void k_gmres_solve_y(GmresScalars *__restrict__ gs) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int m = gs->cols;
+  for (int i = m - 1; i >= 0; --i) {
+    double t = gs->g[i];
+    for (int k = i + 1; k < m; ++k) t -= gs->H[k * kGmresLd + i] * gs->y[k];
+    const double d = gs->H[i * kGmresLd + i];
+    gs->y[i] = d != 0.0 ? t / d : 0.0;
+  }
+}
+
+// t = sum_i y_i v_i over the cycle's columns
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gmres_combine(long n, const T *__restrict__ V, long ldv,
+                                                          const GmresScalars *__restrict__ gs, T *__restrict__ t) {
+  __shared__ double y[kGmresMaxRestart];
+  const int m = gs->cols;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) y[i] = gs->y[i];
+  __syncthreads();
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int k = 0; k < m; ++k) acc += y[k] * (double)V[(long)k * ldv + i];
+  t[i] = (T)acc;
+}
+
+__global__ void k_gmres_set_done(KrylovScalars *__restrict__ sc, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gmres_axpy(long n, T *__restrict__ x, const T *__restrict__ z) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += z[i];
+}
+
+template <typename T, int R>
+static void gmres_dots_update(long n, int ncol, const T *V, long ldv, T *w, double *part, GmresScalars *gs, int j,
+                              KrylovScalars *sc, hipStream_t s) {
+  const int g = (int)((n + (long)kBlock * R - 1) / ((long)kBlock * R));
+  hipLaunchKernelGGL((k_gmres_dots<T, R>), dim3(g), dim3(kBlock), 0, s, n, ncol, V, ldv, w, part, sc);
+  UFE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gmres_dots_finish, dim3(1), dim3(kBlock), 0, s, ncol, g, part, gs, j, sc);
+  UFE_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_gmres_update<T, R>), dim3(g), dim3(kBlock), 0, s, n, ncol, V, ldv, w, gs, j, part, sc);
+  UFE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gmres_norm_finish, dim3(1), dim3(kBlock), 0, s, g, part, gs, j, sc);
+  UFE_LAUNCH_CHECK();
+}
+
+template <typename T>
+SolveStats Krylov::gmres(const BsrT<T> &A, const T *b, T *x, double rtol, double abstol, int maxit, bool zero_guess,
+                         int m, int check_every) {
+  using V2 = typename VecT<T>::V2;
+  m = std::max(1, std::min(m, kGmresMaxRestart));
+  const int nb = A.nb;
+  const long n = 2L * nb;
+  // registers per lane for the two basis passes: the grid must fit the partial slots
+  const int R = n <= (long)kMaxPartials * kBlock * 4 ? 4 : n <= (long)kMaxPartials * kBlock * 16 ? 16 : 0;
+  if (R == 0) throw std::runtime_error("GMRES: system too large for the reduction slots");
+  const size_t vbytes = sizeof(T) * (size_t)n;
+  if (gm_V_.bytes < vbytes * (m + 1)) gm_V_.alloc(vbytes * (m + 1));
+  if (gm_z_.bytes < vbytes) { gm_z_.alloc(vbytes); gm_t_.alloc(vbytes); }
+  if (!gm_sc_.p) {
+    gm_sc_.alloc(sizeof(GmresScalars));
+    gm_part_.alloc(sizeof(double) * (size_t)kMaxPartials * kGmresLd);
+  }
+  T *V = gm_V_.as<T>(), *z = gm_z_.as<T>(), *t = gm_t_.as<T>();
+  GmresScalars *gs = gm_sc_.as<GmresScalars>();
+  double *part = gm_part_.as<double>();
+  KrylovScalars *sc = scal_.as<KrylovScalars>();
+  const bool amg = (pc_ == UFE_PC_AMG) && amg_;
+  const int pcl = amg ? UFE_PC_NONE : pc_;
+  if (zero_guess) {
+    hipLaunchKernelGGL(k_zero2<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb, reinterpret_cast<V2 *>(x));
+    UFE_LAUNCH_CHECK();
+  }
+  SolveStats st;
+  int total = 0;
+  double tol_abs = 0.0;
+  RedSlot sl0 = slot(0);
+  const int pred0 = last_gmres_its_[sizeof(T) == 4];
+  for (int cycle = 0;; ++cycle) {
+    // true residual into v0 (the norm of the system given)
+    spmv_launch<T, OP_RESID>(A, x, V, b, nullptr, sl0, sl0.result, nullptr, s_, 0.0);
+    double h0[3];
+    UFE_HIP(hipMemcpyAsync(h0, sl0.result, sizeof(h0), hipMemcpyDeviceToHost, s_));
+    UFE_HIP(hipStreamSynchronize(s_));
+    const double rn = std::sqrt(h0[0]);
+    if (cycle == 0) {
+      st.r0 = rn;
+      tol_abs = std::max(rtol * rn, abstol);
+    }
+    st.rel = rn;
+    if (!std::isfinite(rn)) { st.status = 2; break; }
+    if (rn <= tol_abs * (cycle ? 1.0001 : 1.0)) { st.status = 1; break; }
+    if (total >= maxit) { st.status = 3; break; }
+    hipLaunchKernelGGL(k_gmres_init<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s_, n, V, sl0.result, tol_abs,
+                       maxit - total, sc, gs);
+    UFE_LAUNCH_CHECK();
+    int j = 0, seen = 0;
+    // first poll where the previous solve stopped, then every check_every steps
+    int next_poll = cycle == 0 && pred0 > 0 ? std::min(pred0, m) : std::min(check_every, m);
+    while (j < m) {
+      for (; j < next_poll; ++j) {
+        T *vj = V + (long)j * n, *w = V + (long)(j + 1) * n;
+        if (amg) {
+          precond_apply<T>(A, vj, z);
+        } else {
+          hipLaunchKernelGGL(k_pc_apply<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
+                             reinterpret_cast<const V2 *>(vj), reinterpret_cast<V2 *>(z), pcl, A.dinv, A.val,
+                             A_diag_, sc);
+          UFE_LAUNCH_CHECK();
+        }
+        spmv_launch<T, OP_PLAIN>(A, z, w, nullptr, nullptr, sl0, nullptr, sc, s_);
+        if (R == 4) gmres_dots_update<T, 4>(n, j + 1, V, n, w, part, gs, j, sc, s_);
+        else gmres_dots_update<T, 16>(n, j + 1, V, n, w, part, gs, j, sc, s_);
+        hipLaunchKernelGGL(k_gmres_scale<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s_, n, w, gs, sc);
+        UFE_LAUNCH_CHECK();
+      }
+      UFE_HIP(hipMemcpyAsync(h_scal_, sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s_));
+      UFE_HIP(hipStreamSynchronize(s_));
+      if (verbose_ > 1)
+        std::fprintf(stderr, "[ufe:gmres%s] cycle %d its %d |r| %.3e tol %.3e done %d\n", sizeof(T) == 4 ? "32" : "64",
+                     cycle, total + h_scal_->its, std::sqrt(h_scal_->rr), tol_abs, h_scal_->done);
+      seen = h_scal_->its;
+      if (h_scal_->done) break;
+      next_poll = std::min(m, j + check_every);
+    }
+    total += seen;
+    const int status = h_scal_->done;
+    // x += M^-1 V y over the columns of this cycle
+    hipLaunchKernelGGL(k_gmres_solve_y, dim3(1), dim3(1), 0, s_, gs);
+    UFE_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gmres_combine<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s_, n, V, n, gs, t);
+    UFE_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gmres_set_done, dim3(1), dim3(1), 0, s_, sc, 0);
+    UFE_LAUNCH_CHECK();
+    if (amg) {
+      precond_apply<T>(A, t, z);
+    } else {
+      hipLaunchKernelGGL(k_pc_apply<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb, reinterpret_cast<const V2 *>(t),
+                         reinterpret_cast<V2 *>(z), pcl, A.dinv, A.val, A_diag_, sc);
+      UFE_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_gmres_axpy<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s_, n, x, z);
+    UFE_LAUNCH_CHECK();
+    if (cycle == 0) last_gmres_its_[sizeof(T) == 4] = seen;
+    if (status == 2 && seen == 0) { st.status = 2; break; }
+  }
+  st.its = total;
+  st.rel = st.r0 > 0 ? st.rel / st.r0 : st.rel;
+  return st;
+}
+
+SolveStats Krylov::gmres64(const BsrT<double> &A, const double *b, double *x, double rtol, double abstol, int maxit,
+                           bool zero_guess, int restart, int check_every) {
+  return gmres<double>(A, b, x, rtol, abstol, maxit, zero_guess, restart, check_every);
+}
+
+SolveStats Krylov::gmres32(const BsrT<float> &A, const float *b, float *x, double rtol, double abstol, int maxit,
+                           bool zero_guess, int restart, int check_every) {
+  return gmres<float>(A, b, x, rtol, abstol, maxit, zero_guess, restart, check_every);
 }
 
 // Generic wrappers declared in kernels.h

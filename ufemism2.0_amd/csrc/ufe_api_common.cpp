@@ -65,3 +65,18 @@ extern "C" void ufe_params_default(ufe_params *p) {
   p->precond_fp32 = 0;
   p->check_every = 8;
 }
+
+// Defaults of the thickness solver's configuration fields
+// (model_configuration.f90:356-364,391-392,437).
+extern "C" void ufe_thickness_params_default(ufe_thickness_params *p) {
+  p->fs = 1.5;
+  for (int i = 0; i < 4; ++i) p->bc_H[i] = UFE_BC_ZERO;
+  p->dt_ice_max = 10.0;
+  p->dt_ice_min = 0.1;
+  p->Hi_min = 0.0;
+  p->rtol = 1e-8;
+  p->abstol = 1e-6;
+  p->krylov = UFE_KRYLOV_GMRES;
+  p->gmres_restart = 30;
+  p->maxit = 10000;
+}

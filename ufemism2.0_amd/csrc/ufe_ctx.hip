@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -58,6 +59,7 @@ struct ufe_ctx {
   bool amg_fold = true;  // UFE_AMG_FOLD=0: keep the separate level-0 pre-smoothing kernel
   double bbox[4] = {0, 1, 0, 1};  // xmin, xmax, ymin, ymax of the vertices
   DBuf feat;                     // learned-guess features [nT][16] fp32 (device order)
+  std::shared_ptr<void> thickness;  // state of the thickness solver (thickness.hip), set up on demand
   hipEvent_t ev[8];
 };
 
@@ -163,6 +165,17 @@ namespace ufe {
 void launch_f64_to_f32(int n, const double *a, float *b, hipStream_t s);
 }
 
+// FP64 Krylov solve of the assembled system into c->x with the chosen method
+// (the reference's PETSc default is GMRES(30), petsc_basic.f90:106-119).
+static SolveStats krylov64(ufe_ctx *c, const BsrT<double> &A, double rtol, bool zero_guess) {
+  const ufe_params &q = c->prm;
+  if (q.krylov == UFE_KRYLOV_GMRES)
+    return c->kry->gmres64(A, c->rhs_s.as<double>(), c->x.as<double>(), rtol, q.abstol, q.krylov_maxit, zero_guess,
+                           q.gmres_restart, q.check_every);
+  return c->kry->bicgstab64(A, c->rhs_s.as<double>(), c->x.as<double>(), rtol, q.abstol, q.krylov_maxit, zero_guess,
+                            q.check_every);
+}
+
 // ---------------------------------------------------------------------------
 // Linear solve of one Picard iteration: x := solution of A x = rhs.
 // ---------------------------------------------------------------------------
@@ -185,8 +198,7 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
     BsrT<double> A = sys64(c);
     K.set_amg_fine32(q.precond_fp32 && q.precond == UFE_PC_AMG ? c->A32.as<float4>() : nullptr);
     if (q.precond == UFE_PC_AMG) K.amg_setup64(A);
-    SolveStats st = K.bicgstab64(A, c->rhs_s.as<double>(), c->x.as<double>(), q.rtol, q.abstol, q.krylov_maxit,
-                                 !warm, q.check_every);
+    SolveStats st = krylov64(c, A, q.rtol, !warm);
     *its = st.its;
     return;
   }
@@ -207,7 +219,10 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
     // each FP32 solve reduces the residual by at most inner_rtol (well above
     // FP32 round-off); the FP64 refinement loop takes as many steps as needed
     double inner = std::max(0.5 * target / rn, q.inner_rtol);
-    SolveStats st = K.bicgstab32(A32, r32, d32, inner, 0.0, q.krylov_maxit - *its, true, q.check_every);
+    SolveStats st = q.krylov == UFE_KRYLOV_GMRES
+                        ? K.gmres32(A32, r32, d32, inner, 0.0, q.krylov_maxit - *its, true, q.gmres_restart,
+                                    q.check_every)
+                        : K.bicgstab32(A32, r32, d32, inner, 0.0, q.krylov_maxit - *its, true, q.check_every);
     *its += st.its;
     hipLaunchKernelGGL(k_add_f32, dim3(blocks_for(n)), dim3(kBlock), 0, c->s, n, c->x.as<double>(), d32);
     UFE_LAUNCH_CHECK();
@@ -223,8 +238,7 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
                        warm ? c->u.as<double>() : (const double *)nullptr, c->x.as<double>());
     UFE_LAUNCH_CHECK();
     if (q.precond == UFE_PC_AMG) K.amg_setup64(A);
-    SolveStats st = K.bicgstab64(A, c->rhs_s.as<double>(), c->x.as<double>(), q.rtol, q.abstol, q.krylov_maxit,
-                                 !warm, q.check_every);
+    SolveStats st = krylov64(c, A, q.rtol, !warm);
     *its += st.its;
   }
 }
@@ -241,6 +255,41 @@ static double rel_residual(ufe_ctx *c, int *n_clamped = nullptr) {
   c->kry->residual64(sys64(c), c->u.as<double>(), c->rhs_s.as<double>(), c->r64.as<double>(), &rr, &bb,
                      c->prm.vel_max, n_clamped, c->Dblk.as<double4>(), 1);
   return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
+// ||r|| and ||b|| of the unscaled system over rows whose velocity is below
+// vel_max (the rows rel_residual measures), as per-workgroup partials.
+__global__ void __launch_bounds__(kBlock) k_unscaled_norms(int nT, const double2 *__restrict__ r,
+                                                           const double2 *__restrict__ b,
+                                                           const double2 *__restrict__ u, double vmax, RedSlot slot) {
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nT; i += gridDim.x * blockDim.x) {
+    const double2 uu = u[i];
+    if (vmax > 0.0 && sqrt(uu.x * uu.x + uu.y * uu.y) >= vmax * (1.0 - 1e-9)) continue;
+    const double2 rr = r[i], bb = b[i];
+    acc[0] += rr.x * rr.x + rr.y * rr.y;
+    acc[1] += bb.x * bb.x + bb.y * bb.y;
+  }
+  __shared__ double lds[2][kBlock / 64];
+  block_sum<2>(acc, lds);
+  block_partials<2>(slot, acc);
+}
+
+// ||b - A u|| / ||b|| of the unscaled system from the residual the last
+// rel_residual call left in r64 and the unscaled right-hand side.
+static double unscaled_rel_residual(ufe_ctx *c) {
+  const int nT = c->m.nT;
+  const int nb = std::min(blocks_for(nT), kMaxPartials);
+  RedSlot sl = c->kry->slot(3);
+  hipLaunchKernelGGL(k_unscaled_norms, dim3(nb), dim3(kBlock), 0, c->s, nT, c->r64.as<double2>(),
+                     c->rhs.as<double2>(), c->u.as<double2>(), c->prm.vel_max, sl);
+  UFE_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_partials_out<2>, dim3(1), dim3(kBlock), 0, c->s, sl, nb, sl.result);
+  UFE_LAUNCH_CHECK();
+  double h[2];
+  UFE_HIP(hipMemcpyAsync(h, sl.result, sizeof(h), hipMemcpyDeviceToHost, c->s));
+  UFE_HIP(hipStreamSynchronize(c->s));
+  return h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
 }
 
 // Everything up to the assembled linear system at the current iterate.
@@ -470,6 +519,7 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
     launch_triangle_3d(c->m, c->eta3a.as<double>(), c->F1a.as<double>(), c->F2a.as<double>(),
                        c->tmp3.as<double>(), c->F1b.as<double>(), c->F2b0.as<double>(), 1, s);
   }
+  double resid_unscaled = -1.0;
   if (rc >= 0 && (q.stop_rule != UFE_STOP_RESIDUAL || !converged)) {
     // Diagnostic residual at the returned iterate.  The evaluation overwrites
     // the DIVA state, so eta_3D_b (returned, and the next call's warm start)
@@ -482,13 +532,17 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
     }
     picard_state(c, diva, eps0, false);
     resid = rel_residual(c, &n_clamped);
+    resid_unscaled = unscaled_rel_residual(c);
     if (diva) {
       UFE_HIP(hipMemcpyAsync(c->eta3b.p, c->tmp3.p, e_bytes, hipMemcpyDeviceToDevice, s));
       UFE_HIP(hipMemcpyAsync(c->beta_basal_b.p, c->beta_save.p, b_bytes, hipMemcpyDeviceToDevice, s));
     }
   }
+  // converged by the residual rule: r64 and rhs still hold the last evaluation at u
+  if (rc >= 0 && resid_unscaled < 0.0 && resid >= 0.0) resid_unscaled = unscaled_rel_residual(c);
   UFE_HIP(hipStreamSynchronize(s));
   inf.n_visc_its = it;
+  inf.rel_residual_unscaled = resid_unscaled;
   inf.n_Axb_its = n_axb;
   inf.n_refinements = n_ref;
   inf.L2_uv = L2;
@@ -636,6 +690,9 @@ Krylov *ctx_krylov(const ufe_ctx *c) { return c->kry.get(); }
 int ctx_fail(ufe_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
 const std::vector<int> &ctx_vperm(const ufe_ctx *c) { return c->vperm; }
 const std::vector<int> &ctx_tperm(const ufe_ctx *c) { return c->tperm; }
+std::shared_ptr<void> &ctx_thickness(ufe_ctx *c) { return c->thickness; }
+const double *ctx_velocity(const ufe_ctx *c) { return c->u.as<double>(); }
+int ctx_guard(ufe_ctx *c, const std::function<int()> &f) { return guarded(c, f); }
 }  // namespace ufe
 
 // ===========================================================================
@@ -772,6 +829,8 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
       c->kry->set_amg_options(om, 0, cs);
       if ((e = std::getenv("UFE_AMG_TAIL_ROWS"))) c->kry->set_amg_tail(std::max(0, std::atoi(e)));
       if ((e = std::getenv("UFE_AMG_FOLD"))) c->amg_fold = std::atoi(e) != 0;
+      if ((e = std::getenv("UFE_DIVA_COLUMNS"))) c->m.diva_columns = std::atoi(e) != 0;
+      if ((e = std::getenv("UFE_GRAPHS"))) c->kry->set_graphs(std::atoi(e) != 0);
     }
     c->kry->set_diag(m.bb_diag);
     {
@@ -815,11 +874,22 @@ extern "C" int ufe_set_amg_options(ufe_ctx *c, double omega, double coarse_scale
   });
 }
 
+extern "C" int ufe_set_graph_launch(ufe_ctx *c, int enable) {
+  if (!c) return fail(c, UFE_ERR_ARGUMENT, "null argument");
+  return guarded(c, [&]() -> int {
+    c->kry->set_graphs(enable != 0);
+    return UFE_OK;
+  });
+}
+
 extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
   if (!c || !p) return fail(c, UFE_ERR_ARGUMENT, "null argument");
   return guarded(c, [&]() -> int {
     if (p->precond == UFE_PC_AMG && !c->amg) return fail(c, UFE_ERR_UNSUPPORTED, "no aggregation hierarchy for this mesh");
-    if (p->krylov != UFE_KRYLOV_BICGSTAB) return fail(c, UFE_ERR_UNSUPPORTED, "only BiCGStab is implemented");
+    if (p->krylov != UFE_KRYLOV_BICGSTAB && p->krylov != UFE_KRYLOV_GMRES)
+      return fail(c, UFE_ERR_ARGUMENT, "krylov must be UFE_KRYLOV_BICGSTAB or UFE_KRYLOV_GMRES");
+    if (p->krylov == UFE_KRYLOV_GMRES && (p->gmres_restart < 1 || p->gmres_restart > 64))
+      return fail(c, UFE_ERR_ARGUMENT, "gmres_restart must be 1..64");
     if (p->check_every < 1 || p->krylov_maxit < 1) return fail(c, UFE_ERR_ARGUMENT, "invalid Krylov controls");
     if (p->accel_depth < 0 || p->accel_depth > UFE_ACCEL_MAX_DEPTH)
       return fail(c, UFE_ERR_ARGUMENT, "accel_depth out of range");
@@ -1129,9 +1199,24 @@ extern "C" int ufe_csr_matvec(int m, int n, int nnz, const int *ptr, const int *
 
 // Solve A x = b for a general square CSR matrix: rows/columns are paired into
 // 2x2 blocks (explicit zeros where needed) and the BSR BiCGStab is used.
+extern "C" int ufe_csr_solve_ex(int n, int nnz, const int *ptr, const int *ind, const double *val, const double *b,
+                                double *x, double rtol, double abstol, int maxit, int precond, int krylov,
+                                int restart, int *n_its, double *rel_residual);
+
 extern "C" int ufe_csr_solve(int n, int nnz, const int *ptr, const int *ind, const double *val, const double *b,
                              double *x, double rtol, double abstol, int maxit, int precond, int *n_its,
                              double *rel_residual) {
+  return ufe_csr_solve_ex(n, nnz, ptr, ind, val, b, x, rtol, abstol, maxit, precond, UFE_KRYLOV_BICGSTAB, 30, n_its,
+                          rel_residual);
+}
+
+extern "C" int ufe_csr_solve_ex(int n, int nnz, const int *ptr, const int *ind, const double *val, const double *b,
+                                double *x, double rtol, double abstol, int maxit, int precond, int krylov,
+                                int restart, int *n_its, double *rel_residual) {
+  if (krylov != UFE_KRYLOV_BICGSTAB && krylov != UFE_KRYLOV_GMRES)
+    return fail(nullptr, UFE_ERR_ARGUMENT, "krylov must be UFE_KRYLOV_BICGSTAB or UFE_KRYLOV_GMRES");
+  if (krylov == UFE_KRYLOV_GMRES && (restart < 1 || restart > 64))
+    return fail(nullptr, UFE_ERR_ARGUMENT, "restart must be 1..64");
   if (n <= 0 || nnz < 0 || !ptr || !ind || !val || !b || !x) return fail(nullptr, UFE_ERR_ARGUMENT, "invalid CSR arguments");
   if (ptr[0] != 0 || ptr[n] != nnz) return fail(nullptr, UFE_ERR_ARGUMENT, "bad row pointer");
   for (int i = 0; i < n; ++i)
@@ -1197,7 +1282,9 @@ extern "C" int ufe_csr_solve(int n, int nnz, const int *ptr, const int *ind, con
       BsrT<double> A;
       A.nb = nb; A.nnzb = (int)bcol.size(); A.ptr = dptr.as<int>(); A.col = dcol.as<int>();
       A.val = dval.as<double4>(); A.dinv = ddinv.as<double4>();
-      SolveStats st = K.bicgstab64(A, db.as<double>(), dx.as<double>(), rtol, abstol, maxit, false, 8);
+      SolveStats st = krylov == UFE_KRYLOV_GMRES
+                          ? K.gmres64(A, db.as<double>(), dx.as<double>(), rtol, abstol, maxit, false, restart, 8)
+                          : K.bicgstab64(A, db.as<double>(), dx.as<double>(), rtol, abstol, maxit, false, 8);
       UFE_HIP(hipMemcpy(xx.data(), dx.p, dx.bytes, hipMemcpyDeviceToHost));
       if (n_its) *n_its = st.its;
       if (rel_residual) *rel_residual = st.rel;

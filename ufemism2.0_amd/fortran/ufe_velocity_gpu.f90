@@ -62,6 +62,7 @@ module ufe_velocity_gpu
     integer(c_int) :: n_visc_its, n_Axb_its, n_refinements
     real(c_double) :: L2_uv, rel_residual, visc_it_relax_applied, epsilon_sq_0_applied, seconds
     integer(c_int) :: n_clamped, n_fallbacks
+    real(c_double) :: rel_residual_unscaled
   end type ufe_gpu_info
 
   ! struct ufe_heat_inputs

@@ -227,3 +227,96 @@ def generate_mesh(xmin, xmax, ymin, ymax, size_fn, seed=0, smooth_sweeps=4) -> M
 
 def uniform_mesh(xmin, xmax, ymin, ymax, h, seed=0) -> Mesh:
     return generate_mesh(xmin, xmax, ymin, ymax, lambda x, y: np.full(np.shape(x), float(h)), seed)
+
+
+# --------------------------------------------------------------------------
+# Voronoi geometry of the vertices (the mass-conservation operators)
+# --------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class VertexGeometry:
+    """Per-vertex Voronoi data of the thickness solver, in the C (neighbour)
+    order of the mesh: A (cell areas), Cw (shared boundary length per
+    connection) and ETri (the triangles either side of each connection's
+    edge, -1 where the edge is on the border)."""
+    A: np.ndarray       # (nV,)
+    Cw: np.ndarray      # (nC,)
+    ETri: np.ndarray    # (nC, 2) int32
+
+
+def circumcentres(mesh: Mesh) -> np.ndarray:
+    """Triangle circumcentres (the reference's mesh%Tricc)."""
+    a, b, c = (mesh.V[mesh.Tri[:, k]] for k in range(3))
+    d = 2.0 * (a[:, 0] * (b[:, 1] - c[:, 1]) + b[:, 0] * (c[:, 1] - a[:, 1]) + c[:, 0] * (a[:, 1] - b[:, 1]))
+    a2, b2, c2 = (np.sum(p * p, axis=1) for p in (a, b, c))
+    ux = (a2 * (b[:, 1] - c[:, 1]) + b2 * (c[:, 1] - a[:, 1]) + c2 * (a[:, 1] - b[:, 1])) / d
+    uy = (a2 * (c[:, 0] - b[:, 0]) + b2 * (a[:, 0] - c[:, 0]) + c2 * (b[:, 0] - a[:, 0])) / d
+    return np.stack([ux, uy], axis=1)
+
+
+def vertex_geometry(mesh: Mesh) -> VertexGeometry:
+    """Voronoi cell areas and connection widths of the vertices.
+
+    A: the circumcentric dual cell of each vertex, i.e. for every triangle
+    around the vertex the two triangles (vertex, edge midpoint, circumcentre),
+    signed, which is the Voronoi cell clipped to the domain the reference
+    builds from the circumcentres of the surrounding triangles
+    (calc_Voronoi_cell, mesh_utilities.f90:72-303; calc_Voronoi_cell_areas,
+    mesh_secondary.f90:137-186).  The cells tile the domain.
+    Cw: the shared Voronoi boundary of the connection's edge
+    (find_shared_Voronoi_boundary, mesh_utilities.f90:305-372): between the
+    circumcentres of the two triangles either side, cropped to the domain;
+    for an edge on the border, from the triangle's circumcentre (clamped into
+    the domain) to a point dx = (width + height) / 100 outside the border,
+    perpendicular to it, as the reference draws it
+    (calc_connection_widths, mesh_secondary.f90:246-298)."""
+    V, Tri = mesh.V, mesh.Tri
+    nV, nT = mesh.nV, mesh.nT
+    cc = circumcentres(mesh)
+    # edges -> the triangles either side: a CCW triangle has each of its
+    # directed edges (va, vb) on its left
+    nC = int(mesh.C_ptr[-1])
+    ea = np.concatenate([Tri[:, n] for n in range(3)]).astype(np.int64)
+    eb = np.concatenate([Tri[:, (n + 1) % 3] for n in range(3)]).astype(np.int64)
+    et = np.tile(np.arange(nT, dtype=np.int64), 3)
+    keys = ea * nV + eb
+    order = np.argsort(keys)
+    keys, et = keys[order], et[order]
+    src = np.repeat(np.arange(nV, dtype=np.int64), np.diff(mesh.C_ptr))
+    dst = mesh.C_ind.astype(np.int64)
+
+    def lookup(k):
+        pos = np.minimum(np.searchsorted(keys, k), len(keys) - 1)
+        return np.where(keys[pos] == k, et[pos], -1).astype(np.int32)
+
+    ETri = np.stack([lookup(src * nV + dst), lookup(dst * nV + src)], axis=1)
+    # cell areas: signed sub-triangles (vertex, edge midpoint, circumcentre)
+    def area2(p, q, r):
+        return (q[:, 0] - p[:, 0]) * (r[:, 1] - p[:, 1]) - (q[:, 1] - p[:, 1]) * (r[:, 0] - p[:, 0])
+    A = np.zeros(nV)
+    for n in range(3):
+        vi, vb, vc = Tri[:, n], Tri[:, (n + 1) % 3], Tri[:, (n + 2) % 3]
+        p = V[vi]
+        mb, mc = 0.5 * (p + V[vb]), 0.5 * (p + V[vc])
+        s = 0.5 * (area2(p, mb, cc) + area2(p, cc, mc))
+        np.add.at(A, vi, s)
+    # connection widths
+    xmin, xmax, ymin, ymax = mesh.xmin, mesh.xmax, mesh.ymin, mesh.ymax
+    dx = ((xmax - xmin) + (ymax - ymin)) / 100.0
+    tol = 1e-9 * max(xmax - xmin, ymax - ymin)
+    lo, hi = np.array([xmin, ymin]), np.array([xmax, ymax])
+    tl, tr = ETri[:, 0], ETri[:, 1]
+    interior = (tl >= 0) & (tr >= 0)
+    p = np.clip(cc[np.where(tl >= 0, tl, tr)], lo, hi)
+    q = np.clip(cc[np.where(tr >= 0, tr, tl)], lo, hi)
+    a, b = V[src], V[dst]
+    north = (np.abs(a[:, 1] - ymax) < tol) & (np.abs(b[:, 1] - ymax) < tol)
+    east = (np.abs(a[:, 0] - xmax) < tol) & (np.abs(b[:, 0] - xmax) < tol)
+    south = (np.abs(a[:, 1] - ymin) < tol) & (np.abs(b[:, 1] - ymin) < tol)
+    qb = np.where(north[:, None], np.stack([p[:, 0], np.full(nC, ymax + dx)], 1),
+                  np.where(east[:, None], np.stack([np.full(nC, xmax + dx), p[:, 1]], 1),
+                           np.where(south[:, None], np.stack([p[:, 0], np.full(nC, ymin - dx)], 1),
+                                    np.stack([np.full(nC, xmin - dx), p[:, 1]], 1))))
+    q = np.where(interior[:, None], q, qb)
+    Cw = np.hypot(p[:, 0] - q[:, 0], p[:, 1] - q[:, 1])
+    return VertexGeometry(A, Cw, ETri)

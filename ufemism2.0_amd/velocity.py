@@ -62,6 +62,10 @@ class VelocitySolver:
                 setattr(self.params, k, v)
         self._check(self.lib.ufe_set_params(self._ctx, C.byref(self.params)))
 
+    def set_graph_launch(self, enable: bool):
+        """Replay chunks of Krylov iterations from captured HIP graphs."""
+        self._check(self.lib.ufe_set_graph_launch(self._ctx, int(bool(enable))))
+
     def _check(self, rc):
         if rc < 0:
             raise VelocitySolverError(f"libufe_hip error {rc}: {self.lib.ufe_last_error(self._ctx).decode()}")
@@ -282,6 +286,49 @@ class VelocitySolver:
                                                L.ptr(out)))
         return out
 
+    # -- ice thickness -------------------------------------------------------
+    def thickness_setup(self, geometry=None):
+        """Attach the Voronoi geometry of the mesh (cell areas, connection
+        widths, edge triangles; computed from the mesh when not given)."""
+        from .mesh import vertex_geometry
+        m = self.md.mesh
+        g = geometry if geometry is not None else vertex_geometry(m)
+        self._thk_keep = (L.i32(m.C_ptr), L.i32(m.C_ind), L.i32(m.VBI), L.f64(g.A), L.f64(g.Cw), L.i32(g.ETri))
+        cp, ci, vbi, A, Cw, et = self._thk_keep
+        self._thk_geom = L.VertexGeometry(int(cp[-1]), L.ptr(cp, L.ip), L.ptr(ci, L.ip), L.ptr(vbi, L.ip),
+                                          L.ptr(A), L.ptr(Cw), L.ptr(et, L.ip))
+        self._check(self.lib.ufe_thickness_setup(self._ctx, C.byref(self._thk_geom)))
+        return g
+
+    def calc_dHi_dt_semiimplicit(self, Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, LMB, fraction_margin, mask_noice,
+                                 dt, dHi_dt_target=None, BC_prescr_mask=None, BC_prescr_Hi=None,
+                                 dHi_dt_guess=None, **params):
+        """One semi-implicit thickness step (calc_dHi_dt_semiimplicit,
+        conservation_of_mass_semiimplicit.f90:23-159): returns
+        (dHi_dt, Hi_tplusdt, divQ, dt_max, n_its).  u_vav_b = v_vav_b = None
+        takes the velocity currently held by this solver.  BC_prescr_mask and
+        BC_prescr_Hi go together, as in the reference.  params override
+        ufe_thickness_params (fs, bc_H, dt_ice_max, dt_ice_min, Hi_min, rtol,
+        abstol, krylov, gmres_restart, maxit); bc_H takes 'zero'/'infinite'
+        names or the integer codes."""
+        if (BC_prescr_mask is None) != (BC_prescr_Hi is None):
+            raise VelocitySolverError("need to provide prescribed both Hi and mask!")
+        if getattr(self, "_thk_geom", None) is None:
+            self.thickness_setup()
+        nV = self.md.mesh.nV
+        p = thickness_params(**params)
+        inp, _keep = make_thickness_inputs(nV, Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, LMB, fraction_margin,
+                                           mask_noice, dt, dHi_dt_target, BC_prescr_mask, BC_prescr_Hi,
+                                           dHi_dt_guess)
+        Hn, rate, divQ = np.zeros(nV), np.zeros(nV), np.zeros(nV)
+        dt_max, its = C.c_double(), C.c_int()
+        rc = self._check(self.lib.ufe_thickness_semiimplicit(self._ctx, C.byref(p), C.byref(inp), L.ptr(Hn),
+                                                             L.ptr(rate), L.ptr(divQ), C.byref(dt_max),
+                                                             C.byref(its)))
+        if rc == L.UFE_WARN_NOT_CONVERGED:
+            warnings.warn("thickness solve did not reach its tolerance", ViscosityIterationWarning)
+        return rate, Hn, divQ, dt_max.value, its.value
+
     def close(self):
         if self._ctx:
             self.lib.ufe_ctx_destroy(self._ctx)
@@ -292,6 +339,42 @@ class VelocitySolver:
             self.close()
         except Exception:
             pass
+
+
+def thickness_params(**params) -> L.ThicknessParams:
+    """ufe_thickness_params with the library defaults, overridden by params
+    (bc_H takes 'zero'/'infinite' names or the integer codes)."""
+    p = L.default_thickness_params()
+    names = dict(L.ThicknessParams._fields_)
+    for k, v in params.items():
+        if k == "bc_H":
+            for i, x in enumerate(v):
+                p.bc_H[i] = L.BC[x] if isinstance(x, str) else int(x)
+        elif k in names:
+            setattr(p, k, v)
+        else:
+            raise KeyError(f"unknown thickness parameter {k}")
+    return p
+
+
+def make_thickness_inputs(nV, Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, LMB, fraction_margin, mask_noice, dt,
+                          dHi_dt_target=None, BC_prescr_mask=None, BC_prescr_Hi=None, dHi_dt_guess=None):
+    """ufe_thickness_inputs over contiguous copies of the fields; returns the
+    structure and the arrays it points into (keep them alive while it is used)."""
+    keep = [L.f64(a) for a in (Hi, Hb, SL, SMB, BMB, LMB, fraction_margin,
+                               np.zeros(nV) if dHi_dt_target is None else dHi_dt_target)]
+    for a in keep:
+        if a.shape != (nV,):
+            raise VelocitySolverError(f"thickness input of shape {a.shape}, expected ({nV},)")
+    uv = (None, None) if u_vav_b is None else (L.f64(u_vav_b), L.f64(v_vav_b))
+    noice = L.i32(np.asarray(mask_noice).astype(np.int32))
+    bcm = None if BC_prescr_mask is None else L.i32(np.asarray(BC_prescr_mask).astype(np.int32))
+    bch = None if BC_prescr_Hi is None else L.f64(BC_prescr_Hi)
+    guess = None if dHi_dt_guess is None else L.f64(dHi_dt_guess)
+    inp = L.ThicknessInputs(L.ptr(keep[0]), L.ptr(keep[1]), L.ptr(keep[2]), L.ptr(uv[0]), L.ptr(uv[1]),
+                            L.ptr(keep[3]), L.ptr(keep[4]), L.ptr(keep[5]), L.ptr(keep[6]), L.ptr(keep[7]),
+                            L.ptr(noice, L.ip), L.ptr(bcm, L.ip), L.ptr(bch), L.ptr(guess), float(dt))
+    return inp, (keep, uv, noice, bcm, bch, guess)
 
 
 def make_heat_inputs(nV, nz, zeta, fields: dict, dt: float, Hi_min_thermo: float = 10.0):
@@ -409,7 +492,10 @@ def csr_matvec(A, x):
     return y
 
 
-def csr_solve(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, precond=L.PC_BLOCK_JACOBI):
+def csr_solve(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, precond=L.PC_BLOCK_JACOBI, krylov=None,
+              restart=30):
+    """Solve A x = b on the device (ufe_csr_solve, or ufe_csr_solve_ex when a
+    Krylov method is named: L.KRYLOV_BICGSTAB or L.KRYLOV_GMRES)."""
     import scipy.sparse as sp
     A = sp.csr_matrix(A)
     lib = L.lib()
@@ -417,8 +503,13 @@ def csr_solve(A, b, x0=None, rtol=1e-8, abstol=0.0, maxit=10000, precond=L.PC_BL
     ptr, ind, val = L.i32(A.indptr), L.i32(A.indices), L.f64(A.data)
     its = C.c_int(0)
     rel = C.c_double(0)
-    rc = lib.ufe_csr_solve(A.shape[0], A.nnz, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val), L.ptr(L.f64(b)),
-                           L.ptr(x), rtol, abstol, maxit, precond, C.byref(its), C.byref(rel))
+    if krylov is None:
+        rc = lib.ufe_csr_solve(A.shape[0], A.nnz, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val), L.ptr(L.f64(b)),
+                               L.ptr(x), rtol, abstol, maxit, precond, C.byref(its), C.byref(rel))
+    else:
+        rc = lib.ufe_csr_solve_ex(A.shape[0], A.nnz, L.ptr(ptr, L.ip), L.ptr(ind, L.ip), L.ptr(val),
+                                  L.ptr(L.f64(b)), L.ptr(x), rtol, abstol, maxit, precond, krylov, restart,
+                                  C.byref(its), C.byref(rel))
     if rc < 0:
         raise VelocitySolverError(lib.ufe_last_error(None).decode())
     return x, its.value, rel.value

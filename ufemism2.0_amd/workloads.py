@@ -339,3 +339,32 @@ def heat_columns_inputs(nV: int, nz: int = 12, seed: int = 6, dt: float = 10.0, 
         out["mask"] = np.where(on_gl, 3, out["mask"]).astype(np.int32)
         out["fraction_gr"] = rg.uniform(0.0, 1.0, nV)
     return {"nV": nV, "nz": nz, "zeta": zeta, "fields": out, "dt": dt}
+
+
+def thickness_step_inputs(w: Workload, u_vav_b: np.ndarray, v_vav_b: np.ndarray, seed: int = 8,
+                          dt: float = 5.0) -> dict:
+    """Inputs of one semi-implicit thickness step (calc_dHi_dt_semiimplicit,
+    conservation_of_mass_semiimplicit.f90:23-40) on a workload: its Hi, Hb,
+    SL, the given vertically averaged velocities, a surface mass balance of
+    +0.3 m/yr on thick ice falling to -2 m/yr on thin ice, basal melt of up to
+    5 m/yr under floating ice, no lateral mass balance, fraction_margin 1
+    where a vertex and all its neighbours carry ice, 0.5 on the ice margin
+    and 0 without ice (the reference's fraction of the cell covered), and no
+    target rate.  Units: m, m/yr, yr."""
+    rng = np.random.default_rng(seed)
+    m = w.mesh
+    nV = m.nV
+    Hi = np.ascontiguousarray(w.Hi, dtype=np.float64)
+    ice = Hi > 0.0
+    nb_ice = np.ones(nV, bool)
+    src = np.repeat(np.arange(nV), np.diff(m.C_ptr))
+    np.logical_and.at(nb_ice, src, ice[m.C_ind])
+    fm = np.where(ice & nb_ice, 1.0, np.where(ice, 0.5, 0.0))
+    SMB = 0.3 - 2.3 * np.exp(-Hi / 300.0) + rng.normal(0.0, 0.05, nV)
+    BMB = np.where(is_floating(Hi, w.Hb, w.SL) & ice, -rng.uniform(0.0, 5.0, nV), 0.0)
+    return {"Hi": Hi, "Hb": np.ascontiguousarray(w.Hb, dtype=np.float64),
+            "SL": np.ascontiguousarray(w.SL, dtype=np.float64),
+            "u_vav_b": np.ascontiguousarray(u_vav_b, dtype=np.float64),
+            "v_vav_b": np.ascontiguousarray(v_vav_b, dtype=np.float64),
+            "SMB": SMB, "BMB": BMB, "LMB": np.zeros(nV), "fraction_margin": fm,
+            "mask_noice": np.zeros(nV, np.int32), "dt": float(dt)}
