@@ -38,7 +38,9 @@ the GPU's plain and chosen forms on the same dome.
 Other keys: `roofline_out_of_cache` (the Krylov block product on the
 continental mesh, whose 326 MB per product exceed the 256 MB Infinity Cache),
 `oracle_parity_at_scale` (the chosen form against the committed oracle
-solution of a 20k-vertex DIVA channel), and the unscaled residual
+solution of a 20k-vertex DIVA channel), `thickness_step` (the semi-implicit
+ice-thickness step on the same mesh with the velocity the solve left on the
+device, host arrays for the other fields), and the unscaled residual
 ||b - A u|| / ||b|| of every solve next to the block-scaled one.
 """
 
@@ -378,6 +380,8 @@ def main():
         live[kern] = (n_t, ms_t, by_t)
     spmv_ms, spmv_bytes = solvers[0][1].profile_spmv(args.spmv_reps)
     heat = time_heat(V, W, w.mesh.nV, 12, args.steps, solvers[0][1]) if args.workload == "continental" else None
+    thickness = (time_thickness(V, W, w, solvers[0][1])
+                 if not args.no_extra and args.workload in ("channel", "continental") else None)
     for _, s in solvers:
         s.close()
     n_solves = args.steps * len(ws)
@@ -439,6 +443,8 @@ def main():
     }
     if heat is not None:
         out["heat_columns"] = heat
+    if thickness is not None:
+        out["thickness_step"] = thickness
     if not args.no_extra and args.workload == "channel":
         log("[bench] out-of-cache roofline (continental mesh)")
         out["roofline_out_of_cache"] = out_of_cache_roofline(V, W)
@@ -452,6 +458,29 @@ def main():
             log("[bench] full dome solves, CPU oracle and GPU")
             out["cpu_baseline"]["measured_full_solves"] = dome_full_solves(V, W)
     print(json.dumps(out), flush=True)
+
+
+def time_thickness(V, W, w, solver, reps=10):
+    """The semi-implicit ice-thickness step after the velocity solve, on the
+    same mesh, with the velocity the solve left on the device (no copy) and
+    the other fields as host arrays (their copies in and the outputs' copies
+    out are inside the time)."""
+    import numpy as np
+    u, v = solver.get_velocity()
+    inp = W.thickness_step_inputs(w, u, v)
+    t0 = time.perf_counter()
+    solver.thickness_setup()
+    t_setup = time.perf_counter() - t0
+    kw = dict(inp, u_vav_b=None, v_vav_b=None)
+    solver.calc_dHi_dt_semiimplicit(**kw)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rate, Hn, divQ, dt_max, its = solver.calc_dHi_dt_semiimplicit(**kw)
+    t = (time.perf_counter() - t0) / reps
+    return {"vertices": w.mesh.nV, "seconds_per_step": t, "steps_per_s": 1.0 / t, "krylov": its,
+            "krylov_method": "GMRES(30), block-Jacobi on vertex pairs", "rtol": 1e-8, "abstol": 1e-6,
+            "dt_years": inp["dt"], "max_abs_dHi_dt": float(np.abs(rate).max()), "setup_seconds": t_setup,
+            "note": "host arrays in and out (PCIe-inclusive), velocity taken from the solver's device state"}
 
 
 def time_heat(V, W, nV, nz, steps, solver):
