@@ -118,3 +118,53 @@ def test_fortran_heat_step_matches_c_abi(V, W, OB, tmp_path, gl_bc):
     rc, To, no = OB.heat_robin(rin, uo, To)
     assert rc == 0 and no == nrep
     assert np.abs(T - To).max() < 1e-9
+
+
+def test_thickness_case_file_layout(W, tmp_path):
+    from ufemism2_amd import fortran_case as FC
+    from ufemism2_amd.mesh import vertex_geometry
+    w = W.dome(h=120e3)
+    m = w.mesh
+    g = vertex_geometry(m)
+    inp = W.thickness_step_inputs(w, np.zeros(m.nT), np.zeros(m.nT))
+    path = str(tmp_path / "thk.bin")
+    FC.write_thickness_case(path, m, g, inp)
+    nC_mem = int(np.diff(m.C_ptr).max())
+    nE = int(m.C_ptr[-1]) // 2
+    n_int = 6 + m.nV + 2 * m.nV * nC_mem + 2 * nE + m.nV + m.nV
+    n_dbl = 1 + m.nV + m.nV * nC_mem + 6 * m.nV
+    assert os.path.getsize(path) == 4 * n_int + 8 * n_dbl
+
+
+@pytest.mark.gpu
+def test_fortran_thickness_step_matches_c_abi(V, W, OB, tmp_path):
+    """calc_dHi_dt_semiimplicit_gpu, set up from the reference's mesh arrays
+    (nC, C, VE, ETri, A, Cw) after a DIVA solve in the same Fortran program,
+    equals the Python host's call on the same velocities bit for bit, and
+    the oracle to 1e-9 of the largest thickness."""
+    from ufemism2_amd import fortran_case as FC
+    from ufemism2_amd.mesh import vertex_geometry
+    w = W.channel_small(target_nV=1500, stress_balance="DIVA")
+    ctl = dict(stop_rule=1, visc_it_nit=800, include_crossterms=0, precond=3, warm_start=1, accel_depth=2,
+               accel_mixing=0.6, nonlinear_rtol=1e-8, vel_max=1e5, abstol=0.0, rtol=0.1)
+    case, out = str(tmp_path / "case.bin"), str(tmp_path / "out.bin")
+    tcase, tout = str(tmp_path / "thk.bin"), str(tmp_path / "thk_out.bin")
+    FC.write_case(case, w, stress="DIVA", **ctl)
+    g = vertex_geometry(w.mesh)
+    # the velocities are not known before the solve; the thickness case holds the rest
+    inp = W.thickness_step_inputs(w, np.zeros(w.mesh.nT), np.zeros(w.mesh.nT))
+    FC.write_thickness_case(tcase, w.mesh, g, inp, bc_H=(0, 1, 0, 1))
+    print(FC.run_driver(case, out, extra=(tcase, tout)), flush=True)
+    uf, vf, _, _, _ = FC.read_result(out, w.mesh.nT)
+    Hf, rf, qf, dtf, itf = FC.read_thickness_result(tout, w.mesh.nV)
+    kw = dict(inp, u_vav_b=uf, v_vav_b=vf)
+    bc = ["infinite", "zero", "infinite", "zero"]
+    s = V.VelocitySolver(w.descriptor())
+    s.thickness_setup(g)
+    rate, Hn, divQ, dt_max, its = s.calc_dHi_dt_semiimplicit(**kw, bc_H=bc, dHi_dt_guess=np.zeros(w.mesh.nV))
+    s.close()
+    assert np.array_equal(Hf, Hn) and np.array_equal(rf, rate) and np.array_equal(qf, divQ)
+    assert dtf == dt_max and itf == its
+    rc, _, Ho, _, _, _ = OB.thickness_semiimplicit(w.mesh, g, **kw, bc_H=bc, rtol=1e-12, abstol=0.0)
+    assert rc == 0
+    assert np.abs(Hf - Ho).max() < 1e-5 * inp["Hi"].max()

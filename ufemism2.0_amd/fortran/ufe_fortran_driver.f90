@@ -1,9 +1,12 @@
 ! ufe_fortran_driver -- runs one velocity solve through ufe_velocity_gpu from a
 ! case file (raw little-endian stream written by the Python host, see
 ! ufemism2.0_amd/fortran_case.py) and writes u_b, v_b and the iteration
-! counts.  Used by the tests to show the Fortran interface end to end.
+! counts.  With a thickness case it then takes one semi-implicit thickness
+! step with the solved velocities (the reference's order: stress balance, then
+! calc_dHi_dt_semiimplicit) and writes Hi_tplusdt, dHi_dt, divQ, dt_max and
+! the Krylov count.  Used by the tests to show the Fortran interface end to end.
 !
-!   ufe_fortran_driver CASE_FILE OUT_FILE
+!   ufe_fortran_driver CASE_FILE OUT_FILE [THICKNESS_CASE THICKNESS_OUT]
 program ufe_fortran_driver
 
   use iso_c_binding
@@ -67,12 +70,51 @@ program ufe_fortran_driver
     call solve_DIVA_gpu(g, Hi, Hs, A_flow, beta_sq, fgr, mask /= 0, u_b, v_b, ub, vb, u3, v3, eta3, tbx, tby, &
                         n_visc_its, n_Axb_its, info=info)
   end if
-  call ufe_gpu_finalise(g)
 
   open(newunit=u_out, file=trim(out_file), access='stream', form='unformatted', status='replace', action='write')
   write(u_out) u_b, v_b, int(n_visc_its, c_int), int(n_Axb_its, c_int), info%rel_residual
   close(u_out)
   write(*, '(a,i0,a,i0,a,es10.3)') 'solve done: picard ', n_visc_its, ' krylov ', n_Axb_its, ' residual ', &
     info%rel_residual
+
+  if (command_argument_count() >= 4) call thickness_step(g, Hi, u_b, v_b)
+  call ufe_gpu_finalise(g)
+
+contains
+
+  subroutine thickness_step(g, Hi, u_b, v_b)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in) :: Hi(:), u_b(:), v_b(:)
+    character(len=4096) :: tcase, tout
+    integer(c_int) :: th(6)
+    integer :: nC_mem, nE, n_its, uin, uout
+    integer(c_int), allocatable :: nC(:), C(:, :), VE(:, :), ETri(:, :), VBI(:), noice(:)
+    real(c_double), allocatable :: A(:), Cw(:, :), Hb(:), SL(:), SMB(:), BMB(:), LMB(:), fm(:), tgt(:)
+    real(c_double), allocatable :: AMB(:), dHi_dt(:), Hn(:), divQ(:)
+    real(c_double) :: dt, dt_max
+    type(ufe_gpu_thickness_params) :: tp
+
+    call get_command_argument(3, tcase)
+    call get_command_argument(4, tout)
+    open(newunit=uin, file=trim(tcase), access='stream', form='unformatted', status='old', action='read')
+    read(uin) th, dt
+    nC_mem = th(1); nE = th(2)
+    allocate(nC(nV), C(nV, nC_mem), VE(nV, nC_mem), ETri(nE, 2), VBI(nV), noice(nV), A(nV), Cw(nV, nC_mem), &
+             Hb(nV), SL(nV), SMB(nV), BMB(nV), LMB(nV), fm(nV), tgt(nV), AMB(nV), dHi_dt(nV), Hn(nV), divQ(nV))
+    read(uin) nC, C, VE, ETri, VBI, A, Cw, Hb, SL, SMB, BMB, LMB, fm, noice
+    close(uin)
+    call ufe_gpu_thickness_setup(g, nC, C, VBI, A, Cw, VE, ETri)
+    tp = ufe_gpu_default_thickness_params()
+    tp%bc_H = th(3:6)
+    tgt = 0.0_c_double
+    AMB = 0.0_c_double
+    dHi_dt = 0.0_c_double
+    call calc_dHi_dt_semiimplicit_gpu(g, Hi, Hb, SL, u_b, v_b, SMB, BMB, LMB, AMB, fm, noice /= 0, dt, dHi_dt, &
+                                      Hn, divQ, tgt, params=tp, dt_max=dt_max, n_Axb_its=n_its)
+    open(newunit=uout, file=trim(tout), access='stream', form='unformatted', status='replace', action='write')
+    write(uout) Hn, dHi_dt, divQ, dt_max, int(n_its, c_int)
+    close(uout)
+    write(*, '(a,i0,a,es10.3)') 'thickness step done: krylov ', n_its, ' dt_max ', dt_max
+  end subroutine thickness_step
 
 end program ufe_fortran_driver

@@ -6,6 +6,8 @@
 !   solve_DIVA_gpu <-> solve_DIVA (DIVA_main.f90:88-262)
 !   solve_3D_heat_equation_gpu <-> the column solve, the 1% rule and the Robin
 !     replacement of solve_3D_heat_equation (thermodynamics_3D_heat_equation.f90:33-307)
+!   calc_dHi_dt_semiimplicit_gpu <-> calc_dHi_dt_semiimplicit
+!     (conservation_of_mass_semiimplicit.f90:23-173), after ufe_gpu_thickness_setup
 ! Arrays are the caller's 1-based Fortran arrays; (n, nz) fields are passed
 ! as they lie in memory (first index fastest).  Mesh connectivity and operator
 ! indices are converted to 0-based once, in ufe_gpu_init.  A failing call
@@ -20,7 +22,8 @@ module ufe_velocity_gpu
   private
   public :: ufe_gpu_context, ufe_gpu_params, ufe_gpu_info, ufe_gpu_init, ufe_gpu_finalise, &
             ufe_gpu_set_params, ufe_gpu_default_params, solve_SSA_gpu, solve_DIVA_gpu, &
-            ufe_gpu_fraction_gr_b, solve_3D_heat_equation_gpu
+            ufe_gpu_fraction_gr_b, solve_3D_heat_equation_gpu, ufe_gpu_thickness_params, &
+            ufe_gpu_default_thickness_params, ufe_gpu_thickness_setup, calc_dHi_dt_semiimplicit_gpu
 
   integer(c_int), parameter, public :: UFE_OK = 0, UFE_WARN_NOT_CONVERGED = 1
 
@@ -80,6 +83,27 @@ module ufe_velocity_gpu
     type(c_ptr)    :: zeta, Hi_eff, T_surf, SMB, geothermal_flux, mask
     real(c_double) :: Hi_min_thermo
   end type ufe_heat_robin_inputs
+
+  ! struct ufe_vertex_geometry
+  type, bind(C) :: ufe_vertex_geometry
+    integer(c_int) :: nC
+    type(c_ptr)    :: C_ptr, C_ind, VBI, A, Cw, ETri
+  end type ufe_vertex_geometry
+
+  ! struct ufe_thickness_params
+  type, bind(C) :: ufe_gpu_thickness_params
+    real(c_double) :: fs
+    integer(c_int) :: bc_H(4)
+    real(c_double) :: dt_ice_max, dt_ice_min, Hi_min, rtol, abstol
+    integer(c_int) :: krylov, gmres_restart, maxit
+  end type ufe_gpu_thickness_params
+
+  ! struct ufe_thickness_inputs
+  type, bind(C) :: ufe_thickness_inputs
+    type(c_ptr)    :: Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, LMB, fraction_margin, dHi_dt_target, &
+                      mask_noice, BC_prescr_mask, BC_prescr_Hi, dHi_dt_guess
+    real(c_double) :: dt
+  end type ufe_thickness_inputs
 
   ! A device context plus the 0-based copies of the mesh arrays it was built from.
   type :: ufe_gpu_context
@@ -142,6 +166,27 @@ module ufe_velocity_gpu
       type(c_ptr), value :: ctx, unstable, Ti
       type(ufe_heat_robin_inputs), intent(in) :: rin
       integer(c_int), intent(out) :: n_replaced
+      integer(c_int) :: rc
+    end function
+    subroutine c_ufe_thickness_params_default(p) bind(C, name='ufe_thickness_params_default')
+      import :: ufe_gpu_thickness_params
+      type(ufe_gpu_thickness_params), intent(out) :: p
+    end subroutine
+    function c_ufe_thickness_setup(ctx, geom) bind(C, name='ufe_thickness_setup') result(rc)
+      import :: c_ptr, c_int, ufe_vertex_geometry
+      type(c_ptr), value :: ctx
+      type(ufe_vertex_geometry), intent(in) :: geom
+      integer(c_int) :: rc
+    end function
+    function c_ufe_thickness_semiimplicit(ctx, p, inp, Hn, dHdt, divQ, dt_max, n_its) &
+        bind(C, name='ufe_thickness_semiimplicit') result(rc)
+      import :: c_ptr, c_int, c_double, ufe_gpu_thickness_params, ufe_thickness_inputs
+      type(c_ptr), value :: ctx
+      type(ufe_gpu_thickness_params), intent(in) :: p
+      type(ufe_thickness_inputs), intent(in) :: inp
+      type(c_ptr), value :: Hn, dHdt, divQ
+      real(c_double), intent(out) :: dt_max
+      integer(c_int), intent(out) :: n_its
       integer(c_int) :: rc
     end function
     function c_ufe_fraction_gr_b(ctx, Hi, Hb, SL, fgr) bind(C, name='ufe_fraction_gr_b') result(rc)
@@ -437,6 +482,118 @@ contains
                   'solve_3D_heat_equation_gpu (unstable columns)')
     n_unstable = int(nun)
   end subroutine solve_3D_heat_equation_gpu
+
+  function ufe_gpu_default_thickness_params() result(p)
+    type(ufe_gpu_thickness_params) :: p
+    call c_ufe_thickness_params_default(p)
+  end function ufe_gpu_default_thickness_params
+
+  ! Attach the Voronoi geometry the thickness step needs, from the reference's
+  ! mesh arrays: nC(nV), C(nV, nC_mem) (1-based neighbours), VBI(nV), A(nV),
+  ! Cw(nV, nC_mem), VE(nV, nC_mem) (1-based edge of each connection) and
+  ! ETri(nE, 2) (triangles either side of each edge, 0 = none).
+  subroutine ufe_gpu_thickness_setup(g, nC, C, VBI, A, Cw, VE, ETri)
+    type(ufe_gpu_context), intent(in) :: g
+    integer, intent(in) :: nC(:), C(:, :), VBI(:), VE(:, :), ETri(:, :)
+    real(c_double), intent(in) :: A(:), Cw(:, :)
+    integer(c_int), allocatable, target :: c_ptr_(:), c_ind(:), vbi0(:), etri_c(:)
+    real(c_double), allocatable, target :: A0(:), cw_c(:)
+    type(ufe_vertex_geometry) :: geom
+    integer :: nV, vi, ci, k, ei
+
+    nV = size(nC)
+    allocate(c_ptr_(nV + 1), c_ind(sum(nC)), cw_c(sum(nC)), etri_c(2 * sum(nC)), vbi0(nV), A0(nV))
+    c_ptr_(1) = 0
+    k = 0
+    do vi = 1, nV
+      do ci = 1, nC(vi)
+        k = k + 1
+        c_ind(k) = C(vi, ci) - 1
+        cw_c(k) = Cw(vi, ci)
+        ei = VE(vi, ci)
+        etri_c(2 * k - 1) = ETri(ei, 1) - 1
+        etri_c(2 * k) = ETri(ei, 2) - 1
+      end do
+      c_ptr_(vi + 1) = k
+    end do
+    vbi0 = VBI
+    A0 = A
+    geom%nC = k
+    geom%C_ptr = c_loc(c_ptr_)
+    geom%C_ind = c_loc(c_ind)
+    geom%VBI = c_loc(vbi0)
+    geom%A = c_loc(A0)
+    geom%Cw = c_loc(cw_c)
+    geom%ETri = c_loc(etri_c)
+    call crash_on(c_ufe_thickness_setup(g%ctx, geom), g%ctx, 'ufe_gpu_thickness_setup')
+  end subroutine ufe_gpu_thickness_setup
+
+  ! calc_dHi_dt_semiimplicit (conservation_of_mass_semiimplicit.f90:23-173):
+  ! dHi_dt is the previous rate on entry (the initial guess of the linear
+  ! solve is Hi + dt dHi_dt, as in the reference) and the new rate on exit;
+  ! AMB comes back as the reference's arithmetic leaves it (dHi_dt minus the
+  ! same rate, zero).  The prescribed-thickness condition needs both
+  ! BC_prescr_mask and BC_prescr_Hi.  params defaults to
+  ! ufe_gpu_default_thickness_params() (dHi_semiimplicit_fs 1.5, 'zero'
+  ! borders, dHi_PETSc_rtol 1e-8, dHi_PETSc_abstol 1e-6).
+  subroutine calc_dHi_dt_semiimplicit_gpu(g, Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, LMB, AMB, fraction_margin, &
+      mask_noice, dt, dHi_dt, Hi_tplusdt, divQ, dHi_dt_target, BC_prescr_mask, BC_prescr_Hi, params, dt_max, &
+      n_Axb_its)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Hi(:), Hb(:), SL(:), u_vav_b(:), v_vav_b(:), SMB(:), BMB(:), LMB(:)
+    real(c_double), intent(inout) :: AMB(:)
+    real(c_double), intent(in), target :: fraction_margin(:)
+    logical, intent(in) :: mask_noice(:)
+    real(c_double), intent(inout) :: dt
+    real(c_double), intent(inout), target :: dHi_dt(:)
+    real(c_double), intent(out), target :: Hi_tplusdt(:), divQ(:)
+    real(c_double), intent(in), target :: dHi_dt_target(:)
+    integer, intent(in), optional :: BC_prescr_mask(:)
+    real(c_double), intent(in), optional, target :: BC_prescr_Hi(:)
+    type(ufe_gpu_thickness_params), intent(in), optional :: params
+    real(c_double), intent(out), optional :: dt_max
+    integer, intent(out), optional :: n_Axb_its
+    type(ufe_gpu_thickness_params) :: p
+    type(ufe_thickness_inputs) :: inp
+    integer(c_int), allocatable, target :: noice(:), bcm(:)
+    real(c_double), allocatable, target :: guess(:), rate(:)
+    real(c_double) :: dtm
+    integer(c_int) :: its, rc
+
+    if (present(BC_prescr_mask) .neqv. present(BC_prescr_Hi)) then
+      write(0, '(a)') 'calc_dHi_dt_semiimplicit_gpu: need to provide prescribed both Hi and mask!'
+      error stop 1
+    end if
+    if (present(params)) then
+      p = params
+    else
+      p = ufe_gpu_default_thickness_params()
+    end if
+    noice = merge(1_c_int, 0_c_int, mask_noice)
+    guess = dHi_dt
+    allocate(rate(size(Hi)))
+    inp%Hi = c_loc(Hi); inp%Hb = c_loc(Hb); inp%SL = c_loc(SL)
+    inp%u_vav_b = c_loc(u_vav_b); inp%v_vav_b = c_loc(v_vav_b)
+    inp%SMB = c_loc(SMB); inp%BMB = c_loc(BMB); inp%LMB = c_loc(LMB)
+    inp%fraction_margin = c_loc(fraction_margin); inp%dHi_dt_target = c_loc(dHi_dt_target)
+    inp%mask_noice = c_loc(noice)
+    inp%BC_prescr_mask = c_null_ptr
+    inp%BC_prescr_Hi = c_null_ptr
+    if (present(BC_prescr_mask)) then
+      bcm = int(BC_prescr_mask, c_int)
+      inp%BC_prescr_mask = c_loc(bcm)
+      inp%BC_prescr_Hi = c_loc(BC_prescr_Hi)
+    end if
+    inp%dHi_dt_guess = c_loc(guess)
+    inp%dt = dt
+    rc = c_ufe_thickness_semiimplicit(g%ctx, p, inp, c_loc(Hi_tplusdt), c_loc(rate), c_loc(divQ), dtm, its)
+    call crash_on(rc, g%ctx, 'calc_dHi_dt_semiimplicit_gpu')
+    if (rc == UFE_WARN_NOT_CONVERGED) write(0, '(a)') 'WARNING: thickness solve did not reach its tolerance'
+    dHi_dt = rate
+    AMB = dHi_dt - rate
+    if (present(dt_max)) dt_max = dtm
+    if (present(n_Axb_its)) n_Axb_its = its
+  end subroutine calc_dHi_dt_semiimplicit_gpu
 
   subroutine bc_pointers(m, u, v, bcm, pbm, pbu, pbv)
     integer, intent(in), optional :: m(:)

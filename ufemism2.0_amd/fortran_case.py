@@ -40,6 +40,50 @@ def write_case(path, w, stress="SSA", **ctl):
             f.write(np.ascontiguousarray(p).tobytes())
 
 
+def write_thickness_case(path, mesh, geom, inp, bc_H=(1, 1, 1, 1)):
+    """The reference's mesh arrays of the thickness step, 1-based: nC(nV),
+    C(nV, nC_mem), VE(nV, nC_mem) (edge of each connection), ETri(nE, 2)
+    (triangles either side of each edge, 0 = none), VBI, A, Cw(nV, nC_mem),
+    then Hb, SL, SMB, BMB, LMB, fraction_margin, mask_noice and, in the
+    header, nC_mem, nE, BC_H (north, east, south, west: 0 infinite, 1 zero)
+    and dt.  Hi and the velocities are those of the velocity case."""
+    nV = mesh.nV
+    cnt = np.diff(mesh.C_ptr)
+    nC_mem = int(cnt.max())
+    src = np.repeat(np.arange(nV), cnt)
+    dst = mesh.C_ind.astype(np.int64)
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    keys, edge = np.unique(lo * nV + hi, return_inverse=True)
+    nE = len(keys)
+    ETri = np.zeros((nE, 2), np.int32)
+    fwd = src < dst   # the edge's own direction: from its lower vertex
+    ETri[edge[fwd]] = geom.ETri[fwd] + 1
+    C = np.zeros((nV, nC_mem), np.int32)
+    VE = np.zeros((nV, nC_mem), np.int32)
+    Cw = np.zeros((nV, nC_mem))
+    col = np.arange(len(dst)) - np.repeat(mesh.C_ptr[:-1], cnt)
+    C[src, col] = dst + 1
+    VE[src, col] = edge + 1
+    Cw[src, col] = geom.Cw
+    ff = lambda a: np.asfortranarray(a).ravel(order="F")  # noqa: E731
+    i32 = lambda a: np.asarray(a, dtype="<i4")  # noqa: E731
+    f64 = lambda a: np.asarray(a, dtype="<f8")  # noqa: E731
+    parts = [i32([nC_mem, nE, *bc_H]), f64([inp["dt"]]), i32(cnt), i32(ff(C)), i32(ff(VE)), i32(ff(ETri)),
+             i32(mesh.VBI), f64(geom.A), f64(ff(Cw))]
+    parts += [f64(inp[k]) for k in ("Hb", "SL", "SMB", "BMB", "LMB", "fraction_margin")]
+    parts.append(i32(inp["mask_noice"]))
+    with open(path, "wb") as f:
+        for p in parts:
+            f.write(np.ascontiguousarray(p).tobytes())
+
+
+def read_thickness_result(path, nV):
+    raw = open(path, "rb").read()
+    a = np.frombuffer(raw, "<f8", 3 * nV + 1, 0)
+    its = int(np.frombuffer(raw, "<i4", 1, 8 * (3 * nV + 1))[0])
+    return a[:nV].copy(), a[nV:2 * nV].copy(), a[2 * nV:3 * nV].copy(), float(a[3 * nV]), its
+
+
 GL_BC = {"grounded": 0, "subgrid": 1, "pmp": 2}
 
 
@@ -69,10 +113,10 @@ def read_heat_result(path, nV, nz):
     return T, n_unstable
 
 
-def run_driver(case, out, timeout=300, driver=DRIVER):
+def run_driver(case, out, timeout=300, driver=DRIVER, extra=()):
     if not os.path.exists(driver):
         raise RuntimeError(f"{driver} is missing: build it first")
-    r = subprocess.run([driver, case, out], capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run([driver, case, out, *extra], capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"Fortran driver failed ({r.returncode}): {r.stdout}\n{r.stderr}")
     return r.stdout
