@@ -38,6 +38,15 @@ extern "C" {
 
 #define UFE_BC_INFINITE            0
 #define UFE_BC_ZERO                1
+/* Lagged copy rows of the idealised experiments (solve_linearised_SSA_DIVA.f90:543-575,
+ * :618-635): the border velocity is set to visc_it_relax * (weighted mean of the
+ * previous iterate around a copy point) + (1 - visc_it_relax) * its previous value.
+ * 'periodic_ISMIP-HOM': copy point = geometric centre shifted by ismip_hom_L / 2 towards
+ * the domain centre in x and y (u and v).  'infinite_SSA_icestream': same y, x at one
+ * third (west half) or two thirds (east half) of the domain (u only, as in the
+ * reference).  Weights 1/d^2 to the triangles around the vertex nearest the copy point. */
+#define UFE_BC_PERIODIC_ISMIP_HOM      2
+#define UFE_BC_INFINITE_SSA_ICESTREAM  3
 
 #define UFE_PREC_FP64              0  /* plain form: FP64 Krylov, reference tolerances */
 #define UFE_PREC_MIXED             1  /* FP32 Krylov inner solves, FP64 refinement residual */
@@ -136,6 +145,7 @@ typedef struct ufe_params {
    * preconditioner need not be exact, so this changes iteration counts at most
    * slightly and halves the bytes of the V-cycle's finest-level passes. */
   int    precond_fp32;               /*                                  0      */
+  double ismip_hom_L;                /* refgeo_idealised_ISMIP_HOM_L     0 m (periodic_ISMIP-HOM needs > 0) */
 } ufe_params;
 
 #define UFE_ACCEL_MAX_DEPTH 8
@@ -386,6 +396,35 @@ typedef struct ufe_heat_robin_inputs {
 } ufe_heat_robin_inputs;
 int ufe_heat_robin(ufe_ctx *ctx, const ufe_heat_robin_inputs *in, const int *is_unstable, double *Ti,
                    int *n_replaced);
+
+/* ---- inputs of the heat columns ----------------------------------------
+ * What the reference computes before the column solve
+ * (thermodynamics_utilities.f90), on the device.  Fields (n, nz) keep the
+ * reference's layout, vertex / triangle index fastest.
+ *   ufe_heat_setup: the triangles around each vertex, counter-clockwise
+ *     (mesh%iTri / mesh%niTri as CSR, 0-based); checked and kept on the device.
+ *   ufe_heat_upwind_derivatives: calc_upwind_heat_flux_derivatives
+ *     (:339-434): per vertex with Hi >= 1 m the first of its triangles whose
+ *     corner at the vertex contains (-u_vav, -v_vav), then per layer
+ *     u_3D_b * ddx_a_b(Ti) and v_3D_b * ddy_a_b(Ti) on that triangle (the
+ *     context's a->b operators); 0 where Hi < 1 m; UFE_ERR_MESH with the
+ *     reference's "could not find upwind triangle!" otherwise.
+ *   ufe_heat_strain_heating: calc_strain_heating (:24-91),
+ *     Phi = 2 A^(-1/n) D^(1/n+1) from the nine 3-D strain rates (order du_dx,
+ *     du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz; each (nV, nz)),
+ *     n = the context's glens_n; 0 where Hi < 0.1 m.
+ *   ufe_heat_frictional_heating: calc_frictional_heating (:93-128), the basal
+ *     friction coefficient times the basal speed where mask_grounded_ice, 0
+ *     elsewhere and everywhere when the context's sliding law is
+ *     UFE_SLIDING_NO_SLIDING. */
+int ufe_heat_setup(ufe_ctx *ctx, const int *iTri_ptr, const int *iTri_ind);
+int ufe_heat_upwind_derivatives(ufe_ctx *ctx, const double *Ti, const double *Hi, const double *u_vav,
+                                const double *v_vav, const double *u_3D_b, const double *v_3D_b,
+                                double *u_times_dTdxp_upwind, double *v_times_dTdyp_upwind);
+int ufe_heat_strain_heating(ufe_ctx *ctx, const double *Hi, const double *A_flow, const double *const *strain_rates,
+                            double *Phi);
+int ufe_heat_frictional_heating(ufe_ctx *ctx, const double *basal_friction_coefficient, const double *uabs_base,
+                                const int *mask_grounded_ice, double *frictional_heating);
 
 /* ---- ice thickness: semi-implicit time step ---------------------------
  * calc_dHi_dt_semiimplicit (conservation_of_mass_semiimplicit.f90:23-159), the

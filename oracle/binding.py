@@ -58,6 +58,9 @@ def lib():
     o.oracle_thickness_semiimplicit.argtypes = [C.c_int, dp, C.POINTER(L.VertexGeometry),
                                                 C.POINTER(L.ThicknessParams), C.POINTER(L.ThicknessInputs),
                                                 dp, dp, dp, dp, ip]
+    o.oracle_heat_upwind.argtypes = [C.POINTER(L.MeshDesc), ip, ip, dp, dp, dp, dp, dp, dp, dp, dp]
+    o.oracle_strain_heating.argtypes = [C.c_int, C.c_int, dp, dp, C.POINTER(dp), C.c_double, dp]
+    o.oracle_frictional_heating.argtypes = [C.c_int, C.c_int, dp, dp, ip, dp]
     o.oracle_zeta_tridiagonals.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
     _o = o
     return o
@@ -232,3 +235,33 @@ def thickness_semiimplicit(mesh, geom, Hi, Hb, SL, u_vav_b, v_vav_b, SMB, BMB, L
     rc = lib().oracle_thickness_semiimplicit(nV, L.ptr(keep[6]), C.byref(g), C.byref(p), C.byref(inp), L.ptr(Hn),
                                              L.ptr(rate), L.ptr(divQ), C.byref(dt_max), C.byref(its))
     return rc, rate, Hn, divQ, dt_max.value, its.value
+
+
+def heat_upwind(md, Ti, Hi, u_vav, v_vav, u_3D_b, v_3D_b):
+    """calc_upwind_heat_flux_derivatives of the oracle: (rc, u_dTdxp, v_dTdyp)."""
+    L = _L()
+    m = md.mesh
+    nz = len(md.zeta)
+    ox, oy = np.zeros(nz * m.nV), np.zeros(nz * m.nV)
+    keep = [L.i32(m.iTri_ptr), L.i32(m.iTri_ind)] + [L.f64(x) for x in (Ti, Hi, u_vav, v_vav, u_3D_b, v_3D_b)]
+    rc = lib().oracle_heat_upwind(md.byref(), L.ptr(keep[0], L.ip), L.ptr(keep[1], L.ip),
+                                  *(L.ptr(x) for x in keep[2:]), L.ptr(ox), L.ptr(oy))
+    return rc, ox, oy
+
+
+def strain_heating(nV, nz, Hi, A_flow, strain_rates, glen_n=3.0):
+    L = _L()
+    keep = [L.f64(x) for x in strain_rates]
+    arr = (L.dp * 9)(*(L.ptr(x) for x in keep))
+    Phi = np.zeros(nz * nV)
+    lib().oracle_strain_heating(nV, nz, L.ptr(L.f64(Hi)), L.ptr(L.f64(A_flow)), arr, glen_n, L.ptr(Phi))
+    return Phi
+
+
+def frictional_heating(beta, uabs_base, mask_grounded, no_sliding=False):
+    L = _L()
+    Q = np.zeros(len(beta))
+    mk = L.i32(np.asarray(mask_grounded).astype(np.int32))
+    lib().oracle_frictional_heating(len(beta), int(no_sliding), L.ptr(L.f64(beta)), L.ptr(L.f64(uabs_base)),
+                                    L.ptr(mk, L.ip), L.ptr(Q))
+    return Q

@@ -326,12 +326,97 @@ static void sliding_beta(const ufe_mesh_desc *m, const ufe_params *p, const doub
 
 static int bc_side(int tribi) { return (tribi - 1) / 2; } /* 0 N, 1 E, 2 S, 3 W */
 
+/* Lagged copy rows of the idealised boundary conditions
+ * (calc_SSA_DIVA_stiffness_matrix_row_BC, solve_linearised_SSA_DIVA.f90:543-575,
+ * :618-635): the velocity at a border triangle is set to a weighted mean of
+ * the previous iterate around a copy point, relaxed with visc_it_relax.  The
+ * copy point (find_ti_copy_ISMIP_HOM_periodic, find_ti_copy_SSA_icestream_infinite,
+ * mesh_utilities.f90:2622-2730): 'periodic_ISMIP-HOM' shifts the triangle's
+ * geometric centre by L/2 towards the domain centre in x and in y;
+ * 'infinite_SSA_icestream' keeps y and takes x at a third (west half) or two
+ * thirds (east half) of the domain.  Its weights: 1/d^2 to the geometric
+ * centres of the triangles around the vertex nearest to the point (the one
+ * whose Voronoi cell holds it), normalised.  copy->ptr[row] .. ptr[row+1]
+ * index the triangles and weights of row 2*ti+uv (empty for other rows). */
+typedef struct {
+  int *ptr, *tri;
+  double *w;
+} copy_lists_t;
+
+static void copy_lists_free(copy_lists_t *c) {
+  free(c->ptr); free(c->tri); free(c->w);
+  c->ptr = c->tri = NULL; c->w = NULL;
+}
+
+static int is_copy_choice(int choice) {
+  return choice == UFE_BC_PERIODIC_ISMIP_HOM || choice == UFE_BC_INFINITE_SSA_ICESTREAM;
+}
+
+static void copy_lists_build(const ufe_mesh_desc *m, const ufe_params *p, copy_lists_t *c) {
+  const int nT = m->nT, nV = m->nV;
+  double xmin = 1e300, xmax = -1e300;
+  for (int vi = 0; vi < nV; ++vi) { xmin = fmin(xmin, m->V[2 * vi]); xmax = fmax(xmax, m->V[2 * vi]); }
+  c->ptr = calloc(2 * (size_t)nT + 1, sizeof(int));
+  int cap = 64, n = 0;
+  c->tri = malloc(sizeof(int) * cap);
+  c->w = malloc(sizeof(double) * cap);
+  for (int row = 0; row < 2 * nT; ++row) {
+    const int ti = row / 2, uv = row % 2;
+    c->ptr[row] = n;
+    if (m->TriBI[ti] <= 0) continue;
+    const int side = bc_side(m->TriBI[ti]);
+    const int choice = uv == 0 ? p->bc_u[side] : p->bc_v[side];
+    if (!is_copy_choice(choice)) continue;
+    const double gx = m->TriGC[2 * ti], gy = m->TriGC[2 * ti + 1];
+    double px, py;
+    if (choice == UFE_BC_PERIODIC_ISMIP_HOM) {
+      px = gx > 0.0 ? gx - p->ismip_hom_L / 2.0 : gx + p->ismip_hom_L / 2.0;
+      py = gy > 0.0 ? gy - p->ismip_hom_L / 2.0 : gy + p->ismip_hom_L / 2.0;
+    } else {
+      py = gy;
+      px = gx < 0.0 ? xmin + (xmax - xmin) * 1.0 / 3.0 : xmin + (xmax - xmin) * 2.0 / 3.0;
+    }
+    int vbest = 0;
+    double dbest = 1e300;
+    for (int vi = 0; vi < nV; ++vi) {
+      const double dx = m->V[2 * vi] - px, dy = m->V[2 * vi + 1] - py, d = dx * dx + dy * dy;
+      if (d < dbest) { dbest = d; vbest = vi; }
+    }
+    const int k0 = m->ba_ptr[vbest], k1 = m->ba_ptr[vbest + 1];
+    if (n + (k1 - k0) > cap) {
+      cap = 2 * (n + (k1 - k0));
+      c->tri = realloc(c->tri, sizeof(int) * cap);
+      c->w = realloc(c->w, sizeof(double) * cap);
+    }
+    double sum = 0.0;
+    for (int k = k0; k < k1; ++k) {
+      const int tj = m->ba_ind[k];
+      const double dx = px - m->TriGC[2 * tj], dy = py - m->TriGC[2 * tj + 1];
+      const double d = sqrt(dx * dx + dy * dy);
+      c->tri[n + (k - k0)] = tj;
+      c->w[n + (k - k0)] = 1.0 / (d * d);
+      sum += 1.0 / (d * d);
+    }
+    for (int k = 0; k < k1 - k0; ++k) c->w[n + k] /= sum;
+    n += k1 - k0;
+  }
+  c->ptr[2 * nT] = n;
+}
+
+static double copy_value(const copy_lists_t *c, const ufe_params *p, int row, const double *u, const double *v) {
+  const double *f = row % 2 == 0 ? u : v;
+  double val = 0.0;
+  for (int k = c->ptr[row]; k < c->ptr[row + 1]; ++k) val += c->w[k] * f[c->tri[k]];
+  return (p->visc_it_relax * val) + ((1.0 - p->visc_it_relax) * f[row / 2]);
+}
+
 /* Assemble A and b in the reference's row order 2*ti+uv and column order
  * (stencil entry k: u column then v column; BC rows: neighbours then diagonal). */
 static void assemble(const ufe_mesh_desc *m, const ufe_params *p, const double *N_b,
                      const double *dNdx, const double *dNdy, const double *beta_b,
                      const double *tdx, const double *tdy, const int *bc_mask, const double *bc_u,
-                     const double *bc_v, int *ptr, int *ind, double *val, double *bb) {
+                     const double *bc_v, const copy_lists_t *copy, const double *ucur, const double *vcur,
+                     int *ptr, int *ind, double *val, double *bb) {
   int nnz = 0;
   ptr[0] = 0;
   for (int row = 0; row < 2 * m->nT; ++row) {
@@ -352,6 +437,9 @@ static void assemble(const ufe_mesh_desc *m, const ufe_params *p, const double *
         }
         ind[nnz] = row; val[nnz] = -(double)nn; ++nnz;
         bb[row] = 0.0;
+      } else if (is_copy_choice(choice)) {
+        ind[nnz] = row; val[nnz] = 1.0; ++nnz;
+        bb[row] = copy_value(copy, p, row, ucur, vcur);
       } else {
         ind[nnz] = row; val[nnz] = 1.0; ++nnz;
         bb[row] = 0.0;
@@ -428,6 +516,9 @@ typedef struct {
   double *eta3a, *F1a, *F2a, *F1b, *F2b, *duz_a, *dvz_a, *tmpb, *tmpa;
   int *Aptr, *Aind;
   double *Aval, *bb, *xx;
+  /* the iterate the system is assembled at (read by the lagged copy rows) */
+  const double *ucur, *vcur;
+  copy_lists_t copy;
 } picard_t;
 
 static void vec_a_b_3d(const ufe_mesh_desc *m, const double *fa, double *fb) {
@@ -532,7 +623,7 @@ static void picard_state(picard_t *s, const double *u, const double *v, double e
 static void picard_assemble(picard_t *s) {
   const double *beta = s->is_diva ? s->beta_eff_b : s->beta_b;
   assemble(s->m, s->p, s->N_b, s->dNdx, s->dNdy, beta, s->tdx, s->tdy, s->bc_mask, s->bc_u, s->bc_v,
-           s->Aptr, s->Aind, s->Aval, s->bb);
+           &s->copy, s->ucur, s->vcur, s->Aptr, s->Aind, s->Aval, s->bb);
 }
 
 /* Block-scaled nonlinear residual ||D^-1 (b(u) - A(u) u)|| / ||D^-1 b(u)||, D the
@@ -590,6 +681,7 @@ static int picard_alloc(picard_t *s) {
   AL(dvz_a, (size_t)nV * nz); AL(tmpb, (size_t)nT * nz); AL(tmpa, (size_t)nV * nz);
   AL(Aptr, 2 * nT + 1); AL(Aind, nnzA); AL(Aval, nnzA); AL(bb, 2 * nT); AL(xx, 2 * nT);
 #undef AL
+  copy_lists_build(m, s->p, &s->copy);
   return 0;
 }
 
@@ -599,6 +691,7 @@ static void picard_free(picard_t *s) {
                 s->F1a, s->F2a, s->F1b, s->F2b, s->duz_a, s->dvz_a, s->tmpb, s->tmpa, s->Aptr,
                 s->Aind, s->Aval, s->bb, s->xx};
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i) free(ps[i]);
+  copy_lists_free(&s->copy);
 }
 
 /* The shared Picard loop of solve_SSA (SSA_main.f90:149-237) and solve_DIVA
@@ -610,6 +703,8 @@ static int picard(picard_t *s, double *u, double *v, double *eta3b, double *tbx,
   int nT = m->nT, nV = m->nV, nz = m->nz;
   double t0 = now_s();
   picard_alloc(s);
+  s->ucur = u;
+  s->vcur = v;
   driving_stress(m, s->Hi, s->Hs, s->tdx, s->tdy);
   if (!s->is_diva)
     for (int vi = 0; vi < nV; ++vi) s->A_vav[vi] = vertical_average(nz, m->zeta, s->A_flow + vi, nV);
@@ -796,6 +891,8 @@ double oracle_ssa_residual(const ufe_mesh_desc *m, const ufe_params *p, const do
   s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
   s.bc_mask = bc_mask; s.bc_u = bc_u; s.bc_v = bc_v; s.is_diva = 0;
   picard_alloc(&s);
+  s.ucur = u;
+  s.vcur = v;
   driving_stress(m, Hi, Hs, s.tdx, s.tdy);
   for (int vi = 0; vi < m->nV; ++vi) s.A_vav[vi] = vertical_average(m->nz, m->zeta, A_flow + vi, m->nV);
   picard_state(&s, u, v, p->epsilon_sq_0, NULL, NULL, NULL, NULL, NULL);
@@ -838,6 +935,8 @@ double oracle_diva_residual(const ufe_mesh_desc *m, const ufe_params *p, const d
   s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
   s.is_diva = 1;
   picard_alloc(&s);
+  s.ucur = u;
+  s.vcur = v;
   int nT = m->nT, nz = m->nz, n = 2 * nT;
   driving_stress(m, Hi, Hs, s.tdx, s.tdy);
   picard_state(&s, u, v, p->epsilon_sq_0, eta3b, tbx, tby, ubase, vbase);
@@ -869,6 +968,8 @@ int oracle_ssa_system(const ufe_mesh_desc *m, const ufe_params *p, const double 
   memset(&s, 0, sizeof(s));
   s.m = m; s.p = p; s.Hi = Hi; s.Hs = Hs; s.A_flow = A_flow; s.roughness = beta_sq; s.fgr_b = fgr_b;
   picard_alloc(&s);
+  s.ucur = u;
+  s.vcur = v;
   driving_stress(m, Hi, Hs, s.tdx, s.tdy);
   for (int vi = 0; vi < m->nV; ++vi) s.A_vav[vi] = vertical_average(m->nz, m->zeta, A_flow + vi, m->nV);
   picard_state(&s, u, v, p->epsilon_sq_0, NULL, NULL, NULL, NULL, NULL);
@@ -1268,4 +1369,81 @@ int oracle_thickness_semiimplicit(int nV, const double *V, const ufe_vertex_geom
   if (n_its) *n_its = its;
   free(ptr); free(ind); free(M); free(Hex); free(rate); free(A); free(bb);
   return its < 0 ? UFE_ERR_ARGUMENT : 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Heat-column inputs (thermodynamics_utilities.f90)                       */
+/* ---------------------------------------------------------------------- */
+
+static double cross2(double ax, double ay, double bx, double by) { return ax * by - ay * bx; }
+
+/* calc_upwind_heat_flux_derivatives (thermodynamics_utilities.f90:339-434):
+ * for each vertex with Hi >= 1 m, the first triangle in its (counter-
+ * clockwise) iTri list whose corner at the vertex contains the upwind vector
+ * (-u_vav, -v_vav); then per layer u_3D_b * ddx_a_b(Ti) and v_3D_b *
+ * ddy_a_b(Ti) on that triangle.  Fields are [k][i], vertex/triangle fastest.
+ * Returns -1 (the reference's crash) when no triangle qualifies. */
+int oracle_heat_upwind(const ufe_mesh_desc *m, const int *iTri_ptr, const int *iTri_ind, const double *Ti,
+                       const double *Hi, const double *u_vav, const double *v_vav, const double *u3, const double *v3,
+                       double *u_dTdxp, double *v_dTdyp) {
+  const int nV = m->nV, nT = m->nT, nz = m->nz;
+  for (int vi = 0; vi < nV; ++vi) {
+    if (Hi[vi] < 1.0) {
+      for (int k = 0; k < nz; ++k) u_dTdxp[(size_t)k * nV + vi] = v_dTdyp[(size_t)k * nV + vi] = 0.0;
+      continue;
+    }
+    const double ux = -u_vav[vi], uy = -v_vav[vi];
+    int tup = -1;
+    for (int q = iTri_ptr[vi]; q < iTri_ptr[vi + 1] && tup < 0; ++q) {
+      const int ti = iTri_ind[q];
+      int vb = -1, vc = -1;
+      for (int n1 = 0; n1 < 3; ++n1)
+        if (m->Tri[3 * ti + n1] == vi) {
+          vb = m->Tri[3 * ti + (n1 + 1) % 3];
+          vc = m->Tri[3 * ti + (n1 + 2) % 3];
+          break;
+        }
+      const double abx = m->V[2 * vb] - m->V[2 * vi], aby = m->V[2 * vb + 1] - m->V[2 * vi + 1];
+      const double acx = m->V[2 * vc] - m->V[2 * vi], acy = m->V[2 * vc + 1] - m->V[2 * vi + 1];
+      if (cross2(abx, aby, ux, uy) >= 0.0 && cross2(ux, uy, acx, acy) >= 0.0) tup = ti;
+    }
+    if (tup < 0) return -1;
+    for (int k = 0; k < nz; ++k) {
+      double dx = 0.0, dy = 0.0;
+      for (int q = m->ab_ptr[tup]; q < m->ab_ptr[tup + 1]; ++q) {
+        const double t = Ti[(size_t)k * nV + m->ab_ind[q]];
+        dx += m->ab_ddx[q] * t;
+        dy += m->ab_ddy[q] * t;
+      }
+      u_dTdxp[(size_t)k * nV + vi] = u3[(size_t)k * nT + tup] * dx;
+      v_dTdyp[(size_t)k * nV + vi] = v3[(size_t)k * nT + tup] * dy;
+    }
+  }
+  return 0;
+}
+
+/* calc_strain_heating (thermodynamics_utilities.f90:24-91): Phi = 2 A^(-1/n)
+ * D^(1/n + 1) with D the effective strain rate of the nine 3-D strain-rate
+ * fields (order du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy,
+ * dw_dz, each [k][vi]); no heating where Hi < 0.1 m. */
+void oracle_strain_heating(int nV, int nz, const double *Hi, const double *A_flow, const double *const *e,
+                           double glen_n, double *Phi) {
+  for (int vi = 0; vi < nV; ++vi)
+    for (int k = 0; k < nz; ++k) {
+      const size_t i = (size_t)k * nV + vi;
+      if (Hi[vi] < 0.1) { Phi[i] = 0.0; continue; }
+      const double ux = e[0][i], uy = e[1][i], uz = e[2][i], vx = e[3][i], vy = e[4][i], vz = e[5][i];
+      const double wx = e[6][i], wy = e[7][i], wz = e[8][i];
+      const double D = sqrt(0.5 * (ux * ux + vy * vy + wz * wz + 0.5 * (uy + vx) * (uy + vx) +
+                                   0.5 * (uz + wx) * (uz + wx) + 0.5 * (vz + wy) * (vz + wy)));
+      Phi[i] = 2.0 * pow(A_flow[i], -1.0 / glen_n) * pow(D, 1.0 / glen_n + 1.0);
+    }
+}
+
+/* calc_frictional_heating (thermodynamics_utilities.f90:93-128): the basal
+ * friction coefficient times the basal speed where the ice is grounded, zero
+ * elsewhere and everywhere without sliding. */
+void oracle_frictional_heating(int nV, int no_sliding, const double *beta, const double *uabs_base,
+                               const int *mask_grounded, double *Q) {
+  for (int vi = 0; vi < nV; ++vi) Q[vi] = (!no_sliding && mask_grounded[vi]) ? beta[vi] * uabs_base[vi] : 0.0;
 }

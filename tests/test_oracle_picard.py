@@ -46,23 +46,60 @@ def schoof_params(L, w, g):
 
 
 def test_schoof_icestream_oracle(W, L, OB):
-    """The reference's own check (RMSE of u against the closed form, reference
-    L2 stop): the error falls with resolution, and on the finer mesh is of the
-    size the reference scoreboard reports for its meshes of that resolution
-    (its meshes refine to a longest edge below the nominal size, ours have
-    edges near it, so the bound is loose)."""
+    """The reference's own check with its own settings (RMSE of u against the
+    closed form, reference L2 stop, 'infinite_SSA_icestream' east/west): the
+    error falls by more than half from 32 to 16 km and at 16 km is below the
+    reference scoreboard's 16 km value (303.7 m/yr; 181 here, 562 at 32 km
+    against its 400.4: the meshes differ).  With the plain 'infinite' rows
+    instead (u = mean of the neighbours) the errors are 908 and 490 m/yr: the
+    lagged copy rows are what the closed form's du/dx = 0 needs."""
     g = GOLD["schoof_icestream"]
     rmse = {}
     for h in (32e3, 16e3):
         w = W.schoof_icestream(h=h)
+        assert list(w.params["bc_u"]) == [1, 3, 1, 3]
         rc, u, v, info = OB.Oracle(w.descriptor()).solve_ssa(schoof_params(L, w, g), w.inputs())
         assert rc == 0, rc
         yc = w.mesh.TriGC[:, 1]
         ua = schoof_u(yc, g["A"], g["H"], g["dhdx"], g["L"], g["m"])
         rmse[h] = np.sqrt(np.mean((u - ua) ** 2))
         assert np.abs(v).max() < 0.15 * np.abs(u).max()
-    assert rmse[16e3] < 0.7 * rmse[32e3]
-    assert rmse[16e3] < 2.0 * g["reference_rmse"]["32km"]
+    assert rmse[16e3] < 0.5 * rmse[32e3]
+    assert rmse[16e3] < g["reference_rmse"]["16km"]
+
+
+def test_copy_rows_take_the_relaxed_copy_of_the_iterate(W, L, OB):
+    """The lagged copy rows (solve_linearised_SSA_DIVA.f90:562-575): in the
+    assembled system at a given iterate, every 'infinite_SSA_icestream' row is
+    an identity row whose value is visc_it_relax * (the 1/d^2-weighted mean of
+    the iterate on the triangles around the vertex nearest the copy point) +
+    (1 - visc_it_relax) * the row's own value, the copy point being at the
+    same y and a third / two thirds of the domain in x.  Checked against a
+    brute-force numpy evaluation of that definition."""
+    w = W.schoof_icestream(h=64e3)
+    m = w.mesh
+    p = schoof_params(L, w, GOLD["schoof_icestream"])
+    rng = np.random.default_rng(3)
+    u, v = rng.normal(0, 100, m.nT), rng.normal(0, 10, m.nT)
+    A, b = OB.Oracle(w.descriptor()).system(p, w.inputs(), u, v)
+    ops = w.ops
+    xmin, xmax = m.V[:, 0].min(), m.V[:, 0].max()
+    n = 0
+    for ti in np.flatnonzero(m.TriBI > 0):
+        side = (m.TriBI[ti] - 1) // 2
+        if p.bc_u[side] != 3:
+            continue
+        gx, gy = m.TriGC[ti]
+        px = xmin + (xmax - xmin) * (1 / 3 if gx < 0 else 2 / 3)
+        vi = int(np.argmin(np.hypot(m.V[:, 0] - px, m.V[:, 1] - gy)))
+        tris = ops.ba.ind[ops.ba.ptr[vi]:ops.ba.ptr[vi + 1]]
+        wts = 1.0 / np.hypot(px - m.TriGC[tris, 0], gy - m.TriGC[tris, 1]) ** 2
+        want = p.visc_it_relax * (wts @ u[tris]) / wts.sum() + (1 - p.visc_it_relax) * u[ti]
+        row = 2 * ti
+        assert A.indptr[row + 1] - A.indptr[row] == 1 and A.indices[A.indptr[row]] == row
+        assert abs(b[row] - want) < 1e-12 * max(1.0, abs(want))
+        n += 1
+    assert n > 10
 
 
 def frictionless_slab(W):
@@ -95,3 +132,34 @@ def test_diva_reduces_to_ssa_without_friction(W, L, OB):
     # no vertical shear: every layer ([k][ti] layout) moves with the vertical mean
     U = st["u3d"].reshape(w.nz, w.mesh.nT)
     assert max_rel(U, np.broadcast_to(st["u"], U.shape)) < 1e-12
+
+
+def periodic_slab(W, Lx=80e3, h=8e3):
+    """Uniform slab on a uniform slope over the ISMIP-HOM square [-L/2, L/2]^2
+    with 'periodic_ISMIP-HOM' on every side: the driving stress is uniform,
+    so with a linear sliding law (Weertman m = 1, beta = beta_sq) the SSA
+    solution is the uniform velocity u = tau_d / beta_sq, and the periodic
+    copy rows (relaxed copies of interior values) are satisfied by it."""
+    mesh = W.uniform_mesh(-Lx / 2, Lx / 2, -Lx / 2, Lx / 2, h, seed=12)
+    nV, nz = mesh.nV, 6
+    x = mesh.V[:, 0]
+    slope = 0.005
+    H = np.full(nV, 1000.0)
+    w = W.Workload("periodic_slab", mesh, W.build_operators(mesh), W.zeta_regular(nz), H, -slope * x,
+                   np.full(nV, -1e4), np.full(nz * nV, 1e-16), np.full(nV, 1e3), 12, "SSA",
+                   {"sliding_law": 1, "slid_weertman_m": 1.0, "bc_u": [2, 2, 2, 2], "bc_v": [2, 2, 2, 2],
+                    "ismip_hom_L": Lx})
+    u_exact = W.RHO_I * 9.81 * 1000.0 * slope / 1e3
+    return w, u_exact
+
+
+def test_periodic_ismip_hom_slab_is_uniform(W, L, OB):
+    w, ue = periodic_slab(W)
+    # unrelaxed copies: the lagged copy rows pass the periodic information on
+    # by one copy per Picard iteration (relaxed by visc_it_relax = 0.2 this
+    # takes thousands of iterations on this stiff, nearly strain-free slab)
+    p = params(L, **w.params, stop_rule=1, nonlinear_rtol=1e-10, visc_it_nit=600, rtol=1e-12, abstol=0.0,
+               include_crossterms=1, visc_it_relax=1.0)
+    rc, u, v, info = OB.Oracle(w.descriptor()).solve_ssa(p, w.inputs())
+    assert rc == 0, rc
+    assert np.abs(u - ue).max() < 1e-6 * ue and np.abs(v).max() < 1e-6 * ue

@@ -27,7 +27,7 @@ UFE_ERR_UNSTABLE = -5
 UFE_ERR_MESH = -6
 
 SLIDING = {"no_sliding": 0, "Weertman": 1, "Budd": 2, "Coulomb": 3}
-BC = {"infinite": 0, "zero": 1}
+BC = {"infinite": 0, "zero": 1, "periodic_ISMIP-HOM": 2, "infinite_SSA_icestream": 3}
 PREC_FP64, PREC_MIXED = 0, 1
 KRYLOV_BICGSTAB, KRYLOV_GMRES = 0, 1
 PC_NONE, PC_JACOBI, PC_BLOCK_JACOBI, PC_AMG = 0, 1, 2, 3
@@ -64,7 +64,7 @@ class Params(C.Structure):
         ("accel_mixing", C.c_double),
         ("accel_restart", C.c_double),
         ("accel_continuation", C.c_int),
-        ("precond_fp32", C.c_int),
+        ("precond_fp32", C.c_int), ("ismip_hom_L", C.c_double),
     ]
 
 
@@ -209,6 +209,10 @@ def lib():
     _sig(L, 'ufe_mesh_operators_ab', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_ba', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_bb2', argtypes=[C.c_int, ip, dp, ip, ip, ip, dp, dp, dp, dp, dp])
+    _sig(L, 'ufe_heat_setup', argtypes=[C.c_void_p, ip, ip])
+    _sig(L, 'ufe_heat_upwind_derivatives', argtypes=[C.c_void_p, dp, dp, dp, dp, dp, dp, dp, dp])
+    _sig(L, 'ufe_heat_strain_heating', argtypes=[C.c_void_p, dp, dp, C.POINTER(dp), dp])
+    _sig(L, 'ufe_heat_frictional_heating', argtypes=[C.c_void_p, dp, dp, ip, dp])
     _sig(L, 'ufe_thickness_params_default', argtypes=[C.POINTER(ThicknessParams)])
     _sig(L, 'ufe_thickness_setup', argtypes=[C.c_void_p, C.POINTER(VertexGeometry)])
     _sig(L, 'ufe_thickness_semiimplicit', argtypes=[C.c_void_p, C.POINTER(ThicknessParams),
@@ -227,6 +231,7 @@ EXPORTED = [
     "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_set_graph_launch", "ufe_ssa_residual", "ufe_diva_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_csr_solve_ex", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_columns_device_noalloc", "ufe_heat_robin",
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
     "ufe_thickness_params_default", "ufe_thickness_setup", "ufe_thickness_semiimplicit",
+    "ufe_heat_setup", "ufe_heat_upwind_derivatives", "ufe_heat_strain_heating", "ufe_heat_frictional_heating",
 ]
 
 

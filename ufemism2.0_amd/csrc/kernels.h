@@ -13,7 +13,10 @@ constexpr double kT0 = 273.16;              // parameters.f90:18
 constexpr double kClausiusClapeyron = 8.7e-4;
 
 // Row types of the linearised system (solve_linearised_SSA_DIVA.f90:94-147).
-enum : signed char { ROW_FREE = 0, ROW_DIRICHLET = 1, ROW_INFINITE = 2, ROW_ZERO = 3 };
+// ROW_COPY: the lagged copy rows of the idealised boundary conditions (identity
+// row, value from the previous iterate, :543-575 and :618-635), carried in the
+// Dirichlet value arrays.
+enum : signed char { ROW_FREE = 0, ROW_DIRICHLET = 1, ROW_INFINITE = 2, ROW_ZERO = 3, ROW_COPY = 4 };
 
 // Device copies of the mesh and operators (pointers into HBM).
 struct DevMesh {

@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(kBlock) k_triangle(
   for (int uv = 0; uv < 2; ++uv) {
     int rt = rowtype[2 * ti + uv];
     double r;
-    if (rt == ROW_DIRICHLET) r = uv == 0 ? bc_u[ti] : bc_v[ti];
+    if (rt == ROW_DIRICHLET || rt == ROW_COPY) r = uv == 0 ? bc_u[ti] : bc_v[ti];
     else if (rt != ROW_FREE) r = 0.0;
     else {
       double t = uv == 0 ? tdx[ti] : tdy[ti];
@@ -149,7 +149,7 @@ __device__ __forceinline__ void system_block(int ti, int tj, const signed char *
       }
       a_same = diag ? -(double)nn : (nb ? 1.0 : 0.0);
       a_other = 0.0;
-    } else {  // ROW_ZERO, ROW_DIRICHLET
+    } else {  // ROW_ZERO, ROW_DIRICHLET, ROW_COPY
       a_same = diag ? 1.0 : 0.0;
       a_other = 0.0;
     }

@@ -333,7 +333,7 @@ extern "C" int ufe_thickness_setup(ufe_ctx *c, const ufe_vertex_geometry *g) {
     std::vector<int> t_o2n(nT);
     for (int i = 0; i < nV; ++i) t->v_o2n[vperm[i]] = i;
     for (int i = 0; i < nT; ++i) t_o2n[tperm[i]] = i;
-    // vertex coordinates in the caller's order, for the unit vectors
+    // vertex coordinates (device order), for the unit vectors
     std::vector<double> Vd(2 * (size_t)nV);
     UFE_HIP(hipMemcpy(Vd.data(), m.V, Vd.size() * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<int> row_ptr(nV + 1), vbi(nV), c_ind(nC), c_tri(2 * (size_t)nC);

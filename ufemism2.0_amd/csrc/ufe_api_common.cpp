@@ -63,6 +63,7 @@ extern "C" void ufe_params_default(ufe_params *p) {
   p->accel_restart = 10.0;
   p->accel_continuation = 0;
   p->precond_fp32 = 0;
+  p->ismip_hom_L = 0.0;
   p->check_every = 8;
 }
 

@@ -31,6 +31,12 @@ struct ufe_ctx {
   DevMesh m;
   std::vector<std::unique_ptr<DBuf>> own;  // mesh/operator storage
   std::vector<int> h_TriBI, h_TriC;
+  // host copies (device order) for the copy points of the idealised boundary conditions
+  std::vector<double> h_V, h_TriGC;
+  std::vector<int> h_ba_ptr, h_ba_ind;
+  // lagged copy rows: row index, and per row the triangles and weights
+  DBuf copy_row, copy_ptr, copy_tri, copy_w;
+  int n_copy = 0;
   // locality ordering: device index -> caller index
   std::vector<int> vperm, tperm;
   // system
@@ -60,6 +66,7 @@ struct ufe_ctx {
   double bbox[4] = {0, 1, 0, 1};  // xmin, xmax, ymin, ymax of the vertices
   DBuf feat;                     // learned-guess features [nT][16] fp32 (device order)
   std::shared_ptr<void> thickness;  // state of the thickness solver (thickness.hip), set up on demand
+  std::shared_ptr<void> heat_mesh;  // vertex-triangle lists of the heat-column inputs (heat_inputs.hip)
   hipEvent_t ev[8];
 };
 
@@ -111,9 +118,64 @@ BsrT<float> sys32(ufe_ctx *c) {
 }
 
 // Row types from border indices, BC choices and the prescribed-velocity mask.
+// Vertex nearest to (px, py): the one whose Voronoi cell holds the point
+// (find_containing_vertex, mesh_utilities.f90).  Brute force over a bucket
+// grid of the vertices, searched ring by ring until no closer bucket remains.
+struct VertexGrid {
+  double x0, y0, h;
+  int nx, ny;
+  std::vector<int> start, items;
+  const std::vector<double> *V;
+  explicit VertexGrid(const std::vector<double> &Vv) : V(&Vv) {
+    const int nV = (int)(Vv.size() / 2);
+    double x1 = -1e300, y1 = -1e300;
+    x0 = y0 = 1e300;
+    for (int i = 0; i < nV; ++i) {
+      x0 = std::min(x0, Vv[2 * i]); x1 = std::max(x1, Vv[2 * i]);
+      y0 = std::min(y0, Vv[2 * i + 1]); y1 = std::max(y1, Vv[2 * i + 1]);
+    }
+    const int side = std::max(1, (int)std::sqrt((double)nV / 4.0));
+    h = std::max(x1 - x0, y1 - y0) / side * (1.0 + 1e-9) + 1e-300;
+    nx = (int)((x1 - x0) / h) + 1;
+    ny = (int)((y1 - y0) / h) + 1;
+    start.assign((size_t)nx * ny + 1, 0);
+    items.resize(nV);
+    for (int i = 0; i < nV; ++i) ++start[cell(Vv[2 * i], Vv[2 * i + 1]) + 1];
+    for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < nV; ++i) items[fill[cell(Vv[2 * i], Vv[2 * i + 1])]++] = i;
+  }
+  int cx(double x) const { return std::min(nx - 1, std::max(0, (int)((x - x0) / h))); }
+  int cy(double y) const { return std::min(ny - 1, std::max(0, (int)((y - y0) / h))); }
+  int cell(double x, double y) const { return cy(y) * nx + cx(x); }
+  int nearest(double px, double py) const {
+    const int ix = cx(px), iy = cy(py);
+    int best = -1;
+    double dbest = 1e300;
+    for (int r = 0; r <= std::max(nx, ny); ++r) {
+      for (int j = iy - r; j <= iy + r; ++j)
+        for (int i = ix - r; i <= ix + r; ++i) {
+          if (i < 0 || j < 0 || i >= nx || j >= ny) continue;
+          if (std::max(std::abs(i - ix), std::abs(j - iy)) != r) continue;
+          for (int k = start[(size_t)j * nx + i]; k < start[(size_t)j * nx + i + 1]; ++k) {
+            const int v = items[k];
+            const double dx = (*V)[2 * v] - px, dy = (*V)[2 * v + 1] - py, d = dx * dx + dy * dy;
+            if (d < dbest || (d == dbest && v < best)) { dbest = d; best = v; }
+          }
+        }
+      // every vertex outside ring r is at least r*h away from the point's cell
+      if (best >= 0 && (double)r * h >= std::sqrt(dbest)) break;
+    }
+    return best;
+  }
+};
+
 void build_rowtypes(ufe_ctx *c, const int *bc_mask) {
   int nT = c->m.nT;
   c->h_rowtype.assign(2 * (size_t)nT, ROW_FREE);
+  std::vector<int> crow, cptr(1, 0), ctri;
+  std::vector<double> cw;
+  std::unique_ptr<VertexGrid> grid;
   for (int ti = 0; ti < nT; ++ti) {
     for (int uv = 0; uv < 2; ++uv) {
       signed char t = ROW_FREE;
@@ -121,12 +183,71 @@ void build_rowtypes(ufe_ctx *c, const int *bc_mask) {
       else if (c->h_TriBI[ti] > 0) {
         int side = (c->h_TriBI[ti] - 1) / 2;  // 1,2 N; 3,4 E; 5,6 S; 7,8 W
         int choice = uv == 0 ? c->prm.bc_u[side] : c->prm.bc_v[side];
-        t = choice == UFE_BC_ZERO ? ROW_ZERO : ROW_INFINITE;
+        if (choice == UFE_BC_PERIODIC_ISMIP_HOM || choice == UFE_BC_INFINITE_SSA_ICESTREAM) {
+          t = ROW_COPY;
+          // copy point (find_ti_copy_ISMIP_HOM_periodic / find_ti_copy_SSA_icestream_infinite)
+          const double gx = c->h_TriGC[2 * ti], gy = c->h_TriGC[2 * ti + 1];
+          double px, py;
+          if (choice == UFE_BC_PERIODIC_ISMIP_HOM) {
+            const double L = c->prm.ismip_hom_L;
+            px = gx > 0.0 ? gx - L / 2.0 : gx + L / 2.0;
+            py = gy > 0.0 ? gy - L / 2.0 : gy + L / 2.0;
+          } else {
+            py = gy;
+            px = gx < 0.0 ? c->bbox[0] + (c->bbox[1] - c->bbox[0]) * 1.0 / 3.0
+                          : c->bbox[0] + (c->bbox[1] - c->bbox[0]) * 2.0 / 3.0;
+          }
+          if (!grid) grid = std::make_unique<VertexGrid>(c->h_V);
+          const int vi = grid->nearest(px, py);
+          const int k0 = c->h_ba_ptr[vi], k1 = c->h_ba_ptr[vi + 1];
+          double sum = 0.0;
+          const size_t base = cw.size();
+          for (int k = k0; k < k1; ++k) {
+            const int tj = c->h_ba_ind[k];
+            const double dx = px - c->h_TriGC[2 * tj], dy = py - c->h_TriGC[2 * tj + 1];
+            const double d = std::sqrt(dx * dx + dy * dy);
+            ctri.push_back(tj);
+            cw.push_back(1.0 / (d * d));
+            sum += 1.0 / (d * d);
+          }
+          for (size_t k = base; k < cw.size(); ++k) cw[k] /= sum;
+          crow.push_back(2 * ti + uv);
+          cptr.push_back((int)ctri.size());
+        } else {
+          t = choice == UFE_BC_ZERO ? ROW_ZERO : ROW_INFINITE;
+        }
       }
       c->h_rowtype[2 * ti + uv] = t;
     }
   }
   h2d(c->rowtype, c->h_rowtype.data(), c->h_rowtype.size(), c->s);
+  c->n_copy = (int)crow.size();
+  if (c->n_copy) {
+    h2d(c->copy_row, crow.data(), crow.size() * sizeof(int), c->s);
+    h2d(c->copy_ptr, cptr.data(), cptr.size() * sizeof(int), c->s);
+    h2d(c->copy_tri, ctri.data(), ctri.size() * sizeof(int), c->s);
+    h2d(c->copy_w, cw.data(), cw.size() * sizeof(double), c->s);
+    UFE_HIP(hipStreamSynchronize(c->s));  // the host lists end with this call
+  }
+}
+
+// Values of the lagged copy rows at the current iterate, written into the
+// Dirichlet value arrays (the rows are identity rows with that value).
+__global__ void __launch_bounds__(kBlock) k_copy_rows(int n, const int *__restrict__ row, const int *__restrict__ ptr,
+                                                      const int *__restrict__ tri, const double *__restrict__ w,
+                                                      const double2 *__restrict__ u, double relax,
+                                                      double *__restrict__ bcu, double *__restrict__ bcv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int rw = row[r], ti = rw / 2, uv = rw % 2;
+  double val = 0.0;
+  for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+    const double2 q = u[tri[k]];
+    val += w[k] * (uv == 0 ? q.x : q.y);
+  }
+  const double2 self = u[ti];
+  val = (relax * val) + ((1.0 - relax) * (uv == 0 ? self.x : self.y));
+  (uv == 0 ? bcu : bcv)[ti] = val;
 }
 
 PicardParams picard_params(const ufe_params &q, double eps0) {
@@ -310,6 +431,12 @@ static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
                        c->F1b.as<double>(), c->F2b0.as<double>(), need_F1 ? 1 : 0, s);
   }
   UFE_HIP(hipEventRecord(c->ev[1], s));
+  if (c->n_copy) {
+    hipLaunchKernelGGL(k_copy_rows, dim3(blocks_for(c->n_copy)), dim3(kBlock), 0, s, c->n_copy,
+                       c->copy_row.as<int>(), c->copy_ptr.as<int>(), c->copy_tri.as<int>(), c->copy_w.as<double>(),
+                       c->u.as<double2>(), q.visc_it_relax, c->bcu.as<double>(), c->bcv.as<double>());
+    UFE_LAUNCH_CHECK();
+  }
   launch_triangle(c->m, c->N_a.as<double>(), c->beta_a.as<double>(), diva ? c->beta_eff_a.as<double>() : nullptr,
                   c->fgr.as<double>(), c->tdx.as<double>(), c->tdy.as<double>(), c->rowtype.as<signed char>(),
                   c->bcu.as<double>(), c->bcv.as<double>(), p, c->tri_coef.as<double4>(),
@@ -691,6 +818,8 @@ int ctx_fail(ufe_ctx *c, int code, const std::string &msg) { return fail(c, code
 const std::vector<int> &ctx_vperm(const ufe_ctx *c) { return c->vperm; }
 const std::vector<int> &ctx_tperm(const ufe_ctx *c) { return c->tperm; }
 std::shared_ptr<void> &ctx_thickness(ufe_ctx *c) { return c->thickness; }
+std::shared_ptr<void> &ctx_heat_mesh(ufe_ctx *c) { return c->heat_mesh; }
+const ufe_params &ctx_params(const ufe_ctx *c) { return c->prm; }
 const double *ctx_velocity(const ufe_ctx *c) { return c->u.as<double>(); }
 int ctx_guard(ufe_ctx *c, const std::function<int()> &f) { return guarded(c, f); }
 }  // namespace ufe
@@ -742,6 +871,10 @@ extern "C" int ufe_ctx_create(ufe_ctx **out, int device, const ufe_mesh_desc *md
     md = &pm.desc;
     c->h_TriBI.assign(md->TriBI, md->TriBI + nT);
     c->h_TriC.assign(md->TriC, md->TriC + 3 * nT);
+    c->h_V.assign(md->V, md->V + 2 * (size_t)nV);
+    c->h_TriGC.assign(md->TriGC, md->TriGC + 2 * (size_t)nT);
+    c->h_ba_ptr.assign(md->ba_ptr, md->ba_ptr + nV + 1);
+    c->h_ba_ind.assign(md->ba_ind, md->ba_ind + md->ba_nnz);
     m.ab_nnz = md->ab_nnz; m.ba_nnz = md->ba_nnz; m.bb_nnz = md->bb_nnz;
     m.ab_ptr = dev_copy(c.get(), md->ab_ptr, nT + 1);
     m.ab_ind = dev_copy(c.get(), md->ab_ind, md->ab_nnz);
@@ -896,6 +1029,15 @@ extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
     if (p->accel_continuation < 0 || p->accel_continuation > 12 ||
         (p->accel_continuation > 0 && p->accel_depth == 0))
       return fail(c, UFE_ERR_ARGUMENT, "accel_continuation needs accel_depth > 0 and at most 12 decades");
+    for (int side = 0; side < 4; ++side) {
+      if (p->bc_u[side] < UFE_BC_INFINITE || p->bc_u[side] > UFE_BC_INFINITE_SSA_ICESTREAM)
+        return fail(c, UFE_ERR_ARGUMENT, "unknown choice_BC_u");
+      if (p->bc_v[side] < UFE_BC_INFINITE || p->bc_v[side] > UFE_BC_PERIODIC_ISMIP_HOM)
+        return fail(c, UFE_ERR_ARGUMENT, "unknown choice_BC_v");
+      if ((p->bc_u[side] == UFE_BC_PERIODIC_ISMIP_HOM || p->bc_v[side] == UFE_BC_PERIODIC_ISMIP_HOM) &&
+          !(p->ismip_hom_L > 0.0))
+        return fail(c, UFE_ERR_ARGUMENT, "periodic_ISMIP-HOM needs ismip_hom_L > 0");
+    }
     if (p->accel_depth > 0 && (p->stop_rule != UFE_STOP_RESIDUAL || !(p->accel_mixing > 0.0) ||
                                !(p->accel_mixing <= 1.0) || !(p->accel_restart > 1.0)))
       return fail(c, UFE_ERR_ARGUMENT,

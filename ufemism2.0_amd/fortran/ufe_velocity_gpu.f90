@@ -58,6 +58,7 @@ module ufe_velocity_gpu
     real(c_double) :: accel_mixing, accel_restart
     integer(c_int) :: accel_continuation
     integer(c_int) :: precond_fp32
+    real(c_double) :: ismip_hom_L
   end type ufe_gpu_params
 
   ! struct ufe_solve_info

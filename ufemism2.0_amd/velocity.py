@@ -286,6 +286,51 @@ class VelocitySolver:
                                                L.ptr(out)))
         return out
 
+    # -- inputs of the heat columns ---------------------------------------------
+    def heat_setup(self):
+        """Attach the triangles around each vertex (the mesh's iTri, counter-
+        clockwise) for the upwind advection terms."""
+        m = self.md.mesh
+        self._heat_keep = (L.i32(m.iTri_ptr), L.i32(m.iTri_ind))
+        self._check(self.lib.ufe_heat_setup(self._ctx, L.ptr(self._heat_keep[0], L.ip),
+                                            L.ptr(self._heat_keep[1], L.ip)))
+
+    def upwind_heat_flux_derivatives(self, Ti, Hi, u_vav, v_vav, u_3D_b, v_3D_b):
+        """calc_upwind_heat_flux_derivatives (thermodynamics_utilities.f90:339-434):
+        (u_times_dTdxp_upwind, v_times_dTdyp_upwind), each (nz*nV,) with the
+        vertex fastest; Ti likewise, u_3D_b / v_3D_b (nz*nT,) triangle fastest."""
+        if getattr(self, "_heat_keep", None) is None:
+            self.heat_setup()
+        m = self.md.mesh
+        nz = len(self.md.zeta)
+        ox, oy = np.zeros(nz * m.nV), np.zeros(nz * m.nV)
+        a = [L.f64(x) for x in (Ti, Hi, u_vav, v_vav, u_3D_b, v_3D_b)]
+        self._check(self.lib.ufe_heat_upwind_derivatives(self._ctx, *(L.ptr(x) for x in a), L.ptr(ox), L.ptr(oy)))
+        return ox, oy
+
+    def strain_heating(self, Hi, A_flow, strain_rates):
+        """calc_strain_heating (thermodynamics_utilities.f90:24-91): Phi (nz*nV,)
+        from the nine 3-D strain-rate fields in the order du_dx, du_dy, du_dz,
+        dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz, with this solver's glens_n."""
+        m = self.md.mesh
+        nz = len(self.md.zeta)
+        if len(strain_rates) != 9:
+            raise VelocitySolverError("strain_heating needs the nine 3-D strain-rate fields")
+        keep = [L.f64(x) for x in strain_rates]
+        arr = (L.dp * 9)(*(L.ptr(x) for x in keep))
+        Phi = np.zeros(nz * m.nV)
+        self._check(self.lib.ufe_heat_strain_heating(self._ctx, L.ptr(L.f64(Hi)), L.ptr(L.f64(A_flow)), arr,
+                                                     L.ptr(Phi)))
+        return Phi
+
+    def frictional_heating(self, basal_friction_coefficient, uabs_base, mask_grounded_ice):
+        """calc_frictional_heating (thermodynamics_utilities.f90:93-128)."""
+        Q = np.zeros(self.md.mesh.nV)
+        mk = L.i32(np.asarray(mask_grounded_ice).astype(np.int32))
+        self._check(self.lib.ufe_heat_frictional_heating(self._ctx, L.ptr(L.f64(basal_friction_coefficient)),
+                                                         L.ptr(L.f64(uabs_base)), L.ptr(mk, L.ip), L.ptr(Q)))
+        return Q
+
     # -- ice thickness -------------------------------------------------------
     def thickness_setup(self, geometry=None):
         """Attach the Voronoi geometry of the mesh (cell areas, connection

@@ -283,9 +283,11 @@ def schoof_icestream(h: float = 16e3, seed: int = 5, nz: int = 10, A: float = 1e
     x, y = mesh.V[:, 0], mesh.V[:, 1]
     f = -RHO_I * 9.81 * H * dhdx
     tau_y = f * np.abs(y / L) ** m
-    # u: 'zero' north/south, 'infinite' east/west (du/dx = 0); v: 'zero'
+    # the reference's border conditions (SSA_icestream/config_0*.cfg): u 'zero'
+    # north/south and 'infinite_SSA_icestream' east/west (the lagged copy of
+    # the interior velocity at the same y); v 'zero'
     params = {"sliding_law": 3, "visc_it_relax": 0.3, "slid_delta_v": 1e-3, "epsilon_sq_0": 1e-10,
-              "bc_u": [1, 0, 1, 0], "bc_v": [1, 1, 1, 1]}
+              "bc_u": [1, 3, 1, 3], "bc_v": [1, 1, 1, 1]}
     return Workload(f"schoof_icestream_{int(h / 1e3)}km", mesh, build_operators(mesh), zeta_regular(nz),
                     np.full(nV, H), dhdx * x, np.full(nV, -10000.0), np.full(nz * nV, A), tau_y, seed, "SSA",
                     params)
