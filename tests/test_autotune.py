@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TIGHT = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinear_rtol=1e-8, visc_it_nit=800,
              krylov_maxit=20000)
 CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.01, accel_depth=2, accel_mixing=1.0, accel_continuation=5,
-              accel_restart=3.0)
+              accel_restart=5.0)
 
 
 def test_autotune_keeps_the_oracle_solution(V, W):
