@@ -506,6 +506,10 @@ int ufe_mesh_operators_ba(int nV, int nT, const double *V, const int *TriC, cons
 int ufe_mesh_operators_bb2(int nT, const int *TriC, const double *TriGC, int *nnz, int *ptr, int *ind,
                            double *val_ddx, double *val_ddy, double *val_d2dx2, double *val_d2dxdy,
                            double *val_d2dy2);
+/* calc_matrix_operators_mesh_a_a (mesh_disc_calc_matrix_operators_2D.f90:61-196):
+ * a->a ddx/ddy (nV rows, home vertex first), read by the zeta gradients. */
+int ufe_mesh_operators_aa(int nV, const double *V, const int *C_ptr, const int *C_ind, int *nnz, int *ptr,
+                          int *ind, double *val_ddx, double *val_ddy);
 
 /* ---- learned initial guess (engine extension, no reference counterpart) --
  * A message-passing network on the triangle graph (3 layers, hidden 64,

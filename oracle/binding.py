@@ -265,3 +265,126 @@ def frictional_heating(beta, uabs_base, mask_grounded, no_sliding=False):
     lib().oracle_frictional_heating(len(beta), int(no_sliding), L.ptr(L.f64(beta)), L.ptr(L.f64(uabs_base)),
                                     L.ptr(mk, L.ip), L.ptr(Q))
     return Q
+
+
+def _aa_args(aa, zs):
+    L = _L()
+    keep = [L.i32(aa.ptr), L.i32(aa.ind), L.f64(aa.vals["ddx"]), L.f64(aa.vals["ddy"]), L.f64(zs)]
+    return keep, [aa.nnz, L.ptr(keep[0], L.ip), L.ptr(keep[1], L.ip), L.ptr(keep[2]), L.ptr(keep[3]),
+                  L.ptr(keep[4])]
+
+
+def _setup_bpa_sigs(o):
+    if getattr(o, "_bpa", False):
+        return
+    L = _L()
+    dp, ip = L.dp, L.ip
+    PD = C.POINTER(dp)
+    md, pr = C.POINTER(L.MeshDesc), C.POINTER(L.Params)
+    aa = [C.c_int, ip, ip, dp, dp, dp]
+    o.oracle_zeta_gradients.argtypes = [md, *aa, dp, dp, dp, dp, PD, PD, PD]
+    o.oracle_ops3d_apply.argtypes = [md, *aa, dp, dp, dp, dp, dp, PD, PD, PD, dp, PD, dp, dp]
+    o.oracle_solve_bpa.argtypes = [md, pr, *aa, dp, dp, dp, dp, dp, ip, ip, dp, dp, dp, dp,
+                                   C.POINTER(L.SolveInfo)]
+    o.oracle_bpa_residual.restype = C.c_double
+    o.oracle_bpa_residual.argtypes = [md, pr, *aa, dp, dp, dp, dp, dp, ip, dp, dp, dp, dp, dp, dp, dp, dp]
+    o._bpa = True
+
+
+def _ptrs(arrs):
+    L = _L()
+    return (L.dp * len(arrs))(*[L.ptr(a) for a in arrs])
+
+
+def zeta_gradients(md, aa, zs, Hi, Hs, dHi_dt=None, dHs_dt=None):
+    """calc_zeta_gradients; returns (ak [7][nz*nV], bk [6][nz*nT], bks [6][(nz-1)*nT]) (first index
+    fastest)."""
+    L = _L()
+    o = lib(); _setup_bpa_sigs(o)
+    nV, nT, nz = md.mesh.nV, md.mesh.nT, len(md.zeta)
+    ak = [np.zeros(nV * nz) for _ in range(7)]
+    bk = [np.zeros(nT * nz) for _ in range(6)]
+    bks = [np.zeros(nT * (nz - 1)) for _ in range(6)]
+    keep, args = _aa_args(aa, zs)
+    ins = [L.f64(a) if a is not None else None for a in (Hi, Hs, dHi_dt, dHs_dt)]
+    rc = o.oracle_zeta_gradients(md.byref(), *args, *(L.ptr(a) for a in ins), _ptrs(ak), _ptrs(bk), _ptrs(bks))
+    assert rc == 0
+    return ak, bk, bks
+
+
+def ops3d_apply(md, aa, zs, Hi, Hs, f_bk, f_ak, f_bks):
+    """The 3-D operators of the BPA applied to fields (first index fastest):
+    dict with bk_bk [7], bk_ak [2], ak_bk [2], bk_bks, bks_bk [2] (map, ddz),
+    map_bks_ak, map_ak_bks."""
+    L = _L()
+    o = lib(); _setup_bpa_sigs(o)
+    nV, nT, nz = md.mesh.nV, md.mesh.nT, len(md.zeta)
+    out = {"bk_bk": [np.zeros(nT * nz) for _ in range(7)], "bk_ak": [np.zeros(nV * nz) for _ in range(2)],
+           "ak_bk": [np.zeros(nT * nz) for _ in range(2)], "bk_bks": np.zeros(nT * (nz - 1)),
+           "bks_bk": [np.zeros(nT * nz) for _ in range(2)], "map_bks_ak": np.zeros(nV * nz),
+           "map_ak_bks": np.zeros(nT * (nz - 1))}
+    keep, args = _aa_args(aa, zs)
+    ins = [L.f64(a) for a in (Hi, Hs, f_bk, f_ak, f_bks)]
+    rc = o.oracle_ops3d_apply(md.byref(), *args, *(L.ptr(a) for a in ins), _ptrs(out["bk_bk"]),
+                              _ptrs(out["bk_ak"]), _ptrs(out["ak_bk"]), L.ptr(out["bk_bks"]),
+                              _ptrs(out["bks_bk"]), L.ptr(out["map_bks_ak"]), L.ptr(out["map_ak_bks"]))
+    assert rc == 0
+    return out
+
+
+def solve_bpa(md, aa, zs, params, inp, u0=None, v0=None):
+    """solve_BPA; inp as Oracle.solve_ssa's plus bc_mask/bc_u/bc_v on the bk
+    grid ([nz*nT], first index fastest).  Returns (rc, u_bk, v_bk, info)."""
+    L = _L()
+    o = lib(); _setup_bpa_sigs(o)
+    nT, nz = md.mesh.nT, len(md.zeta)
+    u = L.f64(np.zeros(nT * nz) if u0 is None else u0).copy()
+    v = L.f64(np.zeros(nT * nz) if v0 is None else v0).copy()
+    info = L.SolveInfo()
+    keep, args = _aa_args(aa, zs)
+    rc = o.oracle_solve_bpa(md.byref(), C.byref(params), *args, L.ptr(inp["Hi"]), L.ptr(inp["Hs"]),
+                            L.ptr(inp["A_flow"]), L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]),
+                            L.ptr(inp.get("mask_grounded"), L.ip), L.ptr(inp.get("bc_mask"), L.ip),
+                            L.ptr(inp.get("bc_u")), L.ptr(inp.get("bc_v")), L.ptr(u), L.ptr(v), C.byref(info))
+    return rc, u, v, info
+
+
+def bpa_residual(md, aa, zs, params, inp, u, v):
+    """One BPA state evaluation at (u, v) [nz*nT]: dict with rel_residual, r,
+    b [2*nT*nz] (rows 2*(ti*nz+k)+uv), eta_bk [nz*nT], beta_b [nT]."""
+    L = _L()
+    o = lib(); _setup_bpa_sigs(o)
+    nT, nz = md.mesh.nT, len(md.zeta)
+    out = {"r": np.zeros(2 * nT * nz), "b": np.zeros(2 * nT * nz), "eta_bk": np.zeros(nT * nz),
+           "beta_b": np.zeros(nT)}
+    keep, args = _aa_args(aa, zs)
+    out["rel_residual"] = o.oracle_bpa_residual(
+        md.byref(), C.byref(params), *args, L.ptr(inp["Hi"]), L.ptr(inp["Hs"]), L.ptr(inp["A_flow"]),
+        L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]), L.ptr(inp.get("bc_mask"), L.ip), L.ptr(inp.get("bc_u")),
+        L.ptr(inp.get("bc_v")), L.ptr(L.f64(u)), L.ptr(L.f64(v)), L.ptr(out["r"]), L.ptr(out["b"]),
+        L.ptr(out["eta_bk"]), L.ptr(out["beta_b"]))
+    return out
+
+
+def bpa_system(md, aa, zs, params, inp, u, v):
+    """The BPA system assembled at (u, v): (scipy CSR A, b), rows 2*(ti*nz+k)+uv."""
+    import scipy.sparse as sp
+    L = _L()
+    o = lib(); _setup_bpa_sigs(o)
+    if not getattr(o, "_bpa_sys", False):
+        dp, ip = L.dp, L.ip
+        o.oracle_bpa_system.argtypes = [C.POINTER(L.MeshDesc), C.POINTER(L.Params), C.c_int, ip, ip, dp, dp, dp,
+                                        dp, dp, dp, dp, dp, ip, dp, dp, dp, dp, C.c_int, ip, ip, dp, dp]
+        o._bpa_sys = True
+    nT, nz = md.mesh.nT, len(md.zeta)
+    n = 2 * nT * nz
+    cap = 2 * 3 * 2 * md.ops.bb.nnz * nz + 8 * n
+    ptr = np.zeros(n + 1, np.int32); ind = np.zeros(cap, np.int32); val = np.zeros(cap); b = np.zeros(n)
+    keep, args = _aa_args(aa, zs)
+    nnz = o.oracle_bpa_system(md.byref(), C.byref(params), *args, L.ptr(inp["Hi"]), L.ptr(inp["Hs"]),
+                              L.ptr(inp["A_flow"]), L.ptr(inp["beta_sq"]), L.ptr(inp["fgr_b"]),
+                              L.ptr(inp.get("bc_mask"), L.ip), L.ptr(inp.get("bc_u")), L.ptr(inp.get("bc_v")),
+                              L.ptr(L.f64(u)), L.ptr(L.f64(v)), cap, L.ptr(ptr, L.ip), L.ptr(ind, L.ip),
+                              L.ptr(val), L.ptr(b))
+    assert nnz > 0
+    return sp.csr_matrix((val[:nnz], ind[:nnz], ptr), shape=(n, n)), b

@@ -45,9 +45,13 @@ static double now_s(void) {
 /* ---------------------------------------------------------------------- */
 
 /* multiply_CSR_matrix_with_vector_1D (CSR_matrix_vector_multiplication.f90:198-334) */
+/* Loops shorter than OMP_MIN_N run on one thread: a parallel region costs
+ * more than such a loop, most of all on a loaded machine. */
+#define OMP_MIN_N 32768
+
 void oracle_csr_matvec(int m, const int *ptr, const int *ind, const double *val, const double *x,
                        double *y) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (m >= OMP_MIN_N)
   for (int i = 0; i < m; ++i) {
     double s = 0.0;
     for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[ind[k]];
@@ -57,7 +61,7 @@ void oracle_csr_matvec(int m, const int *ptr, const int *ind, const double *val,
 
 static double dot(int n, const double *a, const double *b) {
   double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
+#pragma omp parallel for reduction(+ : s) schedule(static) if (n >= OMP_MIN_N)
   for (int i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
@@ -203,7 +207,7 @@ int oracle_gmres_ilu0(int n, const int *ptr, const int *ind, const double *val, 
         double *vj = V + (size_t)j * n;
         double h = dot(n, w, vj);
         H[j * m + k] = h;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (n >= OMP_MIN_N)
         for (int i = 0; i < n; ++i) w[i] -= h * vj[i];
       }
       double hk1 = sqrt(dot(n, w, w));
@@ -236,7 +240,7 @@ int oracle_gmres_ilu0(int n, const int *ptr, const int *ind, const double *val, 
     }
     for (int j = 0; j < kk; ++j) {
       double *vj = V + (size_t)j * n;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (n >= OMP_MIN_N)
       for (int i = 0; i < n; ++i) x[i] += y[j] * vj[i];
     }
     free(y);
@@ -1446,4 +1450,913 @@ void oracle_strain_heating(int nV, int nz, const double *Hi, const double *A_flo
 void oracle_frictional_heating(int nV, int no_sliding, const double *beta, const double *uabs_base,
                                const int *mask_grounded, double *Q) {
   for (int vi = 0; vi < nV; ++vi) Q[vi] = (!no_sliding && mask_grounded[vi]) ? beta[vi] * uabs_base[vi] : 0.0;
+}
+
+/* ====================================================================== */
+/* BPA 3-D stress balance (BPA_main.f90) and what it runs on: the zeta     */
+/* gradients (zeta_gradients.f90:19-212), the 1-D zeta operators           */
+/* (mesh_zeta.f90:287-482) and the 3-D operators                           */
+/* (mesh_disc_calc_matrix_operators_3D.f90).                               */
+/* Internal 3-D layouts follow the reference's translation tables          */
+/* (mesh_translation_tables.f90:120-240): vik = vi*nz+k, tik = ti*nz+k,    */
+/* tiks = ti*(nz-1)+ks, tikuv = 2*tik+uv.  The public arrays are the       */
+/* Fortran ones, first index fastest ([k*nV+vi], [k*nT+ti]).               */
+/* ====================================================================== */
+
+#define BPA_MAX_NZ 64
+
+/* calc_shape_functions_1D_reg_2nd_order (shape_functions.f90:17-79) for two
+ * neighbours: d/dz and d2/dz2 coefficients of the neighbours (fc, fcc) and of
+ * the home point (minus their sums). */
+static void shape_1d_2nd(double z, const double zc[2], double *fi, double *fii, double fc[2], double fcc[2]) {
+  double dx[2], w[2], A[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 2; ++i) {
+    dx[i] = zc[i] - z;
+    w[i] = 1.0 / pow(fabs(dx[i]), 1.5);
+    double w2 = w[i] * w[i];
+    A[0] += w2 * dx[i] * dx[i];
+    A[1] += w2 * dx[i] * 0.5 * dx[i] * dx[i];
+    A[2] += w2 * 0.5 * dx[i] * dx[i] * dx[i];
+    A[3] += w2 * 0.5 * dx[i] * dx[i] * 0.5 * dx[i] * dx[i];
+  }
+  double det = A[0] * A[3] - A[1] * A[2];
+  double M[4] = {A[3] / det, -A[1] / det, -A[2] / det, A[0] / det};
+  for (int i = 0; i < 2; ++i) {
+    double w2 = w[i] * w[i];
+    fc[i] = w2 * (M[0] * dx[i] + M[1] * 0.5 * dx[i] * dx[i]);
+    fcc[i] = w2 * (M[2] * dx[i] + M[3] * 0.5 * dx[i] * dx[i]);
+  }
+  *fi = -(fc[0] + fc[1]);
+  *fii = -(fcc[0] + fcc[1]);
+}
+
+typedef struct {
+  int nz;
+  const double *zeta, *zs;
+  /* M_ddzeta_k_k_1D / M_d2dzeta2_k_k_1D (calc_vertical_operators_reg_1D,
+   * mesh_zeta.f90:287-365): row k = home, then its two neighbours */
+  int kc[BPA_MAX_NZ][3];
+  double kd[BPA_MAX_NZ][3], kdd[BPA_MAX_NZ][3];
+  /* M_map_k_ks_1D / M_ddzeta_k_ks_1D (:396-431): row ks, columns ks, ks+1 */
+  double kkm[BPA_MAX_NZ][2], kkd[BPA_MAX_NZ][2];
+  /* M_map_ks_k_1D / M_ddzeta_ks_k_1D (:433-480): row k, columns ksc[k][0..1] */
+  int ksc[BPA_MAX_NZ][2];
+  double ksm[BPA_MAX_NZ][2], ksd[BPA_MAX_NZ][2];
+} zops_t;
+
+static void zops_build(int nz, const double *zeta, const double *zs, zops_t *o) {
+  o->nz = nz; o->zeta = zeta; o->zs = zs;
+  for (int k = 0; k < nz; ++k) {
+    int c[2];
+    if (k == 0) { c[0] = 1; c[1] = 2; }
+    else if (k == nz - 1) { c[0] = nz - 3; c[1] = nz - 2; }
+    else { c[0] = k - 1; c[1] = k + 1; }
+    double zc[2] = {zeta[c[0]], zeta[c[1]]}, fi, fii, fc[2], fcc[2];
+    shape_1d_2nd(zeta[k], zc, &fi, &fii, fc, fcc);
+    o->kc[k][0] = k; o->kd[k][0] = fi; o->kdd[k][0] = fii;
+    for (int i = 0; i < 2; ++i) { o->kc[k][1 + i] = c[i]; o->kd[k][1 + i] = fc[i]; o->kdd[k][1 + i] = fcc[i]; }
+  }
+  for (int ks = 0; ks < nz - 1; ++ks) {
+    double dz = zeta[ks + 1] - zeta[ks];
+    double wlo = (zeta[ks + 1] - zs[ks]) / dz;
+    o->kkm[ks][0] = wlo; o->kkm[ks][1] = 1.0 - wlo;
+    o->kkd[ks][0] = -1.0 / dz; o->kkd[ks][1] = 1.0 / dz;
+  }
+  for (int k = 0; k < nz; ++k) {
+    int lo, hi;
+    if (k == 0) { lo = 0; hi = 1; }
+    else if (k == nz - 1) { lo = nz - 3; hi = nz - 2; }
+    else { lo = k - 1; hi = k; }
+    double dz = zs[hi] - zs[lo];
+    double wlo = (zs[hi] - zeta[k]) / dz;
+    o->ksc[k][0] = lo; o->ksc[k][1] = hi;
+    o->ksm[k][0] = wlo; o->ksm[k][1] = 1.0 - wlo;
+    o->ksd[k][0] = -1.0 / dz; o->ksd[k][1] = 1.0 / dz;
+  }
+}
+
+/* zeta gradients in the internal layouts */
+typedef struct {
+  double *ak[7];   /* dt, dx, dy, dz, d2x2, d2xy, d2y2  [vi*nz+k]      */
+  double *bk[6];   /* dx, dy, dz, d2x2, d2xy, d2y2      [ti*nz+k]      */
+  double *bks[6];  /* the same                          [ti*(nz-1)+ks] */
+} zgrad_t;
+
+static void zgrad_alloc(zgrad_t *g, int nV, int nT, int nz) {
+  for (int i = 0; i < 7; ++i) g->ak[i] = calloc((size_t)nV * nz, sizeof(double));
+  for (int i = 0; i < 6; ++i) g->bk[i] = calloc((size_t)nT * nz, sizeof(double));
+  for (int i = 0; i < 6; ++i) g->bks[i] = calloc((size_t)nT * (nz - 1), sizeof(double));
+}
+static void zgrad_free(zgrad_t *g) {
+  for (int i = 0; i < 7; ++i) free(g->ak[i]);
+  for (int i = 0; i < 6; ++i) { free(g->bk[i]); free(g->bks[i]); }
+}
+
+typedef struct {
+  int nnz;
+  const int *ptr, *ind;
+  const double *ddx, *ddy;
+} aa_ops_t;
+
+/* calc_zeta_gradients (zeta_gradients.f90:19-212) */
+static void zeta_gradients(const ufe_mesh_desc *m, const aa_ops_t *aa, const zops_t *z, const double *Hi,
+                           const double *Hs, const double *dHi_dt, const double *dHs_dt, zgrad_t *g) {
+  const int nV = m->nV, nT = m->nT, nz = m->nz;
+  const double *H[2] = {Hi, Hs};
+  double *dx_a[2], *dy_a[2], *xx_a[2], *xy_a[2], *yy_a[2], *dx_b[2], *dy_b[2], *xx_b[2], *xy_b[2], *yy_b[2];
+  double *Hb = malloc(sizeof(double) * nT);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_map, Hi, Hb);
+  for (int f = 0; f < 2; ++f) {
+    dx_a[f] = malloc(sizeof(double) * nV); dy_a[f] = malloc(sizeof(double) * nV);
+    xx_a[f] = malloc(sizeof(double) * nV); xy_a[f] = malloc(sizeof(double) * nV);
+    yy_a[f] = malloc(sizeof(double) * nV);
+    dx_b[f] = malloc(sizeof(double) * nT); dy_b[f] = malloc(sizeof(double) * nT);
+    xx_b[f] = malloc(sizeof(double) * nT); xy_b[f] = malloc(sizeof(double) * nT);
+    yy_b[f] = malloc(sizeof(double) * nT);
+    apply_csr(nV, aa->ptr, aa->ind, aa->ddx, H[f], dx_a[f]);
+    apply_csr(nV, aa->ptr, aa->ind, aa->ddy, H[f], dy_a[f]);
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddx, H[f], dx_b[f]);
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddy, H[f], dy_b[f]);
+    apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_ddx, dx_b[f], xx_a[f]);
+    apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_ddy, dx_b[f], xy_a[f]);
+    apply_csr(nV, m->ba_ptr, m->ba_ind, m->ba_ddy, dy_b[f], yy_a[f]);
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddx, dx_a[f], xx_b[f]);
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddy, dx_a[f], xy_b[f]);
+    apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddy, dy_a[f], yy_b[f]);
+  }
+  for (int vi = 0; vi < nV; ++vi) {
+    double h = fmax(10.0, Hi[vi]);
+    for (int k = 0; k < nz; ++k) {
+      double zt = z->zeta[k];
+      size_t i = (size_t)vi * nz + k;
+      double zx = (1.0 / h) * (dx_a[1][vi] - zt * dx_a[0][vi]);
+      double zy = (1.0 / h) * (dy_a[1][vi] - zt * dy_a[0][vi]);
+      g->ak[0][i] = (1.0 / h) * ((dHs_dt ? dHs_dt[vi] : 0.0) - zt * (dHi_dt ? dHi_dt[vi] : 0.0));
+      g->ak[1][i] = zx;
+      g->ak[2][i] = zy;
+      g->ak[3][i] = -1.0 / h;
+      g->ak[4][i] = (dx_a[0][vi] * -1.0 / h) * zx + (1.0 / h) * (xx_a[1][vi] - zt * xx_a[0][vi]);
+      g->ak[5][i] = (dy_a[0][vi] * -1.0 / h) * zx + (1.0 / h) * (xy_a[1][vi] - zt * xy_a[0][vi]);
+      g->ak[6][i] = (dy_a[0][vi] * -1.0 / h) * zy + (1.0 / h) * (yy_a[1][vi] - zt * yy_a[0][vi]);
+    }
+  }
+  for (int ti = 0; ti < nT; ++ti) {
+    double h = fmax(10.0, Hb[ti]);
+    for (int pass = 0; pass < 2; ++pass) {
+      int nl = pass == 0 ? nz : nz - 1;
+      double **o = pass == 0 ? g->bk : g->bks;
+      for (int k = 0; k < nl; ++k) {
+        double zt = pass == 0 ? z->zeta[k] : z->zs[k];
+        size_t i = (size_t)ti * nl + k;
+        double zx = (1.0 / h) * (dx_b[1][ti] - zt * dx_b[0][ti]);
+        double zy = (1.0 / h) * (dy_b[1][ti] - zt * dy_b[0][ti]);
+        o[0][i] = zx;
+        o[1][i] = zy;
+        o[2][i] = -1.0 / h;
+        o[3][i] = (dx_b[0][ti] * -1.0 / h) * zx + (1.0 / h) * (xx_b[1][ti] - zt * xx_b[0][ti]);
+        o[4][i] = (dy_b[0][ti] * -1.0 / h) * zx + (1.0 / h) * (xy_b[1][ti] - zt * xy_b[0][ti]);
+        o[5][i] = (dy_b[0][ti] * -1.0 / h) * zy + (1.0 / h) * (yy_b[1][ti] - zt * yy_b[0][ti]);
+      }
+    }
+  }
+  for (int f = 0; f < 2; ++f) {
+    free(dx_a[f]); free(dy_a[f]); free(xx_a[f]); free(xy_a[f]); free(yy_a[f]);
+    free(dx_b[f]); free(dy_b[f]); free(xx_b[f]); free(xy_b[f]); free(yy_b[f]);
+  }
+  free(Hb);
+}
+
+/* Growable CSR with up to 7 value arrays sharing one pattern. */
+typedef struct {
+  int rows, nv, nnz, cap;
+  int *ptr, *ind;
+  double *val[7];
+} csr_t;
+
+static void csr_init(csr_t *a, int rows, int nv, int cap) {
+  a->rows = 0; a->nv = nv; a->nnz = 0; a->cap = cap > 16 ? cap : 16;
+  a->ptr = malloc(sizeof(int) * ((size_t)rows + 1));
+  a->ptr[0] = 0;
+  a->ind = malloc(sizeof(int) * a->cap);
+  for (int i = 0; i < nv; ++i) a->val[i] = malloc(sizeof(double) * a->cap);
+}
+static void csr_push(csr_t *a, int col, const double *v) {
+  if (a->nnz == a->cap) {
+    a->cap *= 2;
+    a->ind = realloc(a->ind, sizeof(int) * a->cap);
+    for (int i = 0; i < a->nv; ++i) a->val[i] = realloc(a->val[i], sizeof(double) * a->cap);
+  }
+  a->ind[a->nnz] = col;
+  for (int i = 0; i < a->nv; ++i) a->val[i][a->nnz] = v[i];
+  a->nnz++;
+}
+static void csr_endrow(csr_t *a) { a->ptr[++a->rows] = a->nnz; }
+static void csr_free(csr_t *a) {
+  free(a->ptr); free(a->ind);
+  for (int i = 0; i < a->nv; ++i) free(a->val[i]);
+  memset(a, 0, sizeof(*a));
+}
+
+/* The 3-D operators of calc_3D_matrix_operators_mesh
+ * (mesh_disc_calc_matrix_operators_3D.f90:15-59) */
+typedef struct {
+  csr_t bk_ak;    /* ddx, ddy              (:61-215)   */
+  csr_t ak_bk;    /* ddx, ddy              (:217-371)  */
+  csr_t bk_bks;   /* ddz                   (:373-490)  */
+  csr_t bks_bk;   /* map, ddz              (:492-618)  */
+  csr_t bks_ak;   /* map                   (:620-724)  */
+  csr_t ak_bks;   /* map                   (:726-830)  */
+  csr_t bk_bk;    /* ddx, ddy, ddz, d2dx2, d2dxdy, d2dy2, d2dz2 (:832-1072) */
+} ops3d_t;
+
+static void ops3d_free(ops3d_t *o) {
+  csr_free(&o->bk_ak); csr_free(&o->ak_bk); csr_free(&o->bk_bks); csr_free(&o->bks_bk);
+  csr_free(&o->bks_ak); csr_free(&o->ak_bks); csr_free(&o->bk_bk);
+}
+
+static void ops3d_build(const ufe_mesh_desc *m, const zops_t *z, const zgrad_t *g, ops3d_t *o) {
+  const int nV = m->nV, nT = m->nT, nz = m->nz;
+  double v[7];
+  csr_init(&o->bk_ak, nV * nz, 2, m->ba_nnz * nz * 3);
+  for (int vi = 0; vi < nV; ++vi)
+    for (int k = 0; k < nz; ++k) {
+      size_t r = (size_t)vi * nz + k;
+      double zx = g->ak[1][r], zy = g->ak[2][r];
+      for (int e = m->ba_ptr[vi]; e < m->ba_ptr[vi + 1]; ++e) {
+        int ti = m->ba_ind[e];
+        for (int jj = 0; jj < 3; ++jj) {
+          int kk = z->kc[k][jj];
+          double cd = z->kd[k][jj];
+          v[0] = (kk == k ? m->ba_ddx[e] : 0.0) + zx * m->ba_map[e] * cd;
+          v[1] = (kk == k ? m->ba_ddy[e] : 0.0) + zy * m->ba_map[e] * cd;
+          csr_push(&o->bk_ak, ti * nz + kk, v);
+        }
+      }
+      csr_endrow(&o->bk_ak);
+    }
+  csr_init(&o->ak_bk, nT * nz, 2, m->ab_nnz * nz * 3);
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k) {
+      size_t r = (size_t)ti * nz + k;
+      double zx = g->bk[0][r], zy = g->bk[1][r];
+      for (int e = m->ab_ptr[ti]; e < m->ab_ptr[ti + 1]; ++e) {
+        int vi = m->ab_ind[e];
+        for (int jj = 0; jj < 3; ++jj) {
+          int kk = z->kc[k][jj];
+          double cd = z->kd[k][jj];
+          v[0] = (kk == k ? m->ab_ddx[e] : 0.0) + zx * m->ab_map[e] * cd;
+          v[1] = (kk == k ? m->ab_ddy[e] : 0.0) + zy * m->ab_map[e] * cd;
+          csr_push(&o->ak_bk, vi * nz + kk, v);
+        }
+      }
+      csr_endrow(&o->ak_bk);
+    }
+  csr_init(&o->bk_bks, nT * (nz - 1), 1, nT * (nz - 1) * 2);
+  for (int ti = 0; ti < nT; ++ti)
+    for (int ks = 0; ks < nz - 1; ++ks) {
+      double zz = g->bks[2][(size_t)ti * (nz - 1) + ks];
+      for (int jj = 0; jj < 2; ++jj) {
+        v[0] = zz * z->kkd[ks][jj];
+        csr_push(&o->bk_bks, ti * nz + ks + jj, v);
+      }
+      csr_endrow(&o->bk_bks);
+    }
+  csr_init(&o->bks_bk, nT * nz, 2, nT * nz * 2);
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k) {
+      double zz = g->bk[2][(size_t)ti * nz + k];
+      for (int jj = 0; jj < 2; ++jj) {
+        v[0] = z->ksm[k][jj];
+        v[1] = zz * z->ksd[k][jj];
+        csr_push(&o->bks_bk, ti * (nz - 1) + z->ksc[k][jj], v);
+      }
+      csr_endrow(&o->bks_bk);
+    }
+  csr_init(&o->bks_ak, nV * nz, 1, m->ba_nnz * nz * 2);
+  for (int vi = 0; vi < nV; ++vi)
+    for (int k = 0; k < nz; ++k) {
+      for (int e = m->ba_ptr[vi]; e < m->ba_ptr[vi + 1]; ++e) {
+        int ti = m->ba_ind[e];
+        for (int jj = 0; jj < 2; ++jj) {
+          v[0] = m->ba_map[e] * z->ksm[k][jj];
+          csr_push(&o->bks_ak, ti * (nz - 1) + z->ksc[k][jj], v);
+        }
+      }
+      csr_endrow(&o->bks_ak);
+    }
+  csr_init(&o->ak_bks, nT * (nz - 1), 1, m->ab_nnz * (nz - 1) * 2);
+  for (int ti = 0; ti < nT; ++ti)
+    for (int ks = 0; ks < nz - 1; ++ks) {
+      for (int e = m->ab_ptr[ti]; e < m->ab_ptr[ti + 1]; ++e) {
+        int vi = m->ab_ind[e];
+        for (int jj = 0; jj < 2; ++jj) {
+          v[0] = m->ab_map[e] * z->kkm[ks][jj];
+          csr_push(&o->ak_bks, vi * nz + ks + jj, v);
+        }
+      }
+      csr_endrow(&o->ak_bks);
+    }
+  csr_init(&o->bk_bk, nT * nz, 7, m->bb_nnz * nz * 3);
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k) {
+      size_t r = (size_t)ti * nz + k;
+      double zx = g->bk[0][r], zy = g->bk[1][r], zz = g->bk[2][r];
+      double zxx = g->bk[3][r], zxy = g->bk[4][r], zyy = g->bk[5][r];
+      for (int e = m->bb_ptr[ti]; e < m->bb_ptr[ti + 1]; ++e) {
+        int tj = m->bb_ind[e];
+        double hx = m->bb_ddx[e], hy = m->bb_ddy[e];
+        double hxx = m->bb_d2dx2[e], hxy = m->bb_d2dxdy[e], hyy = m->bb_d2dy2[e];
+        for (int jj = 0; jj < 3; ++jj) {
+          int kk = z->kc[k][jj];
+          double cz = z->kd[k][jj], czz = z->kdd[k][jj];
+          for (int i = 0; i < 7; ++i) v[i] = 0.0;
+          if (kk == k) { v[0] += hx; v[1] += hy; v[3] += hxx; v[4] += hxy; v[5] += hyy; }
+          if (tj == ti) {
+            v[0] += zx * cz;
+            v[1] += zy * cz;
+            v[2] += zz * cz;
+            v[3] += zxx * cz + zx * zx * czz;
+            v[4] += zxy * cz + zx * zy * czz;
+            v[5] += zyy * cz + zy * zy * czz;
+            v[6] += zz * zz * czz;
+          }
+          v[3] += 2.0 * zx * hx * cz;
+          v[4] += zx * hy * cz + zy * hx * cz;
+          v[5] += 2.0 * zy * hy * cz;
+          csr_push(&o->bk_bk, tj * nz + kk, v);
+        }
+      }
+      csr_endrow(&o->bk_bk);
+    }
+}
+
+static void csr_apply(const csr_t *a, int which, const double *x, double *y) {
+  oracle_csr_matvec(a->rows, a->ptr, a->ind, a->val[which], x, y);
+}
+
+/* calc_zeta_gradients on the public layouts (first index fastest): for the
+ * parity tests of the engine's zeta-gradient entry point.  ak [7][nV*nz],
+ * bk [6][nT*nz], bks [6][nT*(nz-1)]; any may be NULL. */
+int oracle_zeta_gradients(const ufe_mesh_desc *m, int aa_nnz, const int *aa_ptr, const int *aa_ind,
+                          const double *aa_ddx, const double *aa_ddy, const double *zeta_stag, const double *Hi,
+                          const double *Hs, const double *dHi_dt, const double *dHs_dt, double *const *ak,
+                          double *const *bk, double *const *bks) {
+  const int nV = m->nV, nT = m->nT, nz = m->nz;
+  if (nz < 3 || nz > BPA_MAX_NZ) return -1;
+  aa_ops_t aa = {aa_nnz, aa_ptr, aa_ind, aa_ddx, aa_ddy};
+  zops_t z;
+  zops_build(nz, m->zeta, zeta_stag, &z);
+  zgrad_t g;
+  zgrad_alloc(&g, nV, nT, nz);
+  zeta_gradients(m, &aa, &z, Hi, Hs, dHi_dt, dHs_dt, &g);
+  for (int f = 0; f < 7; ++f)
+    if (ak && ak[f])
+      for (int vi = 0; vi < nV; ++vi)
+        for (int k = 0; k < nz; ++k) ak[f][(size_t)k * nV + vi] = g.ak[f][(size_t)vi * nz + k];
+  for (int f = 0; f < 6; ++f) {
+    if (bk && bk[f])
+      for (int ti = 0; ti < nT; ++ti)
+        for (int k = 0; k < nz; ++k) bk[f][(size_t)k * nT + ti] = g.bk[f][(size_t)ti * nz + k];
+    if (bks && bks[f])
+      for (int ti = 0; ti < nT; ++ti)
+        for (int k = 0; k < nz - 1; ++k) bks[f][(size_t)k * nT + ti] = g.bks[f][(size_t)ti * (nz - 1) + k];
+  }
+  zgrad_free(&g);
+  return 0;
+}
+
+/* The 3-D operators applied to fields, for the operator pins: with f_bk on
+ * the bk grid and f_ak on the ak grid (public layouts), out_bk [7][nT*nz] =
+ * bk->bk ddx, ddy, ddz, d2dx2, d2dxdy, d2dy2, d2dz2 of f_bk; out_ak [2][nV*nz]
+ * = bk->ak ddx, ddy of f_bk; out_akbk [2][nT*nz] = ak->bk ddx, ddy of f_ak;
+ * out_bks [1][nT*(nz-1)] = bk->bks ddz of f_bk; out_bksbk [2][nT*nz] = bks->bk
+ * map, ddz of f_bks; out_map_bks_ak [nV*nz] of f_bks; out_map_ak_bks
+ * [nT*(nz-1)] of f_ak.  Zeta gradients from Hi, Hs (no time derivative). */
+int oracle_ops3d_apply(const ufe_mesh_desc *m, int aa_nnz, const int *aa_ptr, const int *aa_ind,
+                       const double *aa_ddx, const double *aa_ddy, const double *zeta_stag, const double *Hi,
+                       const double *Hs, const double *f_bk, const double *f_ak, const double *f_bks,
+                       double *const *out_bk, double *const *out_ak, double *const *out_akbk, double *out_bks,
+                       double *const *out_bksbk, double *out_map_bks_ak, double *out_map_ak_bks) {
+  const int nV = m->nV, nT = m->nT, nz = m->nz;
+  if (nz < 3 || nz > BPA_MAX_NZ) return -1;
+  aa_ops_t aa = {aa_nnz, aa_ptr, aa_ind, aa_ddx, aa_ddy};
+  zops_t z;
+  zops_build(nz, m->zeta, zeta_stag, &z);
+  zgrad_t g;
+  zgrad_alloc(&g, nV, nT, nz);
+  zeta_gradients(m, &aa, &z, Hi, Hs, NULL, NULL, &g);
+  ops3d_t o;
+  ops3d_build(m, &z, &g, &o);
+  size_t nbk = (size_t)nT * nz, nak = (size_t)nV * nz, nbks = (size_t)nT * (nz - 1);
+  double *xb = malloc(sizeof(double) * nbk), *xa = malloc(sizeof(double) * nak);
+  double *xs = malloc(sizeof(double) * nbks);
+  double *y = malloc(sizeof(double) * (nbk > nak ? nbk : nak));
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k) xb[(size_t)ti * nz + k] = f_bk[(size_t)k * nT + ti];
+  for (int vi = 0; vi < nV; ++vi)
+    for (int k = 0; k < nz; ++k) xa[(size_t)vi * nz + k] = f_ak[(size_t)k * nV + vi];
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz - 1; ++k) xs[(size_t)ti * (nz - 1) + k] = f_bks[(size_t)k * nT + ti];
+#define OUT_BK(dst) for (int ti = 0; ti < nT; ++ti) for (int k = 0; k < nz; ++k) (dst)[(size_t)k * nT + ti] = y[(size_t)ti * nz + k]
+#define OUT_AK(dst) for (int vi = 0; vi < nV; ++vi) for (int k = 0; k < nz; ++k) (dst)[(size_t)k * nV + vi] = y[(size_t)vi * nz + k]
+#define OUT_BKS(dst) for (int ti = 0; ti < nT; ++ti) for (int k = 0; k < nz - 1; ++k) (dst)[(size_t)k * nT + ti] = y[(size_t)ti * (nz - 1) + k]
+  for (int i = 0; i < 7; ++i) { csr_apply(&o.bk_bk, i, xb, y); OUT_BK(out_bk[i]); }
+  for (int i = 0; i < 2; ++i) { csr_apply(&o.bk_ak, i, xb, y); OUT_AK(out_ak[i]); }
+  for (int i = 0; i < 2; ++i) { csr_apply(&o.ak_bk, i, xa, y); OUT_BK(out_akbk[i]); }
+  csr_apply(&o.bk_bks, 0, xb, y); OUT_BKS(out_bks);
+  for (int i = 0; i < 2; ++i) { csr_apply(&o.bks_bk, i, xs, y); OUT_BK(out_bksbk[i]); }
+  csr_apply(&o.bks_ak, 0, xs, y); OUT_AK(out_map_bks_ak);
+  csr_apply(&o.ak_bks, 0, xa, y); OUT_BKS(out_map_ak_bks);
+#undef OUT_BK
+#undef OUT_AK
+#undef OUT_BKS
+  free(xb); free(xa); free(xs); free(y);
+  ops3d_free(&o);
+  zgrad_free(&g);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* solve_BPA (BPA_main.f90:90-252)                                         */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+  const ufe_mesh_desc *m;
+  const ufe_params *p;
+  int nV, nT, nz;
+  zops_t z;
+  zgrad_t g;
+  ops3d_t o;
+  const double *Hi, *Hs, *rough, *fgr;
+  double *A_ak, *A_bks, *Hib;
+  double *dhx, *dhy, *dbx, *dby, *tdx, *tdy;
+  double *sr_ak[6], *sr_bks[6];   /* du/dx, du/dy, du/dz, dv/dx, dv/dy, dv/dz */
+  double *eta_ak, *eta_bks, *eta_bk, *dex, *dey, *dez, *tmp_bk;
+  double *beta_a, *beta_b, *ub, *vb;
+  double *u, *v, *up, *vp;        /* [ti*nz+k] */
+  int *bcm;
+  double *bcu, *bcv;
+  copy_lists_t copy;
+  csr_t A;
+  double *b, *x;
+} bpa_t;
+
+static void bpa_free(bpa_t *s) {
+  zgrad_free(&s->g);
+  ops3d_free(&s->o);
+  free(s->A_ak); free(s->A_bks); free(s->Hib);
+  free(s->dhx); free(s->dhy); free(s->dbx); free(s->dby); free(s->tdx); free(s->tdy);
+  for (int i = 0; i < 6; ++i) { free(s->sr_ak[i]); free(s->sr_bks[i]); }
+  free(s->eta_ak); free(s->eta_bks); free(s->eta_bk); free(s->dex); free(s->dey); free(s->dez); free(s->tmp_bk);
+  free(s->beta_a); free(s->beta_b); free(s->ub); free(s->vb);
+  free(s->u); free(s->v); free(s->up); free(s->vp);
+  free(s->bcm); free(s->bcu); free(s->bcv);
+  copy_lists_free(&s->copy);
+  if (s->A.ptr) csr_free(&s->A);
+  free(s->b); free(s->x);
+}
+
+/* Set-up of one solve_BPA call: zeta gradients and 3-D operators for the
+ * current geometry (:144-151), driving stress (calc_driving_stress,
+ * :1727-1756), inputs into the internal layouts. */
+static int bpa_setup(bpa_t *s, const ufe_mesh_desc *m, const ufe_params *p, const aa_ops_t *aa,
+                     const double *zeta_stag, const double *Hi, const double *Hs, const double *A_flow,
+                     const double *rough, const double *fgr, const int *bc_mask, const double *bc_u,
+                     const double *bc_v) {
+  memset(s, 0, sizeof(*s));
+  const int nV = m->nV, nT = m->nT, nz = m->nz;
+  if (nz < 3 || nz > BPA_MAX_NZ) return UFE_ERR_ARGUMENT;
+  for (int side = 0; side < 4; ++side) {
+    int cu = p->bc_u[side], cv = p->bc_v[side];
+    if (cu == UFE_BC_INFINITE_SSA_ICESTREAM || cv == UFE_BC_INFINITE_SSA_ICESTREAM) return UFE_ERR_ARGUMENT;
+  }
+  s->m = m; s->p = p; s->nV = nV; s->nT = nT; s->nz = nz;
+  s->Hi = Hi; s->Hs = Hs; s->rough = rough; s->fgr = fgr;
+  size_t nak = (size_t)nV * nz, nbk = (size_t)nT * nz, nbks = (size_t)nT * (nz - 1);
+  zops_build(nz, m->zeta, zeta_stag, &s->z);
+  zgrad_alloc(&s->g, nV, nT, nz);
+  zeta_gradients(m, aa, &s->z, Hi, Hs, NULL, NULL, &s->g);
+  ops3d_build(m, &s->z, &s->g, &s->o);
+  s->A_ak = malloc(sizeof(double) * nak);
+  for (int vi = 0; vi < nV; ++vi)
+    for (int k = 0; k < nz; ++k) s->A_ak[(size_t)vi * nz + k] = A_flow[(size_t)k * nV + vi];
+  s->A_bks = malloc(sizeof(double) * nbks);
+  csr_apply(&s->o.ak_bks, 0, s->A_ak, s->A_bks);
+  s->Hib = malloc(sizeof(double) * nV);
+  for (int vi = 0; vi < nV; ++vi) s->Hib[vi] = Hs[vi] - Hi[vi];
+  s->dhx = malloc(sizeof(double) * nT); s->dhy = malloc(sizeof(double) * nT);
+  s->dbx = malloc(sizeof(double) * nT); s->dby = malloc(sizeof(double) * nT);
+  s->tdx = malloc(sizeof(double) * nT); s->tdy = malloc(sizeof(double) * nT);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddx, Hs, s->dhx);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddy, Hs, s->dhy);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddx, s->Hib, s->dbx);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_ddy, s->Hib, s->dby);
+  for (int ti = 0; ti < nT; ++ti) {
+    s->tdx[ti] = -ICE_DENSITY * GRAV * s->dhx[ti];
+    s->tdy[ti] = -ICE_DENSITY * GRAV * s->dhy[ti];
+  }
+  for (int i = 0; i < 6; ++i) {
+    s->sr_ak[i] = malloc(sizeof(double) * nak);
+    s->sr_bks[i] = malloc(sizeof(double) * nbks);
+  }
+  s->eta_ak = malloc(sizeof(double) * nak);
+  s->eta_bks = malloc(sizeof(double) * nbks);
+  s->eta_bk = malloc(sizeof(double) * nbk); s->dex = malloc(sizeof(double) * nbk);
+  s->dey = malloc(sizeof(double) * nbk); s->dez = malloc(sizeof(double) * nbk);
+  s->tmp_bk = malloc(sizeof(double) * nbk);
+  s->beta_a = malloc(sizeof(double) * nV); s->beta_b = malloc(sizeof(double) * nT);
+  s->ub = malloc(sizeof(double) * nT); s->vb = malloc(sizeof(double) * nT);
+  s->u = malloc(sizeof(double) * nbk); s->v = malloc(sizeof(double) * nbk);
+  s->up = malloc(sizeof(double) * nbk); s->vp = malloc(sizeof(double) * nbk);
+  if (bc_mask) {
+    s->bcm = malloc(sizeof(int) * nbk);
+    s->bcu = malloc(sizeof(double) * nbk);
+    s->bcv = malloc(sizeof(double) * nbk);
+    for (int ti = 0; ti < nT; ++ti)
+      for (int k = 0; k < nz; ++k) {
+        size_t i = (size_t)ti * nz + k, j = (size_t)k * nT + ti;
+        s->bcm[i] = bc_mask[j]; s->bcu[i] = bc_u[j]; s->bcv[i] = bc_v[j];
+      }
+  }
+  copy_lists_build(m, p, &s->copy);
+  s->b = malloc(sizeof(double) * 2 * nbk);
+  s->x = malloc(sizeof(double) * 2 * nbk);
+  return 0;
+}
+
+/* calc_strain_rates (:1758-1804), calc_effective_viscosity (:1806-1931) and
+ * calc_applied_basal_friction_coefficient (:1933-1978) at the state's u, v. */
+static void bpa_state(bpa_t *s, double eps0_applied) {
+  const ufe_mesh_desc *m = s->m;
+  const ufe_params *p = s->p;
+  const int nV = s->nV, nT = s->nT, nz = s->nz;
+  size_t nak = (size_t)nV * nz, nbk = (size_t)nT * nz, nbks = (size_t)nT * (nz - 1);
+  double *f[2] = {s->u, s->v};
+  for (int c = 0; c < 2; ++c) {
+    csr_apply(&s->o.bk_ak, 0, f[c], s->sr_ak[3 * c + 0]);
+    csr_apply(&s->o.bk_ak, 1, f[c], s->sr_ak[3 * c + 1]);
+    csr_apply(&s->o.bk_bks, 0, f[c], s->sr_bks[3 * c + 2]);
+    csr_apply(&s->o.ak_bks, 0, s->sr_ak[3 * c + 0], s->sr_bks[3 * c + 0]);
+    csr_apply(&s->o.ak_bks, 0, s->sr_ak[3 * c + 1], s->sr_bks[3 * c + 1]);
+    csr_apply(&s->o.bks_ak, 0, s->sr_bks[3 * c + 2], s->sr_ak[3 * c + 2]);
+  }
+  const double n = p->glens_n;
+  const double eta_max = 0.5 * pow(1e-18, -1.0 / n) * pow(eps0_applied, (1.0 - n) / (2.0 * n));
+  for (size_t i = 0; i < nak; ++i) {
+    double e = glen_3d(eps0_applied, s->sr_ak[0][i], s->sr_ak[1][i], s->sr_ak[2][i], s->sr_ak[3][i],
+                       s->sr_ak[4][i], s->sr_ak[5][i], s->A_ak[i], n);
+    s->eta_ak[i] = fmin(fmax(e, p->visc_eff_min), eta_max);
+  }
+  /* the staggered viscosity uses the configured regularisation (:1876-1880) */
+  for (size_t i = 0; i < nbks; ++i) {
+    double e = glen_3d(p->epsilon_sq_0, s->sr_bks[0][i], s->sr_bks[1][i], s->sr_bks[2][i], s->sr_bks[3][i],
+                       s->sr_bks[4][i], s->sr_bks[5][i], s->A_bks[i], n);
+    s->eta_bks[i] = fmin(fmax(e, p->visc_eff_min), eta_max);
+  }
+  csr_apply(&s->o.ak_bk, 0, s->eta_ak, s->dex);
+  csr_apply(&s->o.ak_bk, 1, s->eta_ak, s->dey);
+  csr_apply(&s->o.bks_bk, 1, s->eta_bks, s->dez);
+  /* map_a_b_3D of eta_ak and map_bks_bk of eta_bks, blended by the shear factor */
+  csr_apply(&s->o.bks_bk, 0, s->eta_bks, s->tmp_bk);
+  for (int ti = 0; ti < nT; ++ti) {
+    size_t t0 = (size_t)ti * nz, tb = t0 + nz - 1;
+    double us = sqrt(0.1 + s->u[t0] * s->u[t0] + s->v[t0] * s->v[t0]);
+    double ubs = sqrt(0.1 + s->u[tb] * s->u[tb] + s->v[tb] * s->v[tb]);
+    double R = ubs / us;
+    for (int k = 0; k < nz; ++k) {
+      double ea = 0.0;
+      for (int e = m->ab_ptr[ti]; e < m->ab_ptr[ti + 1]; ++e) ea += m->ab_map[e] * s->eta_ak[(size_t)m->ab_ind[e] * nz + k];
+      s->eta_bk[t0 + k] = R * ea + (1.0 - R) * s->tmp_bk[t0 + k];
+    }
+  }
+  for (int ti = 0; ti < nT; ++ti) { s->ub[ti] = s->u[(size_t)ti * nz + nz - 1]; s->vb[ti] = s->v[(size_t)ti * nz + nz - 1]; }
+  sliding_beta(m, p, s->ub, s->vb, s->rough, s->beta_a);
+  apply_csr(nT, m->ab_ptr, m->ab_ind, m->ab_map, s->beta_a, s->beta_b);
+  if (p->do_gl_subgrid_friction)
+    for (int ti = 0; ti < nT; ++ti) s->beta_b[ti] *= pow(s->fgr[ti], p->subgrid_friction_exponent);
+  (void)nbk;
+}
+
+/* solve_BPA_linearised's assembly (:330-478) with the row routines
+ * calc_BPA_stiffness_matrix_row_free (:480-644), _BC_surf (:646-889),
+ * _BC_base (:891-1163) and _BC_west/east/south/north (:1165-1723).
+ * Returns 0 or UFE_ERR_ARGUMENT for a border with no neighbour. */
+static int bpa_assemble(bpa_t *s) {
+  const ufe_mesh_desc *m = s->m;
+  const ufe_params *p = s->p;
+  const int nT = s->nT, nz = s->nz;
+  const csr_t *B = &s->o.bk_bk;
+  if (s->A.ptr) csr_free(&s->A);
+  csr_init(&s->A, 2 * nT * nz, 1, B->nnz * 2 * 2);
+  double val;
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k)
+      for (int uv = 0; uv < 2; ++uv) {
+        const size_t tik = (size_t)ti * nz + k;
+        const int row = (int)(2 * tik + uv);
+        double rhs = 0.0;
+        if (s->bcm && s->bcm[tik] == 1) {
+          val = 1.0; csr_push(&s->A, row, &val);
+          rhs = uv == 0 ? s->bcu[tik] : s->bcv[tik];
+        } else if (m->TriBI[ti] > 0) {
+          const int side = bc_side(m->TriBI[ti]);
+          const int choice = uv == 0 ? p->bc_u[side] : p->bc_v[side];
+          if (choice == UFE_BC_INFINITE) {
+            int nn = 0;
+            for (int c = 0; c < 3; ++c) {
+              int tj = m->TriC[3 * ti + c];
+              if (tj < 0) continue;
+              ++nn;
+              val = 1.0; csr_push(&s->A, (int)(2 * ((size_t)tj * nz + k) + uv), &val);
+            }
+            if (nn == 0) return UFE_ERR_ARGUMENT;
+            val = -(double)nn; csr_push(&s->A, row, &val);
+          } else if (choice == UFE_BC_ZERO) {
+            val = 1.0; csr_push(&s->A, row, &val);
+          } else if (choice == UFE_BC_PERIODIC_ISMIP_HOM) {
+            val = 1.0; csr_push(&s->A, row, &val);
+            const double *f = uv == 0 ? s->up : s->vp;
+            const int cr = 2 * ti + uv;
+            double cv = 0.0;
+            for (int e = s->copy.ptr[cr]; e < s->copy.ptr[cr + 1]; ++e)
+              cv += s->copy.w[e] * f[(size_t)s->copy.tri[e] * nz + k];
+            rhs = (p->visc_it_relax * cv) + ((1.0 - p->visc_it_relax) * f[tik]);
+          } else {
+            return UFE_ERR_ARGUMENT;
+          }
+        } else {
+          const double tdx = s->tdx[ti], tdy = s->tdy[ti];
+          double eta, dex = s->dex[tik], dey = s->dey[tik], dez = s->dez[tik];
+          /* coefficients of du/dx, du/dy, dv/dx, dv/dy, d2u/dx2, d2u/dy2, d2v/dxdy
+           * (x-row) or of dv/dy, dv/dx, du/dy, du/dx, d2v/dy2, d2v/dx2, d2u/dxdy
+           * (y-row), plus the column terms of the own triangle */
+          double c_own_x = 0, c_own_y = 0, c_own_xx = 0, c_own_yy = 0, c_oth_x = 0, c_oth_y = 0, c_oth_xy = 0;
+          double c_k = 0, c_kn = 0;
+          int kn = -1, free_row = 0;
+          if (k == 0) {
+            eta = s->eta_bks[(size_t)ti * (nz - 1)];
+            const double hx = s->dhx[ti], hy = s->dhy[ti];
+            const double zz = s->g.bk[2][tik], dz = m->zeta[1] - m->zeta[0];
+            const double q = 2.0 * eta / dz * zz;
+            const double hs = uv == 0 ? hx : hy, ho = uv == 0 ? hy : hx;   /* own / other direction */
+            const double ds = uv == 0 ? dex : dey, dox = uv == 0 ? dey : dex;
+            c_own_x = 4.0 * ds - q * 4.0 * hs + 4.0 * dez * hs;   /* d(own)/d(own dir) */
+            c_own_y = dox - q * ho + dez * ho;                   /* d(own)/d(other dir) */
+            c_oth_y = 2.0 * ds - q * 2.0 * hs + 2.0 * dez * hs;  /* d(other)/d(other dir) */
+            c_oth_x = dox - q * ho + dez * ho;                   /* d(other)/d(own dir) */
+            c_own_xx = 4.0 * eta; c_own_yy = eta; c_oth_xy = 3.0 * eta;
+            c_k = -2.0 * eta / (dz * dz) * zz * zz;
+            c_kn = 2.0 * eta / (dz * dz) * zz * zz;
+            kn = 1;
+          } else if (k == nz - 1) {
+            if (p->sliding_law == UFE_SLIDING_NO_SLIDING) {
+              val = 1.0; csr_push(&s->A, row, &val);
+              s->b[row] = 0.0;
+              csr_endrow(&s->A);
+              continue;
+            }
+            eta = s->eta_bks[(size_t)ti * (nz - 1) + nz - 2];
+            const double bx = s->dbx[ti], by = s->dby[ti];
+            const double zz = s->g.bk[2][tik], dz = m->zeta[nz - 1] - m->zeta[nz - 2];
+            const double Q = 2.0 * eta / (dz * dz) * zz * zz;
+            const double R = 2.0 * eta / dz * zz + dez;
+            const double bs = uv == 0 ? bx : by, bo = uv == 0 ? by : bx;
+            const double ds = uv == 0 ? dex : dey, dox = uv == 0 ? dey : dex;
+            c_own_x = 4.0 * ds + 4.0 * R * bs;
+            c_own_y = dox + R * bo;
+            c_oth_y = 2.0 * ds + 2.0 * R * bs;
+            c_oth_x = dox + R * bo;
+            c_own_xx = 4.0 * eta; c_own_yy = eta; c_oth_xy = 3.0 * eta;
+            c_k = (R * s->beta_b[ti] / eta) - Q;
+            c_kn = Q;
+            kn = nz - 2;
+          } else {
+            eta = s->eta_bk[tik];
+            free_row = 1;
+          }
+          for (int e = B->ptr[tik]; e < B->ptr[tik + 1]; ++e) {
+            const int col = B->ind[e], tj = col / nz, kk = col % nz;
+            const double ox = B->val[0][e], oy = B->val[1][e], oz = B->val[2][e];
+            const double oxx = B->val[3][e], oxy = B->val[4][e], oyy = B->val[5][e], ozz = B->val[6][e];
+            /* own-direction and other-direction first/second derivatives */
+            const double os = uv == 0 ? ox : oy, oo = uv == 0 ? oy : ox;
+            const double oss = uv == 0 ? oxx : oyy, ooo = uv == 0 ? oyy : oxx;
+            double a_own, a_oth;
+            if (free_row) {
+              const double ds = uv == 0 ? dex : dey, dox = uv == 0 ? dey : dex;
+              a_own = 4.0 * eta * oss + 4.0 * ds * os + eta * ooo + dox * oo + eta * ozz + dez * oz;
+              a_oth = 3.0 * eta * oxy + 2.0 * ds * oo + dox * os;
+            } else {
+              a_own = c_own_x * os + c_own_y * oo + c_own_xx * oss + c_own_yy * ooo;
+              if (tj == ti && kk == k) a_own += c_k;
+              if (tj == ti && kk == kn) a_own += c_kn;
+              a_oth = c_oth_y * oo + c_oth_x * os + c_oth_xy * oxy;
+            }
+            const int col_own = (int)(2 * (size_t)col + uv), col_oth = (int)(2 * (size_t)col + (1 - uv));
+            csr_push(&s->A, col_own, &a_own);
+            csr_push(&s->A, col_oth, &a_oth);
+          }
+          rhs = uv == 0 ? -tdx : -tdy;
+        }
+        s->b[row] = rhs;
+        csr_endrow(&s->A);
+      }
+  return 0;
+}
+
+static double bpa_residual_norm(const bpa_t *s, double *r_out) {
+  const int n = 2 * s->nT * s->nz;
+  double *ax = malloc(sizeof(double) * n);
+  oracle_csr_matvec(n, s->A.ptr, s->A.ind, s->A.val[0], s->x, ax);
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double r = s->b[i] - ax[i];
+    if (r_out) r_out[i] = r;
+    rr += r * r;
+    bb += s->b[i] * s->b[i];
+  }
+  free(ax);
+  return sqrt(rr) / fmax(sqrt(bb), 1e-300);
+}
+
+static void bpa_load_x(bpa_t *s) {
+  size_t nbk = (size_t)s->nT * s->nz;
+  for (size_t i = 0; i < nbk; ++i) { s->x[2 * i] = s->u[i]; s->x[2 * i + 1] = s->v[i]; }
+}
+
+/* solve_BPA (:90-252).  u_bk, v_bk [nT*nz] (first index fastest): in the
+ * initial velocity, out the solution.  The a->a operators and zeta_stag are
+ * the mesh's (calc_zeta_gradients needs them). */
+int oracle_solve_bpa(const ufe_mesh_desc *m, const ufe_params *p, int aa_nnz, const int *aa_ptr,
+                     const int *aa_ind, const double *aa_ddx, const double *aa_ddy, const double *zeta_stag,
+                     const double *Hi, const double *Hs, const double *A_flow, const double *beta_sq,
+                     const double *fgr_b, const int *mask_grounded, const int *bc_mask, const double *bc_u,
+                     const double *bc_v, double *u_bk, double *v_bk, ufe_solve_info *info) {
+  const int nT = m->nT, nz = m->nz;
+  const size_t nbk = (size_t)nT * nz;
+  if (!any_grounded(m->nV, mask_grounded)) {
+    memset(u_bk, 0, sizeof(double) * nbk);
+    memset(v_bk, 0, sizeof(double) * nbk);
+    if (info) memset(info, 0, sizeof(*info));
+    return 0;
+  }
+  double t0 = now_s();
+  aa_ops_t aa = {aa_nnz, aa_ptr, aa_ind, aa_ddx, aa_ddy};
+  bpa_t s;
+  int rc = bpa_setup(&s, m, p, &aa, zeta_stag, Hi, Hs, A_flow, beta_sq, fgr_b, bc_mask, bc_u, bc_v);
+  if (rc) { bpa_free(&s); return rc; }
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k) {
+      s.u[(size_t)ti * nz + k] = u_bk[(size_t)k * nT + ti];
+      s.v[(size_t)ti * nz + k] = v_bk[(size_t)k * nT + ti];
+    }
+  double resid = 1e9, resid_prev;
+  int nit_div = 0, it = 0, n_axb = 0, converged = 0;
+  double relax = p->visc_it_relax, eps0 = p->epsilon_sq_0;
+  while (!converged) {
+    ++it;
+    bpa_state(&s, eps0);
+    /* solve_BPA_linearised: previous solution, system, solve */
+    memcpy(s.up, s.u, sizeof(double) * nbk);
+    memcpy(s.vp, s.v, sizeof(double) * nbk);
+    rc = bpa_assemble(&s);
+    if (rc) break;
+    bpa_load_x(&s);
+    int its = oracle_gmres_ilu0((int)(2 * nbk), s.A.ptr, s.A.ind, s.A.val[0], s.b, s.x, p->rtol, p->abstol,
+                                p->krylov_maxit, p->gmres_restart, !p->warm_start, NULL);
+    if (its < 0) { rc = UFE_ERR_ARGUMENT; break; }
+    n_axb += its;
+    for (size_t i = 0; i < nbk; ++i) { s.u[i] = s.x[2 * i]; s.v[i] = s.x[2 * i + 1]; }
+    /* apply_velocity_limits (:2051-2084) */
+    for (size_t i = 0; i < nbk; ++i) {
+      double ua = sqrt(s.u[i] * s.u[i] + s.v[i] * s.v[i]);
+      if (ua > p->vel_max) { s.u[i] = s.u[i] * p->vel_max / ua; s.v[i] = s.v[i] * p->vel_max / ua; }
+    }
+    /* relax_viscosity_iterations (:1982-2005) */
+    for (size_t i = 0; i < nbk; ++i) {
+      s.u[i] = relax * s.u[i] + (1.0 - relax) * s.up[i];
+      s.v[i] = relax * s.v[i] + (1.0 - relax) * s.vp[i];
+    }
+    /* calc_visc_iter_UV_resid (:2007-2049) */
+    resid_prev = resid;
+    double r1 = 0, r2 = 0;
+    for (size_t i = 0; i < nbk; ++i) {
+      r1 += (s.u[i] - s.up[i]) * (s.u[i] - s.up[i]) + (s.v[i] - s.vp[i]) * (s.v[i] - s.vp[i]);
+      r2 += (s.u[i] + s.up[i]) * (s.u[i] + s.up[i]) + (s.v[i] + s.vp[i]) * (s.v[i] + s.vp[i]);
+    }
+    resid = 2.0 * r1 / fmax(r2, 1e-8);
+    if (resid > resid_prev) ++nit_div; else nit_div = 0;
+    if (nit_div > 2) { nit_div = 0; relax *= 0.9; eps0 *= 1.2; }
+    if (relax <= 0.05 || eps0 >= 1e-5) {
+      if (relax < 0.05 || eps0 > 1e-5) { rc = UFE_ERR_DIVERGED; break; }
+    }
+    if (resid < p->visc_it_norm_dUV_tol) converged = 1;
+    if (it > p->visc_it_nit) { rc = UFE_WARN_NOT_CONVERGED; break; }
+  }
+  if (rc >= 0) {
+    for (int ti = 0; ti < nT; ++ti)
+      for (int k = 0; k < nz; ++k) {
+        u_bk[(size_t)k * nT + ti] = s.u[(size_t)ti * nz + k];
+        v_bk[(size_t)k * nT + ti] = s.v[(size_t)ti * nz + k];
+      }
+  }
+  if (info) {
+    double rel = -1.0;
+    if (rc >= 0) {
+      memcpy(s.up, s.u, sizeof(double) * nbk);
+      memcpy(s.vp, s.v, sizeof(double) * nbk);
+      bpa_state(&s, eps0);
+      if (bpa_assemble(&s) == 0) { bpa_load_x(&s); rel = bpa_residual_norm(&s, NULL); }
+    }
+    memset(info, 0, sizeof(*info));
+    info->n_visc_its = it;
+    info->n_Axb_its = n_axb;
+    info->L2_uv = resid;
+    info->rel_residual = rel;
+    info->rel_residual_unscaled = rel;
+    info->visc_it_relax_applied = relax;
+    info->epsilon_sq_0_applied = eps0;
+    info->seconds = now_s() - t0;
+  }
+  bpa_free(&s);
+  return rc;
+}
+
+/* One evaluation of the viscosity iteration's state at (u_bk, v_bk) with the
+ * linearised system assembled there and applied to (u, v): r = b - A u and b
+ * in the reference row order (tikuv = 2*(ti*nz+k)+uv), eta_bk [nT*nz] (first
+ * index fastest) and beta_b [nT]; returns ||r||/||b|| (negative on error).
+ * The lagged copy rows read (u_bk, v_bk) as the previous iterate. */
+double oracle_bpa_residual(const ufe_mesh_desc *m, const ufe_params *p, int aa_nnz, const int *aa_ptr,
+                           const int *aa_ind, const double *aa_ddx, const double *aa_ddy,
+                           const double *zeta_stag, const double *Hi, const double *Hs, const double *A_flow,
+                           const double *beta_sq, const double *fgr_b, const int *bc_mask, const double *bc_u,
+                           const double *bc_v, const double *u_bk, const double *v_bk, double *r, double *b,
+                           double *eta_bk, double *beta_b) {
+  const int nT = m->nT, nz = m->nz;
+  const size_t nbk = (size_t)nT * nz;
+  aa_ops_t aa = {aa_nnz, aa_ptr, aa_ind, aa_ddx, aa_ddy};
+  bpa_t s;
+  if (bpa_setup(&s, m, p, &aa, zeta_stag, Hi, Hs, A_flow, beta_sq, fgr_b, bc_mask, bc_u, bc_v)) {
+    bpa_free(&s);
+    return -1.0;
+  }
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k) {
+      s.u[(size_t)ti * nz + k] = u_bk[(size_t)k * nT + ti];
+      s.v[(size_t)ti * nz + k] = v_bk[(size_t)k * nT + ti];
+    }
+  memcpy(s.up, s.u, sizeof(double) * nbk);
+  memcpy(s.vp, s.v, sizeof(double) * nbk);
+  bpa_state(&s, p->epsilon_sq_0);
+  double rel = -1.0;
+  if (bpa_assemble(&s) == 0) {
+    bpa_load_x(&s);
+    rel = bpa_residual_norm(&s, r);
+    if (b) memcpy(b, s.b, sizeof(double) * 2 * nbk);
+    if (eta_bk)
+      for (int ti = 0; ti < nT; ++ti)
+        for (int k = 0; k < nz; ++k) eta_bk[(size_t)k * nT + ti] = s.eta_bk[(size_t)ti * nz + k];
+    if (beta_b) memcpy(beta_b, s.beta_b, sizeof(double) * nT);
+  }
+  bpa_free(&s);
+  return rel;
+}
+
+/* The BPA system assembled at (u_bk, v_bk) (first index fastest), in the
+ * reference's row order tikuv = 2*(ti*nz+k)+uv: CSR into ptr [2*nT*nz+1],
+ * ind/val [cap], b [2*nT*nz].  Returns nnz, or -1 if cap is too small or the
+ * assembly fails. */
+int oracle_bpa_system(const ufe_mesh_desc *m, const ufe_params *p, int aa_nnz, const int *aa_ptr,
+                      const int *aa_ind, const double *aa_ddx, const double *aa_ddy, const double *zeta_stag,
+                      const double *Hi, const double *Hs, const double *A_flow, const double *beta_sq,
+                      const double *fgr_b, const int *bc_mask, const double *bc_u, const double *bc_v,
+                      const double *u_bk, const double *v_bk, int cap, int *ptr, int *ind, double *val,
+                      double *b) {
+  const int nT = m->nT, nz = m->nz;
+  const size_t nbk = (size_t)nT * nz;
+  aa_ops_t aa = {aa_nnz, aa_ptr, aa_ind, aa_ddx, aa_ddy};
+  bpa_t s;
+  if (bpa_setup(&s, m, p, &aa, zeta_stag, Hi, Hs, A_flow, beta_sq, fgr_b, bc_mask, bc_u, bc_v)) {
+    bpa_free(&s);
+    return -1;
+  }
+  for (int ti = 0; ti < nT; ++ti)
+    for (int k = 0; k < nz; ++k) {
+      s.u[(size_t)ti * nz + k] = u_bk[(size_t)k * nT + ti];
+      s.v[(size_t)ti * nz + k] = v_bk[(size_t)k * nT + ti];
+    }
+  memcpy(s.up, s.u, sizeof(double) * nbk);
+  memcpy(s.vp, s.v, sizeof(double) * nbk);
+  bpa_state(&s, p->epsilon_sq_0);
+  int nnz = -1;
+  if (bpa_assemble(&s) == 0 && s.A.nnz <= cap) {
+    nnz = s.A.nnz;
+    memcpy(ptr, s.A.ptr, sizeof(int) * (2 * nbk + 1));
+    memcpy(ind, s.A.ind, sizeof(int) * nnz);
+    memcpy(val, s.A.val[0], sizeof(double) * nnz);
+    memcpy(b, s.b, sizeof(double) * 2 * nbk);
+  }
+  bpa_free(&s);
+  return nnz;
 }

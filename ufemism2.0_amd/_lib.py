@@ -209,6 +209,7 @@ def lib():
     _sig(L, 'ufe_mesh_operators_ab', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_ba', argtypes=[C.c_int, C.c_int, dp, ip, dp, ip, ip, ip, ip, ip, dp, dp, dp])
     _sig(L, 'ufe_mesh_operators_bb2', argtypes=[C.c_int, ip, dp, ip, ip, ip, dp, dp, dp, dp, dp])
+    _sig(L, 'ufe_mesh_operators_aa', argtypes=[C.c_int, dp, ip, ip, ip, ip, ip, dp, dp])
     _sig(L, 'ufe_heat_setup', argtypes=[C.c_void_p, ip, ip])
     _sig(L, 'ufe_heat_upwind_derivatives', argtypes=[C.c_void_p, dp, dp, dp, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_heat_strain_heating', argtypes=[C.c_void_p, dp, dp, C.POINTER(dp), dp])
@@ -229,7 +230,7 @@ EXPORTED = [
     "ufe_kernel_timing_read", "ufe_set_amg_options", "ufe_mpnn_num_weights", "ufe_mpnn_create",
     "ufe_mpnn_destroy", "ufe_mpnn_get_weights", "ufe_mpnn_forward", "ufe_mpnn_loss_grad", "ufe_mpnn_train",
     "ufe_mpnn_features", "ufe_mpnn_initial_guess", "ufe_set_graph_launch", "ufe_ssa_residual", "ufe_diva_residual", "ufe_csr_matvec", "ufe_csr_solve", "ufe_csr_solve_ex", "ufe_fraction_gr_b", "ufe_heat_columns", "ufe_heat_columns_device", "ufe_heat_columns_device_noalloc", "ufe_heat_robin",
-    "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2",
+    "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2", "ufe_mesh_operators_aa",
     "ufe_thickness_params_default", "ufe_thickness_setup", "ufe_thickness_semiimplicit",
     "ufe_heat_setup", "ufe_heat_upwind_derivatives", "ufe_heat_strain_heating", "ufe_heat_frictional_heating",
 ]

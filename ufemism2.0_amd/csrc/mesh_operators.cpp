@@ -5,6 +5,7 @@
 //   b->a  map/ddx/ddy   (mesh_disc_calc_matrix_operators_2D.f90:337-474)
 //   b->b  2nd-order ddx/ddy/d2dx2/d2dxdy/d2dy2
 //                       (mesh_disc_calc_matrix_operators_2D.f90:612-764)
+//   a->a  ddx/ddy       (mesh_disc_calc_matrix_operators_2D.f90:61-196)
 // using weighted least-squares shape functions with distance weights
 // w = 1/|d|^q, q = 1.5 (shape_functions.f90:13, :218-364 and :366-451).
 //
@@ -86,6 +87,33 @@ bool shape_stag_1st(double x, double y, int nc, const double* xc, const double* 
     Nfx[i] = w2[i] * (M[3] + M[4] * dx[i] + M[5] * dy[i]);
     Nfy[i] = w2[i] * (M[6] + M[7] * dx[i] + M[8] * dy[i]);
   }
+  return true;
+}
+
+// calc_shape_functions_2D_reg_1st_order (shape_functions.f90:140-216):
+// d/dx, d/dy of a first-order fit through the home point; home coefficients
+// are minus the sums of the neighbours'.
+bool shape_reg_1st(double x, double y, int nc, const double* xc, const double* yc, double* Nfx_i,
+                   double* Nfy_i, double* Nfx, double* Nfy) {
+  double A[4] = {0, 0, 0, 0};
+  std::vector<double> w2(nc), dx(nc), dy(nc);
+  for (int i = 0; i < nc; ++i) {
+    dx[i] = xc[i] - x; dy[i] = yc[i] - y;
+    double w = 1.0 / std::pow(std::sqrt(dx[i] * dx[i] + dy[i] * dy[i]), kQ);
+    w2[i] = w * w;
+    A[0] += w2[i] * dx[i] * dx[i]; A[1] += w2[i] * dx[i] * dy[i];
+    A[2] += w2[i] * dy[i] * dx[i]; A[3] += w2[i] * dy[i] * dy[i];
+  }
+  double det = A[0] * A[3] - A[1] * A[2];
+  if (std::fabs(det) < 2.2250738585072014e-308) return false;
+  double M[4] = {A[3] / det, -A[1] / det, -A[2] / det, A[0] / det};
+  double sx = 0.0, sy = 0.0;
+  for (int i = 0; i < nc; ++i) {
+    Nfx[i] = w2[i] * (M[0] * dx[i] + M[1] * dy[i]);
+    Nfy[i] = w2[i] * (M[2] * dx[i] + M[3] * dy[i]);
+    sx += Nfx[i]; sy += Nfy[i];
+  }
+  *Nfx_i = -sx; *Nfy_i = -sy;
   return true;
 }
 
@@ -274,6 +302,53 @@ extern "C" int ufe_mesh_operators_bb2(int nT, const int* TriC, const double* Tri
     }
     nnz += nc + 1;
     if (ptr) ptr[ti + 1] = nnz;
+  }
+  *nnz_out = nnz;
+  return 0;
+}
+
+// a->a ddx/ddy (calc_matrix_operators_mesh_a_a,
+// mesh_disc_calc_matrix_operators_2D.f90:61-196): the home vertex first, then
+// its flood-fill neighbourhood (at least two neighbours, grown while the fit
+// is singular).
+extern "C" int ufe_mesh_operators_aa(int nV, const double* V, const int* C_ptr, const int* C_ind,
+                                     int* nnz_out, int* ptr, int* ind, double* val_ddx, double* val_ddy) {
+  const int n_min = 2;
+  std::vector<int> map(nV, 0), stack;
+  std::vector<double> xc, yc, fx, fy;
+  int nnz = 0;
+  if (ptr) ptr[0] = 0;
+  for (int vi = 0; vi < nV; ++vi) {
+    clear(map, stack);
+    stack.clear();
+    map[vi] = 1;
+    stack.push_back(vi);
+    while ((int)stack.size() - 1 < n_min) {
+      size_t before = stack.size();
+      extend_a(C_ptr, C_ind, map, stack);
+      if (stack.size() == before) return -1;
+    }
+    double fxi = 0.0, fyi = 0.0;
+    while (true) {
+      int nc = (int)stack.size() - 1;
+      xc.resize(nc); yc.resize(nc); fx.resize(nc); fy.resize(nc);
+      for (int i = 0; i < nc; ++i) { xc[i] = V[2 * stack[i + 1]]; yc[i] = V[2 * stack[i + 1] + 1]; }
+      if (shape_reg_1st(V[2 * vi], V[2 * vi + 1], nc, xc.data(), yc.data(), &fxi, &fyi, fx.data(), fy.data())) break;
+      size_t before = stack.size();
+      extend_a(C_ptr, C_ind, map, stack);
+      if (stack.size() == before) return -1;
+    }
+    int nc = (int)stack.size() - 1;
+    if (ind) {
+      ind[nnz] = vi;
+      val_ddx[nnz] = fxi; val_ddy[nnz] = fyi;
+      for (int i = 0; i < nc; ++i) {
+        ind[nnz + 1 + i] = stack[i + 1];
+        val_ddx[nnz + 1 + i] = fx[i]; val_ddy[nnz + 1 + i] = fy[i];
+      }
+    }
+    nnz += nc + 1;
+    if (ptr) ptr[vi + 1] = nnz;
   }
   *nnz_out = nnz;
   return 0;

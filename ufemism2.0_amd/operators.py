@@ -73,6 +73,23 @@ def build_operators(mesh: Mesh) -> MeshOperators:
     return MeshOperators(ab, ba, bb)
 
 
+def build_aa(mesh: Mesh) -> OperatorFamily:
+    """a->a ddx/ddy (mesh%M_ddx_a_a, mesh%M_ddy_a_a;
+    mesh_disc_calc_matrix_operators_2D.f90:61-196), read by the zeta gradients
+    of the BPA and the heat columns."""
+    lib = L.lib()
+    V, cp, ci = L.f64(mesh.V), L.i32(mesh.C_ptr), L.i32(mesh.C_ind)
+    return _family(lib.ufe_mesh_operators_aa, mesh.nV, ["ddx", "ddy"], mesh.nV, L.ptr(V), L.ptr(cp, L.ip),
+                   L.ptr(ci, L.ip))
+
+
+def zeta_stag(zeta: np.ndarray) -> np.ndarray:
+    """Staggered vertical grid of the regular zeta grids: the layer midpoints
+    (mesh_zeta.f90:79-80)."""
+    zeta = np.asarray(zeta, dtype=np.float64)
+    return 0.5 * (zeta[:-1] + zeta[1:])
+
+
 class MeshDescriptor:
     """Owns the arrays behind a ufe_mesh_desc so the pointers stay valid."""
 
