@@ -493,6 +493,68 @@ int ufe_thickness_setup(ufe_ctx *ctx, const ufe_vertex_geometry *g);
 int ufe_thickness_semiimplicit(ufe_ctx *ctx, const ufe_thickness_params *p, const ufe_thickness_inputs *in,
                                double *Hi_tplusdt, double *dHi_dt, double *divQ, double *dt_max, int *n_its);
 
+/* ---- zeta gradients and the BPA 3-D stress balance ---------------------
+ * Both run on the context's mesh and layers (3 <= nz <= 32).
+ *
+ * Set-up, once per context: the a->a derivative operators of the mesh
+ * (mesh%M_ddx_a_a, mesh%M_ddy_a_a; nV rows, built by ufe_mesh_operators_aa)
+ * and the staggered layers mesh%zeta_stag [nz-1] (NULL: the midpoints of
+ * zeta, as mesh_zeta.f90:79-80 sets them for the regular grids).  The 1-D
+ * zeta operators are those of calc_vertical_operators_reg_1D / _stag_1D
+ * (mesh_zeta.f90:287-482). */
+int ufe_zeta_setup(ufe_ctx *ctx, int aa_nnz, const int *aa_ptr, const int *aa_ind, const double *aa_ddx,
+                   const double *aa_ddy, const double *zeta_stag);
+
+/* calc_zeta_gradients (zeta_gradients.f90:19-212).  Hi, Hs, dHi_dt, dHs_dt:
+ * [nV] (the time derivatives may be NULL = 0).  Outputs, first index fastest,
+ * each pointer may be NULL:
+ *   ak  [7][nV*nz]:    dzeta_dt, dzeta_dx, dzeta_dy, dzeta_dz, d2zeta_dx2,
+ *                      d2zeta_dxdy, d2zeta_dy2 on the vertices
+ *   bk  [6][nT*nz]:    dzeta_dx, dzeta_dy, dzeta_dz, d2zeta_dx2, d2zeta_dxdy,
+ *                      d2zeta_dy2 on the triangles
+ *   bks [6][nT*(nz-1)]: the same on the staggered layers
+ * (ice%dzeta_*_ak / _bk / _bks, read by the heat columns and the BPA). */
+int ufe_zeta_gradients(ufe_ctx *ctx, const double *Hi, const double *Hs, const double *dHi_dt,
+                       const double *dHs_dt, double *const *ak, double *const *bk, double *const *bks);
+
+/* Inputs of solve_BPA (BPA_main.f90:90-110): Hi, Hs, beta_sq (or
+ * till_yield_stress), mask_grounded_ice [nV] (mask NULL = all grounded);
+ * A_flow [nV*nz] (vertex fastest); fraction_gr_b [nT] (NULL = 1);
+ * BC_prescr_mask_bk, _u_bk, _v_bk [nT*nz] (triangle fastest) or all NULL. */
+typedef struct ufe_bpa_inputs {
+  const double *Hi, *Hs, *A_flow, *beta_sq, *fraction_gr_b;
+  const int    *mask_grounded_ice;
+  const int    *BC_prescr_mask_bk;
+  const double *BC_prescr_u_bk, *BC_prescr_v_bk;
+} ufe_bpa_inputs;
+
+/* solve_BPA (BPA_main.f90:90-252): the viscosity iteration of the
+ * Blatter-Pattyn approximation on the bk grid.  u_bk, v_bk [nT*nz] (triangle
+ * fastest, u_bk(ti,k) of the reference): in the starting velocity (BPA%u_bk),
+ * out the solution.  Parameters from the context: glens_n, epsilon_sq_0,
+ * visc_eff_min, vel_max, visc_it_*, rtol/abstol (stress_balance_PETSc_*),
+ * sliding law, sub-grid friction, the border choices ('infinite', 'zero',
+ * 'periodic_ISMIP-HOM' with ismip_hom_L), krylov_maxit, gmres_restart,
+ * warm_start.  The linear solves are GMRES(gmres_restart) preconditioned by
+ * an exact solve of each triangle's column (the layers' block tridiagonal
+ * part of the system); precond and krylov are not read.  Returns
+ * UFE_WARN_NOT_CONVERGED when visc_it_nit is reached; info as for SSA/DIVA,
+ * rel_residual = ||b - A u|| / ||b|| of the system assembled at the returned
+ * velocity. */
+int ufe_solve_bpa(ufe_ctx *ctx, const ufe_bpa_inputs *in, double *u_bk, double *v_bk, ufe_solve_info *info);
+
+/* One evaluation of the viscosity iteration's state at (u_bk, v_bk) [nT*nz]
+ * (strain rates, viscosity on the ak, bks and bk grids, basal friction;
+ * BPA_main.f90:1758-1978) with the stiffness matrix and load vector of
+ * solve_BPA_linearised (:330-1723) assembled there and applied to it, at the
+ * configured epsilon_sq_0; the lagged copy rows read (u_bk, v_bk) as the
+ * previous iterate.  Outputs, each may be NULL: r = b - A u and b
+ * [2*nT*nz] in the reference's row order 2*(ti*nz+k)+uv (mesh%tikuv2n);
+ * eta_bk [nT*nz] (triangle fastest); beta_b [nT] (the applied basal friction
+ * coefficient); *rel_residual = ||r|| / ||b||. */
+int ufe_bpa_residual(ufe_ctx *ctx, const ufe_bpa_inputs *in, const double *u_bk, const double *v_bk, double *r,
+                     double *b, double *eta_bk, double *beta_b, double *rel_residual);
+
 /* ---- mesh operator construction (host, once per mesh) -----------------
  * calc_matrix_operators_mesh_a_b / _b_a / _b_b_2nd_order
  * (mesh_disc_calc_matrix_operators_2D.f90:198-474,612-764).  Two-call

@@ -111,6 +111,13 @@ class ThicknessParams(C.Structure):
     ]
 
 
+class BpaInputs(C.Structure):
+    _fields_ = [
+        ("Hi", dp), ("Hs", dp), ("A_flow", dp), ("beta_sq", dp), ("fraction_gr_b", dp),
+        ("mask_grounded_ice", ip), ("BC_prescr_mask_bk", ip), ("BC_prescr_u_bk", dp), ("BC_prescr_v_bk", dp),
+    ]
+
+
 class ThicknessInputs(C.Structure):
     _fields_ = [
         ("Hi", dp), ("Hb", dp), ("SL", dp), ("u_vav_b", dp), ("v_vav_b", dp), ("SMB", dp), ("BMB", dp),
@@ -215,6 +222,11 @@ def lib():
     _sig(L, 'ufe_heat_strain_heating', argtypes=[C.c_void_p, dp, dp, C.POINTER(dp), dp])
     _sig(L, 'ufe_heat_frictional_heating', argtypes=[C.c_void_p, dp, dp, ip, dp])
     _sig(L, 'ufe_thickness_params_default', argtypes=[C.POINTER(ThicknessParams)])
+    _sig(L, 'ufe_zeta_setup', argtypes=[C.c_void_p, C.c_int, ip, ip, dp, dp, dp])
+    _sig(L, 'ufe_zeta_gradients', argtypes=[C.c_void_p, dp, dp, dp, dp, C.POINTER(dp), C.POINTER(dp),
+                                            C.POINTER(dp)])
+    _sig(L, 'ufe_solve_bpa', argtypes=[C.c_void_p, C.POINTER(BpaInputs), dp, dp, C.POINTER(SolveInfo)])
+    _sig(L, 'ufe_bpa_residual', argtypes=[C.c_void_p, C.POINTER(BpaInputs), dp, dp, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_thickness_setup', argtypes=[C.c_void_p, C.POINTER(VertexGeometry)])
     _sig(L, 'ufe_thickness_semiimplicit', argtypes=[C.c_void_p, C.POINTER(ThicknessParams),
                                                     C.POINTER(ThicknessInputs), dp, dp, dp, dp, ip])
@@ -233,6 +245,7 @@ EXPORTED = [
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2", "ufe_mesh_operators_aa",
     "ufe_thickness_params_default", "ufe_thickness_setup", "ufe_thickness_semiimplicit",
     "ufe_heat_setup", "ufe_heat_upwind_derivatives", "ufe_heat_strain_heating", "ufe_heat_frictional_heating",
+    "ufe_zeta_setup", "ufe_zeta_gradients", "ufe_solve_bpa", "ufe_bpa_residual",
 ]
 
 

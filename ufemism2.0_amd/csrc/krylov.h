@@ -114,6 +114,9 @@ class Krylov {
   // Run the coarse levels of the V-cycle with at most max_rows block rows in
   // one workgroup (0: one launch per pass on every level).
   void set_amg_tail(int max_rows) { amg_tail_rows_ = max_rows; }
+  // A preconditioner of the caller's, z = M^-1 r on the FP64 vectors, used by
+  // gmres64 in place of the built-in ones while set (empty function: off).
+  void set_custom_pc(std::function<void(const double *r, double *z)> f) { custom_pc_ = std::move(f); }
   // Launch BiCGStab iteration chunks as replays of captured HIP graphs
   // (same kernels in the same order: identical results).
   void set_graphs(bool on) { use_graphs_ = on; }
@@ -180,6 +183,7 @@ class Krylov {
 
   int nb_;
   hipStream_t s_;
+  std::function<void(const double *, double *)> custom_pc_;
   int pc_ = UFE_PC_BLOCK_JACOBI;
   const AmgHierarchy *amg_ = nullptr;
   const int *A_diag_ = nullptr;

@@ -1711,7 +1711,9 @@ SolveStats Krylov::gmres(const BsrT<T> &A, const T *b, T *x, double rtol, double
     while (j < m) {
       for (; j < next_poll; ++j) {
         T *vj = V + (long)j * n, *w = V + (long)(j + 1) * n;
-        if (amg) {
+        if (custom_pc_ && sizeof(T) == sizeof(double)) {
+          custom_pc_(reinterpret_cast<const double *>(vj), reinterpret_cast<double *>(z));
+        } else if (amg) {
           precond_apply<T>(A, vj, z);
         } else {
           hipLaunchKernelGGL(k_pc_apply<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb,
@@ -1743,7 +1745,9 @@ SolveStats Krylov::gmres(const BsrT<T> &A, const T *b, T *x, double rtol, double
     UFE_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gmres_set_done, dim3(1), dim3(1), 0, s_, sc, 0);
     UFE_LAUNCH_CHECK();
-    if (amg) {
+    if (custom_pc_ && sizeof(T) == sizeof(double)) {
+      custom_pc_(reinterpret_cast<const double *>(t), reinterpret_cast<double *>(z));
+    } else if (amg) {
       precond_apply<T>(A, t, z);
     } else {
       hipLaunchKernelGGL(k_pc_apply<T>, dim3(blocks_for(nb)), dim3(kBlock), 0, s_, nb, reinterpret_cast<const V2 *>(t),

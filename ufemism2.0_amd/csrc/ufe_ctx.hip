@@ -67,6 +67,7 @@ struct ufe_ctx {
   DBuf feat;                     // learned-guess features [nT][16] fp32 (device order)
   std::shared_ptr<void> thickness;  // state of the thickness solver (thickness.hip), set up on demand
   std::shared_ptr<void> heat_mesh;  // vertex-triangle lists of the heat-column inputs (heat_inputs.hip)
+  std::shared_ptr<void> bpa;        // zeta operators and BPA solver state (bpa.hip), set up on demand
   hipEvent_t ev[8];
 };
 
@@ -170,6 +171,39 @@ struct VertexGrid {
   }
 };
 
+// Triangles and weights of the lagged copy row of border triangle ti whose BC
+// choice is a copy choice: the copy point (find_ti_copy_ISMIP_HOM_periodic /
+// find_ti_copy_SSA_icestream_infinite, mesh_utilities.f90:2622-2730), the
+// vertex nearest to it, and 1/d^2 weights to the triangles around that vertex.
+void copy_entry(ufe_ctx *c, int ti, int choice, std::unique_ptr<VertexGrid> &grid, std::vector<int> &ctri,
+                std::vector<double> &cw) {
+  const double gx = c->h_TriGC[2 * ti], gy = c->h_TriGC[2 * ti + 1];
+  double px, py;
+  if (choice == UFE_BC_PERIODIC_ISMIP_HOM) {
+    const double L = c->prm.ismip_hom_L;
+    px = gx > 0.0 ? gx - L / 2.0 : gx + L / 2.0;
+    py = gy > 0.0 ? gy - L / 2.0 : gy + L / 2.0;
+  } else {
+    py = gy;
+    px = gx < 0.0 ? c->bbox[0] + (c->bbox[1] - c->bbox[0]) * 1.0 / 3.0
+                  : c->bbox[0] + (c->bbox[1] - c->bbox[0]) * 2.0 / 3.0;
+  }
+  if (!grid) grid = std::make_unique<VertexGrid>(c->h_V);
+  const int vi = grid->nearest(px, py);
+  const int k0 = c->h_ba_ptr[vi], k1 = c->h_ba_ptr[vi + 1];
+  double sum = 0.0;
+  const size_t base = cw.size();
+  for (int k = k0; k < k1; ++k) {
+    const int tj = c->h_ba_ind[k];
+    const double dx = px - c->h_TriGC[2 * tj], dy = py - c->h_TriGC[2 * tj + 1];
+    const double d = std::sqrt(dx * dx + dy * dy);
+    ctri.push_back(tj);
+    cw.push_back(1.0 / (d * d));
+    sum += 1.0 / (d * d);
+  }
+  for (size_t k = base; k < cw.size(); ++k) cw[k] /= sum;
+}
+
 void build_rowtypes(ufe_ctx *c, const int *bc_mask) {
   int nT = c->m.nT;
   c->h_rowtype.assign(2 * (size_t)nT, ROW_FREE);
@@ -185,32 +219,7 @@ void build_rowtypes(ufe_ctx *c, const int *bc_mask) {
         int choice = uv == 0 ? c->prm.bc_u[side] : c->prm.bc_v[side];
         if (choice == UFE_BC_PERIODIC_ISMIP_HOM || choice == UFE_BC_INFINITE_SSA_ICESTREAM) {
           t = ROW_COPY;
-          // copy point (find_ti_copy_ISMIP_HOM_periodic / find_ti_copy_SSA_icestream_infinite)
-          const double gx = c->h_TriGC[2 * ti], gy = c->h_TriGC[2 * ti + 1];
-          double px, py;
-          if (choice == UFE_BC_PERIODIC_ISMIP_HOM) {
-            const double L = c->prm.ismip_hom_L;
-            px = gx > 0.0 ? gx - L / 2.0 : gx + L / 2.0;
-            py = gy > 0.0 ? gy - L / 2.0 : gy + L / 2.0;
-          } else {
-            py = gy;
-            px = gx < 0.0 ? c->bbox[0] + (c->bbox[1] - c->bbox[0]) * 1.0 / 3.0
-                          : c->bbox[0] + (c->bbox[1] - c->bbox[0]) * 2.0 / 3.0;
-          }
-          if (!grid) grid = std::make_unique<VertexGrid>(c->h_V);
-          const int vi = grid->nearest(px, py);
-          const int k0 = c->h_ba_ptr[vi], k1 = c->h_ba_ptr[vi + 1];
-          double sum = 0.0;
-          const size_t base = cw.size();
-          for (int k = k0; k < k1; ++k) {
-            const int tj = c->h_ba_ind[k];
-            const double dx = px - c->h_TriGC[2 * tj], dy = py - c->h_TriGC[2 * tj + 1];
-            const double d = std::sqrt(dx * dx + dy * dy);
-            ctri.push_back(tj);
-            cw.push_back(1.0 / (d * d));
-            sum += 1.0 / (d * d);
-          }
-          for (size_t k = base; k < cw.size(); ++k) cw[k] /= sum;
+          copy_entry(c, ti, choice, grid, ctri, cw);
           crow.push_back(2 * ti + uv);
           cptr.push_back((int)ctri.size());
         } else {
@@ -819,6 +828,28 @@ const std::vector<int> &ctx_vperm(const ufe_ctx *c) { return c->vperm; }
 const std::vector<int> &ctx_tperm(const ufe_ctx *c) { return c->tperm; }
 std::shared_ptr<void> &ctx_thickness(ufe_ctx *c) { return c->thickness; }
 std::shared_ptr<void> &ctx_heat_mesh(ufe_ctx *c) { return c->heat_mesh; }
+std::shared_ptr<void> &ctx_bpa(ufe_ctx *c) { return c->bpa; }
+// Lagged copy lists of every border triangle whose BC choice (per component)
+// is a copy choice, with no prescribed-velocity mask: rows 2*ti+uv (device
+// order), their triangles and weights (CSR).
+void ctx_copy_lists(ufe_ctx *c, std::vector<int> &crow, std::vector<int> &cptr, std::vector<int> &ctri,
+                    std::vector<double> &cw) {
+  crow.clear(); ctri.clear(); cw.clear();
+  cptr.assign(1, 0);
+  std::unique_ptr<VertexGrid> grid;
+  for (int ti = 0; ti < c->m.nT; ++ti) {
+    if (c->h_TriBI[ti] <= 0) continue;
+    const int side = (c->h_TriBI[ti] - 1) / 2;
+    for (int uv = 0; uv < 2; ++uv) {
+      const int choice = uv == 0 ? c->prm.bc_u[side] : c->prm.bc_v[side];
+      if (choice != UFE_BC_PERIODIC_ISMIP_HOM && choice != UFE_BC_INFINITE_SSA_ICESTREAM) continue;
+      copy_entry(c, ti, choice, grid, ctri, cw);
+      crow.push_back(2 * ti + uv);
+      cptr.push_back((int)ctri.size());
+    }
+  }
+}
+const std::vector<int> &ctx_TriBI(const ufe_ctx *c) { return c->h_TriBI; }
 const ufe_params &ctx_params(const ufe_ctx *c) { return c->prm; }
 const double *ctx_velocity(const ufe_ctx *c) { return c->u.as<double>(); }
 int ctx_guard(ufe_ctx *c, const std::function<int()> &f) { return guarded(c, f); }

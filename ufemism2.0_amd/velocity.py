@@ -331,6 +331,86 @@ class VelocitySolver:
                                                          L.ptr(L.f64(uabs_base)), L.ptr(mk, L.ip), L.ptr(Q)))
         return Q
 
+    # -- zeta gradients and the BPA ---------------------------------------------
+    def zeta_setup(self, aa=None, zeta_stag=None):
+        """Attach the a->a operators (built from the mesh when not given) and
+        the staggered layers (midpoints when not given)."""
+        from .operators import build_aa
+        aa = aa if aa is not None else build_aa(self.md.mesh)
+        zs = None if zeta_stag is None else L.f64(zeta_stag)
+        self._zeta_keep = (L.i32(aa.ptr), L.i32(aa.ind), L.f64(aa.vals["ddx"]), L.f64(aa.vals["ddy"]), zs)
+        k = self._zeta_keep
+        self._check(self.lib.ufe_zeta_setup(self._ctx, aa.nnz, L.ptr(k[0], L.ip), L.ptr(k[1], L.ip), L.ptr(k[2]),
+                                            L.ptr(k[3]), L.ptr(zs)))
+
+    def _need_zeta(self):
+        if getattr(self, "_zeta_keep", None) is None:
+            self.zeta_setup()
+
+    def zeta_gradients(self, Hi, Hs, dHi_dt=None, dHs_dt=None):
+        """calc_zeta_gradients (zeta_gradients.f90:19-212): (ak, bk, bks) lists of
+        7, 6 and 6 fields (nz*nV,), (nz*nT,), ((nz-1)*nT,), first index fastest:
+        ak = dzeta_dt, dzeta_dx, dzeta_dy, dzeta_dz, d2zeta_dx2, d2zeta_dxdy,
+        d2zeta_dy2; bk and bks = the same without dzeta_dt."""
+        self._need_zeta()
+        m = self.md.mesh
+        nz = len(self.md.zeta)
+        ak = [np.zeros(nz * m.nV) for _ in range(7)]
+        bk = [np.zeros(nz * m.nT) for _ in range(6)]
+        bks = [np.zeros((nz - 1) * m.nT) for _ in range(6)]
+        ins = [None if a is None else L.f64(a) for a in (Hi, Hs, dHi_dt, dHs_dt)]
+        arr = [(L.dp * len(x))(*(L.ptr(a) for a in x)) for x in (ak, bk, bks)]
+        self._check(self.lib.ufe_zeta_gradients(self._ctx, *(L.ptr(a) for a in ins), *arr))
+        return ak, bk, bks
+
+    def _bpa_inputs(self, Hi, Hs, A_flow, beta_sq, fraction_gr_b=None, mask_grounded_ice=None,
+                    BC_prescr_mask_bk=None, BC_prescr_u_bk=None, BC_prescr_v_bk=None):
+        bcm, bcu, bcv = self._bc(BC_prescr_mask_bk, BC_prescr_u_bk, BC_prescr_v_bk)
+        keep = [L.f64(Hi), L.f64(Hs), L.f64(A_flow), L.f64(beta_sq),
+                None if fraction_gr_b is None else L.f64(fraction_gr_b),
+                None if mask_grounded_ice is None else L.i32(mask_grounded_ice), bcm, bcu, bcv]
+        s = L.BpaInputs()
+        s.Hi, s.Hs, s.A_flow, s.beta_sq, s.fraction_gr_b = (L.ptr(a) for a in keep[:5])
+        s.mask_grounded_ice = L.ptr(keep[5], L.ip)
+        s.BC_prescr_mask_bk = L.ptr(keep[6], L.ip)
+        s.BC_prescr_u_bk, s.BC_prescr_v_bk = L.ptr(keep[7]), L.ptr(keep[8])
+        return s, keep
+
+    def solve_BPA(self, Hi, Hs, A_flow, beta_sq, fraction_gr_b=None, mask_grounded_ice=None, u_bk=None, v_bk=None,
+                  BC_prescr_mask_bk=None, BC_prescr_u_bk=None, BC_prescr_v_bk=None):
+        """solve_BPA (BPA_main.f90:90-252).  3-D arrays (nz*nT,), triangle
+        fastest.  Returns (u_bk, v_bk, info); a ViscosityIterationWarning when
+        visc_it_nit is reached."""
+        self._need_zeta()
+        nT, nz = self.md.mesh.nT, len(self.md.zeta)
+        u = L.f64(np.zeros(nz * nT) if u_bk is None else u_bk).copy()
+        v = L.f64(np.zeros(nz * nT) if v_bk is None else v_bk).copy()
+        s, keep = self._bpa_inputs(Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice, BC_prescr_mask_bk,
+                                   BC_prescr_u_bk, BC_prescr_v_bk)
+        info = L.SolveInfo()
+        rc = self._check(self.lib.ufe_solve_bpa(self._ctx, C.byref(s), L.ptr(u), L.ptr(v), C.byref(info)))
+        if rc == 1:
+            warnings.warn(f"BPA viscosity iteration failed to converge within {self.params.visc_it_nit} iterations",
+                          ViscosityIterationWarning)
+        return u, v, info
+
+    def bpa_residual(self, u_bk, v_bk, Hi, Hs, A_flow, beta_sq, fraction_gr_b=None, BC_prescr_mask_bk=None,
+                     BC_prescr_u_bk=None, BC_prescr_v_bk=None):
+        """One BPA state evaluation at (u_bk, v_bk): dict with rel_residual, r,
+        b (rows 2*(ti*nz+k)+uv), eta_bk (nz*nT,) and beta_b (nT,)."""
+        self._need_zeta()
+        nT, nz = self.md.mesh.nT, len(self.md.zeta)
+        s, keep = self._bpa_inputs(Hi, Hs, A_flow, beta_sq, fraction_gr_b, None, BC_prescr_mask_bk,
+                                   BC_prescr_u_bk, BC_prescr_v_bk)
+        out = {"r": np.zeros(2 * nT * nz), "b": np.zeros(2 * nT * nz), "eta_bk": np.zeros(nT * nz),
+               "beta_b": np.zeros(nT)}
+        rel = C.c_double(0.0)
+        self._check(self.lib.ufe_bpa_residual(self._ctx, C.byref(s), L.ptr(L.f64(u_bk)), L.ptr(L.f64(v_bk)),
+                                              L.ptr(out["r"]), L.ptr(out["b"]), L.ptr(out["eta_bk"]),
+                                              L.ptr(out["beta_b"]), C.byref(rel)))
+        out["rel_residual"] = rel.value
+        return out
+
     # -- ice thickness -------------------------------------------------------
     def thickness_setup(self, geometry=None):
         """Attach the Voronoi geometry of the mesh (cell areas, connection
