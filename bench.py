@@ -41,7 +41,10 @@ continental mesh, whose 326 MB per product exceed the 256 MB Infinity Cache),
 `oracle_parity_at_scale` (the chosen form against the committed oracle
 solution of a 20k-vertex DIVA channel), `thickness_step` (the semi-implicit
 ice-thickness step on the same mesh with the velocity the solve left on the
-device, host arrays for the other fields), and the unscaled residual
+device, host arrays for the other fields), `bpa` (the BPA 3-D stress
+balance on ISMIP-HOM experiment A, the reference's BPA integrated test: a
+1 km mesh on the GPU with its block product's roofline, and the reference's
+8 km mesh on the CPU oracle and the GPU), and the unscaled residual
 ||b - A u|| / ||b|| of every solve next to the block-scaled one.
 """
 
@@ -331,6 +334,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="tune the chosen form's Anderson depth and multigrid knobs per workload before timing "
                          "(correctness-gated coordinate search; not part of the default measurement)")
+    ap.add_argument("--bpa-only", action="store_true",
+                    help="only the BPA leg (ISMIP-HOM A), printed as one JSON object (not the bench line)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="one chosen-form solve, then only the block-product and copy-kernel launches "
                          "(for hardware counter runs)")
@@ -344,6 +349,10 @@ def main():
     import __graft_entry__ as ge
     ge.load_package()
     from ufemism2_amd import velocity as V, workloads as W
+
+    if args.bpa_only:
+        print(json.dumps(bpa_leg(V, W, cpu=not args.no_cpu_baseline)), flush=True)
+        return
 
     t0 = time.perf_counter()
     name, ws = build_workloads(W, args)
@@ -449,6 +458,8 @@ def main():
     if thickness is not None:
         out["thickness_step"] = thickness
     if not args.no_extra and args.workload == "channel":
+        log("[bench] BPA: ISMIP-HOM A on the GPU (1 km mesh) and GPU vs CPU oracle (the reference's 8 km mesh)")
+        out["bpa"] = bpa_leg(V, W, cpu=not args.no_cpu_baseline)
         log("[bench] out-of-cache roofline (continental mesh)")
         out["roofline_out_of_cache"] = out_of_cache_roofline(V, W)
         log("[bench] oracle parity at scale (20k-vertex channel)")
@@ -461,6 +472,76 @@ def main():
             log("[bench] full dome solves, CPU oracle and GPU")
             out["cpu_baseline"]["measured_full_solves"] = dome_full_solves(V, W)
     print(json.dumps(out), flush=True)
+
+
+def bpa_leg(V, W, h=1e3, cpu=True):
+    """The BPA 3-D stress balance (solve_BPA) on the reference's own BPA
+    integrated test, ISMIP-HOM experiment A at L = 160 km with its settings:
+    one solve from zero velocity on a 1 km mesh (timed after one untimed
+    solve), its stage times, and the BPA block product timed back to back
+    (HIP events; algorithmic bytes: blocks + column indices + row pointers +
+    x + y).  With cpu: the oracle's restatement of the reference iteration
+    (GMRES(30) + ILU(0)) and the engine on the reference's 8 km mesh, and the
+    largest velocity difference between the two."""
+    import numpy as np
+    from ufemism2_amd.operators import build_aa, zeta_stag
+    res = {}
+    w = W.ismip_hom_a(h=h)
+    aa, zs = build_aa(w.mesh), zeta_stag(w.zeta)
+    inp = w.inputs()
+    s = V.VelocitySolver(w.descriptor(), **w.params)
+    s.zeta_setup(aa, zs)
+    args = (inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    s.solve_BPA(*args)
+    t0 = time.perf_counter()
+    u, v, info = s.solve_BPA(*args)
+    t = time.perf_counter() - t0
+    stages = s.bpa_timings()
+    spmv_ms, spmv_bytes = s.bpa_profile_spmv(100)
+    s.close()
+    nbk = w.mesh.nT * w.nz
+    res["gpu"] = {"workload": w.name, "nV": w.mesh.nV, "nTri": w.mesh.nT, "nz": w.nz, "unknowns": 2 * nbk,
+                  "seconds_per_solve": t, "solves_per_s": 1.0 / t, "picard": info.n_visc_its,
+                  "krylov": info.n_Axb_its, "rel_residual": info.rel_residual, "stage_ms": stages,
+                  "max_surface_speed": float(np.hypot(u, v).reshape(w.nz, -1)[0].max())}
+    res["roofline_block_product"] = {"kernel": "k_bsr_spmv<double, OP_PLAIN> on the BPA system (back to back)",
+                                     "bound": "hbm", "avg_ms": spmv_ms, "algorithmic_bytes": spmv_bytes,
+                                     "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if cpu:
+        from oracle import binding as OB
+        from ufemism2_amd import _lib as L
+        w8 = W.ismip_hom_a(h=8e3)
+        aa8, zs8 = build_aa(w8.mesh), zeta_stag(w8.zeta)
+        i8 = w8.inputs()
+        p = L.default_params()
+        for k, val in w8.params.items():
+            if k in ("bc_u", "bc_v"):
+                for i, x in enumerate(val):
+                    getattr(p, k)[i] = int(x)
+            else:
+                setattr(p, k, val)
+        t0 = time.perf_counter()
+        rc, uo, vo, io = OB.solve_bpa(w8.descriptor(), aa8, zs8, p, i8)
+        tc = time.perf_counter() - t0
+        s8 = V.VelocitySolver(w8.descriptor(), **w8.params)
+        s8.zeta_setup(aa8, zs8)
+        a8 = (i8["Hi"], i8["Hs"], i8["A_flow"], i8["beta_sq"], i8["fgr_b"], i8["mask_grounded"])
+        s8.solve_BPA(*a8)
+        t0 = time.perf_counter()
+        ug, vg, ig = s8.solve_BPA(*a8)
+        tg = time.perf_counter() - t0
+        s8.close()
+        scale = float(np.hypot(uo, vo).max())
+        res["reference_mesh_8km"] = {
+            "nV": w8.mesh.nV, "nTri": w8.mesh.nT, "unknowns": 2 * w8.mesh.nT * w8.nz,
+            "cpu_oracle": {"seconds": tc, "picard": io.n_visc_its, "krylov": io.n_Axb_its, "rc": rc,
+                           "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                           "kind": "port"},
+            "gpu": {"seconds": tg, "picard": ig.n_visc_its, "krylov": ig.n_Axb_its},
+            "speedup": tc / tg,
+            "max_rel_vel_diff_gpu_vs_oracle": float(max(np.abs(ug - uo).max(), np.abs(vg - vo).max()) / scale)}
+    return res
 
 
 def time_thickness(V, W, w, solver, reps=10):

@@ -554,6 +554,17 @@ int ufe_solve_bpa(ufe_ctx *ctx, const ufe_bpa_inputs *in, double *u_bk, double *
  * coefficient); *rel_residual = ||r|| / ||b||. */
 int ufe_bpa_residual(ufe_ctx *ctx, const ufe_bpa_inputs *in, const double *u_bk, const double *v_bk, double *r,
                      double *b, double *eta_bk, double *beta_b, double *rel_residual);
+/* Measurement hooks (no reference counterpart).  Time of the last
+ * ufe_solve_bpa per stage, summed over its viscosity iterations, in ms (HIP
+ * events on the context's stream): [0] strain rates, viscosity and basal
+ * friction, [1] assembly of the system and load vector, [2] factorisation of
+ * the column preconditioner, [3] GMRES, [4] velocity limits, relaxation and
+ * the L2 change.  And the average duration in ms of `reps` back-to-back FP64
+ * block products y = A x with the BPA system assembled last, with their
+ * algorithmic HBM bytes per launch (blocks + column indices + row pointers +
+ * x + y). */
+int ufe_bpa_last_timings(const ufe_ctx *ctx, double *ms5);
+int ufe_bpa_profile_spmv(ufe_ctx *ctx, int reps, double *avg_ms, double *bytes);
 
 /* ---- mesh operator construction (host, once per mesh) -----------------
  * calc_matrix_operators_mesh_a_b / _b_a / _b_b_2nd_order

@@ -232,3 +232,33 @@ def test_state_and_system_at_scale(V, OB, L, W, nV):
     scale = max(np.abs(ora["b"]).max(), np.abs(ora["r"]).max())
     assert np.abs(gpu["r"] - ora["r"]).max() < 1e-10 * scale
     s.close()
+
+
+@pytest.mark.parametrize("tight", [True, False])
+def test_ismip_hom_a_reference_configuration(V, OB, L, W, tight):
+    """The reference's BPA integrated test, ISMIP-HOM A at L = 160 km on its
+    8 km mesh with its settings (no sliding, periodic borders, relaxation 0.4
+    to 5e-7, Krylov rtol 1e-6): the engine against the oracle.  Tight: both
+    solve to 1e-10 / 1e-13 and agree to 1e-6 of the largest speed; at the
+    reference's tolerances the two stop after nearly the same number of
+    viscosity iterations on different linear-solver round-off, within 1e-3."""
+    from ufemism2_amd.operators import build_aa, zeta_stag
+    w = W.ismip_hom_a(h=8e3)
+    kw = dict(w.params)
+    if tight:
+        kw.update(rtol=1e-10, abstol=0.0, visc_it_norm_dUV_tol=1e-13)
+    aa, zs = build_aa(w.mesh), zeta_stag(w.zeta)
+    inp = w.inputs()
+    rc, uo, vo, io = OB.solve_bpa(w.descriptor(), aa, zs, params(L, **kw), inp)
+    assert rc == 0
+    s = V.VelocitySolver(w.descriptor(), **kw)
+    s.zeta_setup(aa, zs)
+    print(f"ISMIP-HOM A 160 km nT={w.mesh.nT} tight={tight}", flush=True)
+    u, v, info = s.solve_BPA(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    scale = np.hypot(uo, vo).max()
+    err = max(np.abs(u - uo).max(), np.abs(v - vo).max()) / scale
+    assert err < (1e-6 if tight else 1e-3), err
+    assert abs(info.n_visc_its - io.n_visc_its) <= (2 if tight else 3)
+    U = u.reshape(w.nz, -1)
+    assert 1.0 < U[0].min() and U[0].max() < 120.0   # surface speeds of the experiment (m/yr)
+    s.close()

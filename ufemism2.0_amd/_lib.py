@@ -227,6 +227,8 @@ def lib():
                                             C.POINTER(dp)])
     _sig(L, 'ufe_solve_bpa', argtypes=[C.c_void_p, C.POINTER(BpaInputs), dp, dp, C.POINTER(SolveInfo)])
     _sig(L, 'ufe_bpa_residual', argtypes=[C.c_void_p, C.POINTER(BpaInputs), dp, dp, dp, dp, dp, dp, dp])
+    _sig(L, 'ufe_bpa_last_timings', argtypes=[C.c_void_p, dp])
+    _sig(L, 'ufe_bpa_profile_spmv', argtypes=[C.c_void_p, C.c_int, dp, dp])
     _sig(L, 'ufe_thickness_setup', argtypes=[C.c_void_p, C.POINTER(VertexGeometry)])
     _sig(L, 'ufe_thickness_semiimplicit', argtypes=[C.c_void_p, C.POINTER(ThicknessParams),
                                                     C.POINTER(ThicknessInputs), dp, dp, dp, dp, ip])
@@ -245,7 +247,8 @@ EXPORTED = [
     "ufe_mesh_operators_ab", "ufe_mesh_operators_ba", "ufe_mesh_operators_bb2", "ufe_mesh_operators_aa",
     "ufe_thickness_params_default", "ufe_thickness_setup", "ufe_thickness_semiimplicit",
     "ufe_heat_setup", "ufe_heat_upwind_derivatives", "ufe_heat_strain_heating", "ufe_heat_frictional_heating",
-    "ufe_zeta_setup", "ufe_zeta_gradients", "ufe_solve_bpa", "ufe_bpa_residual",
+    "ufe_zeta_setup", "ufe_zeta_gradients", "ufe_solve_bpa", "ufe_bpa_residual", "ufe_bpa_last_timings",
+    "ufe_bpa_profile_spmv",
 ]
 
 

@@ -681,6 +681,13 @@ struct BpaState {
   // iteration
   DBuf x, xprev, rhs, sr4, eta_ak, eta_bks, coef, beta_a, beta_b, A, cDinv, cW, cU, part, out, r;
   std::unique_ptr<Krylov> kry;
+  // stage timings of the last solve (HIP events on the context's stream)
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double t_ms[5] = {0, 0, 0, 0, 0};
+  ~BpaState() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 
   ZOps zops() const {
     ZOps z;
@@ -838,6 +845,7 @@ void bpa_pattern(ufe_ctx *c, BpaState &b) {
   ensure(b.part, sizeof(double) * kMaxPartials * 2);
   ensure(b.out, sizeof(double) * 2);
   b.kry = std::make_unique<Krylov>(b.nb, s);
+  for (hipEvent_t &ev : b.ev) UFE_HIP(hipEventCreate(&ev));
 }
 
 BpaParams bpa_params(const ufe_params &p, double eps0_applied) {
@@ -908,6 +916,7 @@ void bpa_state_and_system(ufe_ctx *c, BpaState &b, const BpaParams &p, const int
                      b.eta_bks.as<double>(), b.beta_a.as<double>(), b.fgr.as<double>(), p, b.coef.as<double4>(),
                      b.beta_b.as<double>());
   UFE_LAUNCH_CHECK();
+  if (b.ev[1]) UFE_HIP(hipEventRecord(b.ev[1], s));
   const BpaSys sys = bpa_sys(c, b, TriBI_dev);
   hipLaunchKernelGGL(k_bpa_assemble, dim3(blocks_for(b.nnzb)), dim3(kBlock), 0, s, sys, z, p, b.nnzb,
                      b.A.as<double4>());
@@ -1216,13 +1225,18 @@ extern "C" int ufe_solve_bpa(ufe_ctx *c, const ufe_bpa_inputs *in, double *u_bk,
     bool converged = false;
     rc = UFE_OK;
     RedSlot slot{b.part.as<double>(), b.out.as<double>()};
+    for (double &t : b.t_ms) t = 0.0;
     while (!converged) {
       ++it;
       UFE_HIP(hipMemcpyAsync(b.xprev.p, b.x.p, sizeof(double2) * nbk, hipMemcpyDeviceToDevice, s));
+      UFE_HIP(hipEventRecord(b.ev[0], s));
       bpa_state_and_system(c, b, bpa_params(p, eps0), b.tribi.as<int>());
+      UFE_HIP(hipEventRecord(b.ev[2], s));
       col_factor(c, b);
+      UFE_HIP(hipEventRecord(b.ev[3], s));
       const SolveStats st = b.kry->gmres64(A, b.rhs.as<double>(), b.x.as<double>(), p.rtol, p.abstol,
                                            p.krylov_maxit, !p.warm_start, p.gmres_restart, std::max(1, p.check_every));
+      UFE_HIP(hipEventRecord(b.ev[4], s));
       n_axb += st.its;
       // L2 change of the relaxed iterate; one pass per kMaxPartials * kBlock rows
       double r1 = 0.0, r2 = 0.0;
@@ -1239,6 +1253,13 @@ extern "C" int ufe_solve_bpa(ufe_ctx *c, const ufe_bpa_inputs *in, double *u_bk,
         UFE_HIP(hipStreamSynchronize(s));
         r1 += h2[0];
         r2 += h2[1];
+      }
+      UFE_HIP(hipEventRecord(b.ev[5], s));
+      UFE_HIP(hipEventSynchronize(b.ev[5]));
+      for (int q = 0; q < 5; ++q) {
+        float ms = 0.0f;
+        UFE_HIP(hipEventElapsedTime(&ms, b.ev[q], b.ev[q + 1]));
+        b.t_ms[q] += ms;
       }
       resid_prev = resid;
       resid = 2.0 * r1 / std::max(r2, 1e-8);
@@ -1325,6 +1346,24 @@ extern "C" int ufe_bpa_residual(ufe_ctx *c, const ufe_bpa_inputs *in, const doub
       UFE_HIP(hipStreamSynchronize(s));
       for (int i = 0; i < nT; ++i) beta_b[tperm[i]] = h[i];
     }
+    return UFE_OK;
+  });
+}
+
+extern "C" int ufe_bpa_last_timings(const ufe_ctx *c, double *ms5) {
+  if (!c || !ms5) return ctx_fail(const_cast<ufe_ctx *>(c), UFE_ERR_ARGUMENT, "null argument");
+  BpaState *b = state(const_cast<ufe_ctx *>(c));
+  if (!b) return ctx_fail(const_cast<ufe_ctx *>(c), UFE_ERR_ARGUMENT, "ufe_zeta_setup has not been called");
+  for (int i = 0; i < 5; ++i) ms5[i] = b->t_ms[i];
+  return UFE_OK;
+}
+
+extern "C" int ufe_bpa_profile_spmv(ufe_ctx *c, int reps, double *avg_ms, double *bytes) {
+  if (!c || !avg_ms || !bytes || reps < 1) return ctx_fail(c, UFE_ERR_ARGUMENT, "bad argument");
+  return ctx_guard(c, [&]() -> int {
+    BpaState *b = state(c);
+    if (!b || !b->nb) return ctx_fail(c, UFE_ERR_ARGUMENT, "no BPA system has been assembled");
+    *avg_ms = b->kry->time_spmv64(bpa_bsr(*b), b->x.as<double>(), b->r.as<double>(), reps, bytes);
     return UFE_OK;
   });
 }

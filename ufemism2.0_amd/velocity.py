@@ -411,6 +411,19 @@ class VelocitySolver:
         out["rel_residual"] = rel.value
         return out
 
+    def bpa_timings(self):
+        """Stage times (ms) of the last solve_BPA: state, assembly, column
+        factorisation, GMRES, update."""
+        ms = np.zeros(5)
+        self._check(self.lib.ufe_bpa_last_timings(self._ctx, L.ptr(ms)))
+        return dict(zip(("state", "assembly", "column_factor", "gmres", "update"), ms.tolist()))
+
+    def bpa_profile_spmv(self, reps=100):
+        """(average ms, algorithmic bytes) of the BPA block product."""
+        ms, by = C.c_double(0.0), C.c_double(0.0)
+        self._check(self.lib.ufe_bpa_profile_spmv(self._ctx, int(reps), C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
     # -- ice thickness -------------------------------------------------------
     def thickness_setup(self, geometry=None):
         """Attach the Voronoi geometry of the mesh (cell areas, connection

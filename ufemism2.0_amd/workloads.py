@@ -293,6 +293,26 @@ def schoof_icestream(h: float = 16e3, seed: int = 5, nz: int = 10, A: float = 1e
                     params)
 
 
+def ismip_hom_a(L: float = 160e3, h: float = 8e3, nz: int = 12, seed: int = 8) -> Workload:
+    """ISMIP-HOM experiment A, the reference's BPA integrated test
+    (automated_testing/integrated_tests/idealised/ISMIP-HOM/config_ISMIP_HOM_A_*_BPA.cfg,
+    idealised_geometries.f90:243-266): Hs = 2000 - x tan(0.5 deg), Hi = 1000 -
+    500 sin(2 pi x / L) sin(2 pi y / L) on [-L, L]^2, no sliding, periodic
+    borders, Glen n = 3 with A = 1e-16 and epsilon_sq_0 = 1e-12, nz = 12
+    regular layers, viscosity relaxation 0.4 to 5e-7, Krylov rtol 1e-6 and
+    abstol 1e-4; the reference runs it on an 8 km mesh."""
+    mesh = uniform_mesh(-L, L, -L, L, h, seed=seed)
+    nV = mesh.nV
+    x, y = mesh.V[:, 0], mesh.V[:, 1]
+    Hs = 2000.0 - x * np.tan(0.5 * np.pi / 180.0)
+    Hb = Hs - 1000.0 + 500.0 * np.sin(x * 2.0 * np.pi / L) * np.sin(y * 2.0 * np.pi / L)
+    params = {"glens_n": 3.0, "epsilon_sq_0": 1e-12, "sliding_law": 0, "visc_it_relax": 0.4,
+              "visc_it_norm_dUV_tol": 5e-7, "visc_it_nit": 5000, "rtol": 1e-6, "abstol": 1e-4,
+              "bc_u": [2, 2, 2, 2], "bc_v": [2, 2, 2, 2], "ismip_hom_L": L}
+    return Workload(f"ismip_hom_a_{int(L / 1e3)}km_{h / 1e3:g}km", mesh, build_operators(mesh), zeta_regular(nz),
+                    Hs - Hb, Hb, np.full(nV, -10000.0), np.full(nz * nV, 1e-16), np.zeros(nV), seed, "BPA", params)
+
+
 def heat_columns_inputs(nV: int, nz: int = 12, seed: int = 6, dt: float = 10.0, gl: float = 0.0) -> dict:
     """Synthetic inputs of the vertical heat columns for nV vertices, in the
     reference's [k][vi] layer-major layout (vertex fastest): a cold surface,
