@@ -168,3 +168,48 @@ def test_fortran_thickness_step_matches_c_abi(V, W, OB, tmp_path):
     rc, _, Ho, _, _, _ = OB.thickness_semiimplicit(w.mesh, g, **kw, bc_H=bc, rtol=1e-12, abstol=0.0)
     assert rc == 0
     assert np.abs(Hf - Ho).max() < 1e-5 * inp["Hi"].max()
+
+
+def _bpa_ctl(w):
+    ctl = dict(stop_rule=0, visc_it_nit=w.params["visc_it_nit"], include_crossterms=1, precond=2, warm_start=0,
+               accel_depth=0, nonlinear_rtol=1e-6, vel_max=5000.0, abstol=w.params["abstol"],
+               rtol=w.params["rtol"], accel_mixing=0.5)
+    ctl.update({k: w.params[k] for k in ("sliding_law", "bc_u", "bc_v", "glens_n", "epsilon_sq_0", "visc_it_relax",
+                                         "visc_it_norm_dUV_tol", "ismip_hom_L")})
+    return ctl
+
+
+def test_bpa_case_file_layout(W, tmp_path):
+    from ufemism2_amd import fortran_case as FC
+    from ufemism2_amd.operators import build_aa
+    w = W.ismip_hom_a(h=40e3)
+    path = str(tmp_path / "bpa.bin")
+    FC.write_case(path, w, stress="BPA", **_bpa_ctl(w))
+    m, ops = w.mesh, w.ops
+    aa = build_aa(m)
+    n_int = (7 + 3 * m.nT + 3 * m.nT + m.nT + (m.nT + 1) + ops.ab.nnz + (m.nV + 1) + ops.ba.nnz + (m.nT + 1)
+             + ops.bb.nnz + 6 + m.nV) + 10 + (m.nV + 1) + aa.nnz
+    n_dbl = (2 * m.nV + 2 * m.nT + w.nz + 3 * ops.ab.nnz + 3 * ops.ba.nnz + 5 * ops.bb.nnz + 5 + 2 * m.nV
+             + w.nz * m.nV + m.nV + m.nT) + 5 + 2 * aa.nnz + (w.nz - 1)
+    assert os.path.getsize(path) == 4 * n_int + 8 * n_dbl
+
+
+@pytest.mark.gpu
+def test_fortran_bpa_matches_c_abi(V, W, tmp_path):
+    """solve_BPA_gpu (after ufe_gpu_zeta_setup with the 1-based a->a operators)
+    on the reference's ISMIP-HOM A configuration equals the Python host's call
+    bit for bit."""
+    from ufemism2_amd import fortran_case as FC
+    w = W.ismip_hom_a(h=8e3)
+    ctl = _bpa_ctl(w)
+    case, out = str(tmp_path / "bpa.bin"), str(tmp_path / "bpa_out.bin")
+    FC.write_case(case, w, stress="BPA", **ctl)
+    print(FC.run_driver(case, out), flush=True)
+    n = w.mesh.nT * w.nz
+    uf, vf, nvi, nax, res = FC.read_result(out, n)
+    s = V.VelocitySolver(w.descriptor(), **ctl)
+    inp = w.inputs()
+    u, v, info = s.solve_BPA(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
+    s.close()
+    assert nvi == info.n_visc_its and nax == info.n_Axb_its
+    assert np.array_equal(uf, u) and np.array_equal(vf, v)

@@ -593,30 +593,56 @@ __global__ void __launch_bounds__(kBlock) k_col_factor(int nT, int nz, const int
   }
 }
 
-// z = M^-1 r with the column factors (forward sweep into z, backward in place)
-__global__ void __launch_bounds__(kBlock) k_col_apply(int nT, int nz, const double4 *__restrict__ Dinv,
-                                                      const double4 *__restrict__ W, const double4 *__restrict__ U,
-                                                      const double2 *__restrict__ r, double2 *__restrict__ z) {
-  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ti >= nT) return;
-  const double2 *rc = r + (size_t)ti * nz;
-  double2 *zc = z + (size_t)ti * nz;
-  double2 y = rc[0];
-  zc[0] = y;
-  for (int k = 1; k < nz; ++k) {
-    const double2 wy = m2vec(W[(size_t)k * nT + ti], y);
-    const double2 rk = rc[k];
-    y = make_double2(rk.x - wy.x, rk.y - wy.y);
-    zc[k] = y;
+// z = M^-1 r with the column factors.  One wavefront per 64 consecutive
+// triangles: their columns of r are one contiguous stretch of memory, loaded
+// into LDS with coalesced reads (a triangle's column is strided by nz * 16 B
+// across lanes in memory), solved per lane in LDS (forward sweep, then
+// backward in place; columns padded to an odd number of doubles so the lanes
+// hit different banks), and stored back coalesced.
+constexpr int kColWave = 64;
+constexpr int kColPad = 2 * kBpaMaxNz + 1;
+
+__global__ void __launch_bounds__(kColWave) k_col_apply(int nT, int nz, const double4 *__restrict__ Dinv,
+                                                        const double4 *__restrict__ W,
+                                                        const double4 *__restrict__ U,
+                                                        const double *__restrict__ r, double *__restrict__ z) {
+  __shared__ double col[kColWave * kColPad];
+  const int lane = threadIdx.x;
+  const int t0 = blockIdx.x * kColWave;
+  const int nt = min(kColWave, nT - t0);
+  const int nd = 2 * nz;                       // doubles per column
+  const long base = (long)t0 * nd;
+  for (int q = lane; q < nt * nd; q += kColWave) col[(q / nd) * kColPad + q % nd] = r[base + q];
+  __syncthreads();
+  if (lane < nt) {
+    const int ti = t0 + lane;
+    double *c = col + lane * kColPad;
+    double px = c[0], py = c[1];  // y_0 = r_0
+    for (int k = 1; k < nz; ++k) {  // y_k = r_k - W_k y_{k-1}
+      const double4 w = W[(size_t)k * nT + ti];
+      const double nx = c[2 * k] - (w.x * px + w.y * py);
+      const double ny = c[2 * k + 1] - (w.z * px + w.w * py);
+      c[2 * k] = nx;
+      c[2 * k + 1] = ny;
+      px = nx;
+      py = ny;
+    }
+    double4 d = Dinv[(size_t)(nz - 1) * nT + ti];
+    double zx = d.x * px + d.y * py, zy = d.z * px + d.w * py;
+    c[2 * (nz - 1)] = zx;
+    c[2 * (nz - 1) + 1] = zy;
+    for (int k = nz - 2; k >= 0; --k) {  // z_k = D'_k^-1 (y_k - U_k z_{k+1})
+      const double4 u = U[(size_t)k * nT + ti];
+      const double ax = c[2 * k] - (u.x * zx + u.y * zy), ay = c[2 * k + 1] - (u.z * zx + u.w * zy);
+      d = Dinv[(size_t)k * nT + ti];
+      zx = d.x * ax + d.y * ay;
+      zy = d.z * ax + d.w * ay;
+      c[2 * k] = zx;
+      c[2 * k + 1] = zy;
+    }
   }
-  double2 zn = m2vec(Dinv[(size_t)(nz - 1) * nT + ti], y);
-  zc[nz - 1] = zn;
-  for (int k = nz - 2; k >= 0; --k) {
-    const double2 uz = m2vec(U[(size_t)k * nT + ti], zn);
-    const double2 yk = zc[k];
-    zn = m2vec(Dinv[(size_t)k * nT + ti], make_double2(yk.x - uz.x, yk.y - uz.y));
-    zc[k] = zn;
-  }
+  __syncthreads();
+  for (int q = lane; q < nt * nd; q += kColWave) z[base + q] = col[(q / nd) * kColPad + q % nd];
 }
 
 // ---------------------------------------------------------------------------
@@ -1002,9 +1028,8 @@ void set_column_pc(ufe_ctx *c, BpaState &b) {
   BpaState *bp = &b;
   hipStream_t s = ctx_stream(c);
   b.kry->set_custom_pc([bp, s](const double *r, double *z) {
-    hipLaunchKernelGGL(k_col_apply, dim3(blocks_for(bp->nT)), dim3(kBlock), 0, s, bp->nT, bp->nz,
-                       bp->cDinv.as<double4>(), bp->cW.as<double4>(), bp->cU.as<double4>(),
-                       reinterpret_cast<const double2 *>(r), reinterpret_cast<double2 *>(z));
+    hipLaunchKernelGGL(k_col_apply, dim3(blocks_for(bp->nT, kColWave)), dim3(kColWave), 0, s, bp->nT, bp->nz,
+                       bp->cDinv.as<double4>(), bp->cW.as<double4>(), bp->cU.as<double4>(), r, z);
     UFE_LAUNCH_CHECK();
   });
 }

@@ -26,6 +26,10 @@ program ufe_fortran_driver
   real(c_double), allocatable :: Hi(:), Hs(:), A_flow(:, :), beta_sq(:), fgr(:)
   integer(c_int), allocatable :: mask(:)
   real(c_double), allocatable :: u_b(:), v_b(:), ub(:), vb(:), u3(:, :), v3(:, :), eta3(:, :), tbx(:), tby(:)
+  integer(c_int) :: bip(10)
+  real(c_double) :: brp(5)
+  integer(c_int), allocatable :: aa_ptr(:), aa_ind(:)
+  real(c_double), allocatable :: aa_ddx(:), aa_ddy(:), zeta_stag(:), u_bk(:, :), v_bk(:, :)
   type(ufe_gpu_context) :: g
   type(ufe_gpu_params) :: p
   type(ufe_gpu_info) :: info
@@ -48,6 +52,13 @@ program ufe_fortran_driver
   read(u_in) ip, rp
   allocate(Hi(nV), Hs(nV), A_flow(nV, nz), beta_sq(nV), fgr(nT), mask(nV))
   read(u_in) Hi, Hs, A_flow, beta_sq, fgr, mask
+  if (stress == 2) then
+    ! BPA: sliding law, border choices, rheology and iteration controls, the
+    ! a->a operators (1-based CSR) and the staggered layers
+    read(u_in) bip, brp
+    allocate(aa_ptr(nV + 1), aa_ind(bip(1)), aa_ddx(bip(1)), aa_ddy(bip(1)), zeta_stag(nz - 1))
+    read(u_in) aa_ptr, aa_ind, aa_ddx, aa_ddy, zeta_stag
+  end if
   close(u_in)
 
   call ufe_gpu_init(g, 0, V, Tri, TriC, TriBI, TriGC, zeta, ab_ptr, ab_ind, ab_map, ab_ddx, ab_ddy, &
@@ -57,7 +68,30 @@ program ufe_fortran_driver
   p%stop_rule = ip(1); p%visc_it_nit = ip(2); p%include_crossterms = ip(3)
   p%precond = ip(4); p%warm_start = ip(5); p%accel_depth = ip(6)
   p%nonlinear_rtol = rp(1); p%vel_max = rp(2); p%abstol = rp(3); p%rtol = rp(4); p%accel_mixing = rp(5)
+  if (stress == 2) then
+    p%sliding_law = bip(2)
+    p%bc_u = bip(3:6)
+    p%bc_v = bip(7:10)
+    p%glens_n = brp(1); p%epsilon_sq_0 = brp(2); p%visc_it_relax = brp(3)
+    p%visc_it_norm_dUV_tol = brp(4); p%ismip_hom_L = brp(5)
+  end if
   call ufe_gpu_set_params(g, p)
+
+  if (stress == 2) then
+    call ufe_gpu_zeta_setup(g, aa_ptr, aa_ind, aa_ddx, aa_ddy, zeta_stag)
+    allocate(u_bk(nT, nz), v_bk(nT, nz))
+    u_bk = 0.0_c_double
+    v_bk = 0.0_c_double
+    call solve_BPA_gpu(g, Hi, Hs, A_flow, beta_sq, fgr, mask /= 0, u_bk, v_bk, n_visc_its, n_Axb_its, info=info)
+    open(newunit=u_out, file=trim(out_file), access='stream', form='unformatted', status='replace', &
+         action='write')
+    write(u_out) u_bk, v_bk, int(n_visc_its, c_int), int(n_Axb_its, c_int), info%rel_residual
+    close(u_out)
+    write(*, '(a,i0,a,i0,a,es10.3)') 'BPA solve done: viscosity iterations ', n_visc_its, ' krylov ', &
+      n_Axb_its, ' residual ', info%rel_residual
+    call ufe_gpu_finalise(g)
+    stop
+  end if
 
   allocate(u_b(nT), v_b(nT))
   u_b = 0.0_c_double

@@ -8,6 +8,7 @@
 !     replacement of solve_3D_heat_equation (thermodynamics_3D_heat_equation.f90:33-307)
 !   calc_dHi_dt_semiimplicit_gpu <-> calc_dHi_dt_semiimplicit
 !     (conservation_of_mass_semiimplicit.f90:23-173), after ufe_gpu_thickness_setup
+!   solve_BPA_gpu <-> solve_BPA (BPA_main.f90:90-252), after ufe_gpu_zeta_setup
 ! Arrays are the caller's 1-based Fortran arrays; (n, nz) fields are passed
 ! as they lie in memory (first index fastest).  Mesh connectivity and operator
 ! indices are converted to 0-based once, in ufe_gpu_init.  A failing call
@@ -23,7 +24,8 @@ module ufe_velocity_gpu
   public :: ufe_gpu_context, ufe_gpu_params, ufe_gpu_info, ufe_gpu_init, ufe_gpu_finalise, &
             ufe_gpu_set_params, ufe_gpu_default_params, solve_SSA_gpu, solve_DIVA_gpu, &
             ufe_gpu_fraction_gr_b, solve_3D_heat_equation_gpu, ufe_gpu_thickness_params, &
-            ufe_gpu_default_thickness_params, ufe_gpu_thickness_setup, calc_dHi_dt_semiimplicit_gpu
+            ufe_gpu_default_thickness_params, ufe_gpu_thickness_setup, calc_dHi_dt_semiimplicit_gpu, &
+            ufe_gpu_zeta_setup, solve_BPA_gpu
 
   integer(c_int), parameter, public :: UFE_OK = 0, UFE_WARN_NOT_CONVERGED = 1
 
@@ -105,6 +107,12 @@ module ufe_velocity_gpu
                       mask_noice, BC_prescr_mask, BC_prescr_Hi, dHi_dt_guess
     real(c_double) :: dt
   end type ufe_thickness_inputs
+
+  ! struct ufe_bpa_inputs
+  type, bind(C) :: ufe_bpa_inputs
+    type(c_ptr) :: Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice, &
+                   BC_prescr_mask_bk, BC_prescr_u_bk, BC_prescr_v_bk
+  end type ufe_bpa_inputs
 
   ! A device context plus the 0-based copies of the mesh arrays it was built from.
   type :: ufe_gpu_context
@@ -188,6 +196,19 @@ module ufe_velocity_gpu
       type(c_ptr), value :: Hn, dHdt, divQ
       real(c_double), intent(out) :: dt_max
       integer(c_int), intent(out) :: n_its
+      integer(c_int) :: rc
+    end function
+    function c_ufe_zeta_setup(ctx, nnz, ptr, ind, ddx, ddy, zs) bind(C, name='ufe_zeta_setup') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, ptr, ind, ddx, ddy, zs
+      integer(c_int), value :: nnz
+      integer(c_int) :: rc
+    end function
+    function c_ufe_solve_bpa(ctx, inp, u, v, info) bind(C, name='ufe_solve_bpa') result(rc)
+      import :: c_ptr, c_int, ufe_bpa_inputs, ufe_gpu_info
+      type(c_ptr), value :: ctx, u, v
+      type(ufe_bpa_inputs), intent(in) :: inp
+      type(ufe_gpu_info), intent(out) :: info
       integer(c_int) :: rc
     end function
     function c_ufe_fraction_gr_b(ctx, Hi, Hb, SL, fgr) bind(C, name='ufe_fraction_gr_b') result(rc)
@@ -399,6 +420,67 @@ contains
     n_Axb_its = inf%n_Axb_its
     if (present(info)) info = inf
   end subroutine solve_DIVA_gpu
+
+  ! Set-up of the zeta gradients and the BPA, once per context: the a->a
+  ! operators M_ddx_a_a, M_ddy_a_a as 1-based CSR (one pattern; nV rows) and
+  ! mesh%zeta_stag(nz-1).
+  subroutine ufe_gpu_zeta_setup(g, aa_ptr, aa_ind, aa_ddx, aa_ddy, zeta_stag)
+    type(ufe_gpu_context), intent(in) :: g
+    integer(c_int), intent(in) :: aa_ptr(:), aa_ind(:)
+    real(c_double), intent(in), target :: aa_ddx(:), aa_ddy(:), zeta_stag(:)
+    integer(c_int), allocatable, target :: p0(:), i0(:)
+    p0 = aa_ptr - 1
+    i0 = aa_ind - 1
+    call crash_on(c_ufe_zeta_setup(g%ctx, int(size(aa_ind), c_int), c_loc(p0), c_loc(i0), c_loc(aa_ddx), &
+                                   c_loc(aa_ddy), c_loc(zeta_stag)), g%ctx, 'ufe_gpu_zeta_setup')
+  end subroutine ufe_gpu_zeta_setup
+
+  ! solve_BPA (BPA_main.f90:90-252): u_bk, v_bk(nT, nz) are the starting
+  ! velocity on entry and the solution on exit; the prescribed-velocity
+  ! boundary condition needs all three of BC_prescr_mask_bk, BC_prescr_u_bk,
+  ! BC_prescr_v_bk (nT, nz).  ufe_gpu_zeta_setup must have been called.
+  subroutine solve_BPA_gpu(g, Hi, Hs, A_flow, beta_sq, fraction_gr_b, mask_grounded_ice, u_bk, v_bk, &
+      n_visc_its, n_Axb_its, BC_prescr_mask_bk, BC_prescr_u_bk, BC_prescr_v_bk, info)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Hi(:), Hs(:), A_flow(:, :), beta_sq(:), fraction_gr_b(:)
+    logical, intent(in) :: mask_grounded_ice(:)
+    real(c_double), intent(inout), target :: u_bk(:, :), v_bk(:, :)
+    integer, intent(out) :: n_visc_its, n_Axb_its
+    integer, intent(in), optional :: BC_prescr_mask_bk(:, :)
+    real(c_double), intent(in), optional, target :: BC_prescr_u_bk(:, :), BC_prescr_v_bk(:, :)
+    type(ufe_gpu_info), intent(out), optional :: info
+    integer(c_int), allocatable, target :: mask(:), bcm(:, :)
+    type(ufe_bpa_inputs) :: inp
+    type(ufe_gpu_info) :: inf
+    integer(c_int) :: rc
+
+    mask = merge(1_c_int, 0_c_int, mask_grounded_ice)
+    inp%Hi = c_loc(Hi)
+    inp%Hs = c_loc(Hs)
+    inp%A_flow = c_loc(A_flow)
+    inp%beta_sq = c_loc(beta_sq)
+    inp%fraction_gr_b = c_loc(fraction_gr_b)
+    inp%mask_grounded_ice = c_loc(mask)
+    inp%BC_prescr_mask_bk = c_null_ptr
+    inp%BC_prescr_u_bk = c_null_ptr
+    inp%BC_prescr_v_bk = c_null_ptr
+    if (present(BC_prescr_mask_bk) .or. present(BC_prescr_u_bk) .or. present(BC_prescr_v_bk)) then
+      if (.not. (present(BC_prescr_mask_bk) .and. present(BC_prescr_u_bk) .and. present(BC_prescr_v_bk))) then
+        write(0, '(a)') 'solve_BPA_gpu: need to provide prescribed u,v fields and mask!'
+        error stop 1
+      end if
+      bcm = int(BC_prescr_mask_bk, c_int)
+      inp%BC_prescr_mask_bk = c_loc(bcm)
+      inp%BC_prescr_u_bk = c_loc(BC_prescr_u_bk)
+      inp%BC_prescr_v_bk = c_loc(BC_prescr_v_bk)
+    end if
+    rc = c_ufe_solve_bpa(g%ctx, inp, c_loc(u_bk), c_loc(v_bk), inf)
+    call crash_on(rc, g%ctx, 'solve_BPA_gpu')
+    if (rc == UFE_WARN_NOT_CONVERGED) write(0, '(a)') 'WARNING: viscosity iteration failed to converge!'
+    n_visc_its = inf%n_visc_its
+    n_Axb_its = inf%n_Axb_its
+    if (present(info)) info = inf
+  end subroutine solve_BPA_gpu
 
   ! calc_grounded_fractions_bilin_interp_TAF_b (subgrid_grounded_fractions_bilin_TAF.f90:182-250)
   subroutine ufe_gpu_fraction_gr_b(g, Hi, Hb, SL, fraction_gr_b)

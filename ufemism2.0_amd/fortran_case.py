@@ -1,7 +1,8 @@
 """Case files for the Fortran drivers, each one raw little-endian stream in
 Fortran array order.  ufe_fortran_driver: the mesh in the reference's 1-based
 layout (V(nV,2), Tri/TriC(nT,3) with TriC = 0 for none, TriGC(nT,2)), the
-operators as 1-based CSR, a few solver controls and the inputs.
+operators as 1-based CSR, a few solver controls and the inputs; for the BPA
+also its controls, the a->a operators and zeta_stag.
 ufe_fortran_heat_driver: the column fields (nV,nz), vertex index fastest,
 and the per-vertex fields of one heat-equation step."""
 
@@ -23,7 +24,8 @@ def write_case(path, w, stress="SSA", **ctl):
     i32 = lambda a: np.asarray(a, dtype="<i4")  # noqa: E731
     f64 = lambda a: np.asarray(a, dtype="<f8")  # noqa: E731
     ff = lambda a: np.asfortranarray(a).ravel(order="F")  # noqa: E731
-    parts = [i32([m.nV, m.nT, w.nz, ops.ab.nnz, ops.ba.nnz, ops.bb.nnz, 1 if stress == "DIVA" else 0]),
+    code = {"SSA": 0, "DIVA": 1, "BPA": 2}[stress]
+    parts = [i32([m.nV, m.nT, w.nz, ops.ab.nnz, ops.ba.nnz, ops.bb.nnz, code]),
              f64(ff(m.V)), i32(ff(m.Tri + 1)), i32(ff(m.TriC + 1)), i32(m.TriBI), f64(ff(m.TriGC)), f64(w.zeta)]
     for fam, names in ((ops.ab, ["map", "ddx", "ddy"]), (ops.ba, ["map", "ddx", "ddy"]),
                        (ops.bb, ["ddx", "ddy", "d2dx2", "d2dxdy", "d2dy2"])):
@@ -35,6 +37,18 @@ def write_case(path, w, stress="SSA", **ctl):
     inp = w.inputs()
     parts += [f64(inp["Hi"]), f64(inp["Hs"]), f64(inp["A_flow"]), f64(inp["beta_sq"]), f64(inp["fgr_b"]),
               i32(inp["mask_grounded"])]
+    if stress == "BPA":
+        # sliding law, borders (u then v: north, east, south, west), rheology and
+        # iteration controls, the a->a operators (1-based CSR), zeta_stag
+        from .operators import build_aa, zeta_stag
+        aa = ctl.get("aa") or build_aa(m)
+        zs = ctl.get("zeta_stag")
+        zs = zeta_stag(w.zeta) if zs is None else zs
+        parts.append(i32([aa.nnz, ctl.get("sliding_law", 1), *ctl.get("bc_u", (0, 0, 0, 0)),
+                          *ctl.get("bc_v", (0, 0, 0, 0))]))
+        parts.append(f64([ctl.get("glens_n", 3.0), ctl.get("epsilon_sq_0", 1e-8), ctl.get("visc_it_relax", 0.2),
+                          ctl.get("visc_it_norm_dUV_tol", 5e-5), ctl.get("ismip_hom_L", 0.0)]))
+        parts += [i32(aa.ptr + 1), i32(aa.ind + 1), f64(aa.vals["ddx"]), f64(aa.vals["ddy"]), f64(zs)]
     with open(path, "wb") as f:
         for p in parts:
             f.write(np.ascontiguousarray(p).tobytes())
