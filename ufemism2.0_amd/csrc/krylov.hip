@@ -42,11 +42,14 @@ constexpr int kLpr = 4;          // lanes per block row in the V-cycle kernels
 // against 4.3-4.5 TB/s with 8 lanes per row (fewer idle lanes on the row's
 // last pass, one shuffle stage less, twice the rows per workgroup), while a
 // plain read stream of the same buffer reaches 7.0 TB/s.
-constexpr int kSpmvLpr = 4;      // lanes per block row in the block product
+constexpr int kSpmvLpr = 4;      // lanes per block row in the block product (stencil matrices)
 constexpr int kSpmvBlock = 256;  // workgroup size of the block product
 constexpr int kSpmvRedGrid = kMaxPartials;  // grid cap of the block product with a reduction epilogue
 
-template <typename T, int OP>
+// Rows far longer than the 2-D stencil's (the BPA's block rows hold three
+// layers of a b->b row, ~30-40 blocks) take more lanes per row
+// (spmv_lanes): 8 lanes read 256 contiguous bytes of a row per pass.
+template <typename T, int OP, int LPR = kSpmvLpr>
 __global__ void __launch_bounds__(kSpmvBlock) k_bsr_spmv(
     int nb, const int *__restrict__ ptr, const int *__restrict__ col,
     const typename VecT<T>::V4 *__restrict__ val, const typename VecT<T>::V2 *__restrict__ x,
@@ -57,21 +60,21 @@ __global__ void __launch_bounds__(kSpmvBlock) k_bsr_spmv(
   using V2 = typename VecT<T>::V2;
   using V4 = typename VecT<T>::V4;
   if (sc && sc->done) return;
-  const int lane = threadIdx.x % kSpmvLpr;
-  constexpr int kRowsPerPass = kSpmvBlock / kSpmvLpr;
-  // one row per kSpmvLpr lanes; consecutive rows per workgroup, workgroups ordered
+  const int lane = threadIdx.x % LPR;
+  constexpr int kRowsPerPass = kSpmvBlock / LPR;
+  // one row per LPR lanes; consecutive rows per workgroup, workgroups ordered
   // per XCD, grid-stride when the grid is capped
   const int stride = gridDim.x * kRowsPerPass;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int row = xcd_block() * kRowsPerPass + (int)threadIdx.x / kSpmvLpr; row < nb; row += stride) {
+  for (int row = xcd_block() * kRowsPerPass + (int)threadIdx.x / LPR; row < nb; row += stride) {
     T y0 = 0, y1 = 0;
     if (x) {
       const int k1 = ptr[row + 1];
       // two blocks' gathers in flight per lane; the sums keep the block order
       int k = ptr[row] + lane;
-      for (; k + kSpmvLpr < k1; k += 2 * kSpmvLpr) {
-        const V4 a0 = val[k], a1 = val[k + kSpmvLpr];
-        const int c0 = col[k], c1 = col[k + kSpmvLpr];
+      for (; k + LPR < k1; k += 2 * LPR) {
+        const V4 a0 = val[k], a1 = val[k + LPR];
+        const int c0 = col[k], c1 = col[k + LPR];
         const V2 xv0 = x[c0], xv1 = x[c1];
         y0 += a0.x * xv0.x + a0.y * xv0.y;
         y1 += a0.z * xv0.x + a0.w * xv0.y;
@@ -85,9 +88,9 @@ __global__ void __launch_bounds__(kSpmvBlock) k_bsr_spmv(
         y1 += a.z * xv.x + a.w * xv.y;
       }
 #pragma unroll
-      for (int off = kSpmvLpr / 2; off > 0; off >>= 1) {
-        y0 += __shfl_down(y0, off, kSpmvLpr);
-        y1 += __shfl_down(y1, off, kSpmvLpr);
+      for (int off = LPR / 2; off > 0; off >>= 1) {
+        y0 += __shfl_down(y0, off, LPR);
+        y1 += __shfl_down(y1, off, LPR);
       }
     }
     if (lane == 0) {
@@ -158,13 +161,17 @@ __global__ void __launch_bounds__(kFinishBlock) k_spmv_finish(RedSlot slot, int 
   }
 }
 
+// lanes per block row: 8 for rows averaging 20 blocks or more, else kSpmvLpr
+inline int spmv_lanes(int nb, long nnzb) { return nnzb >= 20L * nb ? 8 : kSpmvLpr; }
+
 template <typename T, int OP>
 static void spmv_launch(const BsrT<T> &A, const T *x, T *y, const T *b, const T *w, const RedSlot &slot,
                         double *out, KrylovScalars *sc, hipStream_t s, double vmax = 0.0,
                         const typename VecT<T>::V4 *dblk = nullptr, int write_unscaled = 0,
                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   using V2 = typename VecT<T>::V2;
-  long threads = (long)A.nb * kSpmvLpr;
+  const int lpr = spmv_lanes(A.nb, A.nnzb);
+  long threads = (long)A.nb * lpr;
   // with a reduction epilogue, fewer workgroups with several row passes each
   // amortise the workgroup reduction
   int nblk = xcd_grid(blocks_for(threads, kSpmvBlock));
@@ -174,9 +181,15 @@ static void spmv_launch(const BsrT<T> &A, const T *x, T *y, const T *b, const T 
     nblk = xcd_grid((nblk + passes - 1) / passes);
   }
   if (ev0) UFE_HIP(hipEventRecord(ev0, s));
-  hipLaunchKernelGGL((k_bsr_spmv<T, OP>), dim3(nblk), dim3(kSpmvBlock), 0, s, A.nb, A.ptr, A.col, A.val,
-                     reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y), reinterpret_cast<const V2 *>(b),
-                     reinterpret_cast<const V2 *>(w), slot, out, sc, vmax, dblk, write_unscaled);
+  if (lpr == 8)
+    hipLaunchKernelGGL((k_bsr_spmv<T, OP, 8>), dim3(nblk), dim3(kSpmvBlock), 0, s, A.nb, A.ptr, A.col, A.val,
+                       reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y), reinterpret_cast<const V2 *>(b),
+                       reinterpret_cast<const V2 *>(w), slot, out, sc, vmax, dblk, write_unscaled);
+  else
+    hipLaunchKernelGGL((k_bsr_spmv<T, OP, kSpmvLpr>), dim3(nblk), dim3(kSpmvBlock), 0, s, A.nb, A.ptr, A.col,
+                       A.val, reinterpret_cast<const V2 *>(x), reinterpret_cast<V2 *>(y),
+                       reinterpret_cast<const V2 *>(b), reinterpret_cast<const V2 *>(w), slot, out, sc, vmax, dblk,
+                       write_unscaled);
   UFE_LAUNCH_CHECK();
   if (ev1) UFE_HIP(hipEventRecord(ev1, s));
   if (OP != OP_PLAIN) {
@@ -1503,14 +1516,30 @@ __device__ __forceinline__ double gmres_partial_sum(const double *__restrict__ p
   return s;
 }
 
-__global__ void __launch_bounds__(kBlock) k_gmres_dots_finish(int ncol, int nparts, const double *__restrict__ part,
-                                                              GmresScalars *__restrict__ gs, int j,
-                                                              const KrylovScalars *__restrict__ sc) {
+// All h_ij at once: one wavefront per column (columns round robin over the
+// workgroup's 16 wavefronts), four independent partial sums per lane so the
+// loads of a column are in flight together, a shuffle reduction per column;
+// no workgroup barrier.  The summation order is fixed by nparts.
+constexpr int kGmresFinishBlock = 1024;
+__global__ void __launch_bounds__(kGmresFinishBlock) k_gmres_dots_finish(int ncol, int nparts,
+                                                                         const double *__restrict__ part,
+                                                                         GmresScalars *__restrict__ gs, int j,
+                                                                         const KrylovScalars *__restrict__ sc) {
   if (sc->done) return;
-  __shared__ double lds[kBlock / 64];
-  for (int i = 0; i < ncol; ++i) {
-    const double h = gmres_partial_sum(part + (long)i * kMaxPartials, nparts, lds);
-    if (threadIdx.x == 0) gs->H[j * kGmresLd + i] = h;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int i = wv; i < ncol; i += nw) {
+    const double *__restrict__ p = part + (long)i * kMaxPartials;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = lane;
+    for (; b + 192 < nparts; b += 256) {
+      s0 += p[b];
+      s1 += p[b + 64];
+      s2 += p[b + 128];
+      s3 += p[b + 192];
+    }
+    for (; b < nparts; b += 64) s0 += p[b];
+    const double t = wave_sum((s0 + s1) + (s2 + s3));
+    if (lane == 0) gs->H[j * kGmresLd + i] = t;
   }
 }
 
@@ -1647,7 +1676,7 @@ static void gmres_dots_update(long n, int ncol, const T *V, long ldv, T *w, doub
   const int g = (int)((n + (long)kBlock * R - 1) / ((long)kBlock * R));
   hipLaunchKernelGGL((k_gmres_dots<T, R>), dim3(g), dim3(kBlock), 0, s, n, ncol, V, ldv, w, part, sc);
   UFE_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_gmres_dots_finish, dim3(1), dim3(kBlock), 0, s, ncol, g, part, gs, j, sc);
+  hipLaunchKernelGGL(k_gmres_dots_finish, dim3(1), dim3(kGmresFinishBlock), 0, s, ncol, g, part, gs, j, sc);
   UFE_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_gmres_update<T, R>), dim3(g), dim3(kBlock), 0, s, n, ncol, V, ldv, w, gs, j, part, sc);
   UFE_LAUNCH_CHECK();
