@@ -35,6 +35,14 @@ extern "C" {
 #define UFE_SLIDING_WEERTMAN       1
 #define UFE_SLIDING_BUDD           2  /* needs till_yield_stress in place of beta_sq */
 #define UFE_SLIDING_COULOMB        3  /* needs till_yield_stress in place of beta_sq */
+#define UFE_SLIDING_TSAI2015       4  /* beta_sq, plus the cap alpha_sq * N of ufe_set_sliding_cap */
+#define UFE_SLIDING_SCHOOF2005     5  /* beta_sq, plus the cap alpha_sq * N of ufe_set_sliding_cap */
+#define UFE_SLIDING_ZOET_IVERSON   6  /* needs till_yield_stress in place of beta_sq */
+/* The reference's 'idealised' laws need no choice of their own: ISMIP-HOM C, D
+ * and F prescribe a velocity-independent beta (sliding_laws.f90:499-571), which
+ * is WEERTMAN with slid_weertman_m = 1 and beta_sq = that field (uabs^0 = 1
+ * exactly); 'SSA_icestream' (:462-497) is COULOMB with the analytical till
+ * yield stress of Schoof (2006). */
 
 #define UFE_BC_INFINITE            0
 #define UFE_BC_ZERO                1
@@ -146,6 +154,8 @@ typedef struct ufe_params {
    * slightly and halves the bytes of the V-cycle's finest-level passes. */
   int    precond_fp32;               /*                                  0      */
   double ismip_hom_L;                /* refgeo_idealised_ISMIP_HOM_L     0 m (periodic_ISMIP-HOM needs > 0) */
+  double slid_zi_p;                  /* slid_ZI_p                        5      */
+  double slid_zi_ut;                 /* slid_ZI_ut                       200 m/yr */
 } ufe_params;
 
 #define UFE_ACCEL_MAX_DEPTH 8
@@ -180,6 +190,13 @@ int         ufe_version(void);
 int  ufe_ctx_create(ufe_ctx **ctx, int device, const ufe_mesh_desc *mesh);
 int  ufe_ctx_destroy(ufe_ctx *ctx);
 int  ufe_set_params(ufe_ctx *ctx, const ufe_params *p);
+/* Coulomb limit of the Tsai2015 and Schoof2005 sliding laws
+ * (sliding_laws.f90:244-297, :299-351): the context keeps
+ * alpha_sq * effective_pressure per vertex ([nV] each: bed_roughness%alpha_sq,
+ * ice%effective_pressure) for every later solve until it is set again;
+ * (NULL, NULL) clears it.  Solves with those laws fail with UFE_ERR_ARGUMENT
+ * while it is unset. */
+int  ufe_set_sliding_cap(ufe_ctx *ctx, const double *alpha_sq, const double *effective_pressure);
 /* Multigrid preconditioner knobs (no reference counterpart; the reference's
  * PETSc preconditioner has none of these): damping of the block-Jacobi
  * smoother (0 < omega <= 1, default 0.85), scale of the coarse-level
@@ -583,6 +600,18 @@ int ufe_mesh_operators_bb2(int nT, const int *TriC, const double *TriGC, int *nn
  * a->a ddx/ddy (nV rows, home vertex first), read by the zeta gradients. */
 int ufe_mesh_operators_aa(int nV, const double *V, const int *C_ptr, const int *C_ind, int *nnz, int *ptr,
                           int *ind, double *val_ddx, double *val_ddy);
+
+/* Till yield stress of the COULOMB, BUDD and ZOET_IVERSON laws, which the solves
+ * take in place of beta_sq (sliding_laws.f90:132-242, :353-403): tau_c =
+ * effective_pressure * tan(pi / 180) * till_friction_angle (degrees; the
+ * reference's expression as written), then extend_till_yield_stress_to_neighbours
+ * (:643-700): an ice-free land vertex takes the smallest unextended tau_c of its
+ * grounded neighbours (at most 1000 * rho_i * g), or Hi_min * rho_i * g when
+ * none is grounded.  Host code; [nV] arrays, masks 0/1, C_ptr/C_ind the
+ * 0-based vertex connectivity (mesh%C). */
+int ufe_till_yield_stress(int nV, const int *C_ptr, const int *C_ind, const double *effective_pressure,
+                          const double *till_friction_angle, const int *mask_grounded_ice,
+                          const int *mask_icefree_land, double Hi_min, double *till_yield_stress);
 
 /* ---- learned initial guess (engine extension, no reference counterpart) --
  * A message-passing network on the triangle graph (3 layers, hidden 64,

@@ -62,8 +62,25 @@ def lib():
     o.oracle_strain_heating.argtypes = [C.c_int, C.c_int, dp, dp, C.POINTER(dp), C.c_double, dp]
     o.oracle_frictional_heating.argtypes = [C.c_int, C.c_int, dp, dp, ip, dp]
     o.oracle_zeta_tridiagonals.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp]
+    o.oracle_set_sliding_cap.argtypes = [dp, dp]
     _o = o
     return o
+
+
+_cap_keep = None
+
+
+def set_sliding_cap(alpha_sq=None, effective_pressure=None):
+    """Coulomb limit of the Tsai2015 / Schoof2005 laws for the oracle's later
+    solves (caller-order [nV] arrays, kept alive here); no arguments clears."""
+    global _cap_keep
+    L = _L()
+    if alpha_sq is None:
+        _cap_keep = None
+        lib().oracle_set_sliding_cap(None, None)
+        return
+    _cap_keep = (L.f64(alpha_sq).copy(), L.f64(effective_pressure).copy())
+    lib().oracle_set_sliding_cap(L.ptr(_cap_keep[0]), L.ptr(_cap_keep[1]))
 
 
 def csr_matvec(ptr, ind, val, x):

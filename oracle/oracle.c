@@ -16,7 +16,9 @@
  * answer; Thomas solver against the ut_math_utilities known answer; GMRES
  * against scipy's direct solve; operators against polynomial exactness;
  * assembly against the operator expression of the SSA; Picard against the
- * Schoof (2006) ice-stream closed form; heat columns against the steady
+ * Schoof (2006) ice-stream closed form; the Tsai2015, Schoof2005 and
+ * Zoet-Iverson laws against the pinned Weertman, Coulomb and Budd laws (their
+ * limits and Schoof's harmonic identity); heat columns against the steady
  * diffusion closed form and dense solves; grounded fractions against the
  * geometric closed form and Monte-Carlo areas.
  */
@@ -296,10 +298,23 @@ static double glen_3d(double eps0, double ux, double uy, double uz, double vx, d
   return 0.5 * pow(A, -1.0 / n) * pow(e2, (1.0 - n) / (2.0 * n));
 }
 
+/* Coulomb limit alpha_sq * effective_pressure of the Tsai2015 and Schoof2005
+ * laws (bed_roughness%alpha_sq, ice%effective_pressure), caller-order [nV]
+ * arrays that the caller keeps alive; NULL clears. */
+static const double *g_alpha_sq = NULL, *g_effective_pressure = NULL;
+
+void oracle_set_sliding_cap(const double *alpha_sq, const double *effective_pressure) {
+  g_alpha_sq = alpha_sq;
+  g_effective_pressure = effective_pressure;
+}
+
 /* calc_basal_friction_coefficient + the sliding laws it dispatches to
- * (sliding_laws.f90:25-81 Weertman :86-130, Budd :200-242, Coulomb :132-198);
+ * (sliding_laws.f90:25-81 Weertman :86-130, Budd :200-242, Coulomb :132-198,
+ * Tsai2015 :244-297, Schoof2005 :299-351, Zoet-Iverson :353-403);
  * do_subgrid_friction_on_A_grid is off by default, so bed roughness is used
- * as given.  'roughness' is beta_sq (Weertman) or till_yield_stress. */
+ * as given.  'roughness' is beta_sq (Weertman, Tsai2015, Schoof2005) or
+ * till_yield_stress (Budd, Coulomb, Zoet-Iverson).  The reference's
+ * x * u**(-1) is written x / u. */
 static void sliding_beta(const ufe_mesh_desc *m, const ufe_params *p, const double *u_b,
                          const double *v_b, const double *roughness, double *beta_a) {
   int nV = m->nV;
@@ -317,6 +332,20 @@ static void sliding_beta(const ufe_mesh_desc *m, const ufe_params *p, const doub
             pow(p->slid_budd_u_threshold, p->slid_budd_q_plastic);
         break;
       case UFE_SLIDING_COULOMB: b = roughness[vi] / uabs; break;
+      case UFE_SLIDING_TSAI2015: {
+        double cap = g_alpha_sq[vi] * g_effective_pressure[vi];
+        b = fmin(cap, roughness[vi] * pow(uabs, 1.0 / p->slid_weertman_m)) / uabs;
+        break;
+      }
+      case UFE_SLIDING_SCHOOF2005: {
+        double cap = g_alpha_sq[vi] * g_effective_pressure[vi], m_ = p->slid_weertman_m;
+        b = (roughness[vi] * pow(uabs, 1.0 / m_) * cap) /
+            pow(pow(roughness[vi], m_) * uabs + pow(cap, m_), 1.0 / m_) / uabs;
+        break;
+      }
+      case UFE_SLIDING_ZOET_IVERSON:
+        b = roughness[vi] * pow(uabs, 1.0 / p->slid_zi_p - 1.0) * pow(uabs + p->slid_zi_ut, -1.0 / p->slid_zi_p);
+        break;
       default: b = 0.0;
     }
     beta_a[vi] = fmin(p->slid_beta_max, b);

@@ -26,7 +26,8 @@ UFE_ERR_UNSUPPORTED = -4
 UFE_ERR_UNSTABLE = -5
 UFE_ERR_MESH = -6
 
-SLIDING = {"no_sliding": 0, "Weertman": 1, "Budd": 2, "Coulomb": 3}
+SLIDING = {"no_sliding": 0, "Weertman": 1, "Budd": 2, "Coulomb": 3, "Tsai2015": 4, "Schoof2005": 5,
+           "Zoet-Iverson": 6}
 BC = {"infinite": 0, "zero": 1, "periodic_ISMIP-HOM": 2, "infinite_SSA_icestream": 3}
 PREC_FP64, PREC_MIXED = 0, 1
 KRYLOV_BICGSTAB, KRYLOV_GMRES = 0, 1
@@ -65,6 +66,7 @@ class Params(C.Structure):
         ("accel_restart", C.c_double),
         ("accel_continuation", C.c_int),
         ("precond_fp32", C.c_int), ("ismip_hom_L", C.c_double),
+        ("slid_zi_p", C.c_double), ("slid_zi_ut", C.c_double),
     ]
 
 
@@ -199,6 +201,8 @@ def lib():
     _sig(L, 'ufe_mpnn_features', argtypes=[C.c_void_p, fp])
     _sig(L, 'ufe_mpnn_initial_guess', argtypes=[C.c_void_p, C.c_void_p])
     _sig(L, 'ufe_set_graph_launch', argtypes=[C.c_void_p, C.c_int])
+    _sig(L, 'ufe_set_sliding_cap', argtypes=[C.c_void_p, dp, dp])
+    _sig(L, 'ufe_till_yield_stress', argtypes=[C.c_int, ip, ip, dp, dp, ip, ip, C.c_double, dp])
     _sig(L, 'ufe_ssa_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp])
     _sig(L, 'ufe_diva_residual', argtypes=[C.c_void_p, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
                                            dp])
@@ -248,7 +252,7 @@ EXPORTED = [
     "ufe_thickness_params_default", "ufe_thickness_setup", "ufe_thickness_semiimplicit",
     "ufe_heat_setup", "ufe_heat_upwind_derivatives", "ufe_heat_strain_heating", "ufe_heat_frictional_heating",
     "ufe_zeta_setup", "ufe_zeta_gradients", "ufe_solve_bpa", "ufe_bpa_residual", "ufe_bpa_last_timings",
-    "ufe_bpa_profile_spmv",
+    "ufe_bpa_profile_spmv", "ufe_set_sliding_cap", "ufe_till_yield_stress",
 ]
 
 

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "krylov.h"
+#include "sliding.h"
 #include "ufe_internal.h"
 
 struct ufe_ctx;
@@ -46,6 +47,7 @@ const std::vector<int> &ctx_vperm(const ufe_ctx *c);
 const std::vector<int> &ctx_tperm(const ufe_ctx *c);
 std::shared_ptr<void> &ctx_bpa(ufe_ctx *c);
 const ufe_params &ctx_params(const ufe_ctx *c);
+SlidingLaw ctx_sliding_law(const ufe_ctx *c);
 int ctx_guard(ufe_ctx *c, const std::function<int()> &f);
 void ctx_copy_lists(ufe_ctx *c, std::vector<int> &crow, std::vector<int> &cptr, std::vector<int> &ctri,
                     std::vector<double> &cw);
@@ -102,24 +104,12 @@ __device__ __forceinline__ double glen3(double eps0, double ux, double uy, doubl
 
 struct BpaParams {
   double n, eps0_applied, eps0_config, visc_eff_min, eta_max;
-  double delta_v, beta_max, weertman_m, budd_q, budd_u_th, subgrid_exp;
+  double subgrid_exp;
   double relax_copy;  // C%visc_it_relax of the lagged copy rows
-  int sliding_law, do_subgrid;
+  SlidingLaw sl;
+  int do_subgrid;
   int bc_u[4], bc_v[4];
 };
-
-// calc_basal_friction_coefficient (sliding_laws.f90:25-130) at one vertex
-__device__ __forceinline__ double bpa_sliding_beta(const BpaParams &p, double rough, double ua, double va) {
-  const double uabs = sqrt(p.delta_v * p.delta_v + ua * ua + va * va);
-  double b;
-  switch (p.sliding_law) {
-    case UFE_SLIDING_WEERTMAN: b = rough * pow(uabs, 1.0 / p.weertman_m - 1.0); break;
-    case UFE_SLIDING_BUDD: b = rough * pow(uabs, p.budd_q - 1.0) / pow(p.budd_u_th, p.budd_q); break;
-    case UFE_SLIDING_COULOMB: b = rough / uabs; break;
-    default: b = 0.0;
-  }
-  return fmin(p.beta_max, b);
-}
 
 // ---------------------------------------------------------------------------
 // zeta gradients
@@ -281,7 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_bpa_vertex(
   sr4[i] = make_double4(ux, uy, vx, vy);
   const double e = glen3(p.eps0_applied, ux, uy, uz, vx, vy, vz, A_ak[i], p.n);
   eta_ak[i] = fmin(fmax(e, p.visc_eff_min), p.eta_max);
-  if (k == nz - 1) beta_a[vi] = bpa_sliding_beta(p, rough[vi], ua, va);
+  if (k == nz - 1) beta_a[vi] = basal_friction(p.sl, vi, rough[vi], ua, va);
 }
 
 // One thread per (triangle, staggered layer): strain rates mapped ak->bks,
@@ -385,7 +375,7 @@ __device__ __forceinline__ int bpa_row_kind(const BpaSys &s, const BpaParams &p,
     return choice == UFE_BC_INFINITE ? 4 : 3;
   }
   if (k == 0) return 1;
-  if (k == s.nz - 1) return p.sliding_law == UFE_SLIDING_NO_SLIDING ? 3 : 2;
+  if (k == s.nz - 1) return p.sl.law == UFE_SLIDING_NO_SLIDING ? 3 : 2;
   return 0;
 }
 
@@ -874,7 +864,8 @@ void bpa_pattern(ufe_ctx *c, BpaState &b) {
   for (hipEvent_t &ev : b.ev) UFE_HIP(hipEventCreate(&ev));
 }
 
-BpaParams bpa_params(const ufe_params &p, double eps0_applied) {
+BpaParams bpa_params(const ufe_ctx *c, double eps0_applied) {
+  const ufe_params &p = ctx_params(c);
   BpaParams q{};
   q.n = p.glens_n;
   q.eps0_applied = eps0_applied;
@@ -882,14 +873,9 @@ BpaParams bpa_params(const ufe_params &p, double eps0_applied) {
   q.visc_eff_min = p.visc_eff_min;
   q.eta_max = 0.5 * std::pow(1e-18, -1.0 / p.glens_n) *
               std::pow(eps0_applied, (1.0 - p.glens_n) / (2.0 * p.glens_n));  // BPA_main.f90:1831
-  q.delta_v = p.slid_delta_v;
-  q.beta_max = p.slid_beta_max;
-  q.weertman_m = p.slid_weertman_m;
-  q.budd_q = p.slid_budd_q_plastic;
-  q.budd_u_th = p.slid_budd_u_threshold;
+  q.sl = ctx_sliding_law(c);
   q.subgrid_exp = p.subgrid_friction_exponent;
   q.relax_copy = p.visc_it_relax;
-  q.sliding_law = p.sliding_law;
   q.do_subgrid = p.do_gl_subgrid_friction;
   for (int i = 0; i < 4; ++i) { q.bc_u[i] = p.bc_u[i]; q.bc_v[i] = p.bc_v[i]; }
   return q;
@@ -1005,6 +991,7 @@ int check_bpa_params(ufe_ctx *c, const ufe_params &p) {
       return ctx_fail(c, UFE_ERR_UNSUPPORTED,
                       "the BPA borders take 'infinite', 'zero' or 'periodic_ISMIP-HOM' (BPA_main.f90:1165-1723)");
   if (p.gmres_restart < 1 || p.gmres_restart > 64) return ctx_fail(c, UFE_ERR_ARGUMENT, "gmres_restart must be 1..64");
+  ctx_sliding_law(c);  // throws when the law needs the Coulomb cap and none is set
   return UFE_OK;
 }
 
@@ -1255,7 +1242,7 @@ extern "C" int ufe_solve_bpa(ufe_ctx *c, const ufe_bpa_inputs *in, double *u_bk,
       ++it;
       UFE_HIP(hipMemcpyAsync(b.xprev.p, b.x.p, sizeof(double2) * nbk, hipMemcpyDeviceToDevice, s));
       UFE_HIP(hipEventRecord(b.ev[0], s));
-      bpa_state_and_system(c, b, bpa_params(p, eps0), b.tribi.as<int>());
+      bpa_state_and_system(c, b, bpa_params(c, eps0), b.tribi.as<int>());
       UFE_HIP(hipEventRecord(b.ev[2], s));
       col_factor(c, b);
       UFE_HIP(hipEventRecord(b.ev[3], s));
@@ -1310,7 +1297,7 @@ extern "C" int ufe_solve_bpa(ufe_ctx *c, const ufe_bpa_inputs *in, double *u_bk,
     velocity_down(c, b, u_bk, v_bk);
     if (info) {
       UFE_HIP(hipMemcpyAsync(b.xprev.p, b.x.p, sizeof(double2) * nbk, hipMemcpyDeviceToDevice, s));
-      bpa_state_and_system(c, b, bpa_params(p, eps0), b.tribi.as<int>());
+      bpa_state_and_system(c, b, bpa_params(c, eps0), b.tribi.as<int>());
       const double rel = bpa_rel_residual(b);
       info->n_visc_its = it;
       info->n_Axb_its = n_axb;
@@ -1344,7 +1331,7 @@ extern "C" int ufe_bpa_residual(ufe_ctx *c, const ufe_bpa_inputs *in, const doub
     velocity_up(c, b, u_bk, v_bk);
     const size_t nbk = (size_t)b.nb;
     UFE_HIP(hipMemcpyAsync(b.xprev.p, b.x.p, sizeof(double2) * nbk, hipMemcpyDeviceToDevice, s));
-    bpa_state_and_system(c, b, bpa_params(p, p.epsilon_sq_0), b.tribi.as<int>());
+    bpa_state_and_system(c, b, bpa_params(c, p.epsilon_sq_0), b.tribi.as<int>());
     const double rel = bpa_rel_residual(b);
     if (rel_residual) *rel_residual = rel;
     auto rows_down = [&](const DBuf &src, double *dst) {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "../../include/ufe_hip.h"
+#include "sliding.h"
 #include "ufe_device.h"
 
 namespace ufe {
@@ -42,9 +43,9 @@ struct DevMesh {
 // Scalars of one Picard iteration (copied by value into kernels).
 struct PicardParams {
   double n, eps0, visc_eff_min, eta_max;
-  double delta_v, beta_max, weertman_m, budd_q, budd_u_th;
+  SlidingLaw sl;
   double subgrid_exp;
-  int sliding_law, do_subgrid, crossterms, no_sliding;
+  int do_subgrid, crossterms, no_sliding;
 };
 
 // picard kernels

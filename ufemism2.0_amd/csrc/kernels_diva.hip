@@ -88,15 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_diva(
       f2s = -h * I2;
     }
   }
-  double uabs = sqrt(p.delta_v * p.delta_v + ub * ub + vb * vb);
-  double b;
-  switch (p.sliding_law) {
-    case UFE_SLIDING_WEERTMAN: b = rough[vi] * pow(uabs, 1.0 / p.weertman_m - 1.0); break;
-    case UFE_SLIDING_BUDD: b = rough[vi] * pow(uabs, p.budd_q - 1.0) / pow(p.budd_u_th, p.budd_q); break;
-    case UFE_SLIDING_COULOMB: b = rough[vi] / uabs; break;
-    default: b = 0.0;
-  }
-  b = fmin(p.beta_max, b);
+  const double b = basal_friction(p.sl, vi, rough[vi], ub, vb);
   beta_a[vi] = b;
   // F2 at the surface (k = 0); 0 for a single-layer column
   if (nz < 2) f2s = 0.0;
@@ -257,15 +249,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_diva_cols(
   if (k != 0) return;
   const double f2s = nz < 2 ? 0.0 : -h * I2;  // F2 at the surface (k = 0)
   N_a[vi] = evav * fmax((double)0.1f, Hi[vi]);
-  const double uabs = sqrt(p.delta_v * p.delta_v + ub * ub + vb * vb);
-  double b;
-  switch (p.sliding_law) {
-    case UFE_SLIDING_WEERTMAN: b = rough[vi] * pow(uabs, 1.0 / p.weertman_m - 1.0); break;
-    case UFE_SLIDING_BUDD: b = rough[vi] * pow(uabs, p.budd_q - 1.0) / pow(p.budd_u_th, p.budd_q); break;
-    case UFE_SLIDING_COULOMB: b = rough[vi] / uabs; break;
-    default: b = 0.0;
-  }
-  b = fmin(p.beta_max, b);
+  const double b = basal_friction(p.sl, vi, rough[vi], ub, vb);
   beta_a[vi] = b;
   beta_eff_a[vi] = p.no_sliding ? 1.0 / f2s : b / (1.0 + b * f2s);
 }

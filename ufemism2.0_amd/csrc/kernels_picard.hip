@@ -21,18 +21,6 @@ namespace ufe {
 //                                  (SSA_main.f90:356-382, constitutive_equation.f90:25-51)
 //   calc_basal_friction_coefficient (sliding_laws.f90:25-130): u_a = map_b_a(u)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sliding_beta(const PicardParams &p, double rough, double ua, double va) {
-  double uabs = sqrt(p.delta_v * p.delta_v + ua * ua + va * va);
-  double b;
-  switch (p.sliding_law) {
-    case UFE_SLIDING_WEERTMAN: b = rough * pow(uabs, 1.0 / p.weertman_m - 1.0); break;
-    case UFE_SLIDING_BUDD: b = rough * pow(uabs, p.budd_q - 1.0) / pow(p.budd_u_th, p.budd_q); break;
-    case UFE_SLIDING_COULOMB: b = rough / uabs; break;
-    default: b = 0.0;
-  }
-  return fmin(p.beta_max, b);
-}
-
 __global__ void __launch_bounds__(kBlock) k_vertex_ssa(
     int nV, const int *__restrict__ ptr, const int *__restrict__ ind, const double *__restrict__ wmap,
     const double *__restrict__ wdx, const double *__restrict__ wdy, const double *__restrict__ x,
@@ -52,7 +40,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex_ssa(
   double eta = 0.5 * pow(A_vav[vi], -1.0 / p.n) * pow(e2, (1.0 - p.n) / (2.0 * p.n));
   eta = fmin(fmax(eta, p.visc_eff_min), p.eta_max);
   N_a[vi] = eta * fmax(0.1, Hi[vi]);
-  beta_a[vi] = sliding_beta(p, rough[vi], ua, va);
+  beta_a[vi] = basal_friction(p.sl, vi, rough[vi], ua, va);
 }
 
 // ---------------------------------------------------------------------------

@@ -64,6 +64,8 @@ extern "C" void ufe_params_default(ufe_params *p) {
   p->accel_continuation = 0;
   p->precond_fp32 = 0;
   p->ismip_hom_L = 0.0;
+  p->slid_zi_p = 5.0;
+  p->slid_zi_ut = 200.0;
   p->check_every = 8;
 }
 

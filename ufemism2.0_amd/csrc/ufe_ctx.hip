@@ -16,6 +16,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,8 @@ struct ufe_ctx {
   // inputs
   DBuf Hi, Hs, A_vk_vf, A_vk_lf, A_vav, rough, fgr, bcu, bcv, rowtype;
   DBuf Apow_lf;  // A^(-1/n), layer-fastest, formed at the start of each DIVA solve
+  DBuf cap;      // alpha_sq * effective_pressure of TSAI2015 / SCHOOF2005, device vertex order
+  bool has_cap = false;
   std::vector<signed char> h_rowtype;
   bool grounded_exists = true;
   bool inputs_ready = false;
@@ -259,19 +262,23 @@ __global__ void __launch_bounds__(kBlock) k_copy_rows(int n, const int *__restri
   (uv == 0 ? bcu : bcv)[ti] = val;
 }
 
-PicardParams picard_params(const ufe_params &q, double eps0) {
+// The sliding-law constants of a solve; throws (UFE_ERR_ARGUMENT through
+// guarded) when the law needs the Coulomb cap and none is set.
+SlidingLaw ctx_sliding(const ufe_ctx *c) {
+  if (sliding_law_needs_cap(c->prm.sliding_law) && !c->has_cap)
+    throw std::invalid_argument("the Tsai2015 and Schoof2005 sliding laws need ufe_set_sliding_cap first");
+  return make_sliding_law(c->prm, c->has_cap ? c->cap.as<double>() : nullptr);
+}
+
+PicardParams picard_params(const ufe_ctx *c, double eps0) {
+  const ufe_params &q = c->prm;
   PicardParams p;
   p.n = q.glens_n;
   p.eps0 = eps0;
   p.visc_eff_min = q.visc_eff_min;
   p.eta_max = 0.5 * std::pow(1e-18, -1.0 / q.glens_n) * std::pow(eps0, (1.0 - q.glens_n) / (2.0 * q.glens_n));
-  p.delta_v = q.slid_delta_v;
-  p.beta_max = q.slid_beta_max;
-  p.weertman_m = q.slid_weertman_m;
-  p.budd_q = q.slid_budd_q_plastic;
-  p.budd_u_th = q.slid_budd_u_threshold;
+  p.sl = ctx_sliding(c);
   p.subgrid_exp = q.subgrid_friction_exponent;
-  p.sliding_law = q.sliding_law;
   p.do_subgrid = q.do_gl_subgrid_friction;
   p.crossterms = q.include_crossterms;
   p.no_sliding = q.sliding_law == UFE_SLIDING_NO_SLIDING;
@@ -425,7 +432,7 @@ static double unscaled_rel_residual(ufe_ctx *c) {
 // Everything up to the assembled linear system at the current iterate.
 static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
   const ufe_params &q = c->prm;
-  PicardParams p = picard_params(q, eps0);
+  PicardParams p = picard_params(c, eps0);
   hipStream_t s = c->s;
   UFE_HIP(hipEventRecord(c->ev[0], s));
   if (!diva) {
@@ -566,6 +573,7 @@ static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
   int nT = c->m.nT;
   ufe_solve_info inf{};
   if (!c->inputs_ready) return fail(c, UFE_ERR_ARGUMENT, "inputs not uploaded");
+  ctx_sliding(c);  // throws when the law needs the Coulomb cap and none is set
   if (!c->grounded_exists || (!diva && q.sliding_law == UFE_SLIDING_NO_SLIDING)) {
     // solve_SSA / solve_DIVA early exit: no grounded ice -> zero velocities
     UFE_HIP(hipMemsetAsync(c->u.p, 0, c->u.bytes, s));
@@ -851,6 +859,7 @@ void ctx_copy_lists(ufe_ctx *c, std::vector<int> &crow, std::vector<int> &cptr, 
 }
 const std::vector<int> &ctx_TriBI(const ufe_ctx *c) { return c->h_TriBI; }
 const ufe_params &ctx_params(const ufe_ctx *c) { return c->prm; }
+SlidingLaw ctx_sliding_law(const ufe_ctx *c) { return ctx_sliding(c); }
 const double *ctx_velocity(const ufe_ctx *c) { return c->u.as<double>(); }
 int ctx_guard(ufe_ctx *c, const std::function<int()> &f) { return guarded(c, f); }
 }  // namespace ufe
@@ -1038,6 +1047,24 @@ extern "C" int ufe_set_amg_options(ufe_ctx *c, double omega, double coarse_scale
   });
 }
 
+extern "C" int ufe_set_sliding_cap(ufe_ctx *c, const double *alpha_sq, const double *effective_pressure) {
+  if (!c || (!alpha_sq) != (!effective_pressure)) return fail(c, UFE_ERR_ARGUMENT, "need both alpha_sq and effective_pressure, or neither");
+  return guarded(c, [&]() -> int {
+    if (!alpha_sq) {
+      c->has_cap = false;
+      return UFE_OK;
+    }
+    UFE_HIP(hipSetDevice(c->device));
+    const int nV = c->m.nV;
+    std::vector<double> h(nV);
+    for (int i = 0; i < nV; ++i) h[i] = alpha_sq[c->vperm[i]] * effective_pressure[c->vperm[i]];
+    h2d(c->cap, h.data(), sizeof(double) * nV, c->s);
+    UFE_HIP(hipStreamSynchronize(c->s));
+    c->has_cap = true;
+    return UFE_OK;
+  });
+}
+
 extern "C" int ufe_set_graph_launch(ufe_ctx *c, int enable) {
   if (!c) return fail(c, UFE_ERR_ARGUMENT, "null argument");
   return guarded(c, [&]() -> int {
@@ -1069,6 +1096,10 @@ extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
           !(p->ismip_hom_L > 0.0))
         return fail(c, UFE_ERR_ARGUMENT, "periodic_ISMIP-HOM needs ismip_hom_L > 0");
     }
+    if (p->sliding_law < UFE_SLIDING_NO_SLIDING || p->sliding_law > UFE_SLIDING_ZOET_IVERSON)
+      return fail(c, UFE_ERR_ARGUMENT, "unknown choice_sliding_law");
+    if (p->sliding_law == UFE_SLIDING_ZOET_IVERSON && !(p->slid_zi_p > 0.0 && p->slid_zi_ut >= 0.0))
+      return fail(c, UFE_ERR_ARGUMENT, "Zoet-Iverson needs slid_ZI_p > 0 and slid_ZI_ut >= 0");
     if (p->accel_depth > 0 && (p->stop_rule != UFE_STOP_RESIDUAL || !(p->accel_mixing > 0.0) ||
                                !(p->accel_mixing <= 1.0) || !(p->accel_restart > 1.0)))
       return fail(c, UFE_ERR_ARGUMENT,

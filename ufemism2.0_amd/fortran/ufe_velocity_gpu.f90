@@ -25,7 +25,7 @@ module ufe_velocity_gpu
             ufe_gpu_set_params, ufe_gpu_default_params, solve_SSA_gpu, solve_DIVA_gpu, &
             ufe_gpu_fraction_gr_b, solve_3D_heat_equation_gpu, ufe_gpu_thickness_params, &
             ufe_gpu_default_thickness_params, ufe_gpu_thickness_setup, calc_dHi_dt_semiimplicit_gpu, &
-            ufe_gpu_zeta_setup, solve_BPA_gpu
+            ufe_gpu_zeta_setup, solve_BPA_gpu, ufe_gpu_set_sliding_cap, ufe_gpu_till_yield_stress
 
   integer(c_int), parameter, public :: UFE_OK = 0, UFE_WARN_NOT_CONVERGED = 1
 
@@ -61,6 +61,7 @@ module ufe_velocity_gpu
     integer(c_int) :: accel_continuation
     integer(c_int) :: precond_fp32
     real(c_double) :: ismip_hom_L
+    real(c_double) :: slid_zi_p, slid_zi_ut
   end type ufe_gpu_params
 
   ! struct ufe_solve_info
@@ -202,6 +203,19 @@ module ufe_velocity_gpu
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx, ptr, ind, ddx, ddy, zs
       integer(c_int), value :: nnz
+      integer(c_int) :: rc
+    end function
+    function c_ufe_set_sliding_cap(ctx, a, n) bind(C, name='ufe_set_sliding_cap') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, a, n
+      integer(c_int) :: rc
+    end function
+    function c_ufe_till_yield_stress(nV, cptr, cind, en, phi, mg, ml, Hi_min, tau) &
+        bind(C, name='ufe_till_yield_stress') result(rc)
+      import :: c_ptr, c_int, c_double
+      integer(c_int), value :: nV
+      type(c_ptr), value :: cptr, cind, en, phi, mg, ml, tau
+      real(c_double), value :: Hi_min
       integer(c_int) :: rc
     end function
     function c_ufe_solve_bpa(ctx, inp, u, v, info) bind(C, name='ufe_solve_bpa') result(rc)
@@ -420,6 +434,49 @@ contains
     n_Axb_its = inf%n_Axb_its
     if (present(info)) info = inf
   end subroutine solve_DIVA_gpu
+
+  ! Coulomb limit of the Tsai2015 and Schoof2005 sliding laws
+  ! (sliding_laws.f90:244-351): bed_roughness%alpha_sq(nV) and
+  ! ice%effective_pressure(nV), kept by the context for the following solves.
+  subroutine ufe_gpu_set_sliding_cap(g, alpha_sq, effective_pressure)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: alpha_sq(:), effective_pressure(:)
+    call crash_on(c_ufe_set_sliding_cap(g%ctx, c_loc(alpha_sq), c_loc(effective_pressure)), g%ctx, &
+                  'ufe_gpu_set_sliding_cap')
+  end subroutine ufe_gpu_set_sliding_cap
+
+  ! Till yield stress of the Coulomb, Budd and Zoet-Iverson laws, passed to the
+  ! solves in place of beta_sq (sliding_laws.f90:132-242, :353-403, with
+  ! extend_till_yield_stress_to_neighbours :643-700), from mesh%nC, mesh%C
+  ! (1-based neighbours, C(nV, nC_mem)), ice%effective_pressure,
+  ! bed_roughness%till_friction_angle, the grounded and ice-free land masks
+  ! and C%Hi_min.  Host code of the library; needs no context.
+  subroutine ufe_gpu_till_yield_stress(nC, C, effective_pressure, till_friction_angle, mask_grounded_ice, &
+                                       mask_icefree_land, Hi_min, till_yield_stress)
+    integer, intent(in) :: nC(:), C(:, :)
+    real(c_double), intent(in), target :: effective_pressure(:), till_friction_angle(:)
+    logical, intent(in) :: mask_grounded_ice(:), mask_icefree_land(:)
+    real(c_double), intent(in) :: Hi_min
+    real(c_double), intent(out), target :: till_yield_stress(:)
+    integer(c_int), allocatable, target :: cp(:), ci(:), mg(:), ml(:)
+    integer :: nV, vi, k, j
+    nV = size(nC)
+    allocate(cp(nV + 1), ci(max(1, sum(nC))), mg(nV), ml(nV))
+    k = 0
+    cp(1) = 0
+    do vi = 1, nV
+      do j = 1, nC(vi)
+        k = k + 1
+        ci(k) = C(vi, j) - 1
+      end do
+      cp(vi + 1) = k
+    end do
+    mg = merge(1, 0, mask_grounded_ice)
+    ml = merge(1, 0, mask_icefree_land)
+    call crash_on(c_ufe_till_yield_stress(int(nV, c_int), c_loc(cp), c_loc(ci), c_loc(effective_pressure), &
+                                          c_loc(till_friction_angle), c_loc(mg), c_loc(ml), Hi_min, &
+                                          c_loc(till_yield_stress)), c_null_ptr, 'ufe_gpu_till_yield_stress')
+  end subroutine ufe_gpu_till_yield_stress
 
   ! Set-up of the zeta gradients and the BPA, once per context: the a->a
   ! operators M_ddx_a_a, M_ddy_a_a as 1-based CSR (one pattern; nV rows) and

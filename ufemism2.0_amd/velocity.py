@@ -141,6 +141,18 @@ class VelocitySolver:
                                                  int(opts["tail_rows"])))
         self.amg_options = opts
 
+    def set_sliding_cap(self, alpha_sq=None, effective_pressure=None):
+        """Coulomb limit alpha_sq * effective_pressure ([nV] each) of the
+        Tsai2015 and Schoof2005 sliding laws (sliding_laws.f90:244-351), kept
+        for every later solve; no arguments clears it."""
+        if alpha_sq is None:
+            self._check(self.lib.ufe_set_sliding_cap(self._ctx, None, None))
+            return
+        a, n = L.f64(alpha_sq), L.f64(effective_pressure)
+        if a.size != self.md.mesh.nV or n.size != self.md.mesh.nV:
+            raise ValueError("alpha_sq and effective_pressure need nV values")
+        self._check(self.lib.ufe_set_sliding_cap(self._ctx, L.ptr(a), L.ptr(n)))
+
     def _apply(self, cfg):
         self.set_params(**{k: v for k, v in cfg.items() if k in _PARAM_NAMES})
         self.set_amg_options(**{k: v for k, v in cfg.items() if k in AMG_DEFAULTS})
@@ -477,6 +489,26 @@ class VelocitySolver:
             self.close()
         except Exception:
             pass
+
+
+def till_yield_stress(mesh, effective_pressure, till_friction_angle, mask_grounded_ice, mask_icefree_land,
+                      Hi_min=0.0):
+    """Till yield stress of the Coulomb, Budd and Zoet-Iverson laws, the field
+    the solves take in place of beta_sq for them: N tan(pi/180) phi, then
+    extend_till_yield_stress_to_neighbours (sliding_laws.f90:132-242, :353-403,
+    :643-700).  Host code of the library; [nV] arrays."""
+    lib = L.lib()
+    nV = mesh.nV
+    cp, ci = L.i32(mesh.C_ptr), L.i32(mesh.C_ind)
+    N, phi = L.f64(effective_pressure), L.f64(till_friction_angle)
+    mg = L.i32(np.asarray(mask_grounded_ice).astype(np.int32))
+    ml = L.i32(np.asarray(mask_icefree_land).astype(np.int32))
+    if not (N.size == phi.size == mg.size == ml.size == nV):
+        raise ValueError("till_yield_stress needs nV values per field")
+    out = np.zeros(nV)
+    L.check(lib.ufe_till_yield_stress(nV, L.ptr(cp, L.ip), L.ptr(ci, L.ip), L.ptr(N), L.ptr(phi),
+                                      L.ptr(mg, L.ip), L.ptr(ml, L.ip), float(Hi_min), L.ptr(out)))
+    return out
 
 
 def thickness_params(**params) -> L.ThicknessParams:

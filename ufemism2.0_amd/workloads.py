@@ -293,6 +293,23 @@ def schoof_icestream(h: float = 16e3, seed: int = 5, nz: int = 10, A: float = 1e
                     params)
 
 
+def ismip_hom_friction(mesh: Mesh, experiment: str, L: float, A_uniform: float = 1e-16) -> np.ndarray:
+    """beta of the 'idealised' sliding laws of ISMIP-HOM C, D and F
+    (sliding_laws.f90:499-571) at the vertices: 1000 + 1000 sin(2 pi x / L)
+    sin(2 pi y / L), 1000 + 1000 sin(2 pi x / L), and 1 / (1000 A).  These do
+    not depend on the velocity, so they are the Weertman law with
+    slid_weertman_m = 1 and this field as beta_sq (uabs^0 = 1 exactly);
+    slid_beta_max still applies."""
+    x, y = mesh.V[:, 0], mesh.V[:, 1]
+    if experiment == "ISMIP-HOM_C":
+        return 1000.0 + 1000.0 * np.sin(2.0 * np.pi * x / L) * np.sin(2.0 * np.pi * y / L)
+    if experiment == "ISMIP-HOM_D":
+        return 1000.0 + 1000.0 * np.sin(2.0 * np.pi * x / L)
+    if experiment == "ISMIP-HOM_F":
+        return np.full(mesh.nV, 1.0 / (A_uniform * 1000.0))
+    raise ValueError(f"no idealised sliding law for {experiment!r} (ISMIP-HOM_E is not implemented in the reference)")
+
+
 def ismip_hom_a(L: float = 160e3, h: float = 8e3, nz: int = 12, seed: int = 8) -> Workload:
     """ISMIP-HOM experiment A, the reference's BPA integrated test
     (automated_testing/integrated_tests/idealised/ISMIP-HOM/config_ISMIP_HOM_A_*_BPA.cfg,
