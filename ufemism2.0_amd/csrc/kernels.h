@@ -77,6 +77,26 @@ void launch_aa_columns(int nT, double *x, const double *u, double *fprev, double
                        double *out, hipStream_t s);
 void launch_aa_mix(int n, double *x, const double *u, const double *dF, const double *dG, long stride, int ncols,
                    const AAGamma &gam, double mix, hipStream_t s);
+// Host side of one solve's mixing: the ring position, the normal equations
+// of the least-squares problem and the restart state.
+struct AndersonState {
+  int cols = 0, pos = 0;
+  bool have_prev = false;
+  double best = 1e300;
+  double H[kAAMaxDepth][kAAMaxDepth] = {};
+  int n_restarts = 0;
+};
+// Device work arrays of the mixing: depth ring columns dF, dG and the previous
+// f, G of n2 (u, v) pairs each, and the reduction slot of 2 kAAMaxDepth + 1 sums.
+struct AndersonBuffers {
+  double *dF, *dG, *fprev, *gprev, *part, *out;
+};
+// One mixing step over n2 (u, v) pairs: x holds G(u) on entry (clamped to
+// vel_max here) and the next iterate on exit; u is the current iterate and
+// resid the nonlinear residual at u (the history restarts when it exceeds
+// restart times the best one seen).  Synchronises the stream once.
+void anderson_step(AndersonState &aa, const AndersonBuffers &buf, int depth, double mixing, double restart,
+                   double vel_max, long n2, double *x, const double *u, double resid, hipStream_t s);
 
 // DIVA kernels (kernels_diva.hip)
 void launch_vertex_diva(const DevMesh &m, const double *x, const double *xbase, const double *taub,

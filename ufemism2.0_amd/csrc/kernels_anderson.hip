@@ -10,6 +10,7 @@
 // pass over the n = 2 nT unknowns: one writes the new ring column and the dot
 // products of the small normal equations, the other forms the mixed iterate.
 #include <algorithm>
+#include <cmath>
 
 #include "kernels.h"
 
@@ -96,6 +97,77 @@ void launch_aa_mix(int n, double *x, const double *u, const double *dF, const do
                    const AAGamma &gam, double mix, hipStream_t s) {
   hipLaunchKernelGGL(k_aa_mix, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, u, dF, dG, stride, ncols, gam, mix);
   UFE_LAUNCH_CHECK();
+}
+
+// Solves (H + lambda I) g = b by Gaussian elimination with partial pivoting;
+// false if the system is numerically singular.
+static bool small_solve(int m, const double (&H)[kAAMaxDepth][kAAMaxDepth], const double *b, double *g) {
+  double a[kAAMaxDepth][kAAMaxDepth + 1];
+  double tr = 0.0;
+  for (int i = 0; i < m; ++i) tr = std::max(tr, H[i][i]);
+  if (!(tr > 0.0)) return false;
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < m; ++j) a[i][j] = H[i][j] + (i == j ? 1e-12 * tr : 0.0);
+    a[i][m] = b[i];
+  }
+  for (int k = 0; k < m; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < m; ++i)
+      if (std::fabs(a[i][k]) > std::fabs(a[piv][k])) piv = i;
+    if (!(std::fabs(a[piv][k]) > 1e-14 * tr)) return false;
+    if (piv != k)
+      for (int j = 0; j <= m; ++j) std::swap(a[k][j], a[piv][j]);
+    for (int i = k + 1; i < m; ++i) {
+      double f = a[i][k] / a[k][k];
+      for (int j = k; j <= m; ++j) a[i][j] -= f * a[k][j];
+    }
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double t = a[i][m];
+    for (int j = i + 1; j < m; ++j) t -= a[i][j] * g[j];
+    g[i] = t / a[i][i];
+  }
+  return true;
+}
+
+void anderson_step(AndersonState &aa, const AndersonBuffers &buf, int depth, double mixing, double restart,
+                   double vel_max, long n2, double *x, const double *u, double resid, hipStream_t s) {
+  const int m = depth;
+  const long stride = 2L * n2;
+  if (resid > restart * aa.best) {
+    aa.cols = 0;
+    aa.pos = 0;
+    ++aa.n_restarts;
+  }
+  aa.best = std::min(aa.best, resid);
+  int p = -1;
+  if (aa.have_prev) {
+    p = aa.pos;
+    aa.pos = (aa.pos + 1) % m;
+    aa.cols = std::min(aa.cols + 1, m);
+  }
+  RedSlot sl;
+  sl.partials = buf.part;
+  sl.result = buf.out;
+  launch_aa_columns((int)n2, x, u, buf.fprev, buf.gprev, buf.dF, buf.dG, stride, p < 0 ? 0 : p,
+                    aa.have_prev ? 1 : 0, aa.cols, vel_max, sl, buf.out, s);
+  double h[2 * kAAMaxDepth + 1];
+  UFE_HIP(hipMemcpyAsync(h, buf.out, sizeof(h), hipMemcpyDeviceToHost, s));
+  UFE_HIP(hipStreamSynchronize(s));
+  aa.have_prev = true;
+  AAGamma gam{};
+  int ncols = aa.cols;
+  if (p >= 0) {
+    for (int j = 0; j < aa.cols; ++j) aa.H[p][j] = aa.H[j][p] = h[j];
+    if (!small_solve(aa.cols, aa.H, h + kAAMaxDepth, gam.g)) {
+      // degenerate history: plain mixing step, history restarts
+      ncols = 0;
+      aa.cols = 0;
+      aa.pos = 0;
+      ++aa.n_restarts;
+    }
+  }
+  launch_aa_mix((int)stride, x, u, buf.dF, buf.dG, stride, ncols, gam, mixing, s);
 }
 
 }  // namespace ufe

@@ -471,14 +471,6 @@ static void picard_state(ufe_ctx *c, bool diva, double eps0, bool need_F1) {
 // solve on entry and the mixed next iterate on exit.  The normal equations
 // (dF^T dF) gamma = dF^T f_k are at most 8x8 and solved on the host.
 // ---------------------------------------------------------------------------
-struct AndersonState {
-  int cols = 0, pos = 0;
-  bool have_prev = false;
-  double best = 1e300;
-  double H[kAAMaxDepth][kAAMaxDepth] = {};
-  int n_restarts = 0;
-};
-
 static void aa_prepare(ufe_ctx *c) {
   size_t n = 2 * (size_t)c->m.nT;
   size_t cols = (size_t)kAAMaxDepth * n * sizeof(double);
@@ -492,78 +484,12 @@ static void aa_prepare(ufe_ctx *c) {
   }
 }
 
-// Solves (H + lambda I) g = b by Gaussian elimination with partial pivoting;
-// false if the system is numerically singular.
-static bool small_solve(int m, const double (&H)[kAAMaxDepth][kAAMaxDepth], const double *b, double *g) {
-  double a[kAAMaxDepth][kAAMaxDepth + 1];
-  double tr = 0.0;
-  for (int i = 0; i < m; ++i) tr = std::max(tr, H[i][i]);
-  if (!(tr > 0.0)) return false;
-  for (int i = 0; i < m; ++i) {
-    for (int j = 0; j < m; ++j) a[i][j] = H[i][j] + (i == j ? 1e-12 * tr : 0.0);
-    a[i][m] = b[i];
-  }
-  for (int k = 0; k < m; ++k) {
-    int piv = k;
-    for (int i = k + 1; i < m; ++i)
-      if (std::fabs(a[i][k]) > std::fabs(a[piv][k])) piv = i;
-    if (!(std::fabs(a[piv][k]) > 1e-14 * tr)) return false;
-    if (piv != k)
-      for (int j = 0; j <= m; ++j) std::swap(a[k][j], a[piv][j]);
-    for (int i = k + 1; i < m; ++i) {
-      double f = a[i][k] / a[k][k];
-      for (int j = k; j <= m; ++j) a[i][j] -= f * a[k][j];
-    }
-  }
-  for (int i = m - 1; i >= 0; --i) {
-    double t = a[i][m];
-    for (int j = i + 1; j < m; ++j) t -= a[i][j] * g[j];
-    g[i] = t / a[i][i];
-  }
-  return true;
-}
-
 static void aa_step(ufe_ctx *c, AndersonState &aa, double resid) {
   const ufe_params &q = c->prm;
-  const int m = q.accel_depth;
-  const int nT = c->m.nT;
-  const long stride = 2L * nT;
-  if (resid > q.accel_restart * aa.best) {
-    aa.cols = 0;
-    aa.pos = 0;
-    ++aa.n_restarts;
-  }
-  aa.best = std::min(aa.best, resid);
-  int p = -1;
-  if (aa.have_prev) {
-    p = aa.pos;
-    aa.pos = (aa.pos + 1) % m;
-    aa.cols = std::min(aa.cols + 1, m);
-  }
-  RedSlot sl;
-  sl.partials = c->aa_part.as<double>();
-  sl.result = c->aa_out.as<double>();
-  launch_aa_columns(nT, c->x.as<double>(), c->u.as<double>(), c->aa_fprev.as<double>(), c->aa_gprev.as<double>(),
-                    c->aa_dF.as<double>(), c->aa_dG.as<double>(), stride, p < 0 ? 0 : p, aa.have_prev ? 1 : 0,
-                    aa.cols, q.vel_max, sl, c->aa_out.as<double>(), c->s);
-  double h[2 * kAAMaxDepth + 1];
-  UFE_HIP(hipMemcpyAsync(h, c->aa_out.p, sizeof(h), hipMemcpyDeviceToHost, c->s));
-  UFE_HIP(hipStreamSynchronize(c->s));
-  aa.have_prev = true;
-  AAGamma gam{};
-  int ncols = aa.cols;
-  if (p >= 0) {
-    for (int j = 0; j < aa.cols; ++j) aa.H[p][j] = aa.H[j][p] = h[j];
-    if (!small_solve(aa.cols, aa.H, h + kAAMaxDepth, gam.g)) {
-      // degenerate history: plain mixing step, history restarts
-      ncols = 0;
-      aa.cols = 0;
-      aa.pos = 0;
-      ++aa.n_restarts;
-    }
-  }
-  launch_aa_mix((int)stride, c->x.as<double>(), c->u.as<double>(), c->aa_dF.as<double>(), c->aa_dG.as<double>(),
-                stride, ncols, gam, q.accel_mixing, c->s);
+  const AndersonBuffers buf{c->aa_dF.as<double>(), c->aa_dG.as<double>(), c->aa_fprev.as<double>(),
+                            c->aa_gprev.as<double>(), c->aa_part.as<double>(), c->aa_out.as<double>()};
+  anderson_step(aa, buf, q.accel_depth, q.accel_mixing, q.accel_restart, q.vel_max, c->m.nT, c->x.as<double>(),
+                c->u.as<double>(), resid, c->s);
 }
 
 static int run_picard(ufe_ctx *c, bool diva, ufe_solve_info *info) {
