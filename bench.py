@@ -64,6 +64,18 @@ BASE = dict(include_crossterms=0, vel_max=1e5, abstol=0.0, stop_rule=1, nonlinea
 CHOSEN = dict(precision=0, precond=3, warm_start=1, rtol=0.01, accel_depth=2, accel_mixing=1.0, accel_continuation=5,
               accel_restart=5.0)
 PLAIN = dict(precision=0, precond=3, warm_start=0, rtol=1e-7)
+# On the 200k-vertex channel the chosen form also smooths with l1 block Jacobi
+# (40% fewer Krylov iterations there); on the continental mesh that smoother
+# slows the Picard iteration down and on the ensemble members it needs more
+# Krylov iterations, so those keep plain block Jacobi (autotune searches both).
+L1_WORKLOADS = ("channel",)
+
+
+def chosen_for(workload, smoother="auto"):
+    """The chosen form of a workload; smoother overrides the V-cycle smoother
+    ("bj" block Jacobi, "l1" l1 block Jacobi)."""
+    l1 = workload in L1_WORKLOADS if smoother == "auto" else smoother == "l1"
+    return dict(CHOSEN, amg_smoother=1 if l1 else 0)
 METRIC = "converged velocity solves/s at 1e-6 rel. residual, plain vs chosen form, 1 GPU"
 HBM_PEAK_GBS = 8000.0
 
@@ -170,7 +182,7 @@ def oracle_parity_at_scale(V, W):
     if str(f["digest"]) != w.mesh.digest():
         return {"error": "mesh digest differs from the fixture's"}
     inp = w.inputs()
-    s = V.VelocitySolver(w.descriptor(), 0, **dict(BASE, nonlinear_rtol=1e-8), **CHOSEN)
+    s = V.VelocitySolver(w.descriptor(), 0, **dict(BASE, nonlinear_rtol=1e-8), **chosen_for("channel"))
     s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
     s.set_velocity(None, None)
     s.reset_diva_state()
@@ -328,6 +340,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=200)
     ap.add_argument("--nv", type=int, default=200_000, help="target vertices of the channel mesh")
     ap.add_argument("--workload", choices=WORKLOADS, default="channel")
+    ap.add_argument("--smoother", choices=("auto", "bj", "l1"), default="auto",
+                    help="V-cycle smoother of the chosen form (auto: l1 on the channel geometry)")
     ap.add_argument("--members", type=int, default=64, help="ensemble members (--workload ensemble)")
     ap.add_argument("--concurrency", type=int, default=8,
                     help="ensemble members solved at once, one HIP stream each (--workload ensemble)")
@@ -362,7 +376,7 @@ def main():
 
     if args.profile_kernels:
         inp = w.inputs()
-        s = V.VelocitySolver(w.descriptor(), 0, **BASE, **CHOSEN)
+        s = V.VelocitySolver(w.descriptor(), 0, **BASE, **chosen_for(args.workload, args.smoother))
         s.upload(inp["Hi"], inp["Hs"], inp["A_flow"], inp["beta_sq"], inp["fgr_b"], inp["mask_grounded"])
         solve_once(s, w.stress_balance)
         ms, nbytes = s.profile_spmv(args.spmv_reps)
@@ -375,7 +389,8 @@ def main():
         return
 
     conc = args.concurrency if args.workload == "ensemble" else 1
-    t_chosen, infos, vel_c, solvers = time_form(V, ws, CHOSEN, args.steps, args.warmup, "chosen", conc,
+    chosen = chosen_for(args.workload, args.smoother)
+    t_chosen, infos, vel_c, solvers = time_form(V, ws, chosen, args.steps, args.warmup, "chosen", conc,
                                                 autotune=args.autotune)
     # kernels timed live inside the timed solves (HIP events on the solver stream):
     # 1 = the preconditioner's finest-level prolongation + smoothing (the largest
@@ -425,7 +440,8 @@ def main():
                    "stop": "block-scaled nonlinear residual <= 1e-6",
                    "chosen": "Anderson(2, 1.0, history restart at 5x the best residual) Picard after 5-decade "
                              "eps0 continuation, FP64 AMG-BiCGStab warm, "
-                             "rtol 0.01",
+                             "rtol 0.01, V-cycle smoother "
+                             + ("l1 block Jacobi" if chosen["amg_smoother"] else "block Jacobi"),
                    "plain": "relaxed Picard (reference rule), FP64 AMG-BiCGStab zero guess, rtol 1e-7"},
         "chosen": {"picard": [i.n_visc_its for i in infos], "krylov": [i.n_Axb_its for i in infos],
                    "residual": [i.rel_residual for i in infos],

@@ -209,7 +209,8 @@ def channel_fixtures(W):
     return out
 
 
-FORMS = {"plain": PLAIN, "chosen": CHOSEN, "chosen-fp32-precond": dict(CHOSEN, precond_fp32=1),
+FORMS = {"plain": PLAIN, "chosen": CHOSEN, "chosen-l1-smoother": dict(CHOSEN, amg_smoother=1),
+         "chosen-fp32-precond": dict(CHOSEN, precond_fp32=1),
          # FP32 Krylov with the FP32 V-cycle (all levels) and FP64 refinement
          "mixed-amg": dict(CHOSEN, precision=1, inner_rtol=0.1)}
 

@@ -32,6 +32,7 @@ BC = {"infinite": 0, "zero": 1, "periodic_ISMIP-HOM": 2, "infinite_SSA_icestream
 PREC_FP64, PREC_MIXED = 0, 1
 KRYLOV_BICGSTAB, KRYLOV_GMRES = 0, 1
 PC_NONE, PC_JACOBI, PC_BLOCK_JACOBI, PC_AMG = 0, 1, 2, 3
+SMOOTHER_BLOCK_JACOBI, SMOOTHER_L1_BLOCK_JACOBI = 0, 1
 STOP_L2UV, STOP_RESIDUAL = 0, 1
 STRESS_SSA, STRESS_DIVA = 0, 1
 
@@ -67,6 +68,7 @@ class Params(C.Structure):
         ("accel_continuation", C.c_int),
         ("precond_fp32", C.c_int), ("ismip_hom_L", C.c_double),
         ("slid_zi_p", C.c_double), ("slid_zi_ut", C.c_double),
+        ("amg_smoother", C.c_int),
     ]
 
 

@@ -114,6 +114,9 @@ class Krylov {
   // Run the coarse levels of the V-cycle with at most max_rows block rows in
   // one workgroup (0: one launch per pass on every level).
   void set_amg_tail(int max_rows) { amg_tail_rows_ = max_rows; }
+  // l1 block-Jacobi smoothing on every level (true) or plain block Jacobi;
+  // the finest level's pre-smoothing is then not folded into its passes
+  void set_amg_l1(bool on) { amg_l1_ = on; }
   // A preconditioner of the caller's, z = M^-1 r on the FP64 vectors, used by
   // gmres64 in place of the built-in ones while set (empty function: off).
   void set_custom_pc(std::function<void(const double *r, double *z)> f) { custom_pc_ = std::move(f); }
@@ -189,6 +192,7 @@ class Krylov {
   const int *A_diag_ = nullptr;
   const float4 *fine32_ = nullptr;
   int amg_tail_rows_ = 300;
+  bool amg_l1_ = false;
   bool unit_diag0_ = false;
   double amg_omega_ = 0.85;
   int amg_coarse_sweeps_ = 20;

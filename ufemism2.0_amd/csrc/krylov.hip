@@ -393,6 +393,35 @@ __global__ void __launch_bounds__(kBlock) k_galerkin(int nnzc, const int *__rest
   }
 }
 
+// l1 block-Jacobi smoother (Baker, Falgout, Kolev & Yang, SIAM J. Sci. Comput.
+// 33 (2011)): the inverse of D_i + diag(s0, s1), D_i the diagonal block of
+// block row i and s0, s1 the sums of the absolute values of the off-diagonal
+// blocks' entries in its two scalar rows.  Formed on every level after the
+// Galerkin products, once per Picard iteration.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_l1_dinv(int nb, const int *__restrict__ ptr, const int *__restrict__ col,
+                                                    const typename VecT<T>::V4 *__restrict__ val,
+                                                    typename VecT<T>::V4 *__restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  double a = 0, b = 0, c = 0, d = 0, s0 = 0, s1 = 0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const typename VecT<T>::V4 v = val[k];
+    if (col[k] == i) {
+      a = v.x; b = v.y; c = v.z; d = v.w;
+    } else {
+      s0 += fabs((double)v.x) + fabs((double)v.y);
+      s1 += fabs((double)v.z) + fabs((double)v.w);
+    }
+  }
+  a += s0;
+  d += s1;
+  const double det = a * d - b * c, id = 1.0 / det;
+  typename VecT<T>::V4 inv;
+  inv.x = (T)(d * id); inv.y = (T)(-b * id); inv.z = (T)(-c * id); inv.w = (T)(a * id);
+  dinv[i] = inv;
+}
+
 // z0 = omega D0^-1 r0  (pre-smoothing from a zero guess on the finest level)
 // The V-cycle's per-row arithmetic is written with explicit fused multiply-adds
 // and separately rounded products, so every kernel that inlines it (the
@@ -1019,6 +1048,13 @@ void Krylov::amg_setup(const BsrT<T> &A0) {
   }
   lev.assign(L, BsrT<T>());
   lev[0] = A0;
+  if (amg_l1_) {
+    if (av.dinv[0]->bytes < sizeof(V4) * (size_t)A0.nb) av.dinv[0]->alloc(sizeof(V4) * (size_t)A0.nb);
+    hipLaunchKernelGGL(k_l1_dinv<T>, dim3(blocks_for(A0.nb)), dim3(kBlock), 0, s_, A0.nb, A0.ptr, A0.col, A0.val,
+                       av.dinv[0]->template as<V4>());
+    UFE_LAUNCH_CHECK();
+    lev[0].dinv = av.dinv[0]->template as<V4>();
+  }
   for (int l = 1; l < L; ++l) {
     const AmgLevelPattern &P = *amg_->levels[l];
     BsrT<T> B;
@@ -1032,6 +1068,11 @@ void Krylov::amg_setup(const BsrT<T> &A0) {
                        P.glist.as<int>(), lev[l - 1].val, P.crow.as<int>(), P.col.as<int>(),
                        av.val[l]->template as<V4>(), av.dinv[l]->template as<V4>());
     UFE_LAUNCH_CHECK();
+    if (amg_l1_) {
+      hipLaunchKernelGGL(k_l1_dinv<T>, dim3(blocks_for(B.nb)), dim3(kBlock), 0, s_, B.nb, B.ptr, B.col, B.val,
+                         av.dinv[l]->template as<V4>());
+      UFE_LAUNCH_CHECK();
+    }
     lev[l] = B;
   }
   const BsrT<T> &Cc = lev[L - 1];
@@ -1065,7 +1106,7 @@ void Krylov::precond_apply(const BsrT<T> &A, const T *r, T *z, hipEvent_t ev0, h
   // finest-level pre-smoothing from zero, z0 = omega D^-1 r: with identity
   // diagonal blocks (the scaled system) the level-0 passes read omega * r
   // directly instead (bit-identical: 1 * x + 0 * y == x)
-  const bool fold = unit_diag0_;
+  const bool fold = unit_diag0_ && !amg_l1_;
   const V2 *zl0 = fold ? rl : z0;
   const T zs0 = fold ? om : T(1);
   if (!fold) {
@@ -1184,7 +1225,8 @@ std::vector<uintptr_t> Krylov::graph_key(const BsrT<T> &A, const T *b, const T *
   std::vector<uintptr_t> k = {(uintptr_t)A.nb, (uintptr_t)A.nnzb, (uintptr_t)A.ptr, (uintptr_t)A.col,
                               (uintptr_t)A.val, (uintptr_t)A.dinv, (uintptr_t)b, (uintptr_t)x, (uintptr_t)nbx,
                               (uintptr_t)pc_, (uintptr_t)amg_, (uintptr_t)A_diag_, (uintptr_t)fine32_,
-                              (uintptr_t)amg_tail_rows_, (uintptr_t)unit_diag0_, bits(amg_omega_),
+                              (uintptr_t)amg_tail_rows_, (uintptr_t)unit_diag0_, (uintptr_t)amg_l1_,
+                              bits(amg_omega_),
                               bits(amg_coarse_scale_), (uintptr_t)coarse_inv_.p, (uintptr_t)red_part_.p,
                               (uintptr_t)vec64_[0].p, (uintptr_t)vec32_[0].p, (uintptr_t)scal_.p};
   if (pc_ == UFE_PC_AMG && amg_) {

@@ -66,6 +66,7 @@ extern "C" void ufe_params_default(ufe_params *p) {
   p->ismip_hom_L = 0.0;
   p->slid_zi_p = 5.0;
   p->slid_zi_ut = 200.0;
+  p->amg_smoother = UFE_SMOOTHER_BLOCK_JACOBI;
   p->check_every = 8;
 }
 

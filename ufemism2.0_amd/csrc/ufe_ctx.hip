@@ -331,6 +331,7 @@ static void linear_solve(ufe_ctx *c, int *its, int *nref) {
   // KSP test measures with a block-Jacobi preconditioner.
   // the assembled system is S = D^-1 A: identity diagonal blocks
   K.set_unit_diag0(c->amg_fold);
+  K.set_amg_l1(q.amg_smoother == UFE_SMOOTHER_L1_BLOCK_JACOBI);
   if (q.precision == UFE_PREC_FP64) {
     BsrT<double> A = sys64(c);
     K.set_amg_fine32(q.precond_fp32 && q.precond == UFE_PC_AMG ? c->A32.as<float4>() : nullptr);
@@ -1022,6 +1023,8 @@ extern "C" int ufe_set_params(ufe_ctx *c, const ufe_params *p) {
           !(p->ismip_hom_L > 0.0))
         return fail(c, UFE_ERR_ARGUMENT, "periodic_ISMIP-HOM needs ismip_hom_L > 0");
     }
+    if (p->amg_smoother != UFE_SMOOTHER_BLOCK_JACOBI && p->amg_smoother != UFE_SMOOTHER_L1_BLOCK_JACOBI)
+      return fail(c, UFE_ERR_ARGUMENT, "amg_smoother must be UFE_SMOOTHER_BLOCK_JACOBI or UFE_SMOOTHER_L1_BLOCK_JACOBI");
     if (p->sliding_law < UFE_SLIDING_NO_SLIDING || p->sliding_law > UFE_SLIDING_ZOET_IVERSON)
       return fail(c, UFE_ERR_ARGUMENT, "unknown choice_sliding_law");
     if (p->sliding_law == UFE_SLIDING_ZOET_IVERSON && !(p->slid_zi_p > 0.0 && p->slid_zi_ut >= 0.0))

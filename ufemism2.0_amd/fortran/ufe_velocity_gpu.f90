@@ -62,6 +62,7 @@ module ufe_velocity_gpu
     integer(c_int) :: precond_fp32
     real(c_double) :: ismip_hom_L
     real(c_double) :: slid_zi_p, slid_zi_ut
+    integer(c_int) :: amg_smoother
   end type ufe_gpu_params
 
   ! struct ufe_solve_info

@@ -32,7 +32,8 @@ _PARAM_NAMES = [f for f, _ in L.Params._fields_]
 # multigrid preconditioner knobs (ufe_set_amg_options) and their library defaults
 AMG_DEFAULTS = {"omega": 0.85, "coarse_scale": 1.5, "tail_rows": 300}
 # coordinate search of VelocitySolver.autotune: one axis at a time, best kept
-AUTOTUNE_AXES = (("accel_depth", (2, 3)), ("omega", (0.75, 0.85, 0.95)), ("coarse_scale", (1.3, 1.5, 1.7)))
+AUTOTUNE_AXES = (("amg_smoother", (0, 1)), ("accel_depth", (2, 3)), ("omega", (0.75, 0.85, 0.95)),
+                 ("coarse_scale", (1.3, 1.5, 1.7)))
 
 
 class VelocitySolver:
