@@ -1407,11 +1407,12 @@ SolveStats Krylov::bicgstab(const BsrT<T> &A, const T *b, T *x, double rtol, dou
           // map, r (also the pre-smoothed z when folded), D^-1 unless folded, the
           // coarse correction, the output
           const double nc = amg_->n_levels() > 1 ? (double)amg_->levels[1]->nb : 0.0;
+          const bool fold = unit_diag0_ && !amg_l1_;
           tn_[1] += 2 * real;
           tbytes_[1] = (double)A.nnzb * ((fine32_ ? sizeof(float4) : sizeof(double4)) + sizeof(int)) +
                        (double)(A.nb + 1) * sizeof(int) + (double)A.nb * sizeof(int) +
-                       (unit_diag0_ ? 2.0 : 3.0) * A.nb * sizeof(double2) +
-                       (unit_diag0_ ? 0.0 : (double)A.nb * sizeof(double4)) + nc * sizeof(double2);
+                       (fold ? 2.0 : 3.0) * A.nb * sizeof(double2) +
+                       (fold ? 0.0 : (double)A.nb * sizeof(double4)) + nc * sizeof(double2);
         }
       }
       its_seen = h_scal_->its;
