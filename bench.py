@@ -172,7 +172,13 @@ def dome_full_solves(V, W, reps=3):
 def oracle_parity_at_scale(V, W):
     """The chosen form on the 20k-vertex DIVA channel (seed 2) against the
     committed oracle solution of the reference iteration (nonlinear residual
-    1e-8): max relative velocity error, normalised by max |u_oracle|."""
+    1e-8): relative velocity differences, normalised by max |u_oracle|.  Two
+    converged states of these equations differ along a weakly determined mode
+    of the floating shelf (the engine's own reference iteration ends 1.8e-2
+    from the fixture, each state with residual 1e-8 in both implementations),
+    so the median and the share of triangles within 1e-5 are reported beside
+    the maximum; the test suite checks the oracle's residual at the engine's
+    state."""
     import numpy as np
     path = os.path.join(ROOT, "tests", "golden", "oracle_channel_diva_20k.npz")
     if not os.path.exists(path):
@@ -191,8 +197,11 @@ def oracle_parity_at_scale(V, W):
     s.close()
     fu, fv = f["u"].astype(np.float64), f["v"].astype(np.float64)
     sc = max(np.abs(fu).max(), np.abs(fv).max())
+    d = np.hypot(u - fu, v - fv) / sc
     return {"workload": f"channel_diva seed 2 ({w.mesh.nV} vertices, {w.mesh.nT} triangles, nz 10)",
             "max_rel_vel_err_vs_oracle": float(max(np.abs(u - fu).max(), np.abs(v - fv).max()) / sc),
+            "median_rel_vel_err_vs_oracle": float(np.median(d)),
+            "share_of_triangles_within_1e-5": float((d <= 1e-5).mean()),
             "chosen_residual": info.rel_residual, "chosen_residual_unscaled": info.rel_residual_unscaled,
             "oracle_residual": float(f["rel_residual"]), "stop": "block-scaled nonlinear residual <= 1e-8"}
 
