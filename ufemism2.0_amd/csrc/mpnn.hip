@@ -1,7 +1,8 @@
 // Mesh message-passing network on the triangle graph: the learned Picard
 // initial guess.  Inference and training run on the device with hand-written
 // kernels; the dense layers use BF16 matrix-core instructions
-// (mfma_f32_16x16x16bf16_1k) with FP32 accumulation, everything else is FP32.
+// (gfx950's mfma_f32_16x16x32_bf16; 16x16x16bf16_1k for the 16-wide layers)
+// with FP32 accumulation, everything else is FP32.
 //
 // Network (nodes = triangles, neighbours = TriC, mean aggregation):
 //   h0  = relu(X We + be)                                   X [N,16]  We [16,64]
@@ -40,7 +41,24 @@ constexpr int kTargets = 3;  // trained outputs: log10 speed, direction (x, y)
 constexpr int kRows = 64;  // rows per workgroup tile
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// One 16x16 FP32 tile += A (16 x KS) B (KS x 16) on the matrix cores, KS = 32
+// (gfx950's v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][8 (l / 16) ..
+// +7] and B[8 (l / 16) .. +7][l % 16]) or KS = 16 (the 16x16x16 shape, for
+// the 16-wide layers: lane l holds 4 values from 4 (l / 16)).  p and q point
+// at lane l's first A and B values in LDS (16- and 8-byte aligned).
+template <int KS>
+__device__ __forceinline__ f32x4 mfma_tile(const unsigned short *p, const unsigned short *q, f32x4 acc) {
+  if constexpr (KS == 32) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8 *>(p),
+                                                   *reinterpret_cast<const bf16x8 *>(q), acc, 0, 0, 0);
+  } else {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const bf16x4 *>(p),
+                                                     *reinterpret_cast<const bf16x4 *>(q), acc, 0, 0, 0);
+  }
+}
 
 // float32 -> bfloat16, round to nearest even
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -58,13 +76,17 @@ __global__ void __launch_bounds__(256) k_rowgemm(int N, const float *__restrict_
                                                  const unsigned short *__restrict__ Wt,
                                                  const float *__restrict__ bias, const float *__restrict__ Hin,
                                                  float *__restrict__ Zout, float *__restrict__ Out) {
-  constexpr int KP = K + 4;
+  // K steps of 32 where K allows (gfx950's 16x16x32 shape), else of 16; row
+  // pitch K + 8 keeps every lane's 16-byte operand aligned and the 16 rows of
+  // a step on distinct LDS banks
+  constexpr int KS = K % 32 == 0 ? 32 : 16;
+  constexpr int KP = K + 8;
   __shared__ __align__(16) unsigned short sA[kRows * KP];
   __shared__ __align__(16) unsigned short sW[NO * KP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < NO * K; e += 256) sW[(e / K) * KP + (e % K)] = Wt[e];
   const int ar = wave * 16 + (lane & 15);
-  const int kq = 4 * (lane >> 4);
+  const int kq = (KS / 4) * (lane >> 4);
   for (int tile = blockIdx.x; tile * kRows < N; tile += gridDim.x) {
     const int r0 = tile * kRows;
     __syncthreads();
@@ -77,13 +99,10 @@ __global__ void __launch_bounds__(256) k_rowgemm(int N, const float *__restrict_
 #pragma unroll
     for (int t = 0; t < NO / 16; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int kb = 0; kb < K; kb += 16) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&sA[ar * KP + kb + kq]);
+    for (int kb = 0; kb < K; kb += KS) {
 #pragma unroll
-      for (int t = 0; t < NO / 16; ++t) {
-        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&sW[(t * 16 + (lane & 15)) * KP + kb + kq]);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-      }
+      for (int t = 0; t < NO / 16; ++t)
+        acc[t] = mfma_tile<KS>(&sA[ar * KP + kb + kq], &sW[(t * 16 + (lane & 15)) * KP + kb + kq], acc[t]);
     }
     // lane holds rows 4*(lane/16)+i, column lane%16 of each 16x16 tile
 #pragma unroll
@@ -115,12 +134,14 @@ __global__ void __launch_bounds__(256) k_rowgemm(int N, const float *__restrict_
 template <int K, int NO>
 __global__ void __launch_bounds__(256) k_colgemm(int N, const float *__restrict__ A, const float *__restrict__ B,
                                                  float *__restrict__ part) {
-  constexpr int RP = kRows + 4;
+  // the reduction runs over the tile's kRows rows in steps of 32 (16x16x32)
+  static_assert(kRows % 32 == 0, "row tile must be a multiple of 32");
+  constexpr int RP = kRows + 8;
   constexpr int TM = K / 16, TN = NO / 16, TT = TM * TN, PER = (TT + 3) / 4;
   __shared__ __align__(16) unsigned short sAt[K * RP];
   __shared__ __align__(16) unsigned short sBt[NO * RP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int kq = 4 * (lane >> 4);
+  const int kq = 8 * (lane >> 4);
   f32x4 acc[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -147,11 +168,9 @@ __global__ void __launch_bounds__(256) k_colgemm(int N, const float *__restrict_
       if (t < TT) {
         const int tm = t / TN, tn = t % TN;
 #pragma unroll
-        for (int rb = 0; rb < kRows; rb += 16) {
-          const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&sAt[(tm * 16 + (lane & 15)) * RP + rb + kq]);
-          const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&sBt[(tn * 16 + (lane & 15)) * RP + rb + kq]);
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
-        }
+        for (int rb = 0; rb < kRows; rb += 32)
+          acc[q] = mfma_tile<32>(&sAt[(tm * 16 + (lane & 15)) * RP + rb + kq],
+                                 &sBt[(tn * 16 + (lane & 15)) * RP + rb + kq], acc[q]);
       }
     }
   }
