@@ -84,6 +84,49 @@ def test_fortran_solve_matches_c_abi(V, W, tmp_path, stress):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("law", [4, 6], ids=["tsai2015", "zoet_iverson"])
+def test_fortran_sliding_laws_match_c_abi(V, W, tmp_path, law):
+    """solve_SSA_gpu with the Tsai2015 law (ufe_gpu_set_sliding_cap with
+    bed_roughness%alpha_sq and ice%effective_pressure) and with Zoet-Iverson
+    (ufe_gpu_till_yield_stress from mesh%nC / mesh%C, the effective pressure
+    and the till friction angle, in place of beta_sq) equals the Python host
+    bit for bit."""
+    from ufemism2_amd import fortran_case as FC
+    w = W.dome(h=90e3)
+    m = w.mesh
+    inp = w.inputs()
+    rng = np.random.default_rng(5)
+    N = 0.02 * 910.0 * 9.81 * np.maximum(inp["Hi"], 500.0) * rng.uniform(0.8, 1.2, m.nV)
+    # the reference iteration and its stop rule (L2_uv < 5e-5)
+    ctl = dict(stop_rule=0, visc_it_nit=1000, include_crossterms=1, precond=3, warm_start=0, accel_depth=0,
+               nonlinear_rtol=1e-8, vel_max=5000.0, abstol=0.0, rtol=1e-10)
+    land = (inp["Hi"] <= 0).astype(np.int32)
+    if law == 4:
+        alpha = rng.uniform(0.3, 0.7, m.nV)
+        sliding = dict(law=law, m=3.0, alpha_sq=alpha, N=N)
+    else:
+        phi = rng.uniform(10.0, 20.0, m.nV)
+        sliding = dict(law=law, zi_p=5.0, zi_ut=200.0, N=N, phi=phi, icefree_land=land)
+    case, out = str(tmp_path / "case.bin"), str(tmp_path / "out.bin")
+    FC.write_case(case, w, stress="SSA", sliding=sliding, **ctl)
+    print(FC.run_driver(case, out), flush=True)
+    uf, vf, nvi, nax, res = FC.read_result(out, m.nT)
+    s = V.VelocitySolver(w.descriptor(), sliding_law=law, slid_weertman_m=3.0, slid_zi_p=5.0, slid_zi_ut=200.0,
+                         **ctl)
+    beta = inp["beta_sq"]
+    if law == 4:
+        s.set_sliding_cap(alpha, N)
+    else:
+        beta = V.till_yield_stress(m, N, phi, inp["mask_grounded"], land)
+    u, v, nv2, na2, info = s.solve_SSA(inp["Hi"], inp["Hs"], inp["A_flow"], beta, inp["fgr_b"],
+                                       inp["mask_grounded"])
+    s.close()
+    assert nvi == nv2 and nvi < ctl["visc_it_nit"] and abs(res - info.rel_residual) <= 1e-12 * info.rel_residual
+    assert max_rel(uf, u) < 1e-12 and max_rel(vf, v) < 1e-12
+    assert np.hypot(u, v).max() > 1.0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("gl_bc", ["subgrid", "pmp"])
 def test_fortran_heat_step_matches_c_abi(V, W, OB, tmp_path, gl_bc):
     """solve_3D_heat_equation_gpu (column solve with the grounding-line

@@ -28,6 +28,13 @@ program ufe_fortran_driver
   real(c_double), allocatable :: u_b(:), v_b(:), ub(:), vb(:), u3(:, :), v3(:, :), eta3(:, :), tbx(:), tby(:)
   integer(c_int) :: bip(10)
   real(c_double) :: brp(5)
+  ! optional sliding block (SSA/DIVA): law, nC_mem, kind (1 Coulomb cap, 2 till
+  ! yield stress from N and the friction angle); Weertman m, ZI p, ZI u_t
+  integer(c_int) :: sip(3)
+  real(c_double) :: srp(3)
+  real(c_double), allocatable :: s_alpha(:), s_en(:), s_phi(:)
+  integer, allocatable :: s_land(:), s_nC(:), s_C(:, :)
+  logical :: has_sliding
   integer(c_int), allocatable :: aa_ptr(:), aa_ind(:)
   real(c_double), allocatable :: aa_ddx(:), aa_ddy(:), zeta_stag(:), u_bk(:, :), v_bk(:, :)
   type(ufe_gpu_context) :: g
@@ -39,7 +46,9 @@ program ufe_fortran_driver
   call get_command_argument(2, out_file)
   open(newunit=u_in, file=trim(case_file), access='stream', form='unformatted', status='old', action='read')
   read(u_in) hdr
-  nV = hdr(1); nT = hdr(2); nz = hdr(3); ab_nnz = hdr(4); ba_nnz = hdr(5); bb_nnz = hdr(6); stress = hdr(7)
+  nV = hdr(1); nT = hdr(2); nz = hdr(3); ab_nnz = hdr(4); ba_nnz = hdr(5); bb_nnz = hdr(6)
+  stress = iand(hdr(7), 15)
+  has_sliding = iand(hdr(7), 16) /= 0
   allocate(V(nV, 2), Tri(nT, 3), TriC(nT, 3), TriBI(nT), TriGC(nT, 2), zeta(nz))
   read(u_in) V, Tri, TriC, TriBI, TriGC, zeta
   allocate(ab_ptr(nT + 1), ab_ind(ab_nnz), ab_map(ab_nnz), ab_ddx(ab_nnz), ab_ddy(ab_nnz))
@@ -52,6 +61,16 @@ program ufe_fortran_driver
   read(u_in) ip, rp
   allocate(Hi(nV), Hs(nV), A_flow(nV, nz), beta_sq(nV), fgr(nT), mask(nV))
   read(u_in) Hi, Hs, A_flow, beta_sq, fgr, mask
+  if (has_sliding) then
+    read(u_in) sip, srp
+    if (sip(3) == 1) then
+      allocate(s_alpha(nV), s_en(nV))
+      read(u_in) s_alpha, s_en
+    else if (sip(3) == 2) then
+      allocate(s_en(nV), s_phi(nV), s_land(nV), s_nC(nV), s_C(nV, sip(2)))
+      read(u_in) s_en, s_phi, s_land, s_nC, s_C
+    end if
+  end if
   if (stress == 2) then
     ! BPA: sliding law, border choices, rheology and iteration controls, the
     ! a->a operators (1-based CSR) and the staggered layers
@@ -75,7 +94,18 @@ program ufe_fortran_driver
     p%glens_n = brp(1); p%epsilon_sq_0 = brp(2); p%visc_it_relax = brp(3)
     p%visc_it_norm_dUV_tol = brp(4); p%ismip_hom_L = brp(5)
   end if
+  if (has_sliding) then
+    p%sliding_law = sip(1)
+    p%slid_weertman_m = srp(1); p%slid_zi_p = srp(2); p%slid_zi_ut = srp(3)
+  end if
   call ufe_gpu_set_params(g, p)
+  if (has_sliding) then
+    ! sliding_laws.f90: the Coulomb cap of Tsai2015 / Schoof2005, or the till
+    ! yield stress of the yield-stress laws in place of beta_sq
+    if (sip(3) == 1) call ufe_gpu_set_sliding_cap(g, s_alpha, s_en)
+    if (sip(3) == 2) call ufe_gpu_till_yield_stress(s_nC, s_C, s_en, s_phi, mask /= 0, s_land /= 0, &
+                                                    0.0_c_double, beta_sq)
+  end if
 
   if (stress == 2) then
     call ufe_gpu_zeta_setup(g, aa_ptr, aa_ind, aa_ddx, aa_ddy, zeta_stag)

@@ -19,12 +19,21 @@ HEAT_COLUMN_FIELDS = ("Ti", "u_3D", "v_3D", "w_3D", "u_dTdxp_upwind", "v_dTdyp_u
                       "dzeta_dz", "dzeta_dt", "Phi")
 
 
-def write_case(path, w, stress="SSA", **ctl):
+def write_case(path, w, stress="SSA", sliding=None, **ctl):
+    """sliding (SSA / DIVA): dict(law=UFE_SLIDING_*, m=, zi_p=, zi_ut=) with
+    alpha_sq and N (the Coulomb cap of Tsai2015 / Schoof2005) or N, phi and
+    icefree_land (the driver forms the till yield stress from them and the
+    mesh connectivity, in place of beta_sq); flagged by bit 4 of the stress
+    code in the header."""
     m, ops = w.mesh, w.ops
     i32 = lambda a: np.asarray(a, dtype="<i4")  # noqa: E731
     f64 = lambda a: np.asarray(a, dtype="<f8")  # noqa: E731
     ff = lambda a: np.asfortranarray(a).ravel(order="F")  # noqa: E731
     code = {"SSA": 0, "DIVA": 1, "BPA": 2}[stress]
+    if sliding is not None:
+        if stress == "BPA":
+            raise ValueError("the sliding block is read for SSA and DIVA cases")
+        code |= 16
     parts = [i32([m.nV, m.nT, w.nz, ops.ab.nnz, ops.ba.nnz, ops.bb.nnz, code]),
              f64(ff(m.V)), i32(ff(m.Tri + 1)), i32(ff(m.TriC + 1)), i32(m.TriBI), f64(ff(m.TriGC)), f64(w.zeta)]
     for fam, names in ((ops.ab, ["map", "ddx", "ddy"]), (ops.ba, ["map", "ddx", "ddy"]),
@@ -37,6 +46,19 @@ def write_case(path, w, stress="SSA", **ctl):
     inp = w.inputs()
     parts += [f64(inp["Hi"]), f64(inp["Hs"]), f64(inp["A_flow"]), f64(inp["beta_sq"]), f64(inp["fgr_b"]),
               i32(inp["mask_grounded"])]
+    if sliding is not None:
+        kind = 1 if "alpha_sq" in sliding else 2 if "phi" in sliding else 0
+        cnt = np.diff(m.C_ptr)
+        nC_mem = int(cnt.max()) if kind == 2 else 0
+        parts.append(i32([sliding["law"], nC_mem, kind]))
+        parts.append(f64([sliding.get("m", 3.0), sliding.get("zi_p", 5.0), sliding.get("zi_ut", 200.0)]))
+        if kind == 1:
+            parts += [f64(sliding["alpha_sq"]), f64(sliding["N"])]
+        elif kind == 2:
+            C = np.zeros((m.nV, nC_mem), np.int32)
+            src = np.repeat(np.arange(m.nV), cnt)
+            C[src, np.arange(len(src)) - np.repeat(m.C_ptr[:-1], cnt)] = m.C_ind + 1
+            parts += [f64(sliding["N"]), f64(sliding["phi"]), i32(sliding["icefree_land"]), i32(cnt), i32(ff(C))]
     if stress == "BPA":
         # sliding law, borders (u then v: north, east, south, west), rheology and
         # iteration controls, the a->a operators (1-based CSR), zeta_stag
