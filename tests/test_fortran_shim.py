@@ -84,6 +84,47 @@ def test_fortran_solve_matches_c_abi(V, W, tmp_path, stress):
 
 
 @pytest.mark.gpu
+def test_fortran_heat_inputs_match_c_abi(V, W, tmp_path):
+    """The heat columns' inputs through the Fortran host (ufe_gpu_heat_setup
+    from mesh%niTri / mesh%iTri, calc_upwind_heat_flux_derivatives_gpu,
+    calc_strain_heating_gpu, calc_frictional_heating_gpu, and
+    calc_zeta_gradients_gpu after ufe_gpu_zeta_setup) equal the Python host
+    bit for bit."""
+    from ufemism2_amd import fortran_case as FC
+    from ufemism2_amd.operators import build_aa, zeta_stag
+    from test_oracle_heat_inputs import _fields
+    w = W.channel_small(target_nV=1500, stress_balance="DIVA")
+    m, nz = w.mesh, w.nz
+    inp = w.inputs()
+    Ti, _, u, v, _, _ = _fields(w, 4)
+    rng = np.random.default_rng(9)
+    u3 = np.tile(rng.uniform(10.0, 200.0, m.nT), nz)
+    v3 = np.tile(rng.uniform(-50.0, 50.0, m.nT), nz)
+    sr = [rng.normal(0.0, 1e-3, nz * m.nV) for _ in range(9)]
+    f = dict(Ti=Ti, u_vav=u, v_vav=v, u_3D_b=u3, v_3D_b=v3, strain_rates=sr, beta=rng.uniform(1e2, 1e4, m.nV),
+             uabs_base=rng.uniform(0.0, 300.0, m.nV), mask=inp["mask_grounded"], aa=build_aa(m),
+             zeta_stag=zeta_stag(w.zeta), dHi_dt=rng.normal(0.0, 1.0, m.nV), dHs_dt=rng.normal(0.0, 0.5, m.nV))
+    ctl = dict(stop_rule=1, visc_it_nit=800, include_crossterms=0, precond=3, warm_start=1, accel_depth=2,
+               accel_mixing=0.6, nonlinear_rtol=1e-8, vel_max=1e5, abstol=0.0, rtol=0.1)
+    case, out, hcase = str(tmp_path / "case.bin"), str(tmp_path / "out.bin"), str(tmp_path / "heatin.bin")
+    FC.write_case(case, w, stress="DIVA", **ctl)
+    FC.write_heat_inputs_case(hcase, m, nz, f)
+    print(FC.run_driver(case, out, extra=(hcase,)), flush=True)
+    ox_f, oy_f, phi_f, fh_f, ak_f, bk_f, bks_f = FC.read_heat_inputs_result(out + ".heatin", m.nV, m.nT, nz)
+    s = V.VelocitySolver(w.descriptor(), **ctl)
+    ox, oy = s.upwind_heat_flux_derivatives(Ti, inp["Hi"], u, v, u3, v3)
+    phi = s.strain_heating(inp["Hi"], inp["A_flow"], sr)
+    fh = s.frictional_heating(f["beta"], f["uabs_base"], f["mask"])
+    s.zeta_setup(f["aa"], f["zeta_stag"])
+    ak, bk, bks = s.zeta_gradients(inp["Hi"], inp["Hs"], f["dHi_dt"], f["dHs_dt"])
+    s.close()
+    for a, b in ((ox_f, ox), (oy_f, oy), (phi_f, phi), (fh_f, fh), *zip(ak_f, ak), *zip(bk_f, bk),
+                 *zip(bks_f, bks)):
+        assert np.array_equal(a, b)
+    assert np.abs(ox).max() > 0 and np.abs(phi).max() > 0 and np.abs(fh).max() > 0 and np.abs(ak[1]).max() > 0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("law", [4, 6], ids=["tsai2015", "zoet_iverson"])
 def test_fortran_sliding_laws_match_c_abi(V, W, tmp_path, law):
     """solve_SSA_gpu with the Tsai2015 law (ufe_gpu_set_sliding_cap with

@@ -142,9 +142,50 @@ program ufe_fortran_driver
     info%rel_residual
 
   if (command_argument_count() >= 4) call thickness_step(g, Hi, u_b, v_b)
+  if (command_argument_count() == 3) call heat_inputs(g, Hi, Hs, A_flow)
   call ufe_gpu_finalise(g)
 
 contains
+
+  ! The heat columns' inputs of thermodynamics_utilities.f90 and the zeta
+  ! gradients, from a third case file; all outputs to <out>.heatin.
+  subroutine heat_inputs(g, Hi, Hs, A_flow)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in) :: Hi(:), Hs(:), A_flow(:, :)
+    character(len=4096) :: hcase
+    integer(c_int) :: hh(2)
+    integer :: uin, uout, k
+    integer, allocatable :: niTri(:), iTri(:, :), mg(:)
+    integer(c_int), allocatable :: a_ptr(:), a_ind(:)
+    real(c_double), allocatable :: Ti(:, :), uv(:), vv(:), u3b(:, :), v3b(:, :), sr(:, :, :), beta(:), uab(:)
+    real(c_double), allocatable :: a_dx(:), a_dy(:), zs(:), dHi(:), dHs(:), ox(:, :), oy(:, :), Phi(:, :), fh(:)
+    real(c_double), allocatable :: ak(:, :, :), bk(:, :, :), bks(:, :, :)
+    call get_command_argument(3, hcase)
+    open(newunit=uin, file=trim(hcase), access='stream', form='unformatted', status='old', action='read')
+    read(uin) hh
+    allocate(niTri(nV), iTri(nV, hh(1)), Ti(nV, nz), uv(nV), vv(nV), u3b(nT, nz), v3b(nT, nz), sr(nV, nz, 9))
+    allocate(beta(nV), uab(nV), mg(nV), a_ptr(nV + 1), a_ind(hh(2)), a_dx(hh(2)), a_dy(hh(2)), zs(nz - 1))
+    allocate(dHi(nV), dHs(nV))
+    read(uin) niTri, iTri, Ti, uv, vv, u3b, v3b, sr, beta, uab, mg, a_ptr, a_ind, a_dx, a_dy, zs, dHi, dHs
+    close(uin)
+    allocate(ox(nV, nz), oy(nV, nz), Phi(nV, nz), fh(nV), ak(nV, nz, 7), bk(nT, nz, 6), bks(nT, nz - 1, 6))
+    call ufe_gpu_heat_setup(g, niTri, iTri)
+    call calc_upwind_heat_flux_derivatives_gpu(g, Ti, Hi, uv, vv, u3b, v3b, ox, oy)
+    call calc_strain_heating_gpu(g, Hi, A_flow, sr(:, :, 1), sr(:, :, 2), sr(:, :, 3), sr(:, :, 4), sr(:, :, 5), &
+                                 sr(:, :, 6), sr(:, :, 7), sr(:, :, 8), sr(:, :, 9), Phi)
+    call calc_frictional_heating_gpu(g, beta, uab, mg /= 0, fh)
+    call ufe_gpu_zeta_setup(g, a_ptr, a_ind, a_dx, a_dy, zs)
+    call calc_zeta_gradients_gpu(g, Hi, Hs, dHi, dHs, ak(:, :, 1), ak(:, :, 2), ak(:, :, 3), ak(:, :, 4), &
+                                 ak(:, :, 5), ak(:, :, 6), ak(:, :, 7), bk(:, :, 1), bk(:, :, 2), bk(:, :, 3), &
+                                 bk(:, :, 4), bk(:, :, 5), bk(:, :, 6), bks(:, :, 1), bks(:, :, 2), &
+                                 bks(:, :, 3), bks(:, :, 4), bks(:, :, 5), bks(:, :, 6))
+    open(newunit=uout, file=trim(out_file) // '.heatin', access='stream', form='unformatted', &
+         status='replace', action='write')
+    write(uout) ox, oy, Phi, fh, ak, bk, bks
+    close(uout)
+    k = 0
+    write(*, '(a)') 'heat inputs done'
+  end subroutine heat_inputs
 
   subroutine thickness_step(g, Hi, u_b, v_b)
     type(ufe_gpu_context), intent(in) :: g

@@ -25,7 +25,9 @@ module ufe_velocity_gpu
             ufe_gpu_set_params, ufe_gpu_default_params, solve_SSA_gpu, solve_DIVA_gpu, &
             ufe_gpu_fraction_gr_b, solve_3D_heat_equation_gpu, ufe_gpu_thickness_params, &
             ufe_gpu_default_thickness_params, ufe_gpu_thickness_setup, calc_dHi_dt_semiimplicit_gpu, &
-            ufe_gpu_zeta_setup, solve_BPA_gpu, ufe_gpu_set_sliding_cap, ufe_gpu_till_yield_stress
+            ufe_gpu_zeta_setup, solve_BPA_gpu, ufe_gpu_set_sliding_cap, ufe_gpu_till_yield_stress, &
+            ufe_gpu_heat_setup, calc_upwind_heat_flux_derivatives_gpu, calc_strain_heating_gpu, &
+            calc_frictional_heating_gpu, calc_zeta_gradients_gpu
 
   integer(c_int), parameter, public :: UFE_OK = 0, UFE_WARN_NOT_CONVERGED = 1
 
@@ -204,6 +206,34 @@ module ufe_velocity_gpu
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx, ptr, ind, ddx, ddy, zs
       integer(c_int), value :: nnz
+      integer(c_int) :: rc
+    end function
+    function c_ufe_heat_setup(ctx, ptr, ind) bind(C, name='ufe_heat_setup') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, ptr, ind
+      integer(c_int) :: rc
+    end function
+    function c_ufe_heat_upwind(ctx, Ti, Hi, u, v, u3, v3, ox, oy) &
+        bind(C, name='ufe_heat_upwind_derivatives') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, Ti, Hi, u, v, u3, v3, ox, oy
+      integer(c_int) :: rc
+    end function
+    function c_ufe_heat_strain(ctx, Hi, A, sr, Phi) bind(C, name='ufe_heat_strain_heating') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, Hi, A, Phi
+      type(c_ptr), intent(in) :: sr(9)
+      integer(c_int) :: rc
+    end function
+    function c_ufe_heat_friction(ctx, beta, ub, mask, fh) bind(C, name='ufe_heat_frictional_heating') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, beta, ub, mask, fh
+      integer(c_int) :: rc
+    end function
+    function c_ufe_zeta_gradients(ctx, Hi, Hs, dHi, dHs, ak, bk, bks) bind(C, name='ufe_zeta_gradients') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, Hi, Hs, dHi, dHs
+      type(c_ptr), intent(in) :: ak(7), bk(6), bks(6)
       integer(c_int) :: rc
     end function
     function c_ufe_set_sliding_cap(ctx, a, n) bind(C, name='ufe_set_sliding_cap') result(rc)
@@ -435,6 +465,91 @@ contains
     n_Axb_its = inf%n_Axb_its
     if (present(info)) info = inf
   end subroutine solve_DIVA_gpu
+
+  ! The triangles around each vertex, mesh%niTri(nV) and mesh%iTri(nV, :)
+  ! (1-based, counter-clockwise), once per context before the upwind terms.
+  subroutine ufe_gpu_heat_setup(g, niTri, iTri)
+    type(ufe_gpu_context), intent(in) :: g
+    integer, intent(in) :: niTri(:), iTri(:, :)
+    integer(c_int), allocatable, target :: p0(:), i0(:)
+    integer :: nV, vi, j, k
+    nV = size(niTri)
+    allocate(p0(nV + 1), i0(max(1, sum(niTri))))
+    k = 0
+    p0(1) = 0
+    do vi = 1, nV
+      do j = 1, niTri(vi)
+        k = k + 1
+        i0(k) = iTri(vi, j) - 1
+      end do
+      p0(vi + 1) = k
+    end do
+    call crash_on(c_ufe_heat_setup(g%ctx, c_loc(p0), c_loc(i0)), g%ctx, 'ufe_gpu_heat_setup')
+  end subroutine ufe_gpu_heat_setup
+
+  ! calc_upwind_heat_flux_derivatives (thermodynamics_utilities.f90:339-434):
+  ! Ti(nV, nz), Hi(nV), u_vav, v_vav(nV), u_3D_b, v_3D_b(nT, nz) in; the two
+  ! upwind terms (nV, nz) out.  Stops the program where the reference does.
+  subroutine calc_upwind_heat_flux_derivatives_gpu(g, Ti, Hi, u_vav, v_vav, u_3D_b, v_3D_b, &
+                                                   u_times_dTdxp_upwind, v_times_dTdyp_upwind)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Ti(:, :), Hi(:), u_vav(:), v_vav(:), u_3D_b(:, :), v_3D_b(:, :)
+    real(c_double), intent(out), target :: u_times_dTdxp_upwind(:, :), v_times_dTdyp_upwind(:, :)
+    call crash_on(c_ufe_heat_upwind(g%ctx, c_loc(Ti), c_loc(Hi), c_loc(u_vav), c_loc(v_vav), c_loc(u_3D_b), &
+                                    c_loc(v_3D_b), c_loc(u_times_dTdxp_upwind), c_loc(v_times_dTdyp_upwind)), &
+                  g%ctx, 'calc_upwind_heat_flux_derivatives_gpu')
+  end subroutine calc_upwind_heat_flux_derivatives_gpu
+
+  ! calc_strain_heating (thermodynamics_utilities.f90:24-91): the nine 3-D
+  ! strain rates (nV, nz) of ice%du_dx_3D ... ice%dw_dz_3D in; ice%internal_heating out.
+  subroutine calc_strain_heating_gpu(g, Hi, A_flow, du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, &
+                                     dw_dz, internal_heating)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Hi(:), A_flow(:, :), du_dx(:, :), du_dy(:, :), du_dz(:, :), &
+                                          dv_dx(:, :), dv_dy(:, :), dv_dz(:, :), dw_dx(:, :), dw_dy(:, :), &
+                                          dw_dz(:, :)
+    real(c_double), intent(out), target :: internal_heating(:, :)
+    type(c_ptr) :: sr(9)
+    sr = [c_loc(du_dx), c_loc(du_dy), c_loc(du_dz), c_loc(dv_dx), c_loc(dv_dy), c_loc(dv_dz), c_loc(dw_dx), &
+          c_loc(dw_dy), c_loc(dw_dz)]
+    call crash_on(c_ufe_heat_strain(g%ctx, c_loc(Hi), c_loc(A_flow), sr, c_loc(internal_heating)), g%ctx, &
+                  'calc_strain_heating_gpu')
+  end subroutine calc_strain_heating_gpu
+
+  ! calc_frictional_heating (thermodynamics_utilities.f90:93-128).
+  subroutine calc_frictional_heating_gpu(g, basal_friction_coefficient, uabs_base, mask_grounded_ice, &
+                                         frictional_heating)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: basal_friction_coefficient(:), uabs_base(:)
+    logical, intent(in) :: mask_grounded_ice(:)
+    real(c_double), intent(out), target :: frictional_heating(:)
+    integer(c_int), allocatable, target :: m0(:)
+    m0 = merge(1, 0, mask_grounded_ice)
+    call crash_on(c_ufe_heat_friction(g%ctx, c_loc(basal_friction_coefficient), c_loc(uabs_base), c_loc(m0), &
+                                      c_loc(frictional_heating)), g%ctx, 'calc_frictional_heating_gpu')
+  end subroutine calc_frictional_heating_gpu
+
+  ! calc_zeta_gradients (zeta_gradients.f90:19-212), after ufe_gpu_zeta_setup:
+  ! Hi, Hs, dHi_dt, dHs_dt (nV) in; ice%dzeta_*_ak (nV, nz), _bk (nT, nz)
+  ! and _bks (nT, nz-1) out, in the reference's order (dt, dx, dy, dz,
+  ! d2x2, d2xdy, d2y2 on ak; dx, dy, dz, d2x2, d2xdy, d2y2 on bk and bks).
+  subroutine calc_zeta_gradients_gpu(g, Hi, Hs, dHi_dt, dHs_dt, ak_dt, ak_dx, ak_dy, ak_dz, ak_dx2, ak_dxdy, &
+                                     ak_dy2, bk_dx, bk_dy, bk_dz, bk_dx2, bk_dxdy, bk_dy2, bks_dx, bks_dy, &
+                                     bks_dz, bks_dx2, bks_dxdy, bks_dy2)
+    type(ufe_gpu_context), intent(in) :: g
+    real(c_double), intent(in), target :: Hi(:), Hs(:), dHi_dt(:), dHs_dt(:)
+    real(c_double), intent(out), target :: ak_dt(:, :), ak_dx(:, :), ak_dy(:, :), ak_dz(:, :), ak_dx2(:, :), &
+                                           ak_dxdy(:, :), ak_dy2(:, :), bk_dx(:, :), bk_dy(:, :), bk_dz(:, :), &
+                                           bk_dx2(:, :), bk_dxdy(:, :), bk_dy2(:, :), bks_dx(:, :), &
+                                           bks_dy(:, :), bks_dz(:, :), bks_dx2(:, :), bks_dxdy(:, :), &
+                                           bks_dy2(:, :)
+    type(c_ptr) :: ak(7), bk(6), bks(6)
+    ak = [c_loc(ak_dt), c_loc(ak_dx), c_loc(ak_dy), c_loc(ak_dz), c_loc(ak_dx2), c_loc(ak_dxdy), c_loc(ak_dy2)]
+    bk = [c_loc(bk_dx), c_loc(bk_dy), c_loc(bk_dz), c_loc(bk_dx2), c_loc(bk_dxdy), c_loc(bk_dy2)]
+    bks = [c_loc(bks_dx), c_loc(bks_dy), c_loc(bks_dz), c_loc(bks_dx2), c_loc(bks_dxdy), c_loc(bks_dy2)]
+    call crash_on(c_ufe_zeta_gradients(g%ctx, c_loc(Hi), c_loc(Hs), c_loc(dHi_dt), c_loc(dHs_dt), ak, bk, bks), &
+                  g%ctx, 'calc_zeta_gradients_gpu')
+  end subroutine calc_zeta_gradients_gpu
 
   ! Coulomb limit of the Tsai2015 and Schoof2005 sliding laws
   ! (sliding_laws.f90:244-351): bed_roughness%alpha_sq(nV) and

@@ -165,3 +165,42 @@ def read_result(path, nT):
     its = np.frombuffer(raw, "<i4", 2, 16 * nT)
     res = np.frombuffer(raw, "<f8", 1, 16 * nT + 8)[0]
     return u, v, int(its[0]), int(its[1]), float(res)
+
+
+def write_heat_inputs_case(path, mesh, nz, f):
+    """Inputs of the driver's heat-input mode (a third argument): the triangles
+    around each vertex as mesh%niTri / mesh%iTri (1-based), Ti (nV, nz),
+    u_vav, v_vav (nV), u_3D_b, v_3D_b (nT, nz), the nine strain rates
+    (nV, nz, 9), the basal friction coefficient and speed, the grounded mask,
+    the a->a operators (1-based CSR), zeta_stag and dHi_dt, dHs_dt.  Fields
+    (n, nz) are given as (nz*n,) arrays, the first index fastest."""
+    i32 = lambda a: np.asarray(a, dtype="<i4")  # noqa: E731
+    f64 = lambda a: np.asarray(a, dtype="<f8")  # noqa: E731
+    nV = mesh.nV
+    cnt = np.diff(mesh.iTri_ptr)
+    mem = int(cnt.max())
+    it = np.zeros((nV, mem), np.int32)
+    src = np.repeat(np.arange(nV), cnt)
+    it[src, np.arange(len(src)) - np.repeat(mesh.iTri_ptr[:-1], cnt)] = mesh.iTri_ind + 1
+    aa = f["aa"]
+    parts = [i32([mem, aa.nnz]), i32(cnt), i32(np.asfortranarray(it).ravel(order="F")), f64(f["Ti"]),
+             f64(f["u_vav"]), f64(f["v_vav"]), f64(f["u_3D_b"]), f64(f["v_3D_b"])]
+    parts += [f64(np.concatenate([np.asarray(x, np.float64).ravel() for x in f["strain_rates"]]))]
+    parts += [f64(f["beta"]), f64(f["uabs_base"]), i32(f["mask"]), i32(aa.ptr + 1), i32(aa.ind + 1),
+              f64(aa.vals["ddx"]), f64(aa.vals["ddy"]), f64(f["zeta_stag"]), f64(f["dHi_dt"]), f64(f["dHs_dt"])]
+    with open(path, "wb") as fh:
+        for p in parts:
+            fh.write(np.ascontiguousarray(p).tobytes())
+
+
+def read_heat_inputs_result(path, nV, nT, nz):
+    """(u_times_dTdxp_upwind, v_times_dTdyp_upwind, Phi, frictional_heating,
+    ak[7], bk[6], bks[6]) as written by the driver's heat-input mode."""
+    d = np.fromfile(path, dtype="<f8")
+    out, o = [], 0
+    for n in (nV * nz, nV * nz, nV * nz, nV):
+        out.append(d[o:o + n]); o += n
+    ak = [d[o + j * nV * nz:o + (j + 1) * nV * nz] for j in range(7)]; o += 7 * nV * nz
+    bk = [d[o + j * nT * nz:o + (j + 1) * nT * nz] for j in range(6)]; o += 6 * nT * nz
+    bks = [d[o + j * nT * (nz - 1):o + (j + 1) * nT * (nz - 1)] for j in range(6)]
+    return (*out, ak, bk, bks)
