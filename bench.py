@@ -15,7 +15,8 @@ assembled at the final u.  Inputs are uploaded to HBM before the timed region.
   chosen form: Anderson-mixed Picard (depth 2, mixing 1.0, history restarted
                when the residual exceeds 5x the best so far) after a 5-decade
                regularisation continuation, warm-started FP64 BiCGStab with the
-               aggregation-AMG preconditioner, rtol 0.01
+               aggregation-AMG preconditioner, rtol 0.01; on the channel the
+               V-cycle smooths with l1 block Jacobi (--smoother)
   plain form:  the reference iteration (relaxation 0.2 with the adaptive
                relaxation rule, zero Krylov initial guess, rtol 1e-7), FP64
                BiCGStab with the same preconditioner
@@ -39,7 +40,8 @@ the GPU's plain and chosen forms on the same dome.
 Other keys: `roofline_out_of_cache` (the Krylov block product on the
 continental mesh, whose 326 MB per product exceed the 256 MB Infinity Cache),
 `oracle_parity_at_scale` (the chosen form against the committed oracle
-solution of a 20k-vertex DIVA channel), `thickness_step` (the semi-implicit
+solution of a 20k-vertex DIVA channel: maximum, median and share of
+triangles within 1e-5), `thickness_step` (the semi-implicit
 ice-thickness step on the same mesh with the velocity the solve left on the
 device, host arrays for the other fields), `bpa` (the BPA 3-D stress
 balance on ISMIP-HOM experiment A, the reference's BPA integrated test: a
